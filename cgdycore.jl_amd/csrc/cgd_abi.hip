@@ -1,0 +1,362 @@
+// C ABI (include/cgdycore_hip.h): context set-up, argument validation and
+// launch sequencing for the gfx950 kernels in cgd_kernels.hip.
+#include "../../include/cgdycore_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cgd_kernels.hip"
+
+struct cgd_ctx {
+  cgd_desc desc{};
+  KArgs ka{};
+  int N = 0, NN = 0, Nz = 0, NF = 0, NumG = 0;
+  int nzt = 0, nchunk = 0, block = 0;
+  size_t lds1 = 0, lds2 = 0;
+  int ncolor = 0;
+  std::vector<int> color_off;  // host offsets into d_elist per colour
+  int *d_glob = nullptr;       // [NF][NN] 0-based
+  int *d_elist = nullptr;      // elements grouped by colour
+  double *d_met = nullptr, *d_met1 = nullptr, *d_ms = nullptr, *d_bwork = nullptr;
+  bool ok = false;
+  std::string err;
+  // per-pass timing (events on the launch stream)
+  bool prof = false;
+  double prof_ms[3] = {0, 0, 0};
+  long long prof_n[3] = {0, 0, 0};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+static int fail(cgd_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(ctx, CGD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+static inline hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+static int grid_for(size_t n, int block) {
+  size_t g = (n + block - 1) / block;
+  return (int)std::min<size_t>(std::max<size_t>(g, 1), 65536);
+}
+
+extern "C" const char *cgd_last_error(const cgd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
+  if (!out) return CGD_ERR_ARG;
+  *out = nullptr;
+  cgd_ctx *c = new cgd_ctx();
+  *out = c;
+  if (!d) return fail(c, CGD_ERR_ARG, "null descriptor");
+  c->desc = *d;
+  if (d->N < 4 || d->N > 5) return fail(c, CGD_ERR_UNSUPPORTED, "OrdPoly must be 3 or 4 (N = 4 or 5)");
+  if (d->Nz < 3 || d->Nz > 256) return fail(c, CGD_ERR_ARG, "Nz must be in [3, 256]");
+  if (d->NF < 1 || d->NumG < 1) return fail(c, CGD_ERR_ARG, "NF and NumG must be positive");
+  if (d->NumV != 5) return fail(c, CGD_ERR_UNSUPPORTED, "NumV must be 5 (rho, u, v, w, rho*theta)");
+  if (d->NumTr != 0) return fail(c, CGD_ERR_UNSUPPORTED, "tracers (NumTr > 0) are not implemented");
+  if (d->precision != CGD_FP64) return fail(c, CGD_ERR_UNSUPPORTED, "unknown precision mode");
+  if (d->HorLimit || d->Turbulence || d->VerticalDiffusion || d->SurfaceFlux || d->Forcing || d->Microphysics ||
+      d->Damping || d->EDMF)
+    return fail(c, CGD_ERR_UNSUPPORTED, "only the dry CompressibleShallow path without physics hooks is implemented");
+  if (d->HyperDDivW != 0.0) return fail(c, CGD_ERR_UNSUPPORTED, "HyperDDivW > 0 (w hyperviscosity) is not implemented");
+  if (!d->Glob || !d->DS || !d->DW || !d->dXdxI || !d->J || !d->X || !d->M || !d->dz)
+    return fail(c, CGD_ERR_ARG, "null static input pointer");
+  if ((size_t)d->NF * d->N * d->N * d->Nz > (size_t)1 << 40) return fail(c, CGD_ERR_ARG, "grid too large");
+  hipStream_t st = S(stream);
+  c->N = d->N;
+  c->NN = d->N * d->N;
+  c->Nz = d->Nz;
+  c->NF = d->NF;
+  c->NumG = d->NumG;
+  const int NN = c->NN, Nz = c->Nz, NF = c->NF;
+  // level chunks of at most 32 levels (NN * nzt <= 1024 threads, tiles < 64 KiB)
+  int maxl = 1024 / NN;
+  if (maxl > 32) maxl = 32;
+  c->nchunk = (Nz + maxl - 1) / maxl;
+  c->nzt = (Nz + c->nchunk - 1) / c->nchunk;
+  c->block = ((NN * c->nzt + 63) / 64) * 64;
+  c->lds1 = sizeof(double) * (2 * NN + 5 * NN * c->nzt);
+  c->lds2 = sizeof(double) * (2 * NN + 8 * NN * c->nzt);
+
+  // global numbering -> 0-based int32, range check
+  std::vector<int64_t> hglob((size_t)NN * NF);
+  HIPCHK(c, hipMemcpyAsync(hglob.data(), d->Glob, hglob.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  std::vector<double> hds(NN), hdw(NN);
+  HIPCHK(c, hipMemcpyAsync(hds.data(), d->DS, NN * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipMemcpyAsync(hdw.data(), d->DW, NN * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  std::vector<int> g32((size_t)NN * NF);
+  for (size_t i = 0; i < g32.size(); ++i) {
+    if (hglob[i] < 1 || hglob[i] > d->NumG) return fail(c, CGD_ERR_ARG, "Glob entry outside 1..NumG");
+    g32[i] = (int)(hglob[i] - 1);
+  }
+  // greedy element colouring: no two elements of one colour share a node
+  std::vector<int> nptr(d->NumG + 1, 0);
+  for (int v : g32) nptr[v + 1]++;
+  for (int i = 0; i < d->NumG; ++i) nptr[i + 1] += nptr[i];
+  std::vector<int> nel(nptr[d->NumG]);
+  {
+    std::vector<int> fillp(nptr.begin(), nptr.end() - 1);
+    for (int e = 0; e < NF; ++e)
+      for (int i = 0; i < NN; ++i) nel[fillp[g32[(size_t)e * NN + i]]++] = e;
+  }
+  std::vector<int> color(NF, -1);
+  int ncol = 0;
+  std::vector<int> mark(64, -1);
+  for (int e = 0; e < NF; ++e) {
+    for (int i = 0; i < NN; ++i) {
+      int gn = g32[(size_t)e * NN + i];
+      for (int q = nptr[gn]; q < nptr[gn + 1]; ++q) {
+        int o = nel[q];
+        if (o != e && color[o] >= 0) mark[color[o]] = e;
+      }
+    }
+    int col = 0;
+    while (col < 64 && mark[col] == e) ++col;
+    if (col >= 64) return fail(c, CGD_ERR_ARG, "element colouring needs more than 64 colours");
+    color[e] = col;
+    ncol = std::max(ncol, col + 1);
+  }
+  c->ncolor = ncol;
+  c->color_off.assign(ncol + 1, 0);
+  for (int e = 0; e < NF; ++e) c->color_off[color[e] + 1]++;
+  for (int k = 0; k < ncol; ++k) c->color_off[k + 1] += c->color_off[k];
+  std::vector<int> elist(NF);
+  {
+    std::vector<int> fp(c->color_off.begin(), c->color_off.end() - 1);
+    for (int e = 0; e < NF; ++e) elist[fp[color[e]]++] = e;
+  }
+  // device buffers (set-up only; nothing is allocated on the step path)
+  const size_t npt = (size_t)NF * NN * Nz;
+  HIPCHK(c, hipMalloc(&c->d_glob, g32.size() * sizeof(int)));
+  HIPCHK(c, hipMalloc(&c->d_elist, (size_t)NF * sizeof(int)));
+  HIPCHK(c, hipMalloc(&c->d_met, npt * CGD_NMET * sizeof(double)));
+  HIPCHK(c, hipMalloc(&c->d_met1, npt * CGD_NMET1 * sizeof(double)));
+  HIPCHK(c, hipMalloc(&c->d_ms, (size_t)d->NumG * Nz * sizeof(double)));
+  HIPCHK(c, hipMalloc(&c->d_bwork, (size_t)d->NumG * (Nz - 1) * sizeof(double)));
+  HIPCHK(c, hipMemcpyAsync(c->d_glob, g32.data(), g32.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->d_elist, elist.data(), elist.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_relayout_metric, dim3(grid_for(npt, 256)), dim3(256), 0, st, d->dXdxI, d->J, d->X, c->d_met,
+                     c->d_met1, NN, Nz, NF, d->Omega);
+  HIPCHK(c, hipGetLastError());
+  const size_t ng = (size_t)d->NumG * Nz;
+  hipLaunchKernelGGL(k_msum, dim3(grid_for(ng, 256)), dim3(256), 0, st, d->M, c->d_ms, ng);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(st));
+
+  KArgs &a = c->ka;
+  a.Nz = Nz;
+  a.NumG = d->NumG;
+  a.nzt = c->nzt;
+  a.Grav = d->Grav;
+  a.Omega = d->Omega;
+  a.Rd = d->Rd;
+  a.Cpd = d->Cpd;
+  a.p0 = d->p0;
+  a.kappa = d->kappa;
+  a.Kc = d->HyperDCurl;
+  a.Kg = d->HyperDGrad;
+  a.Kd = d->HyperDDiv;
+  // DS, DW arrive column-major (Julia): DS[i,k] at i + N*k
+  for (int i = 0; i < c->N; ++i)
+    for (int k = 0; k < c->N; ++k) {
+      a.D[i * c->N + k] = hds[i + c->N * k];
+      a.DWm[i * c->N + k] = hdw[i + c->N * k];
+    }
+  a.glob = c->d_glob;
+  a.met = c->d_met;
+  a.met1 = c->d_met1;
+  a.Ms = c->d_ms;
+  a.M = d->M;
+  HIPCHK(c, hipEventCreate(&c->ev[0]));
+  HIPCHK(c, hipEventCreate(&c->ev[1]));
+  c->ok = true;
+  return CGD_OK;
+}
+
+extern "C" int cgd_ctx_destroy(cgd_ctx *c) {
+  if (!c) return CGD_ERR_ARG;
+  (void)hipFree(c->d_glob);
+  (void)hipFree(c->d_elist);
+  (void)hipFree(c->d_met);
+  (void)hipFree(c->d_met1);
+  (void)hipFree(c->d_ms);
+  (void)hipFree(c->d_bwork);
+  if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
+  if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
+  delete c;
+  return CGD_OK;
+}
+
+static int check(cgd_ctx *c) {
+  if (!c) return CGD_ERR_ARG;
+  if (!c->ok) return fail(c, CGD_ERR_STATE, "context creation failed: " + c->err);
+  return CGD_OK;
+}
+
+// timing helpers around a group of launches of one pass
+static void prof_begin(cgd_ctx *c, hipStream_t st) {
+  if (c->prof) (void)hipEventRecord(c->ev[0], st);
+}
+static int prof_end(cgd_ctx *c, hipStream_t st, int which, int launches) {
+  if (!c->prof) return CGD_OK;
+  HIPCHK(c, hipEventRecord(c->ev[1], st));
+  HIPCHK(c, hipEventSynchronize(c->ev[1]));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+  c->prof_ms[which] += ms;
+  c->prof_n[which] += launches;
+  return CGD_OK;
+}
+
+extern "C" int cgd_profile_enable(cgd_ctx *c, int on) {
+  int r = check(c);
+  if (r) return r;
+  c->prof = on != 0;
+  for (int i = 0; i < 3; ++i) c->prof_ms[i] = 0, c->prof_n[i] = 0;
+  return CGD_OK;
+}
+
+extern "C" int cgd_profile_read(cgd_ctx *c, int which, double *total_ms, long long *launches) {
+  int r = check(c);
+  if (r) return r;
+  if (which < 0 || which > 2 || !total_ms || !launches) return fail(c, CGD_ERR_ARG, "bad profile query");
+  *total_ms = c->prof_ms[which];
+  *launches = c->prof_n[which];
+  return CGD_OK;
+}
+
+static int fill(cgd_ctx *c, double *x, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, st, x, n, 0.0);
+  HIPCHK(c, hipGetLastError());
+  return CGD_OK;
+}
+
+extern "C" int cgd_prepare(cgd_ctx *c, const double *U, double *AuxG, void *stream) {
+  int r = check(c);
+  if (r) return r;
+  if (!U || !AuxG) return fail(c, CGD_ERR_ARG, "cgd_prepare: null array");
+  const size_t n = (size_t)c->NumG * c->Nz;
+  hipLaunchKernelGGL(k_prepare, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), c->ka, U, AuxG, n);
+  HIPCHK(c, hipGetLastError());
+  return CGD_OK;
+}
+
+extern "C" int cgd_fcn(cgd_ctx *c, double *F, const double *U, double *Temp1, const double *AuxG, void *stream) {
+  int r = check(c);
+  if (r) return r;
+  if (!F || !U || !Temp1 || !AuxG) return fail(c, CGD_ERR_ARG, "cgd_fcn: null array");
+  hipStream_t st = S(stream);
+  const size_t n5 = 5 * (size_t)c->NumG * c->Nz;
+  if ((r = fill(c, Temp1, n5, st))) return r;
+  prof_begin(c, st);
+  int launches = 0;
+  for (int k = 0; k < c->ncolor; ++k) {
+    const int ne = c->color_off[k + 1] - c->color_off[k];
+    if (!ne) continue;
+    const int *el = c->d_elist + c->color_off[k];
+    if (c->N == 5)
+      hipLaunchKernelGGL(k_fcn1<5>, dim3(ne * c->nchunk), dim3(c->block), c->lds1, st, c->ka, el, c->nchunk, U, Temp1);
+    else
+      hipLaunchKernelGGL(k_fcn1<4>, dim3(ne * c->nchunk), dim3(c->block), c->lds1, st, c->ka, el, c->nchunk, U, Temp1);
+    HIPCHK(c, hipGetLastError());
+    ++launches;
+  }
+  if ((r = prof_end(c, st, 0, launches))) return r;
+  if ((r = fill(c, F, n5, st))) return r;
+  prof_begin(c, st);
+  launches = 0;
+  for (int k = 0; k < c->ncolor; ++k) {
+    const int ne = c->color_off[k + 1] - c->color_off[k];
+    if (!ne) continue;
+    const int *el = c->d_elist + c->color_off[k];
+    for (int ch = 0; ch < c->nchunk; ++ch) {
+      if (c->N == 5)
+        hipLaunchKernelGGL(k_fcn2<5>, dim3(ne), dim3(c->block), c->lds2, st, c->ka, el, ch, U, Temp1, AuxG, F);
+      else
+        hipLaunchKernelGGL(k_fcn2<4>, dim3(ne), dim3(c->block), c->lds2, st, c->ka, el, ch, U, Temp1, AuxG, F);
+      HIPCHK(c, hipGetLastError());
+      ++launches;
+    }
+  }
+  return prof_end(c, st, 1, launches);
+}
+
+extern "C" int cgd_jac(cgd_ctx *c, double *JRhoW, double *JWRho, double *JWRhoTh, double *JRhoThW, const double *U,
+                       const double *AuxG, void *stream) {
+  int r = check(c);
+  if (r) return r;
+  if (!JRhoW || !JWRho || !JWRhoTh || !JRhoThW || !U || !AuxG) return fail(c, CGD_ERR_ARG, "cgd_jac: null array");
+  const size_t n = (size_t)c->NumG * (c->Nz - 1);
+  const size_t ng = (size_t)c->NumG * c->Nz;
+  hipLaunchKernelGGL(k_jac, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), c->ka, U, AuxG + 3 * ng, c->desc.dz,
+                     JRhoW, JWRho, JWRhoTh, JRhoThW);
+  HIPCHK(c, hipGetLastError());
+  return CGD_OK;
+}
+
+extern "C" int cgd_schur(cgd_ctx *c, double *k, double *v, const double *JRhoW, const double *JWRho,
+                         const double *JWRhoTh, const double *JRhoThW, double *tri, double fac, int comp_tri,
+                         void *stream) {
+  int r = check(c);
+  if (r) return r;
+  if (!k || !v || !JRhoW || !JWRho || !JWRhoTh || !JRhoThW || !tri) return fail(c, CGD_ERR_ARG, "cgd_schur: null array");
+  if (!(fac > 0.0)) return fail(c, CGD_ERR_ARG, "cgd_schur: fac must be positive");
+  hipStream_t st = S(stream);
+  prof_begin(c, st);
+  hipLaunchKernelGGL(k_schur, dim3(grid_for(c->NumG, 64)), dim3(64), 0, st, c->ka, k, v, JRhoW, JWRho, JWRhoTh,
+                     JRhoThW, tri, c->d_bwork, fac, comp_tri);
+  HIPCHK(c, hipGetLastError());
+  return prof_end(c, st, 2, 1);
+}
+
+extern "C" int cgd_rw_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, void *stream) {
+  int r = check(c);
+  if (r) return r;
+  if (!V || !w || !w->Vn || !w->fV || !w->k || !w->AuxG || !w->Temp1 || !w->JRhoW || !w->JWRho || !w->JWRhoTh ||
+      !w->JRhoThW || !w->tri)
+    return fail(c, CGD_ERR_ARG, "cgd_rw_step: null array");
+  if (!(dt > 0.0)) return fail(c, CGD_ERR_ARG, "cgd_rw_step: dt must be positive");
+  hipStream_t st = S(stream);
+  // SSP-Knoth: a = alpha/Gamma, c = -inv(Gamma), m = Gamma'\b, gamma = 1
+  const double A[3][3] = {{0, 0, 0}, {1.0, 0, 0}, {0.25, 0.25, 0}};
+  const double Cc[3][3] = {{-1.0, 0, 0}, {-0.0, -1.0, 0}, {-0.75, -0.75, -1.0}};
+  const double m3 = 2.0 / 3.0, m2 = 1.0 / 6.0 - (-0.75) * m3, m1 = 1.0 / 6.0 - (-0.75) * m3;
+  const double Mw[3] = {m1, m2, m3};
+  const double gamma = 1.0;
+  const size_t n5 = 5 * (size_t)c->NumG * c->Nz;
+  const int gb = grid_for(n5, 256);
+  double *k[3] = {w->k, w->k + n5, w->k + 2 * n5};
+  HIPCHK(c, hipMemcpyAsync(w->Vn, V, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  for (int is = 0; is < 3; ++is) {
+    if (is > 0) {
+      hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], A[is][0], k[1], A[is][1], is, n5);
+      HIPCHK(c, hipGetLastError());
+    }
+    if ((r = cgd_prepare(c, V, w->AuxG, stream))) return r;
+    if ((r = cgd_fcn(c, w->fV, V, w->Temp1, w->AuxG, stream))) return r;
+    if (is == 0 && (r = cgd_jac(c, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, V, w->AuxG, stream))) return r;
+    if (is > 0) {
+      hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, w->fV, w->fV, k[0], Cc[is][0] / dt, k[1],
+                         Cc[is][1] / dt, is, n5);
+      HIPCHK(c, hipGetLastError());
+    }
+    if ((r = cgd_schur(c, k[is], w->fV, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma, is == 0,
+                       stream)))
+      return r;
+  }
+  hipLaunchKernelGGL(k_axpy3, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], Mw[0], k[1], Mw[1], k[2], Mw[2], n5);
+  HIPCHK(c, hipGetLastError());
+  return CGD_OK;
+}

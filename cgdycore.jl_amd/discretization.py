@@ -1,0 +1,337 @@
+"""Static inputs of the hot path: GLL basis, derivative matrices, sphere metric,
+DSS mass and layer thicknesses.
+
+Host-side setup (runs once).  Array layouts are the reference's Julia
+column-major layouts; a Julia array of dims ``(a, b, c)`` is held here as a
+C-ordered numpy array of shape ``(c, b, a)`` so that the bytes are identical:
+
+==========  ======================  ===========================
+name        Julia dims              numpy shape
+==========  ======================  ===========================
+dXdxI       (3,3,2,N^2,Nz,NF)       (NF, Nz, N^2, 2, 3, 3)
+J           (N^2,2,Nz,NF)           (NF, Nz, 2, N^2)
+X           (N^2,2,3,Nz,NF)         (NF, Nz, 3, 2, N^2)
+M           (Nz,NumG,2)             (2, NumG, Nz)
+dz          (Nz,NumG)               (NumG, Nz)
+Glob        (N^2,NF)  (1-based)     (NF, N^2)
+==========  ======================  ===========================
+
+Reference files restated here:
+``src/DyCore/FiniteElement.jl:34-89`` (CGQuad), ``src/DG/DerivativeMatrixSingle.jl:1-19``,
+``src/DG/DLagrange.jl:1-10``, ``src/Grids/AddVerticalGrid.jl:1-16``,
+``src/Grids/Adapt.jl:13-31`` (Sleve, the default ``AdaptGridType``),
+``src/Grids/JacobiSphere3GPU.jl:102-215,413-417`` (shallow sphere metric),
+``src/DyCore/MassCG.jl:61-75`` and ``src/DyCore/DiscretizationCG.jl:119-144``.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import grid as _grid
+
+
+@dataclass(frozen=True)
+class Phys:
+    """``PhysParameters{FT}()`` (src/DyCore/GlobalVariables.jl:188-230)."""
+
+    RadEarth: float = 6.37122e6
+    Grav: float = 9.80616
+    Cpd: float = 1004.0
+    Cvd: float = 717.0
+    p0: float = 1.0e5
+
+    @property
+    def Rd(self):
+        return self.Cpd - self.Cvd
+
+    @property
+    def kappa(self):
+        return self.Rd / self.Cpd
+
+    @property
+    def Omega(self):
+        return 2 * math.pi / 24.0 / 3600.0
+
+
+def _legendre(n, x):
+    """P_n(x) and P_n'(x) by the three-term recurrence."""
+    p0, p1 = np.ones_like(x), x.copy()
+    if n == 0:
+        return p0, np.zeros_like(x)
+    for k in range(2, n + 1):
+        p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
+    dp = n * (x * p1 - p0) / (x * x - 1.0)
+    return p1, dp
+
+
+def gausslobatto(n: int):
+    """Gauss-Lobatto-Legendre nodes (ascending) and weights on n points.
+
+    Same quadrature as ``FastGaussQuadrature.gausslobatto`` used at
+    FiniteElement.jl:45.  Interior nodes are the roots of P'_{n-1}, found by
+    Newton's method from Chebyshev-Gauss-Lobatto guesses; weights are
+    2 / (n (n-1) P_{n-1}(x)^2).
+    """
+    if n < 2:
+        raise ValueError("gausslobatto needs n >= 2")
+    m = n - 1
+    x = -np.cos(np.pi * np.arange(n) / m)
+    xi = x[1:-1].copy()
+    for _ in range(100):
+        # roots of P'_m: Newton on q = P'_m with q' from the Legendre ODE
+        p, dp = _legendre(m, xi)
+        ddp = (2 * xi * dp - m * (m + 1) * p) / (1.0 - xi * xi)
+        step = dp / ddp
+        xi -= step
+        if np.max(np.abs(step)) < 1e-16:
+            break
+    x[1:-1] = xi
+    x[0], x[-1] = -1.0, 1.0
+    # symmetrise
+    x = 0.5 * (x - x[::-1])
+    p, _ = _legendre(m, x)
+    w = 2.0 / (m * (m + 1) * p * p)
+    w = 0.5 * (w + w[::-1])
+    return x, w
+
+
+def dlagrange(x: float, xp: np.ndarray, ip: int) -> float:
+    """``DLagrange(x,xP,iP)`` (DLagrange.jl:1-10), 0-based ``ip``."""
+    df = 0.0
+    f = 1.0
+    for i in range(len(xp)):
+        if i != ip:
+            df = df * (x - xp[i]) / (xp[ip] - xp[i]) + f / (xp[ip] - xp[i])
+            f = f * (x - xp[i]) / (xp[ip] - xp[i])
+    return df
+
+
+def derivative_matrices(p: int):
+    """``DerivativeMatrixSingle(OrdPoly)`` (DerivativeMatrixSingle.jl:1-19).
+
+    Returns ``(DW, DS, xw, w)`` with ``DS[i,j] = l_j'(x_i)`` and
+    ``DW = -inv(diagm(w)) * DS' * diagm(w)`` evaluated entrywise as
+    ``(-(1/w_i) * DS[j,i]) * w_j``.
+    """
+    xw, w = gausslobatto(p + 1)
+    n = p + 1
+    ds = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            ds[i, j] = dlagrange(xw[i], xw, j)
+    dw = np.zeros((n, n))
+    for i in range(n):
+        for j in range(n):
+            dw[i, j] = (-(1.0 / w[i]) * ds[j, i]) * w[j]
+    return dw, ds, xw, w
+
+
+def vertical_grid(nz: int, H: float):
+    """``AddVerticalGrid!`` (AddVerticalGrid.jl:1-16): uniform interfaces z[0..nz]."""
+    z = np.zeros(nz + 1)
+    dzeta = H / nz
+    for i in range(1, nz + 1):
+        z[i] = z[i - 1] + dzeta
+    return z
+
+
+def sleve(z_ref, zs, H, eta_h=0.7, s=0.8):
+    """``Sleve`` boundary-following height (Adapt.jl:13-31)."""
+    eta = z_ref / H
+    lower = eta * H + zs * np.sinh((eta_h - eta) / s / eta_h) / np.sinh(1.0 / s)
+    return np.where(eta <= eta_h, lower, eta * H)
+
+
+@dataclass
+class Discretization:
+    """All static inputs of the tendency / column kernels for one grid."""
+
+    ne: int
+    p: int
+    nz: int
+    H: float
+    rad: float
+    grid: "_grid.CubedSphere"
+    glob: np.ndarray  # (NF, N^2) int64, 1-based
+    num_g: int
+    DS: np.ndarray
+    DW: np.ndarray
+    xw: np.ndarray
+    w: np.ndarray
+    z: np.ndarray  # (nz+1,)
+    dXdxI: np.ndarray
+    J: np.ndarray
+    X: np.ndarray
+    M: np.ndarray
+    dz: np.ndarray
+
+    @property
+    def N(self):
+        return self.p + 1
+
+    @property
+    def nf(self):
+        return self.glob.shape[0]
+
+    @property
+    def element_points(self):
+        return self.nf * self.N * self.N * self.nz
+
+
+def sphere_metric(face_corners: np.ndarray, xw: np.ndarray, DS: np.ndarray, z: np.ndarray,
+                  rad: float, H: float):
+    """Shallow-sphere metric (JacobiSphere3GPU.jl:102-215) for flat topography.
+
+    ``face_corners``: (NF, 4, 3).  Returns ``X, dXdxI, J`` in the numpy
+    shapes of the module docstring.  The horizontal 2x2 block does not depend
+    on height (``f = Rad/r`` uses the bilinear point), the vertical row is the
+    derivative of the Sleve height with ``zs = 0``.
+    """
+    nf = face_corners.shape[0]
+    n = xw.shape[0]
+    nz = z.shape[0] - 1
+    zeta = np.array([-1.0, 1.0])  # gausslobatto(2)
+    ksi1 = xw[None, None, :]  # I axis (last)
+    ksi2 = xw[None, :, None]  # J axis
+    Fc = face_corners  # (NF, corner, coord)
+    one = 1.0
+
+    def bil(c):
+        f1 = Fc[:, 0, c][:, None, None]
+        f2 = Fc[:, 1, c][:, None, None]
+        f3 = Fc[:, 2, c][:, None, None]
+        f4 = Fc[:, 3, c][:, None, None]
+        return 0.25 * (f1 * (one - ksi1) * (one - ksi2) + f2 * (one + ksi1) * (one - ksi2)
+                       + f3 * (one + ksi1) * (one + ksi2) + f4 * (one - ksi1) * (one + ksi2))
+
+    X1, X2, X3 = bil(0), bil(1), bil(2)  # (NF, J, I)
+    r = np.sqrt(X1 * X1 + X2 * X2 + X3 * X3)
+    f = rad / r
+    # cart2sphere (Geometry.jl:65-89)
+    rr = np.sqrt(X1 * X1 + X2 * X2 + X3 * X3)
+    theta = np.arcsin(X3 / rr)
+    lam = np.arctan2(X2, X1)
+    lam = np.where(X2 < 0.0, lam + 2 * math.pi, lam)
+    sinlam, coslam = np.sin(lam), np.cos(lam)
+    sinth, costh = np.sin(theta), np.cos(theta)
+    a11 = sinlam * sinlam * costh * costh + sinth * sinth
+    a12 = -sinlam * coslam * costh * costh
+    a13 = -coslam * sinth * costh
+    a21 = a12
+    a22 = coslam * coslam * costh * costh + sinth * sinth
+    a23 = -sinlam * sinth * costh
+    a31 = -coslam * sinth
+    a32 = -sinlam * sinth
+    a33 = costh
+    A = [[a11, a12, a13], [a21, a22, a23], [a31, a32, a33]]
+    # ((((1/4 f) DD) A) B) C with DD = [-sin(lam) cos(lam) 0; 0 0 1]
+    s = 0.25 * f
+    DD = [[s * -sinlam, s * coslam, s * 0.0], [s * 0.0, s * 0.0, s * one]]
+    DA = [[DD[i][0] * A[0][j] + DD[i][1] * A[1][j] + DD[i][2] * A[2][j] for j in range(3)]
+          for i in range(2)]
+    B = [[Fc[:, k, c][:, None, None] for k in range(4)] for c in range(3)]
+    DAB = [[DA[i][0] * B[0][k] + DA[i][1] * B[1][k] + DA[i][2] * B[2][k] for k in range(4)]
+           for i in range(2)]
+    C = [[-one + ksi2, -one + ksi1], [one - ksi2, -one - ksi1],
+         [one + ksi2, one + ksi1], [-one - ksi2, one - ksi1]]
+    L = [[DAB[i][0] * C[0][j] + DAB[i][1] * C[1][j] + DAB[i][2] * C[2][j] + DAB[i][3] * C[3][j]
+          for j in range(2)] for i in range(2)]  # each (NF, J, I)
+
+    # vertical: hR[K, Iz] (flat topography -> independent of I, J, IF)
+    zl = np.empty((2, nz))
+    for K in range(2):
+        zl[K] = 0.5 * ((one - zeta[K]) * z[:-1] + (one + zeta[K]) * z[1:])
+    hR = sleve(zl, 0.0, H)  # (2, Nz)
+    # DxhR = sum_k D[I,k] hR (same value at every k), DyhR likewise
+    dxh = np.zeros((n, 2, nz))
+    for I in range(n):
+        acc = DS[I, 0] * hR
+        for k in range(1, n):
+            acc = acc + DS[I, k] * hR
+        dxh[I] = acc
+    dzh = 0.5 * (hR[1] - hR[0])  # (Nz,)
+
+    # assemble per-point quantities, shape (NF, Nz, J, I, K)
+    def hor(a):
+        return a[:, None, :, :, None]
+
+    L11, L12, L21, L22 = hor(L[0][0]), hor(L[0][1]), hor(L[1][0]), hor(L[1][1])
+    L13 = 0.0
+    L23 = 0.0
+    L31 = np.transpose(dxh, (2, 0, 1))[None, :, None, :, :]  # (1, Nz, 1, I, K)
+    L32 = np.transpose(dxh, (2, 0, 1))[None, :, :, None, :]  # (1, Nz, J, 1, K)
+    L33 = np.broadcast_to(dzh[None, :, None, None, None], (1, nz, 1, 1, 2))
+    shape = (nf, nz, n, n, 2)
+
+    def full(a):
+        return np.broadcast_to(a, shape)
+
+    jac = L11 * (L22 * L33 - L23 * L32) - L12 * (L21 * L33 - L23 * L31) + L13 * (L21 * L32 - L22 * L31)
+    d = np.empty(shape + (3, 3))  # [..., j, i] = dXdxI[i, j]
+    d[..., 0, 0] = full(L22 * L33 - L23 * L32)
+    d[..., 0, 1] = full(-(L21 * L33 - L23 * L31))
+    d[..., 0, 2] = full(L21 * L32 - L22 * L31)
+    d[..., 1, 0] = full(-(L12 * L33 - L13 * L32))
+    d[..., 1, 1] = full(L11 * L33 - L13 * L31)
+    d[..., 1, 2] = full(-(L11 * L32 - L12 * L31))
+    d[..., 2, 0] = full(L12 * L23 - L13 * L22)
+    d[..., 2, 1] = full(-(L11 * L23 - L13 * L21))
+    d[..., 2, 2] = full(L11 * L22 - L12 * L21)
+    dXdxI = d.reshape(nf, nz, n * n, 2, 3, 3)
+    J = np.ascontiguousarray(np.transpose(full(jac), (0, 1, 4, 2, 3))).reshape(nf, nz, 2, n * n)
+    # X[ID,K,c,Iz,IF] = Xc / r * (Rad + hR)
+    X = np.empty((nf, nz, 3, 2, n, n))
+    for c, Xc in enumerate((X1, X2, X3)):
+        xr = (Xc / r)[:, None, None, :, :]
+        X[:, :, c] = xr * (rad + hR.T[None, :, :, None, None])
+    X = X.reshape(nf, nz, 3, 2, n * n)
+    return X, dXdxI, J
+
+
+def mass_cg(J: np.ndarray, glob: np.ndarray, num_g: int):
+    """``MassCGKernel!`` (MassCG.jl:61-75): M[Iz,ind,K] = sum of J over copies.
+
+    Summed in ascending (IF, ID) order.  Returns (2, NumG, Nz).
+    """
+    nf, nz, _, nn = J.shape
+    M = np.zeros((2, num_g, nz))
+    idx = glob.reshape(-1) - 1  # (NF*N^2,) in (IF, ID) order
+    for K in range(2):
+        vals = np.transpose(J[:, :, K, :], (0, 2, 1)).reshape(nf * nn, nz)
+        np.add.at(M[K], idx, vals)
+    return M
+
+
+def layer_thickness(X: np.ndarray, glob: np.ndarray, num_g: int):
+    """``GridSizeSphereKernel!`` (DiscretizationCG.jl:119-144): dz = r2 - r1.
+
+    Every element sharing a node writes it; the last writer in (IF, ID)
+    order is kept.  Returns (NumG, Nz).
+    """
+    nf, nz, _, _, nn = X.shape
+    r = np.sqrt(X[:, :, 0] * X[:, :, 0] + X[:, :, 1] * X[:, :, 1] + X[:, :, 2] * X[:, :, 2])
+    dzl = r[:, :, 1, :] - r[:, :, 0, :]  # (NF, Nz, N^2)
+    vals = np.transpose(dzl, (0, 2, 1)).reshape(nf * nn, nz)
+    idx = glob.reshape(-1) - 1
+    dz = np.zeros((num_g, nz))
+    last = np.full(num_g, -1, dtype=np.int64)
+    last[idx] = np.arange(idx.size)  # numpy keeps the last assignment
+    dz[:] = vals[last]
+    return dz
+
+
+def build(ne: int, p: int = 4, nz: int = 64, H: float = 30000.0,
+          rad: float = 6.37122e6) -> Discretization:
+    """Grid + numbering + metric + mass for one configuration."""
+    g = _grid.cubed_sphere(ne, rad)
+    glob, num_g = _grid.numbering_cg_quad(g, p)
+    DW, DS, xw, w = derivative_matrices(p)
+    z = vertical_grid(nz, H)
+    X, dXdxI, J = sphere_metric(g.face_corners(), xw, DS, z, rad, H)
+    M = mass_cg(J, glob, num_g)
+    dz = layer_thickness(X, glob, num_g)
+    return Discretization(ne=ne, p=p, nz=nz, H=H, rad=rad, grid=g, glob=glob, num_g=num_g,
+                          DS=DS, DW=DW, xw=xw, w=w, z=z, dXdxI=dXdxI, J=J, X=X, M=M, dz=dz)

@@ -1,0 +1,400 @@
+"""Host-side mirror of the reference's hot-path interface over the C ABI.
+
+Reference call signatures kept (``!`` dropped, Julia structs mapped to small
+Python classes holding torch device tensors):
+
+* ``FcnPrepareGPU(U, FE, Metric, Phys, Cache, Exchange, Global, Param, DiscType)``
+  -- ``src/GPU/DiagnosticKernel.jl:180``
+* ``FcnGPU(F, U, FE, Metric, Phys, Cache, Exchange, Global, Param, Equation)``
+  -- ``src/GPU/FcnGPU.jl:552``
+* ``JacSchurGPU(J, U, CG, Metric, Phys, Cache, Global, Param, Equation)``
+  -- ``src/DyCore/JacSchur.jl:3``
+* ``SchurSolveGPU(k, v, J, fac, Cache, Global)`` -- ``src/Integration/SchurSolve.jl:149``
+* ``RosenbrockSchur(V, dt, Fcn, FcnPrepare, Jac, CG, Metric, Phys, Cache, JCache,
+  Exchange, Global, Param, DiscType)`` -- ``src/Integration/RosenbrockSchur.jl:1``
+* ``TimeStepper(U, dt, nIter, ...)`` -- the Rosenbrock branch of
+  ``src/Integration/TimeStepper.jl:1-126``
+
+Arrays are torch float64 CUDA tensors whose shapes are the reference's Julia
+dims reversed (same bytes): ``U`` is ``(NumV, NumG, Nz)``, ``k`` is
+``(nStage, NumV, NumG, Nz)`` and so on.  Every native call is ordered on
+torch's current stream.  A non-zero return code of the C ABI raises
+``CGDError`` with the library's message (the reference raises a Julia error).
+
+The library must be built (``__graft_entry__.build()``); there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import lib_path
+from .discretization import Discretization, Phys
+
+_vp = C.c_void_p
+
+
+class CGDError(RuntimeError):
+    pass
+
+
+class _Desc(C.Structure):
+    _fields_ = [("N", C.c_int), ("Nz", C.c_int), ("NF", C.c_int), ("NumG", C.c_int), ("NumV", C.c_int),
+                ("NumTr", C.c_int), ("precision", C.c_int), ("HorLimit", C.c_int), ("Turbulence", C.c_int),
+                ("VerticalDiffusion", C.c_int), ("SurfaceFlux", C.c_int), ("Forcing", C.c_int),
+                ("Microphysics", C.c_int), ("Damping", C.c_int), ("EDMF", C.c_int),
+                ("Grav", C.c_double), ("Omega", C.c_double), ("Rd", C.c_double), ("Cpd", C.c_double),
+                ("p0", C.c_double), ("kappa", C.c_double),
+                ("HyperDCurl", C.c_double), ("HyperDGrad", C.c_double), ("HyperDDiv", C.c_double),
+                ("HyperDDivW", C.c_double),
+                ("Glob", _vp), ("DS", _vp), ("DW", _vp), ("dXdxI", _vp), ("J", _vp), ("X", _vp),
+                ("M", _vp), ("dz", _vp)]
+
+
+class _RWWork(C.Structure):
+    _fields_ = [(n, _vp) for n in ("Vn", "fV", "k", "AuxG", "Temp1", "JRhoW", "JWRho", "JWRhoTh",
+                                   "JRhoThW", "tri")]
+
+
+_LIB = None
+
+
+def native():
+    """Load ``libcgdycore_hip.so``; raises if it was not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = lib_path()
+    if not os.path.exists(path):
+        raise CGDError(f"{path} is missing: build it with __graft_entry__.build() (no CPU fallback exists)")
+    L = C.CDLL(path)
+    L.cgd_ctx_create.argtypes = [C.POINTER(_Desc), C.POINTER(_vp), _vp]
+    L.cgd_ctx_destroy.argtypes = [_vp]
+    L.cgd_last_error.argtypes = [_vp]
+    L.cgd_last_error.restype = C.c_char_p
+    L.cgd_prepare.argtypes = [_vp, _vp, _vp, _vp]
+    L.cgd_fcn.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
+    L.cgd_jac.argtypes = [_vp] * 8
+    L.cgd_schur.argtypes = [_vp] * 8 + [C.c_double, C.c_int, _vp]
+    L.cgd_rw_step.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), _vp]
+    L.cgd_profile_enable.argtypes = [_vp, C.c_int]
+    L.cgd_profile_read.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+    _LIB = L
+    return L
+
+
+def exported_symbols():
+    return ["cgd_ctx_create", "cgd_ctx_destroy", "cgd_last_error", "cgd_prepare", "cgd_fcn", "cgd_jac",
+            "cgd_schur", "cgd_rw_step", "cgd_profile_enable", "cgd_profile_read"]
+
+
+def _stream():
+    return _vp(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t: torch.Tensor):
+    if not t.is_cuda:
+        raise CGDError("device tensor expected")
+    if not t.is_contiguous():
+        raise CGDError("contiguous tensor expected")
+    return _vp(t.data_ptr())
+
+
+# --------------------------------------------------------------------------
+# reference structs (only the fields the hot path reads)
+
+class CompressibleShallow:
+    """``Models.CompressibleShallow`` dispatch marker (src/Models/Equation.jl:5)."""
+
+
+@dataclass
+class ModelStruct:
+    """The ``Global.Model`` switches the path reads (GlobalVariables.jl:248-329)."""
+
+    State: str = "Dry"
+    NumV: int = 5
+    NumTr: int = 0
+    HorLimit: bool = False
+    Turbulence: bool = False
+    VerticalDiffusion: bool = False
+    SurfaceFlux: bool = False
+    Forcing: bool = False
+    Microphysics: bool = False
+    Damping: bool = False
+    EDMF: bool = False
+    HyperDCurl: float = 0.0
+    HyperDGrad: float = 0.0
+    HyperDDiv: float = 0.0
+    HyperDDivW: float = 0.0
+
+
+@dataclass
+class CGQuad:
+    """``CGQuad`` (src/DyCore/FiniteElement.jl:34-165) on the device."""
+
+    OrdPoly: int
+    Glob: torch.Tensor  # int64 (NF, N^2), 1-based
+    DS: torch.Tensor  # (N, N) stored column-major like Julia: tensor of DS^T
+    DW: torch.Tensor
+    M: torch.Tensor  # (2, NumG, Nz)
+    NumG: int
+
+
+@dataclass
+class MetricStruct:
+    """``MetricStruct`` (src/DyCore/GlobalVariables.jl:117-165) on the device."""
+
+    dXdxI: torch.Tensor
+    J: torch.Tensor
+    X: torch.Tensor
+    dz: torch.Tensor
+
+
+@dataclass
+class CacheStruct:
+    """``CacheStruct`` fields of the path (src/Integration/cache.jl:86-145)."""
+
+    AuxG: torch.Tensor  # (4, NumG, Nz)
+    Temp1: torch.Tensor  # (nT, NumG, Nz), nT >= 5
+    k: torch.Tensor  # (3, 5, NumG, Nz)
+    fV: torch.Tensor
+    Vn: torch.Tensor
+
+
+@dataclass
+class JStruct:
+    """``JStruct`` (src/Integration/JacCache.jl:63-101) on the device."""
+
+    JRhoW: torch.Tensor  # (NumG, Nz-1, 2)
+    JWRho: torch.Tensor
+    JWRhoTh: torch.Tensor
+    JRhoThW: torch.Tensor
+    tri: torch.Tensor  # (NumG, Nz-1, 3)
+    CompTri: bool = False
+
+
+@dataclass
+class GlobalStruct:
+    """``GlobalStruct`` subset: model switches, dims and the native context."""
+
+    Model: ModelStruct
+    Nz: int
+    NumG: int
+    NF: int
+    N: int
+    _ctx: object = field(default=None, repr=False)
+
+
+def ssp_knoth():
+    """``RosenbrockStruct{FT}("SSP-Knoth")`` (RosenbrockMethod.jl:24-48).
+
+    ``a = alpha/Gamma``, ``c = -inv(Gamma)``, ``m = Gamma' \\ b`` by back
+    substitution; ``gamma = 1``.
+    """
+    a = np.zeros((3, 3))
+    a[1, 0] = 1.0
+    a[2, 0] = 0.25
+    a[2, 1] = 0.25
+    c = -np.eye(3)
+    c[1, 0] = -0.0
+    c[2, 0] = -0.75
+    c[2, 1] = -0.75
+    m3 = 2.0 / 3.0
+    m = np.array([1.0 / 6.0 - (-0.75) * m3, 1.0 / 6.0 - (-0.75) * m3, m3])
+    return a, c, m, 1.0
+
+
+# --------------------------------------------------------------------------
+class NativeContext:
+    """Owns one ``cgd_ctx``: colouring and re-laid-out metric for one grid."""
+
+    def __init__(self, FE: CGQuad, Metric: MetricStruct, Phys: Phys, Global: GlobalStruct):
+        L = native()
+        M = Global.Model
+        d = _Desc()
+        d.N, d.Nz, d.NF, d.NumG = Global.N, Global.Nz, Global.NF, Global.NumG
+        d.NumV, d.NumTr, d.precision = M.NumV, M.NumTr, 0
+        d.HorLimit = int(M.HorLimit)
+        d.Turbulence, d.VerticalDiffusion = int(M.Turbulence), int(M.VerticalDiffusion)
+        d.SurfaceFlux, d.Forcing, d.Microphysics = int(M.SurfaceFlux), int(M.Forcing), int(M.Microphysics)
+        d.Damping, d.EDMF = int(M.Damping), int(M.EDMF)
+        d.Grav, d.Omega, d.Rd, d.Cpd, d.p0, d.kappa = (Phys.Grav, Phys.Omega, Phys.Rd, Phys.Cpd, Phys.p0,
+                                                       Phys.kappa)
+        d.HyperDCurl, d.HyperDGrad, d.HyperDDiv, d.HyperDDivW = (M.HyperDCurl, M.HyperDGrad, M.HyperDDiv,
+                                                                 M.HyperDDivW)
+        self._keep = (FE.Glob, FE.DS, FE.DW, Metric.dXdxI, Metric.J, Metric.X, FE.M, Metric.dz)
+        d.Glob, d.DS, d.DW = _ptr(FE.Glob), _ptr(FE.DS), _ptr(FE.DW)
+        d.dXdxI, d.J, d.X = _ptr(Metric.dXdxI), _ptr(Metric.J), _ptr(Metric.X)
+        d.M, d.dz = _ptr(FE.M), _ptr(Metric.dz)
+        h = _vp()
+        rc = L.cgd_ctx_create(C.byref(d), C.byref(h), _stream())
+        self.h = h
+        self.L = L
+        if rc != 0:
+            msg = L.cgd_last_error(h).decode() if h.value else "cgd_ctx_create failed"
+            self.close()
+            raise CGDError(f"cgd_ctx_create: {msg} (code {rc})")
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise CGDError(f"{what}: {self.L.cgd_last_error(self.h).decode()} (code {rc})")
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.L.cgd_ctx_destroy(self.h)
+            self.h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def profile(self, on=True):
+        self.check(self.L.cgd_profile_enable(self.h, int(on)), "cgd_profile_enable")
+
+    def profile_read(self, which):
+        ms = C.c_double()
+        n = C.c_longlong()
+        self.check(self.L.cgd_profile_read(self.h, which, C.byref(ms), C.byref(n)), "cgd_profile_read")
+        return ms.value, n.value
+
+
+def _ctx(FE, Metric, Phys, Global) -> NativeContext:
+    if Global._ctx is None:
+        Global._ctx = NativeContext(FE, Metric, Phys, Global)
+    return Global._ctx
+
+
+def _check_state(t: torch.Tensor, Global: GlobalStruct, nv: int, name: str):
+    if t.dtype != torch.float64 or not t.is_cuda:
+        raise CGDError(f"{name}: float64 CUDA tensor expected")
+    if t.dim() != 3 or t.shape[0] < nv or t.shape[1] != Global.NumG or t.shape[2] != Global.Nz:
+        raise CGDError(f"{name}: expected shape ({nv}, {Global.NumG}, {Global.Nz}), got {tuple(t.shape)}")
+
+
+# --------------------------------------------------------------------------
+# the reference entry points
+
+def FcnPrepareGPU(U, FE, Metric, Phys, Cache, Exchange, Global, Param, DiscType):
+    """``FcnPrepareGPU!`` (DiagnosticKernel.jl:180-241): AuxG[:,:,1:3] = p, T, theta."""
+    ctx = _ctx(FE, Metric, Phys, Global)
+    _check_state(U, Global, 5, "U")
+    _check_state(Cache.AuxG, Global, 4, "Cache.AuxG")
+    ctx.check(ctx.L.cgd_prepare(ctx.h, _ptr(U), _ptr(Cache.AuxG), _stream()), "FcnPrepareGPU")
+
+
+def FcnGPU(F, U, FE, Metric, Phys, Cache, Exchange, Global, Param, Equation):
+    """``FcnGPU!`` for ``Models.CompressibleShallow`` (FcnGPU.jl:552-913)."""
+    if not isinstance(Equation, CompressibleShallow):
+        raise CGDError(f"FcnGPU: no method for equation type {type(Equation).__name__}")
+    ctx = _ctx(FE, Metric, Phys, Global)
+    _check_state(F, Global, 5, "F")
+    _check_state(U, Global, 5, "U")
+    _check_state(Cache.Temp1, Global, 5, "Cache.Temp1")
+    ctx.check(ctx.L.cgd_fcn(ctx.h, _ptr(F), _ptr(U), _ptr(Cache.Temp1), _ptr(Cache.AuxG), _stream()), "FcnGPU")
+
+
+def JacSchurGPU(J, U, CG, Metric, Phys, Cache, Global, Param, Equation):
+    """``JacSchurGPU!`` (JacSchur.jl:3-26)."""
+    ctx = _ctx(CG, Metric, Phys, Global)
+    _check_state(U, Global, 5, "U")
+    ctx.check(ctx.L.cgd_jac(ctx.h, _ptr(J.JRhoW), _ptr(J.JWRho), _ptr(J.JWRhoTh), _ptr(J.JRhoThW), _ptr(U),
+                            _ptr(Cache.AuxG), _stream()), "JacSchurGPU")
+
+
+def SchurSolveGPU(k, v, J, fac, Cache, Global):
+    """``SchurSolveGPU!`` (SchurSolve.jl:149-177); sets ``J.CompTri = False``."""
+    ctx = Global._ctx
+    if ctx is None:
+        raise CGDError("SchurSolveGPU: the native context is created by the first Fcn/Jac call")
+    _check_state(k, Global, 5, "k")
+    _check_state(v, Global, 5, "v")
+    ctx.check(ctx.L.cgd_schur(ctx.h, _ptr(k), _ptr(v), _ptr(J.JRhoW), _ptr(J.JWRho), _ptr(J.JWRhoTh),
+                              _ptr(J.JRhoThW), _ptr(J.tri), C.c_double(fac), C.c_int(1 if J.CompTri else 0),
+                              _stream()), "SchurSolveGPU")
+    J.CompTri = False
+
+
+def RosenbrockSchur(V, dt, Fcn, FcnPrepare, Jac, CG, Metric, Phys, Cache, JCache, Exchange, Global, Param,
+                    DiscType):
+    """``RosenbrockSchur!`` (RosenbrockSchur.jl:1-33), one SSP-Knoth step.
+
+    With the native entry points passed as ``Fcn``/``FcnPrepare``/``Jac`` the
+    whole step runs as one ``cgd_rw_step`` call; other callables are
+    composed stage by stage exactly as the reference does.
+    """
+    if Fcn is FcnGPU and FcnPrepare is FcnPrepareGPU and Jac is JacSchurGPU:
+        ctx = _ctx(CG, Metric, Phys, Global)
+        _check_state(V, Global, 5, "V")
+        w = _RWWork(_ptr(Cache.Vn), _ptr(Cache.fV), _ptr(Cache.k), _ptr(Cache.AuxG), _ptr(Cache.Temp1),
+                    _ptr(JCache.JRhoW), _ptr(JCache.JWRho), _ptr(JCache.JWRhoTh), _ptr(JCache.JRhoThW),
+                    _ptr(JCache.tri))
+        ctx.check(ctx.L.cgd_rw_step(ctx.h, _ptr(V), C.c_double(dt), C.byref(w), _stream()), "RosenbrockSchur")
+        JCache.CompTri = False
+        return
+    a, c, m, gamma = ssp_knoth()
+    k, fV, Vn = Cache.k, Cache.fV, Cache.Vn
+    JCache.CompTri = True
+    Vn.copy_(V)
+    for i in range(3):
+        V.copy_(Vn)
+        for j in range(i):
+            V.add_(k[j], alpha=float(a[i, j]))
+        FcnPrepare(V, CG, Metric, Phys, Cache, Exchange, Global, Param, DiscType)
+        Fcn(fV, V, CG, Metric, Phys, Cache, Exchange, Global, Param, DiscType)
+        if i == 0:
+            Jac(JCache, V, CG, Metric, Phys, Cache, Global, Param, DiscType)
+        for j in range(i):
+            fV.add_(k[j], alpha=float(c[i, j] / dt))
+        SchurSolveGPU(k[i], fV, JCache, dt * gamma, Cache, Global)
+    V.copy_(Vn)
+    for i in range(3):
+        V.add_(k[i], alpha=float(m[i]))
+
+
+def TimeStepper(U, dt, nIter, FE, Metric, Phys, Cache, JCache, Global, Exchange=None, Param=None):
+    """Rosenbrock branch of ``TimeStepper!`` (TimeStepper.jl:1-126) without output."""
+    eq = CompressibleShallow()
+    for _ in range(nIter):
+        RosenbrockSchur(U, dt, FcnGPU, FcnPrepareGPU, JacSchurGPU, FE, Metric, Phys, Cache, JCache, Exchange,
+                        Global, Param, eq)
+
+
+# --------------------------------------------------------------------------
+# convenience: device structs from a host Discretization
+
+def hyperdiffusion_for(ne: int) -> float:
+    """Hyperviscosity coefficient used by this build's workloads:
+    5e14 m^4/s at ne = 30 scaled with (30/ne)^3.2."""
+    return 5.0e14 * (30.0 / ne) ** 3.2
+
+
+def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None = None, device="cuda",
+                 nT: int = 5):
+    """Upload the static inputs and allocate ``Cache``/``JCache``."""
+    hd = hyperdiffusion_for(disc.ne) if hyperd is None else hyperd
+    dev = torch.device(device)
+    f64 = dict(dtype=torch.float64, device=dev)
+    FE = CGQuad(OrdPoly=disc.p, Glob=torch.as_tensor(disc.glob, dtype=torch.int64, device=dev).contiguous(),
+                DS=torch.as_tensor(np.ascontiguousarray(disc.DS.T), **f64),
+                DW=torch.as_tensor(np.ascontiguousarray(disc.DW.T), **f64),
+                M=torch.as_tensor(disc.M, **f64), NumG=disc.num_g)
+    Metric = MetricStruct(dXdxI=torch.as_tensor(disc.dXdxI, **f64), J=torch.as_tensor(disc.J, **f64),
+                          X=torch.as_tensor(disc.X, **f64), dz=torch.as_tensor(disc.dz, **f64))
+    Model = ModelStruct(HyperDCurl=hd, HyperDGrad=hd, HyperDDiv=hd)
+    Global = GlobalStruct(Model=Model, Nz=disc.nz, NumG=disc.num_g, NF=disc.nf, N=disc.N)
+    n = (disc.num_g, disc.nz)
+    Cache = CacheStruct(AuxG=torch.zeros((4,) + n, **f64), Temp1=torch.zeros((nT,) + n, **f64),
+                        k=torch.zeros((3, 5) + n, **f64), fV=torch.zeros((5,) + n, **f64),
+                        Vn=torch.zeros((5,) + n, **f64))
+    jn = (disc.num_g, disc.nz - 1)
+    JCache = JStruct(JRhoW=torch.zeros(jn + (2,), **f64), JWRho=torch.zeros(jn + (2,), **f64),
+                     JWRhoTh=torch.zeros(jn + (2,), **f64), JRhoThW=torch.zeros(jn + (2,), **f64),
+                     tri=torch.zeros(jn + (3,), **f64))
+    return FE, Metric, Global, Cache, JCache

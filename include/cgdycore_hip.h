@@ -1,0 +1,113 @@
+/*
+ * cgdycore_hip.h -- C ABI of the MI355X (gfx950) spectral-element tendency and
+ * Rosenbrock-W step for CGDycore.jl's CompressibleShallow / Dry
+ * configuration.
+ *
+ * Every array argument is a DEVICE pointer with the reference's Julia
+ * column-major layout (first index fastest):
+ *   U, F, V, Vn, fV, Temp1 : double (Nz, NumG, 5)       AuxG : double (Nz, NumG, 4)
+ *   k                      : double (Nz, NumG, 5, 3)
+ *   JRhoW, JWRho, JWRhoTh, JRhoThW : double (2, Nz-1, NumG)   tri : double (3, Nz-1, NumG)
+ *   Glob  : int64 (N*N, NF), 1-based      DS, DW : double (N, N)
+ *   dXdxI : double (3,3,2,N*N,Nz,NF)      J : double (N*N,2,Nz,NF)
+ *   X     : double (N*N,2,3,Nz,NF)        M : double (Nz,NumG,2)    dz : double (Nz,NumG)
+ * Temp1 may have more than 5 components (the reference allocates 8+NumTr);
+ * only the first 5 are read or written.
+ *
+ * All calls are ordered on the caller's HIP stream (passed as void*), return
+ * CGD_OK (0) or a negative error code, and allocate nothing after
+ * cgd_ctx_create.  A context is not thread-safe; calls on one context must
+ * be issued from one host thread.
+ */
+#ifndef CGDYCORE_HIP_H
+#define CGDYCORE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CGD_OK = 0,
+  CGD_ERR_ARG = -1,         /* null pointer or bad dimension */
+  CGD_ERR_UNSUPPORTED = -2, /* flag combination outside the Dry/CompressibleShallow path */
+  CGD_ERR_HIP = -3,         /* a HIP runtime call failed */
+  CGD_ERR_STATE = -4        /* context not usable (e.g. creation failed) */
+};
+
+enum { CGD_FP64 = 0 };
+
+typedef struct {
+  /* dimensions (reference names) */
+  int N;      /* OrdPoly + 1, 2..5 */
+  int Nz;     /* vertical layers, 2..256 */
+  int NF;     /* NumFaces (elements) */
+  int NumG;   /* global CG nodes */
+  int NumV;   /* must be 5: rho, u, v, w, rho*theta */
+  int NumTr;  /* must be 0 */
+  int precision; /* CGD_FP64 */
+  /* model switches that select FcnGPU! branches; only the BASELINE set is
+   * implemented: HorLimit=0, KoeffDivW=0, no EDMF/TKE/physics hooks */
+  int HorLimit;
+  int Turbulence, VerticalDiffusion, SurfaceFlux, Forcing, Microphysics, Damping, EDMF;
+  /* PhysParameters (GlobalVariables.jl:188-230) */
+  double Grav, Omega, Rd, Cpd, p0, kappa;
+  /* hyperviscosity (Model.HyperDCurl/HyperDGrad/HyperDDiv/HyperDDivW) */
+  double HyperDCurl, HyperDGrad, HyperDDiv, HyperDDivW;
+  /* static device inputs (reference layouts) */
+  const int64_t *Glob;
+  const double *DS, *DW, *dXdxI, *J, *X, *M, *dz;
+} cgd_desc;
+
+typedef struct cgd_ctx cgd_ctx;
+
+/* Validates the descriptor, builds the element colouring and the
+ * level-fastest copy of the metric.  Replaces the static part of
+ * FcnGPU!'s setup (src/GPU/FcnGPU.jl:552-730) and of CGQuad/MetricStruct
+ * consumption; no reference counterpart as a separate call. */
+int cgd_ctx_create(const cgd_desc *desc, cgd_ctx **ctx, void *stream);
+int cgd_ctx_destroy(cgd_ctx *ctx);
+const char *cgd_last_error(const cgd_ctx *ctx);
+
+/* FcnPrepareGPU!(U,...) -- src/GPU/DiagnosticKernel.jl:180-241 (PressureKernel!
+ * :96-115 with Models.Dry, src/Models/Equation.jl:25-32).  Writes
+ * AuxG[:,:,1:3] = (p, T, theta). */
+int cgd_prepare(cgd_ctx *ctx, const double *U, double *AuxG, void *stream);
+
+/* FcnGPU!(F,U,...,Models.CompressibleShallow) -- src/GPU/FcnGPU.jl:552-913.
+ * Zeroes and overwrites F and Temp1[:,:,1:5]; reads p = AuxG[:,:,1]. */
+int cgd_fcn(cgd_ctx *ctx, double *F, const double *U, double *Temp1, const double *AuxG, void *stream);
+
+/* JacSchurGPU!(J,U,...) -- src/DyCore/JacSchur.jl:3-75.  Reads AuxG[:,:,4]
+ * (identically zero for State=Dry). */
+int cgd_jac(cgd_ctx *ctx, double *JRhoW, double *JWRho, double *JWRhoTh, double *JRhoThW, const double *U,
+            const double *AuxG, void *stream);
+
+/* SchurSolveGPU!(k,v,J,fac,...) -- src/Integration/SchurSolve.jl:149-177.
+ * k: one stage (Nz,NumG,5).  Clobbers v[:,:,1], v[:,:,4], v[:,:,5].
+ * comp_tri != 0 rebuilds tri from the J blocks first (J.CompTri). */
+int cgd_schur(cgd_ctx *ctx, double *k, double *v, const double *JRhoW, const double *JWRho, const double *JWRhoTh,
+              const double *JRhoThW, double *tri, double fac, int comp_tri, void *stream);
+
+/* RosenbrockSchur!(V,dt,...) with SSP-Knoth -- src/Integration/RosenbrockSchur.jl:1-33
+ * and src/Integration/RosenbrockMethod.jl:24-48.  The work buffers are the
+ * reference's Cache.Vn, Cache.fV, Cache.k, Cache.AuxG (4th component must be
+ * zero), Cache.Temp1 and the JCache blocks; all are caller owned. */
+typedef struct {
+  double *Vn, *fV, *k, *AuxG, *Temp1;
+  double *JRhoW, *JWRho, *JWRhoTh, *JRhoThW, *tri;
+} cgd_rw_work;
+int cgd_rw_step(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, void *stream);
+
+/* Per-kernel timing of the tendency passes with HIP events on the launch
+ * stream (used by the benchmark; off by default).  which: 0 = pass 1
+ * (hyperviscosity), 1 = pass 2 (fused tendency), 2 = column solve. */
+int cgd_profile_enable(cgd_ctx *ctx, int on);
+int cgd_profile_read(cgd_ctx *ctx, int which, double *total_ms, long long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

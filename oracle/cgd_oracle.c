@@ -1,0 +1,625 @@
+/*
+ * CPU oracle for the CGDycore.jl spectral-element tendency + Rosenbrock-W
+ * step (plain FP64 C).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity checker for the HIP
+ * path and the timed CPU baseline; the product never links or calls it.
+ *
+ * It restates the reference kernels one by one, with the atomic DSS
+ * scatter-adds replaced by serial adds in ascending (element, level,
+ * node) order:
+ *   PressureKernel!                  src/GPU/DiagnosticKernel.jl:96-115
+ *   Dry EOS / dPresdRhoTh            src/Models/Equation.jl:25-38,79
+ *   FcnGPU! (CompressibleShallow)    src/GPU/FcnGPU.jl:552-913
+ *   HyperViscKernel!                 src/GPU/ViscKernel.jl:1-84
+ *   HyperViscKoeffKernel!            src/GPU/ViscKernel.jl:86-167
+ *   MomentumVectorInvariantCoriolis  src/GPU/MomentumKernel.jl:1-178
+ *   CoriolisShallow                  src/GPU/Coriolis.jl:16-28
+ *   GradFullKernel!                  src/GPU/GradKineticKernel.jl:1-129
+ *   GravitationShallow               src/GPU/Coriolis.jl:59-67
+ *   DivRhoTrUpwind3New2Kernel!       src/GPU/OperatorKernel.jl:637-720
+ *   Contra12/Grad12/Curl12/Rot12/Contra3/RecU4
+ *                                    src/GPU/OperatorKernel.jl:953-1059
+ *   JacSchurKernel!                  src/DyCore/JacSchur.jl:29-75
+ *   triSolve!/TriDiag/SchurSolveF/Tri/B
+ *                                    src/Integration/SchurSolve.jl:1-14,45-177
+ *   RosenbrockSchur!                 src/Integration/RosenbrockSchur.jl:1-33
+ *   SSP-Knoth tableau                src/Integration/RosenbrockMethod.jl:24-48
+ *
+ * The reference's LDS column tiles only change where halo values are read
+ * from; the restatement reads the clamped neighbour levels directly
+ * (Iz-1 -> max(Iz-1,1), Iz+2 -> min(Iz+2,Nz)) which is what every tile
+ * shape computes.  Where a tile-boundary branch of GradFullKernel! sums the
+ * kinetic energy of level Iz-1 in another order (GradKineticKernel.jl:33),
+ * the interior-branch order is used everywhere.
+ *
+ * Arrays use the reference's Julia column-major layouts (see cgd_oracle.h).
+ */
+#include "cgd_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NN (g->N * g->N)
+#define Uv(U, iz, ig, v) (U)[(size_t)(iz) + (size_t)g->Nz * ((size_t)(ig) + (size_t)g->NumG * (size_t)(v))]
+#define DX(i, j, K, ID, iz, IF)                                                                        \
+  g->dXdxI[(size_t)(i) + 3 * ((size_t)(j) + 3 * ((size_t)(K) + 2 * ((size_t)(ID) + (size_t)NN * \
+           ((size_t)(iz) + (size_t)g->Nz * (size_t)(IF)))))]
+#define JJ(ID, K, iz, IF) g->J[(size_t)(ID) + (size_t)NN * ((size_t)(K) + 2 * ((size_t)(iz) + (size_t)g->Nz * (size_t)(IF)))]
+#define XX(ID, K, c, iz, IF) \
+  g->X[(size_t)(ID) + (size_t)NN * ((size_t)(K) + 2 * ((size_t)(c) + 3 * ((size_t)(iz) + (size_t)g->Nz * (size_t)(IF))))]
+#define MM(iz, ig, K) g->M[(size_t)(iz) + (size_t)g->Nz * ((size_t)(ig) + (size_t)g->NumG * (size_t)(K))]
+#define GLOB(ID, IF) ((int64_t)g->Glob[(size_t)(ID) + (size_t)NN * (size_t)(IF)] - 1)
+#define DS(i, k) g->DS[(size_t)(i) + (size_t)g->N * (size_t)(k)]
+#define DW(i, k) g->DW[(size_t)(i) + (size_t)g->N * (size_t)(k)]
+
+#define MAXN 8
+
+/* Julia min/max on non-NaN floats */
+static inline double jmin(double a, double b) { return (b < a) ? b : a; }
+static inline double jmax(double a, double b) { return (b > a) ? b : a; }
+
+/* ---- helpers (OperatorKernel.jl:953-1059); d(a,b,K) = dXdxI[a,b,K,ID,Iz,IF] ---- */
+typedef struct {
+  double d[3][3][2];
+} metric_pt;
+
+static void load_metric(const cgdo_grid *g, int ID, int iz, int IF, metric_pt *m) {
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      for (int K = 0; K < 2; ++K) m->d[a][b][K] = DX(a, b, K, ID, iz, IF);
+}
+
+static inline void contra12(double u, double v, const metric_pt *m, double *uc, double *vc) {
+  *uc = (m->d[0][0][0] + m->d[0][0][1]) * u + (m->d[0][1][0] + m->d[0][1][1]) * v;
+  *vc = (m->d[1][0][0] + m->d[1][0][1]) * u + (m->d[1][1][0] + m->d[1][1][1]) * v;
+}
+static inline void contra12r(double rho, double u, double v, const metric_pt *m, double *uc, double *vc) {
+  *uc = rho * ((m->d[0][0][0] + m->d[0][0][1]) * u + (m->d[0][1][0] + m->d[0][1][1]) * v);
+  *vc = rho * ((m->d[1][0][0] + m->d[1][0][1]) * u + (m->d[1][1][0] + m->d[1][1][1]) * v);
+}
+static inline void grad12j(double u, double v, const metric_pt *m, double j1, double j2, double *gu, double *gv) {
+  *gu = ((m->d[0][0][0] + m->d[0][0][1]) * u + (m->d[1][0][0] + m->d[1][0][1]) * v) / (j1 + j2);
+  *gv = ((m->d[0][1][0] + m->d[0][1][1]) * u + (m->d[1][1][0] + m->d[1][1][1]) * v) / (j1 + j2);
+}
+static inline void grad12(double u, double v, const metric_pt *m, double *gu, double *gv) {
+  *gu = (m->d[0][0][0] + m->d[0][0][1]) * u + (m->d[1][0][0] + m->d[1][0][1]) * v;
+  *gv = (m->d[0][1][0] + m->d[0][1][1]) * u + (m->d[1][1][0] + m->d[1][1][1]) * v;
+}
+static inline void curl12(double u, double v, const metric_pt *m, double *cu, double *cv) {
+  *cu = (m->d[0][0][0] + m->d[0][0][1]) * v - (m->d[0][1][0] + m->d[0][1][1]) * u;
+  *cv = (m->d[1][0][0] + m->d[1][0][1]) * v - (m->d[1][1][0] + m->d[1][1][1]) * u;
+}
+static inline void rot12(double u, double v, const metric_pt *m, double *ru, double *rv) {
+  *ru = (m->d[0][1][0] + m->d[0][1][1]) * u + (m->d[1][1][0] + m->d[1][1][1]) * v;
+  *rv = -(m->d[0][0][0] + m->d[0][0][1]) * u - (m->d[1][0][0] + m->d[1][0][1]) * v;
+}
+
+static void recu4(double cLL, double cL, double cR, double cRR, double JLL, double JL, double JR, double JRR,
+                  double *cFL, double *cFR) {
+  double kR = (JL / (JL + JR)) * ((JLL + JL) / (JLL + JL + JR));
+  double kL = (JL / (JLL + JL)) * (JR / (JLL + JL + JR));
+  double r = (cL - cLL) / (cR - cL + 1.e-20);
+  *cFL = cL + jmax(0.0, jmin(r, jmin(kL * r + kR, 1.0))) * (cR - cL);
+  kL = (JR / (JR + JL)) * ((JRR + JR) / (JL + JR + JRR));
+  kR = (JR / (JRR + JR)) * (JL / (JL + JR + JRR));
+  r = (cR - cRR) / (cL - cR + 1.e-20);
+  *cFR = cR + jmax(0.0, jmin(r, jmin(kR * r + kL, 1.0))) * (cL - cR);
+}
+
+/* ---- EOS (Equation.jl:25-38,79) ---- */
+static inline double eos_pressure(const cgdo_grid *g, double rhoth) {
+  /* p0 * fast_powGPU(Rd*RhoTh/p0, 1/(1-kappa)), fast_powGPU(x,y) = exp(y*log(x)) */
+  double y = 1.0 / (1.0 - g->kappa);
+  return g->p0 * exp(y * log(g->Rd * rhoth / g->p0));
+}
+static inline double dpres_drhoth(const cgdo_grid *g, double rhoth) {
+  return g->Rd * pow(g->Rd * rhoth / g->p0, g->kappa / (1.0 - g->kappa));
+}
+
+void cgdo_prepare(const cgdo_grid *g, const double *U, double *AuxG) {
+  const int Nz = g->Nz;
+  for (int ig = 0; ig < g->NumG; ++ig)
+    for (int iz = 0; iz < Nz; ++iz) {
+      double rho = Uv(U, iz, ig, 0), rhoth = Uv(U, iz, ig, 4);
+      double p = eos_pressure(g, rhoth);
+      double T = p / (g->Rd * rho);
+      double th = pow(g->p0 / p, g->Rd / g->Cpd) * T;
+      Uv(AuxG, iz, ig, 0) = p;
+      Uv(AuxG, iz, ig, 1) = T;
+      Uv(AuxG, iz, ig, 2) = th;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pass 1: HyperViscKernel! (ViscKernel.jl:1-84) and pass 2 hyperviscosity   */
+/* HyperViscKoeffKernel! (ViscKernel.jl:86-167) share the operator body.     */
+static void hypervisc_element(const cgdo_grid *g, int IF, int iz, int koeff, double *Out, const double *U,
+                              const double *Cache) {
+  const int N = g->N;
+  double uC[MAXN][MAXN], vC[MAXN][MAXN], uD[MAXN][MAXN], vD[MAXN][MAXN], Th[MAXN][MAXN];
+  double Curl[MAXN][MAXN], Div[MAXN][MAXN], ThCx[MAXN][MAXN], ThCy[MAXN][MAXN];
+  metric_pt m;
+  /* [I][J] indexing, ID = I + J*N */
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      load_metric(g, ID, iz, IF, &m);
+      if (!koeff) {
+        curl12(Uv(U, iz, ind, 1), Uv(U, iz, ind, 2), &m, &uC[I][J], &vC[I][J]);
+        contra12(Uv(U, iz, ind, 1), Uv(U, iz, ind, 2), &m, &uD[I][J], &vD[I][J]);
+        Th[I][J] = Uv(U, iz, ind, 4) / Uv(U, iz, ind, 0);
+      } else {
+        curl12(Uv(Cache, iz, ind, 0), Uv(Cache, iz, ind, 1), &m, &uC[I][J], &vC[I][J]);
+        contra12(Uv(Cache, iz, ind, 2), Uv(Cache, iz, ind, 3), &m, &uD[I][J], &vD[I][J]);
+        Th[I][J] = Uv(Cache, iz, ind, 4);
+      }
+    }
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      load_metric(g, ID, iz, IF, &m);
+      double Dxc = DS(I, 0) * Th[0][J];
+      double Dyc = DS(J, 0) * Th[I][0];
+      double cu = DS(I, 0) * uC[0][J] + DS(J, 0) * vC[I][0];
+      double dv = DS(I, 0) * uD[0][J] + DS(J, 0) * vD[I][0];
+      for (int k = 1; k < N; ++k) {
+        Dxc += DS(I, k) * Th[k][J];
+        Dyc += DS(J, k) * Th[I][k];
+        cu += DS(I, k) * uC[k][J] + DS(J, k) * vC[I][k];
+        dv += DS(I, k) * uD[k][J] + DS(J, k) * vD[I][k];
+      }
+      double j1 = JJ(ID, 0, iz, IF), j2 = JJ(ID, 1, iz, IF);
+      Curl[I][J] = cu / (j1 + j2);
+      Div[I][J] = dv / (j1 + j2);
+      double gdx, gdy;
+      grad12j(Dxc, Dyc, &m, j1, j2, &gdx, &gdy);
+      if (!koeff)
+        contra12(gdx, gdy, &m, &ThCx[I][J], &ThCy[I][J]);
+      else
+        contra12r(Uv(U, iz, ind, 0), gdx, gdy, &m, &ThCx[I][J], &ThCy[I][J]);
+    }
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      load_metric(g, ID, iz, IF, &m);
+      double DxCurl = DW(I, 0) * Curl[0][J];
+      double DyCurl = DW(J, 0) * Curl[I][0];
+      double DxDiv = DW(I, 0) * Div[0][J];
+      double DyDiv = DW(J, 0) * Div[I][0];
+      double DivTh = DW(I, 0) * ThCx[0][J] + DW(J, 0) * ThCy[I][0];
+      for (int k = 1; k < N; ++k) {
+        DxCurl += DW(I, k) * Curl[k][J];
+        DyCurl += DW(J, k) * Curl[I][k];
+        DxDiv += DW(I, k) * Div[k][J];
+        DyDiv += DW(J, k) * Div[I][k];
+        DivTh += DW(I, k) * ThCx[k][J] + DW(J, k) * ThCy[I][k];
+      }
+      double FuC, FvC, FuD, FvD;
+      rot12(DxCurl, DyCurl, &m, &FuC, &FvC);
+      grad12(DxDiv, DyDiv, &m, &FuD, &FvD);
+      double Ms = MM(iz, ind, 0) + MM(iz, ind, 1);
+      if (!koeff) {
+        Uv(Out, iz, ind, 0) += -FuC / Ms;
+        Uv(Out, iz, ind, 1) += -FvC / Ms;
+        Uv(Out, iz, ind, 2) += FuD / Ms;
+        Uv(Out, iz, ind, 3) += FvD / Ms;
+        Uv(Out, iz, ind, 4) += DivTh / Ms;
+      } else {
+        Uv(Out, iz, ind, 1) += -(-g->KoeffCurl * FuC + g->KoeffGrad * FuD) / Ms;
+        Uv(Out, iz, ind, 2) += -(-g->KoeffCurl * FvC + g->KoeffGrad * FvD) / Ms;
+        Uv(Out, iz, ind, 4) += -g->KoeffDiv * DivTh / Ms;
+      }
+    }
+}
+
+/* free-slip bottom w (MomentumKernel.jl:35-36, GradKineticKernel.jl:26-27) */
+static inline double w_bottom(const cgdo_grid *g, const double *U, int ID, int IF, int64_t ind) {
+  return -(DX(2, 0, 0, ID, 0, IF) * Uv(U, 0, ind, 1) + DX(2, 1, 0, ID, 0, IF) * Uv(U, 0, ind, 2)) /
+         DX(2, 2, 0, ID, 0, IF);
+}
+/* w at the lower / upper interface of layer iz as the kernels see it */
+static inline double w_lo(const cgdo_grid *g, const double *U, int ID, int IF, int64_t ind, int iz) {
+  return iz == 0 ? w_bottom(g, U, ID, IF, ind) : Uv(U, iz - 1, ind, 3);
+}
+
+/* MomentumVectorInvariantCoriolisKernel! (MomentumKernel.jl:1-178) */
+static void momentum_element(const cgdo_grid *g, int IF, int iz, double *F, const double *U) {
+  const int N = g->N, Nz = g->Nz;
+  double t1[3][2][MAXN][MAXN], t2[3][2][MAXN][MAXN]; /* tempXZ1/2[var][K][I][J] */
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      double u = Uv(U, iz, ind, 1), v = Uv(U, iz, ind, 2);
+      double w[2] = {w_lo(g, U, ID, IF, ind, iz), Uv(U, iz, ind, 3)};
+      for (int K = 0; K < 2; ++K) {
+        t1[0][K][I][J] = -DX(0, 1, K, ID, iz, IF) * w[K];
+        t2[0][K][I][J] = -DX(1, 1, K, ID, iz, IF) * w[K];
+        t1[1][K][I][J] = -DX(0, 0, K, ID, iz, IF) * w[K];
+        t2[1][K][I][J] = -DX(1, 0, K, ID, iz, IF) * w[K];
+        t1[2][K][I][J] = DX(0, 1, K, ID, iz, IF) * u - DX(0, 0, K, ID, iz, IF) * v;
+        t2[2][K][I][J] = DX(1, 1, K, ID, iz, IF) * u - DX(1, 0, K, ID, iz, IF) * v;
+      }
+    }
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      double u = Uv(U, iz, ind, 1), v = Uv(U, iz, ind, 2), rho = Uv(U, iz, ind, 0);
+      double wl = w_lo(g, U, ID, IF, ind, iz), wh = Uv(U, iz, ind, 3);
+      /* tempuZ/tempvZ/tempwZ of this layer (K=1,2) and of the neighbours */
+#define TUZ(L, K, uu, vv, wk) (DX(2, 2, K, ID, L, IF) * (vv) - DX(2, 1, K, ID, L, IF) * (wk))
+#define TVZ(L, K, uu, vv, wk) (DX(2, 2, K, ID, L, IF) * (uu) - DX(2, 0, K, ID, L, IF) * (wk))
+#define TWZ(L, K, uu, vv) (DX(2, 1, K, ID, L, IF) * (uu) - DX(2, 0, K, ID, L, IF) * (vv))
+      double uz1 = TUZ(iz, 0, u, v, wl), uz2 = TUZ(iz, 1, u, v, wh);
+      double vz1 = TVZ(iz, 0, u, v, wl), vz2 = TVZ(iz, 1, u, v, wh);
+      double wz1 = TWZ(iz, 0, u, v), wz2 = TWZ(iz, 1, u, v);
+      double FU = 0.0, FV = 0.0, FW1 = 0.0, FW2 = 0.0;
+      if (iz > 0) {
+        double um = Uv(U, iz - 1, ind, 1), vm = Uv(U, iz - 1, ind, 2);
+        /* top node of layer iz-1 uses w at its upper interface = wl */
+        double FluxUZ = 0.5 * (uz1 - TUZ(iz - 1, 1, um, vm, wl));
+        double FluxVZ = 0.5 * (vz1 - TVZ(iz - 1, 1, um, vm, wl));
+        double FluxWZ = 0.5 * (wz1 - TWZ(iz - 1, 1, um, vm));
+        FU += (-v * FluxWZ - wl * FluxVZ);
+        FV += (u * FluxWZ - wl * FluxUZ);
+        FW1 += (u * FluxVZ + v * FluxUZ);
+      }
+      if (iz < Nz - 1) {
+        double up = Uv(U, iz + 1, ind, 1), vp = Uv(U, iz + 1, ind, 2);
+        double FluxUZ = 0.5 * (TUZ(iz + 1, 0, up, vp, wh) - uz2);
+        double FluxVZ = 0.5 * (TVZ(iz + 1, 0, up, vp, wh) - vz2);
+        double FluxWZ = 0.5 * (TWZ(iz + 1, 0, up, vp) - wz2);
+        FU += (-v * FluxWZ - wh * FluxVZ);
+        FV += (u * FluxWZ - wh * FluxUZ);
+        FW2 += (u * FluxVZ + v * FluxUZ);
+      }
+#undef TUZ
+#undef TVZ
+#undef TWZ
+      double U1 = 0.5 * (uz2 - uz1), U2 = U1;
+      double V1 = 0.5 * (vz2 - vz1), V2 = V1;
+      double W1 = 0.5 * (wz2 - wz1), W2 = W1;
+      U1 += DS(I, 0) * t1[0][0][0][J] + DS(J, 0) * t2[0][0][I][0];
+      U2 += DS(I, 0) * t1[0][1][0][J] + DS(J, 0) * t2[0][1][I][0];
+      V1 += DS(I, 0) * t1[1][0][0][J] + DS(J, 0) * t2[1][0][I][0];
+      V2 += DS(I, 0) * t1[1][1][0][J] + DS(J, 0) * t2[1][1][I][0];
+      W1 += DS(I, 0) * t1[2][0][0][J] + DS(J, 0) * t2[2][0][I][0];
+      W2 += DS(I, 0) * t1[2][1][0][J] + DS(J, 0) * t2[2][1][I][0];
+      for (int k = 1; k < N; ++k) {
+        U1 += DS(I, k) * t1[0][0][k][J] + DS(J, k) * t2[0][0][I][k];
+        U2 += DS(I, k) * t1[0][1][k][J] + DS(J, k) * t2[0][1][I][k];
+        V1 += DS(I, k) * t1[1][0][k][J] + DS(J, k) * t2[1][0][I][k];
+        V2 += DS(I, k) * t1[1][1][k][J] + DS(J, k) * t2[1][1][I][k];
+        W1 += DS(I, k) * t1[2][0][k][J] + DS(J, k) * t2[2][0][I][k];
+        W2 += DS(I, k) * t1[2][1][k][J] + DS(J, k) * t2[2][1][I][k];
+      }
+      /* CoriolisShallow at the layer midpoint */
+      double x = 0.5 * (XX(ID, 0, 0, iz, IF) + XX(ID, 1, 0, iz, IF));
+      double y = 0.5 * (XX(ID, 0, 1, iz, IF) + XX(ID, 1, 1, iz, IF));
+      double z = 0.5 * (XX(ID, 0, 2, iz, IF) + XX(ID, 1, 2, iz, IF));
+      double sinlat = z / sqrt(x * x + y * y + z * z);
+      double Wc = 2.0 * g->Omega * sinlat;
+      double FuCor = v * Wc, FvCor = -u * Wc, FwCor = 0.0;
+      double j1 = JJ(ID, 0, iz, IF), j2 = JJ(ID, 1, iz, IF);
+      FU += FuCor * (j1 + j2);
+      FV += FvCor * (j1 + j2);
+      FW1 += FwCor * j1;
+      FW2 += FwCor * j2;
+      double Ms = MM(iz, ind, 0) + MM(iz, ind, 1);
+      Uv(F, iz, ind, 1) += (-v * W1 - wl * V1 - v * W2 - wh * V2 + FU) / Ms;
+      Uv(F, iz, ind, 2) += (u * W1 - wl * U1 + u * W2 - wh * U2 + FV) / Ms;
+      if (iz > 0) {
+        double rm = Uv(U, iz - 1, ind, 0);
+        Uv(F, iz - 1, ind, 3) += rho * (u * V1 + v * U1 + FW1) / (rm * MM(iz - 1, ind, 1) + rho * MM(iz, ind, 0));
+      }
+      if (iz < Nz - 1) {
+        double rp = Uv(U, iz + 1, ind, 0);
+        Uv(F, iz, ind, 3) += rho * (u * V2 + v * U2 + FW2) / (rp * MM(iz + 1, ind, 0) + rho * MM(iz, ind, 1));
+      }
+    }
+}
+
+/* kinetic energy at the bottom (K=0) / top (K=1) node of layer iz */
+static inline double kin_node(const cgdo_grid *g, const double *U, int ID, int IF, int64_t ind, int iz, int K) {
+  double u = Uv(U, iz, ind, 1), v = Uv(U, iz, ind, 2);
+  double kh = 0.5 * (u * u + v * v);
+  double w = (K == 0) ? w_lo(g, U, ID, IF, ind, iz) : Uv(U, iz, ind, 3);
+  return kh + 0.5 * (w * w);
+}
+
+/* GradFullKernel! (GradKineticKernel.jl:1-129) */
+static void grad_element(const cgdo_grid *g, int IF, int iz, double *F, const double *U, const double *p) {
+  const int N = g->N, Nz = g->Nz;
+  double K1[MAXN][MAXN], K2[MAXN][MAXN], pc[MAXN][MAXN];
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      K1[I][J] = kin_node(g, U, ID, IF, ind, iz, 0);
+      K2[I][J] = kin_node(g, U, ID, IF, ind, iz, 1);
+      pc[I][J] = Uv(p, iz, ind, 0);
+    }
+  const double Grav = g->Grav;
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      double rho = Uv(U, iz, ind, 0);
+      double DXp = DS(I, 0) * pc[0][J], DYp = DS(J, 0) * pc[I][0];
+      double DXK1 = DS(I, 0) * K1[0][J], DYK1 = DS(J, 0) * K1[I][0];
+      double DXK2 = DS(I, 0) * K2[0][J], DYK2 = DS(J, 0) * K2[I][0];
+      for (int k = 1; k < N; ++k) {
+        DXK1 += DS(I, k) * K1[k][J];
+        DYK1 += DS(J, k) * K1[I][k];
+        DXK2 += DS(I, k) * K2[k][J];
+        DYK2 += DS(J, k) * K2[I][k];
+        DXp += DS(I, k) * pc[k][J];
+        DYp += DS(J, k) * pc[I][k];
+      }
+#define D_(a, b, K) DX(a, b, K, ID, iz, IF)
+      double GuF1 = -(D_(0, 0, 0) * DXK1 + D_(1, 0, 0) * DYK1);
+      double GvF1 = -(D_(0, 1, 0) * DXK1 + D_(1, 1, 0) * DYK1);
+      double GuF2 = -(D_(0, 0, 1) * DXK2 + D_(1, 0, 1) * DYK2);
+      double GvF2 = -(D_(0, 1, 1) * DXK2 + D_(1, 1, 1) * DYK2);
+      double Gu = (D_(0, 0, 0) + D_(0, 0, 1)) * DXp + (D_(1, 0, 0) + D_(1, 0, 1)) * DYp;
+      double Gv = (D_(0, 1, 0) + D_(0, 1, 1)) * DXp + (D_(1, 1, 0) + D_(1, 1, 1)) * DYp;
+      double GwF1 = 0.0, GwF2 = 0.0, GradZ;
+      double k1 = K1[I][J], k2 = K2[I][J];
+      if (iz > 0) {
+        GradZ = 0.5 * (k1 - kin_node(g, U, ID, IF, ind, iz - 1, 1));
+        GuF1 += -GradZ * D_(2, 0, 0);
+        GvF1 += -GradZ * D_(2, 1, 0);
+        GwF1 += -rho * GradZ * D_(2, 2, 0);
+      }
+      if (iz < Nz - 1) {
+        GradZ = 0.5 * (kin_node(g, U, ID, IF, ind, iz + 1, 0) - k2);
+        GuF2 += -GradZ * D_(2, 0, 1);
+        GvF2 += -GradZ * D_(2, 1, 1);
+        GwF2 += -rho * GradZ * D_(2, 2, 1);
+      }
+      GradZ = 0.5 * (k2 - k1);
+      GuF2 += -GradZ * D_(2, 0, 1);
+      GuF1 += -GradZ * D_(2, 0, 0);
+      GvF2 += -GradZ * D_(2, 1, 1);
+      GvF1 += -GradZ * D_(2, 1, 0);
+      GwF2 += -rho * GradZ * D_(2, 2, 1);
+      GwF1 += -rho * GradZ * D_(2, 2, 0);
+      double j1 = JJ(ID, 0, iz, IF), j2 = JJ(ID, 1, iz, IF);
+      GradZ = -Grav * rho * (j1 + j2) / (D_(2, 2, 0) + D_(2, 2, 1));
+      Gu += GradZ * (D_(2, 0, 0) + D_(2, 0, 1));
+      Gv += GradZ * (D_(2, 1, 0) + D_(2, 1, 1));
+      double Ms = MM(iz, ind, 0) + MM(iz, ind, 1);
+      Uv(F, iz, ind, 1) += (GuF1 + GuF2 - Gu / rho) / Ms;
+      Uv(F, iz, ind, 2) += (GvF1 + GvF2 - Gv / rho) / Ms;
+      if (iz < Nz - 1) {
+        double rp = Uv(U, iz + 1, ind, 0);
+        GradZ = 0.5 * (Uv(p, iz + 1, ind, 0) - Uv(p, iz, ind, 0));
+        double Gradw = GradZ * (D_(2, 2, 1) + DX(2, 2, 0, ID, iz + 1, IF));
+        double Gradp = -(Gradw + Grav * (rho * j2 + rp * JJ(ID, 0, iz + 1, IF)));
+        Uv(F, iz, ind, 3) += (GwF2 + Gradp) / (rp * MM(iz + 1, ind, 0) + rho * MM(iz, ind, 1));
+      }
+      if (iz > 0) {
+        double rm = Uv(U, iz - 1, ind, 0);
+        Uv(F, iz - 1, ind, 3) += GwF1 / (rm * MM(iz - 1, ind, 1) + rho * MM(iz, ind, 0));
+      }
+#undef D_
+    }
+}
+
+/* DivRhoTrUpwind3New2Kernel! with NumTr = 0 (OperatorKernel.jl:637-720) */
+static void div_element(const cgdo_grid *g, int IF, int iz, double *F, const double *U) {
+  const int N = g->N, Nz = g->Nz;
+  double uc[MAXN][MAXN], vc[MAXN][MAXN], cc[MAXN][MAXN];
+  metric_pt m;
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      load_metric(g, ID, iz, IF, &m);
+      double rho = Uv(U, iz, ind, 0);
+      cc[I][J] = Uv(U, iz, ind, 4) / rho;
+      contra12r(-rho, Uv(U, iz, ind, 1), Uv(U, iz, ind, 2), &m, &uc[I][J], &vc[I][J]);
+    }
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      double Ms = MM(iz, ind, 0) + MM(iz, ind, 1);
+      if (iz < Nz - 1) {
+        double rL = Uv(U, iz, ind, 0), rR = Uv(U, iz + 1, ind, 0);
+        double w = Uv(U, iz, ind, 3);
+        double wCon = rL * (DX(2, 0, 1, ID, iz, IF) * Uv(U, iz, ind, 1) + DX(2, 1, 1, ID, iz, IF) * Uv(U, iz, ind, 2) +
+                            DX(2, 2, 1, ID, iz, IF) * w) +
+                      rR * (DX(2, 0, 0, ID, iz + 1, IF) * Uv(U, iz + 1, ind, 1) +
+                            DX(2, 1, 0, ID, iz + 1, IF) * Uv(U, iz + 1, ind, 2) + DX(2, 2, 0, ID, iz + 1, IF) * w);
+        int lm = iz > 0 ? iz - 1 : 0, lp2 = iz + 2 < Nz ? iz + 2 : Nz - 1;
+#define CC(L) (Uv(U, L, ind, 4) / Uv(U, L, ind, 0))
+#define JC(L) (JJ(ID, 0, L, IF) + JJ(ID, 1, L, IF))
+        double cFL, cFR;
+        recu4(CC(lm), CC(iz), CC(iz + 1), CC(lp2), JC(lm), JC(iz), JC(iz + 1), JC(lp2), &cFL, &cFR);
+#undef CC
+#undef JC
+        double Msp = MM(iz + 1, ind, 0) + MM(iz + 1, ind, 1);
+        double Flux = 0.25 * ((fabs(wCon) + wCon) * cFL + (-fabs(wCon) + wCon) * cFR);
+        Uv(F, iz, ind, 4) += -Flux / Ms;
+        Uv(F, iz + 1, ind, 4) += Flux / Msp;
+        Flux = 0.5 * wCon;
+        Uv(F, iz, ind, 0) += -Flux / Ms;
+        Uv(F, iz + 1, ind, 0) += Flux / Msp;
+      }
+      double DivRhoTr = DS(I, 0) * uc[0][J] * cc[0][J] + DS(J, 0) * vc[I][0] * cc[I][0];
+      double DivRho = DS(I, 0) * uc[0][J] + DS(J, 0) * vc[I][0];
+      for (int k = 1; k < N; ++k) {
+        DivRhoTr += DS(I, k) * uc[k][J] * cc[k][J] + DS(J, k) * vc[I][k] * cc[I][k];
+        DivRho += DS(I, k) * uc[k][J] + DS(J, k) * vc[I][k];
+      }
+      Uv(F, iz, ind, 4) += DivRhoTr / Ms;
+      Uv(F, iz, ind, 0) += DivRho / Ms;
+    }
+}
+
+void cgdo_fcn(const cgdo_grid *g, double *F, const double *U, double *Temp1, const double *AuxG) {
+  const size_t ng = (size_t)g->Nz * (size_t)g->NumG;
+  memset(Temp1, 0, 5 * ng * sizeof(double));
+  for (int IF = 0; IF < g->NF; ++IF)
+    for (int iz = 0; iz < g->Nz; ++iz) hypervisc_element(g, IF, iz, 0, Temp1, U, NULL);
+  memset(F, 0, 5 * ng * sizeof(double));
+  for (int IF = 0; IF < g->NF; ++IF)
+    for (int iz = 0; iz < g->Nz; ++iz) hypervisc_element(g, IF, iz, 1, F, U, Temp1);
+  for (int IF = 0; IF < g->NF; ++IF)
+    for (int iz = 0; iz < g->Nz; ++iz) momentum_element(g, IF, iz, F, U);
+  for (int IF = 0; IF < g->NF; ++IF)
+    for (int iz = 0; iz < g->Nz; ++iz) grad_element(g, IF, iz, F, U, AuxG);
+  for (int IF = 0; IF < g->NF; ++IF)
+    for (int iz = 0; iz < g->Nz; ++iz) div_element(g, IF, iz, F, U);
+}
+
+/* JacSchurKernel! (JacSchur.jl:29-75); p = AuxG[:,:,4] for State = Dry */
+void cgdo_jac(const cgdo_grid *g, double *JRhoW, double *JWRho, double *JWRhoTh, double *JRhoThW, const double *U,
+              const double *AuxG) {
+  const int Nz = g->Nz;
+  for (int ig = 0; ig < g->NumG; ++ig)
+    for (int iz = 0; iz < Nz - 1; ++iz) {
+      size_t o = 2 * ((size_t)iz + (size_t)(Nz - 1) * (size_t)ig);
+      double RhoL = Uv(U, iz, ig, 0), RhoR = Uv(U, iz + 1, ig, 0);
+      double RhoThL = Uv(U, iz, ig, 4) + Uv(AuxG, iz, ig, 3);
+      double RhoThR = Uv(U, iz + 1, ig, 4) + Uv(AuxG, iz + 1, ig, 3);
+      double dzL = g->dz[(size_t)iz + (size_t)Nz * ig], dzR = g->dz[(size_t)iz + 1 + (size_t)Nz * ig];
+      double RhoF = (RhoL * dzL + RhoR * dzR) / (dzL + dzR);
+      JRhoW[o] = -RhoF / dzL;
+      JRhoW[o + 1] = RhoF / dzR;
+      double dPdThL = dpres_drhoth(g, RhoThL), dPdThR = dpres_drhoth(g, RhoThR);
+      JWRhoTh[o] = -dPdThR / RhoF / (0.5 * (dzL + dzR));
+      JWRhoTh[o + 1] = dPdThL / RhoF / (0.5 * (dzL + dzR));
+      JWRho[o] = -g->Grav * dzR / RhoF / (dzL + dzR);
+      JWRho[o + 1] = -g->Grav * dzL / RhoF / (dzL + dzR);
+      double RhoThF = (RhoThL * dzL + RhoThR * dzR) / (dzL + dzR);
+      JRhoThW[o] = -RhoThF / dzL;
+      JRhoThW[o + 1] = RhoThF / dzR;
+    }
+}
+
+/* SchurSolveGPU! (SchurSolve.jl:149-177) with its kernels, k: (Nz,NumG,5) */
+void cgdo_schur(const cgdo_grid *g, double *k, double *v, const double *JRhoW, const double *JWRho,
+                const double *JWRhoTh, const double *JRhoThW, double *tri, double fac, int comp_tri) {
+  const int Nz = g->Nz, n = Nz - 1;
+#define JB(A, c, i, ig) (A)[(size_t)(c) + 2 * ((size_t)(i) + (size_t)n * (size_t)(ig))]
+#define TR(c, i, ig) tri[(size_t)(c) + 3 * ((size_t)(i) + (size_t)n * (size_t)(ig))]
+  if (comp_tri) {
+    double invfac = 1.0 / fac, invfac2 = invfac / fac;
+    for (int ig = 0; ig < g->NumG; ++ig)
+      for (int i = 0; i < n; ++i) {
+        double d = invfac2 - JB(JWRho, 1, i, ig) * JB(JRhoW, 0, i, ig) - JB(JWRho, 0, i, ig) * JB(JRhoW, 1, i, ig) -
+                   JB(JWRhoTh, 1, i, ig) * JB(JRhoThW, 0, i, ig) - JB(JWRhoTh, 0, i, ig) * JB(JRhoThW, 1, i, ig);
+        TR(1, i, ig) = d;
+        TR(0, i, ig) = (i == 0) ? 0.0
+                                : -JB(JWRho, 0, i - 1, ig) * JB(JRhoW, 0, i, ig) -
+                                      JB(JWRhoTh, 0, i - 1, ig) * JB(JRhoThW, 0, i, ig);
+        TR(2, i, ig) = (i == n - 1) ? 0.0
+                                    : -JB(JWRho, 1, i + 1, ig) * JB(JRhoW, 1, i, ig) -
+                                          JB(JWRhoTh, 1, i + 1, ig) * JB(JRhoThW, 1, i, ig);
+      }
+  }
+  double *x = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  double *b = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+  for (int ig = 0; ig < g->NumG; ++ig) {
+    /* SchurSolveFKernel! */
+    for (int iz = 0; iz < Nz; ++iz) {
+      if (iz < Nz - 1)
+        Uv(v, iz, ig, 3) = Uv(v, iz, ig, 3) / fac + JB(JWRho, 1, iz, ig) * Uv(v, iz, ig, 0) +
+                           JB(JWRho, 0, iz, ig) * Uv(v, iz + 1, ig, 0) + JB(JWRhoTh, 1, iz, ig) * Uv(v, iz, ig, 4) +
+                           JB(JWRhoTh, 0, iz, ig) * Uv(v, iz + 1, ig, 4);
+      else
+        Uv(k, iz, ig, 3) = 0.0;
+    }
+    /* SchurSolveTriKernel! -> triSolve! on a copy of the rhs */
+    for (int i = 0; i < n; ++i) b[i] = Uv(v, i, ig, 3);
+    x[0] = TR(1, 0, ig);
+    for (int i = 0; i < n - 1; ++i) {
+      x[i + 1] = TR(1, i + 1, ig) - TR(2, i, ig) / x[i] * TR(0, i + 1, ig);
+      b[i + 1] = b[i + 1] - TR(2, i, ig) / x[i] * b[i];
+    }
+    x[n - 1] = b[n - 1] / x[n - 1];
+    for (int i = n - 2; i >= 0; --i) x[i] = (b[i] - TR(0, i + 1, ig) * x[i + 1]) / x[i];
+    for (int i = 0; i < n; ++i) Uv(k, i, ig, 3) = x[i];
+    /* SchurSolveBKernel! */
+    for (int iz = 0; iz < Nz; ++iz) {
+      if (iz == 0) {
+        Uv(v, iz, ig, 0) += JB(JRhoW, 0, iz, ig) * Uv(k, iz, ig, 3);
+        Uv(v, iz, ig, 4) += JB(JRhoThW, 0, iz, ig) * Uv(k, iz, ig, 3);
+      } else if (iz == Nz - 1) {
+        Uv(v, iz, ig, 0) += JB(JRhoW, 1, iz - 1, ig) * Uv(k, iz - 1, ig, 3);
+        Uv(v, iz, ig, 4) += JB(JRhoThW, 1, iz - 1, ig) * Uv(k, iz - 1, ig, 3);
+      } else {
+        Uv(v, iz, ig, 0) += JB(JRhoW, 0, iz, ig) * Uv(k, iz, ig, 3) + JB(JRhoW, 1, iz - 1, ig) * Uv(k, iz - 1, ig, 3);
+        Uv(v, iz, ig, 4) +=
+            JB(JRhoThW, 0, iz, ig) * Uv(k, iz, ig, 3) + JB(JRhoThW, 1, iz - 1, ig) * Uv(k, iz - 1, ig, 3);
+      }
+      Uv(k, iz, ig, 0) = fac * Uv(v, iz, ig, 0);
+      Uv(k, iz, ig, 1) = fac * Uv(v, iz, ig, 1);
+      Uv(k, iz, ig, 2) = fac * Uv(v, iz, ig, 2);
+      Uv(k, iz, ig, 4) = fac * Uv(v, iz, ig, 4);
+    }
+  }
+  free(x);
+  free(b);
+#undef JB
+#undef TR
+}
+
+/* SSP-Knoth (RosenbrockMethod.jl:24-48): a = alpha/Gamma, c = -inv(Gamma),
+ * m = Gamma' \ b evaluated by back substitution, gamma = 1. */
+void cgdo_ssp_knoth(double a[3][3], double c[3][3], double m[3], double *gamma) {
+  memset(a, 0, 9 * sizeof(double));
+  memset(c, 0, 9 * sizeof(double));
+  a[1][0] = 1.0;
+  a[2][0] = 0.25;
+  a[2][1] = 0.25;
+  c[0][0] = -1.0;
+  c[1][1] = -1.0;
+  c[2][2] = -1.0;
+  c[1][0] = -0.0;
+  c[2][0] = -0.75;
+  c[2][1] = -0.75;
+  double b1 = 1.0 / 6.0, b2 = 1.0 / 6.0, b3 = 2.0 / 3.0;
+  m[2] = b3;
+  m[1] = b2 - (-0.75) * m[2];
+  m[0] = b1 - (-0.75) * m[2];
+  *gamma = 1.0;
+}
+
+/* RosenbrockSchur! (RosenbrockSchur.jl:1-33).  Work arrays in cgdo_work. */
+void cgdo_rw_step(const cgdo_grid *g, double *V, double dt, cgdo_work *w) {
+  double a[3][3], c[3][3], m[3], gamma;
+  cgdo_ssp_knoth(a, c, m, &gamma);
+  const size_t n5 = 5 * (size_t)g->Nz * (size_t)g->NumG;
+  memcpy(w->Vn, V, n5 * sizeof(double));
+  for (int is = 0; is < 3; ++is) {
+    memcpy(V, w->Vn, n5 * sizeof(double));
+    for (int js = 0; js < is; ++js) {
+      const double *kj = w->k + (size_t)js * n5;
+      for (size_t i = 0; i < n5; ++i) V[i] = V[i] + a[is][js] * kj[i];
+    }
+    cgdo_prepare(g, V, w->AuxG);
+    cgdo_fcn(g, w->fV, V, w->Temp1, w->AuxG);
+    if (is == 0) cgdo_jac(g, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, V, w->AuxG);
+    for (int js = 0; js < is; ++js) {
+      double fac = c[is][js] / dt;
+      const double *kj = w->k + (size_t)js * n5;
+      for (size_t i = 0; i < n5; ++i) w->fV[i] = w->fV[i] + fac * kj[i];
+    }
+    cgdo_schur(g, w->k + (size_t)is * n5, w->fV, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma,
+               is == 0);
+  }
+  memcpy(V, w->Vn, n5 * sizeof(double));
+  for (int is = 0; is < 3; ++is) {
+    const double *ki = w->k + (size_t)is * n5;
+    for (size_t i = 0; i < n5; ++i) V[i] = V[i] + m[is] * ki[i];
+  }
+}

@@ -1,0 +1,38 @@
+/*
+ * CPU oracle (test infrastructure only) for the CGDycore.jl FP64 tendency
+ * and Rosenbrock-W step.  All arrays are FP64 in the reference's Julia
+ * column-major layouts:
+ *   U, F, fV, V, Vn, Temp1 : (Nz, NumG, NumV)      AuxG : (Nz, NumG, 4)
+ *   dXdxI : (3,3,2,N^2,Nz,NF)  J : (N^2,2,Nz,NF)  X : (N^2,2,3,Nz,NF)
+ *   M : (Nz,NumG,2)  dz : (Nz,NumG)  Glob : int64 (N^2,NF), 1-based
+ *   J blocks : (2,Nz-1,NumG)  tri : (3,Nz-1,NumG)  k : (Nz,NumG,5,3)
+ */
+#ifndef CGD_ORACLE_H
+#define CGD_ORACLE_H
+
+#include <stdint.h>
+
+typedef struct {
+  int N, Nz, NF, NumG;
+  const int64_t *Glob;
+  const double *DS, *DW;
+  const double *dXdxI, *J, *X, *M, *dz;
+  double Grav, Omega, Rd, Cpd, p0, kappa;
+  double KoeffCurl, KoeffGrad, KoeffDiv;
+} cgdo_grid;
+
+typedef struct {
+  double *Vn, *fV, *k, *AuxG, *Temp1;
+  double *JRhoW, *JWRho, *JWRhoTh, *JRhoThW, *tri;
+} cgdo_work;
+
+void cgdo_prepare(const cgdo_grid *g, const double *U, double *AuxG);
+void cgdo_fcn(const cgdo_grid *g, double *F, const double *U, double *Temp1, const double *AuxG);
+void cgdo_jac(const cgdo_grid *g, double *JRhoW, double *JWRho, double *JWRhoTh, double *JRhoThW,
+              const double *U, const double *AuxG);
+void cgdo_schur(const cgdo_grid *g, double *k, double *v, const double *JRhoW, const double *JWRho,
+                const double *JWRhoTh, const double *JRhoThW, double *tri, double fac, int comp_tri);
+void cgdo_ssp_knoth(double a[3][3], double c[3][3], double m[3], double *gamma);
+void cgdo_rw_step(const cgdo_grid *g, double *V, double dt, cgdo_work *w);
+
+#endif

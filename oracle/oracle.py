@@ -1,0 +1,132 @@
+"""ctypes front end of the CPU oracle (``libcgd_oracle.so``).
+
+Test infrastructure only: used by ``tests/``, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` as the checker / CPU baseline.  The
+arrays follow the reference's layouts as numpy arrays (see
+``cgdycore.jl_amd/discretization.py``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libcgd_oracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+
+
+class _Grid(C.Structure):
+    _fields_ = [("N", C.c_int), ("Nz", C.c_int), ("NF", C.c_int), ("NumG", C.c_int),
+                ("Glob", C.POINTER(C.c_int64)), ("DS", _dp), ("DW", _dp),
+                ("dXdxI", _dp), ("J", _dp), ("X", _dp), ("M", _dp), ("dz", _dp),
+                ("Grav", C.c_double), ("Omega", C.c_double), ("Rd", C.c_double),
+                ("Cpd", C.c_double), ("p0", C.c_double), ("kappa", C.c_double),
+                ("KoeffCurl", C.c_double), ("KoeffGrad", C.c_double), ("KoeffDiv", C.c_double)]
+
+
+class _Work(C.Structure):
+    _fields_ = [(n, _dp) for n in ("Vn", "fV", "k", "AuxG", "Temp1", "JRhoW", "JWRho",
+                                   "JWRhoTh", "JRhoThW", "tri")]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _lib.cgdo_ssp_knoth.argtypes = [_dp, _dp, _dp, _dp]
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"] and a.dtype == np.float64
+    return a.ctypes.data_as(_dp)
+
+
+class Oracle:
+    """Binds one discretization (static inputs) to the oracle."""
+
+    def __init__(self, disc, phys=None, koeff=(0.0, 0.0, 0.0)):
+        from_phys = phys
+        if from_phys is None:
+            raise ValueError("phys constants required")
+        self.disc = disc
+        self._keep = [np.ascontiguousarray(x) for x in
+                      (disc.glob.astype(np.int64), np.asfortranarray(disc.DS).ravel(order="K"),
+                       np.asfortranarray(disc.DW).ravel(order="K"), disc.dXdxI, disc.J, disc.X,
+                       disc.M, disc.dz)]
+        glob, DS, DW, dX, J, X, M, dz = self._keep
+        # DS, DW are (N,N) column-major: DS[i + N*k]
+        DSc = np.ascontiguousarray(disc.DS.T).ravel()
+        DWc = np.ascontiguousarray(disc.DW.T).ravel()
+        self._keep[1], self._keep[2] = DSc, DWc
+        g = _Grid()
+        g.N, g.Nz, g.NF, g.NumG = disc.N, disc.nz, disc.nf, disc.num_g
+        g.Glob = glob.ctypes.data_as(C.POINTER(C.c_int64))
+        g.DS, g.DW = _p(DSc), _p(DWc)
+        g.dXdxI, g.J, g.X, g.M, g.dz = _p(dX), _p(J), _p(X), _p(M), _p(dz)
+        g.Grav, g.Omega, g.Rd, g.Cpd = phys.Grav, phys.Omega, phys.Rd, phys.Cpd
+        g.p0, g.kappa = phys.p0, phys.kappa
+        g.KoeffCurl, g.KoeffGrad, g.KoeffDiv = koeff
+        self.g = g
+        self.L = lib()
+
+    # shapes
+    def _state(self, nv=5):
+        return np.zeros((nv, self.disc.num_g, self.disc.nz))
+
+    def prepare(self, U, AuxG=None):
+        AuxG = self._state(4) if AuxG is None else AuxG
+        self.L.cgdo_prepare(C.byref(self.g), _p(U), _p(AuxG))
+        return AuxG
+
+    def fcn(self, U, AuxG):
+        F = self._state(5)
+        Temp1 = self._state(5)
+        self.L.cgdo_fcn(C.byref(self.g), _p(F), _p(U), _p(Temp1), _p(AuxG))
+        return F, Temp1
+
+    def jac(self, U, AuxG):
+        n = (self.disc.num_g, self.disc.nz - 1, 2)
+        blocks = [np.zeros(n) for _ in range(4)]
+        self.L.cgdo_jac(C.byref(self.g), *[_p(b) for b in blocks], _p(U), _p(AuxG))
+        return blocks  # JRhoW, JWRho, JWRhoTh, JRhoThW
+
+    def schur(self, v, blocks, fac, tri=None, comp_tri=True):
+        k = self._state(5)
+        tri = np.zeros((self.disc.num_g, self.disc.nz - 1, 3)) if tri is None else tri
+        self.L.cgdo_schur(C.byref(self.g), _p(k), _p(v), *[_p(b) for b in blocks], _p(tri),
+                          C.c_double(fac), C.c_int(1 if comp_tri else 0))
+        return k, tri
+
+    def rw_step(self, V, dt, nsteps=1):
+        d = self.disc
+        n5 = (5, d.num_g, d.nz)
+        w = dict(Vn=np.zeros(n5), fV=np.zeros(n5), k=np.zeros((3,) + n5),
+                 AuxG=np.zeros((4, d.num_g, d.nz)), Temp1=np.zeros(n5),
+                 JRhoW=np.zeros((d.num_g, d.nz - 1, 2)), JWRho=np.zeros((d.num_g, d.nz - 1, 2)),
+                 JWRhoTh=np.zeros((d.num_g, d.nz - 1, 2)), JRhoThW=np.zeros((d.num_g, d.nz - 1, 2)),
+                 tri=np.zeros((d.num_g, d.nz - 1, 3)))
+        work = _Work(**{k: _p(v) for k, v in w.items()})
+        for _ in range(nsteps):
+            self.L.cgdo_rw_step(C.byref(self.g), _p(V), C.c_double(dt), C.byref(work))
+        return w
+
+    @staticmethod
+    def ssp_knoth():
+        a = np.zeros((3, 3))
+        c = np.zeros((3, 3))
+        m = np.zeros(3)
+        gam = np.zeros(1)
+        lib().cgdo_ssp_knoth(_p(a), _p(c), _p(m), _p(gam))
+        return a, c, m, float(gam[0])
