@@ -1,0 +1,173 @@
+"""Benchmark: SSP-Knoth Rosenbrock-W steps of the spectral-element dynamical
+core on one MI355X, FP64 drop-in (cubed sphere ne=30, p=4, nz=64, dt=100 s,
+seeded baroclinic-jet state).
+
+Prints one JSON line (rank 0): throughput in element points x steps per
+second (Mpts/s, points = 6 ne^2 (p+1)^2 nz), the roofline of the dominant
+kernel (fused tendency pass, HBM bound) and the CPU oracle timed on this
+host as a reported baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0  # MI355X HBM3E peak, GB/s
+
+
+def survey_bytes(ne, p, nz, wt=8, wm=8, wc=8):
+    """Algorithmic HBM bytes (frozen per-unit formulas of the build's survey).
+
+    Returns dict with per-RW-step total and the per-evaluation bytes of the
+    pass-1, pass-2 and column kernels for stage 1.
+    """
+    NE = 6 * ne * ne
+    N = p + 1
+    Ne = NE * N * N * nz
+    Ng = (6 * ne * ne * p * p + 2) * nz
+    glob = NE * N * N * 8
+    pass1 = (4 * Ng * wt + 10 * Ne * wm + 2 * Ng * wm + 5 * Ng * wt) + glob
+    pass2 = (5 * Ng * wt + 5 * Ng * wt + 26 * Ne * wm + 2 * Ng * wm + 5 * Ng * wt) + glob
+    step = 0
+    for i in range(1, 4):
+        col = 5 * Ng * wc + (i - 1) * 5 * Ng * wc + (8 + 3) * Ng * wc + 5 * Ng * wc
+        stage = (5 + (i - 1) * 5 + 5) * Ng * wc if i >= 2 else 0
+        step += pass1 + pass2 + col + stage
+    step += (2 * Ng + Ng) * wc + 8 * Ng * wc  # Jacobian
+    step += 8 * Ng * wc + 3 * Ng * wc  # tri build
+    step += (5 + 15 + 5) * Ng * wc  # final update
+    return dict(step=step, pass1=pass1, pass2=pass2, Ne=Ne, Ng=Ng)
+
+
+def cpu_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ne", type=int, default=30)
+    ap.add_argument("--p", type=int, default=4)
+    ap.add_argument("--nz", type=int, default=64)
+    ap.add_argument("--dt", type=float, default=None,
+                    help="default 100 s at ne<=30 and 3000/ne s above (horizontal acoustic CFL of p=4)")
+    ap.add_argument("--hyperd", type=float, default=None, help="hyperviscosity coefficient (m^4/s)")
+    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle RW steps for the CPU baseline (0: skip)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.dt is None:
+        args.dt = 100.0 if args.ne <= 30 else 3000.0 / args.ne
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU halo exchange is not implemented", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import __graft_entry__ as ge
+
+    if args.calibrate_copy:
+        n = args.calibrate_copy * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": n * 8, "copies": 3}))
+        return
+
+    pkg = ge.load_package()
+    host = pkg.host
+    D = pkg.discretization
+    t0 = time.perf_counter()
+    disc = D.build(args.ne, args.p, args.nz, 30000.0)
+    phys = D.Phys()
+    U0 = pkg.state.jet_state(disc, seed=0)
+    print(f"[bench] setup ne={args.ne} p={args.p} nz={args.nz}: {time.perf_counter() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, hyperd=args.hyperd, device="cuda:0")
+    eq = host.CompressibleShallow()
+    ctx = host.init_context(FE, Metric, phys, Global)
+    V = torch.as_tensor(U0, device="cuda:0").clone()
+
+    def step():
+        host.RosenbrockSchur(V, args.dt, host.FcnGPU, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys,
+                             Cache, JCache, None, Global, None, eq)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    passes = {name: ctx.profile_read(i) for i, name in enumerate(("pass1", "pass2", "column"))}
+    ctx.profile(False)
+    finite = bool(torch.isfinite(V).all().item())
+    print(f"[bench] {args.steps} steps in {wall:.3f}s, finite={finite}, passes={passes}", file=sys.stderr,
+          flush=True)
+
+    sb = survey_bytes(args.ne, args.p, args.nz)
+    Ne = sb["Ne"]
+    ms_per_step = wall / args.steps * 1e3
+    mpts = Ne * args.steps / wall / 1e6
+    # dominant kernel: fused tendency pass (k_fcn2), 3 evaluations per step
+    p2_ms, p2_launch = passes["pass2"]
+    n_eval = 3 * args.steps
+    p2_eval_s = p2_ms / 1e3 / n_eval
+    achieved = sb["pass2"] / p2_eval_s / 1e9
+    roofline = {"bound": "hbm", "kernel": "k_fcn2 (fused tendency pass)", "achieved": round(achieved, 1),
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_eval": sb["pass2"], "launches_per_eval": p2_launch // max(n_eval, 1),
+                "ms_per_eval": round(p2_eval_s * 1e3, 4),
+                "step_roofline_frac": round(sb["step"] / (wall / args.steps) / 1e9 / PEAK_HBM_GBS, 4),
+                "pass_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in passes.items()}}
+
+    cpu = None
+    if args.cpu_steps > 0:
+        O = ge.load_oracle()
+        orc = O.Oracle(disc, phys, (Global.Model.HyperDCurl, Global.Model.HyperDGrad, Global.Model.HyperDDiv))
+        Vo = U0.copy()
+        t0 = time.perf_counter()
+        orc.rw_step(Vo, args.dt, args.cpu_steps)
+        ct = time.perf_counter() - t0
+        cpu = {"value": round(Ne * args.cpu_steps / ct / 1e6, 4), "unit": "Mpts/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} RW step(s) of the C oracle on the same workload "
+                         f"({ct:.1f}s, 1 thread, {cpu_info()})"}
+
+    out = {"metric": "RW step Mpts/s + % HBM roofline, FP64 vs FP32-mixed at stated err",
+           "value": round(mpts, 2), "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64", "data": "synthetic: seeded baroclinic jet (seed 0) on a flat cubed sphere",
+           "config": {"workload": f"rw_step_fp64_ne{args.ne}_p{args.p}_nz{args.nz}", "ne": args.ne, "p": args.p,
+                      "nz": args.nz, "dt": args.dt, "hyperd": Global.Model.HyperDDiv, "points": Ne,
+                      "precision": "fp64", "finite": finite},
+           "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -17,19 +18,25 @@ struct cgd_ctx {
   KArgs ka{};
   int N = 0, NN = 0, Nz = 0, NF = 0, NumG = 0;
   int nzt = 0, nchunk = 0, block = 0;
+  int thomas = 0;  // column solve: 0 = cyclic reduction, 1 = serial Thomas (CGD_THOMAS=1)
   size_t lds1 = 0, lds2 = 0;
   int ncolor = 0;
   std::vector<int> color_off;  // host offsets into d_elist per colour
   int *d_glob = nullptr;       // [NF][NN] 0-based
   int *d_elist = nullptr;      // elements grouped by colour
   double *d_met = nullptr, *d_met1 = nullptr, *d_ms = nullptr, *d_bwork = nullptr;
+  double *d_f4x = nullptr;     // [NF][NN][nchunk] chunk-bottom F4 terms
+  int *d_nptr = nullptr, *d_ncopy = nullptr;  // global node -> element copies (colour order)
+  unsigned char *d_first = nullptr;           // [NF][NN] first copy of its node in colour order
   bool ok = false;
   std::string err;
   // per-pass timing (events on the launch stream)
   bool prof = false;
   double prof_ms[3] = {0, 0, 0};
   long long prof_n[3] = {0, 0, 0};
-  hipEvent_t ev[2] = {nullptr, nullptr};
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<int> prof_which, prof_launch;
+  int prof_cap = 0, prof_used = 0;
 };
 
 static int fail(cgd_ctx *c, int code, const std::string &msg) {
@@ -80,11 +87,20 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   c->NF = d->NF;
   c->NumG = d->NumG;
   const int NN = c->NN, Nz = c->Nz, NF = c->NF;
-  // level chunks of at most 32 levels (NN * nzt <= 1024 threads, tiles < 64 KiB)
-  int maxl = 1024 / NN;
-  if (maxl > 32) maxl = 32;
-  c->nchunk = (Nz + maxl - 1) / maxl;
-  c->nzt = (Nz + c->nchunk - 1) / c->nchunk;
+  // level chunk height (a template constant, 8 or 16 levels: a chunk tiles
+  // the 16-lane DPP rows).  The fused tendency kernel needs ~250 VGPRs, so
+  // 8-level chunks (4 waves per workgroup) keep two workgroups per CU; this
+  // measured faster than 16 at ne=30, nz=64.  CGD_NZT=16 overrides.
+  {
+    int nzt = 8;
+    if (const char *ev = getenv("CGD_NZT")) {
+      const int o = atoi(ev);
+      if (o == 8 || o == 16) nzt = o;
+    }
+    c->nzt = nzt;
+    c->nchunk = (Nz + nzt - 1) / nzt;
+    if (const char *ev = getenv("CGD_THOMAS")) c->thomas = atoi(ev) != 0;
+  }
   c->block = ((NN * c->nzt + 63) / 64) * 64;
   c->lds1 = sizeof(double) * (2 * NN + 5 * NN * c->nzt);
   c->lds2 = sizeof(double) * (2 * NN + 8 * NN * c->nzt);
@@ -137,6 +153,20 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
     std::vector<int> fp(c->color_off.begin(), c->color_off.end() - 1);
     for (int e = 0; e < NF; ++e) elist[fp[color[e]]++] = e;
   }
+  // global node -> element copies (e*NN + node) in colour order
+  std::vector<int> cptr(d->NumG + 1, 0), ccopy((size_t)NF * NN);
+  for (int v : g32) cptr[v + 1]++;
+  for (int i = 0; i < d->NumG; ++i) cptr[i + 1] += cptr[i];
+  {
+    std::vector<int> fp(cptr.begin(), cptr.end() - 1);
+    for (int q = 0; q < NF; ++q) {
+      const int e = elist[q];
+      for (int i = 0; i < NN; ++i) ccopy[fp[g32[(size_t)e * NN + i]]++] = e * NN + i;
+    }
+  }
+  std::vector<unsigned char> firstc((size_t)NF * NN, 0);
+  for (int gn = 0; gn < d->NumG; ++gn)
+    if (cptr[gn + 1] > cptr[gn]) firstc[ccopy[cptr[gn]]] = 1;
   // device buffers (set-up only; nothing is allocated on the step path)
   const size_t npt = (size_t)NF * NN * Nz;
   HIPCHK(c, hipMalloc(&c->d_glob, g32.size() * sizeof(int)));
@@ -145,6 +175,13 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   HIPCHK(c, hipMalloc(&c->d_met1, npt * CGD_NMET1 * sizeof(double)));
   HIPCHK(c, hipMalloc(&c->d_ms, (size_t)d->NumG * Nz * sizeof(double)));
   HIPCHK(c, hipMalloc(&c->d_bwork, (size_t)d->NumG * (Nz - 1) * sizeof(double)));
+  HIPCHK(c, hipMalloc(&c->d_f4x, (size_t)NF * NN * c->nchunk * sizeof(double)));
+  HIPCHK(c, hipMalloc(&c->d_first, firstc.size()));
+  HIPCHK(c, hipMemcpyAsync(c->d_first, firstc.data(), firstc.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMalloc(&c->d_nptr, cptr.size() * sizeof(int)));
+  HIPCHK(c, hipMalloc(&c->d_ncopy, ccopy.size() * sizeof(int)));
+  HIPCHK(c, hipMemcpyAsync(c->d_nptr, cptr.data(), cptr.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->d_ncopy, ccopy.data(), ccopy.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_glob, g32.data(), g32.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_elist, elist.data(), elist.size() * sizeof(int), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_relayout_metric, dim3(grid_for(npt, 256)), dim3(256), 0, st, d->dXdxI, d->J, d->X, c->d_met,
@@ -179,11 +216,12 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   a.met1 = c->d_met1;
   a.Ms = c->d_ms;
   a.M = d->M;
-  HIPCHK(c, hipEventCreate(&c->ev[0]));
-  HIPCHK(c, hipEventCreate(&c->ev[1]));
+  a.first = c->d_first;
   c->ok = true;
   return CGD_OK;
 }
+
+static void prof_free(cgd_ctx *c);
 
 extern "C" int cgd_ctx_destroy(cgd_ctx *c) {
   if (!c) return CGD_ERR_ARG;
@@ -193,8 +231,11 @@ extern "C" int cgd_ctx_destroy(cgd_ctx *c) {
   (void)hipFree(c->d_met1);
   (void)hipFree(c->d_ms);
   (void)hipFree(c->d_bwork);
-  if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
-  if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
+  (void)hipFree(c->d_f4x);
+  (void)hipFree(c->d_nptr);
+  (void)hipFree(c->d_first);
+  (void)hipFree(c->d_ncopy);
+  prof_free(c);
   delete c;
   return CGD_OK;
 }
@@ -205,26 +246,42 @@ static int check(cgd_ctx *c) {
   return CGD_OK;
 }
 
-// timing helpers around a group of launches of one pass
+// Timing of one pass (a group of launches): an event pair is recorded on the
+// launch stream; nothing synchronises until cgd_profile_read.  The events
+// are created by cgd_profile_enable (never on the step path).
 static void prof_begin(cgd_ctx *c, hipStream_t st) {
-  if (c->prof) (void)hipEventRecord(c->ev[0], st);
+  if (!c->prof || c->prof_used >= c->prof_cap) return;
+  (void)hipEventRecord(c->prof_ev[2 * c->prof_used], st);
 }
 static int prof_end(cgd_ctx *c, hipStream_t st, int which, int launches) {
-  if (!c->prof) return CGD_OK;
-  HIPCHK(c, hipEventRecord(c->ev[1], st));
-  HIPCHK(c, hipEventSynchronize(c->ev[1]));
-  float ms = 0;
-  HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-  c->prof_ms[which] += ms;
-  c->prof_n[which] += launches;
+  if (!c->prof || c->prof_used >= c->prof_cap) return CGD_OK;
+  HIPCHK(c, hipEventRecord(c->prof_ev[2 * c->prof_used + 1], st));
+  c->prof_which.push_back(which);
+  c->prof_launch.push_back(launches);
+  c->prof_used++;
   return CGD_OK;
+}
+
+static void prof_free(cgd_ctx *c) {
+  for (hipEvent_t e : c->prof_ev) (void)hipEventDestroy(e);
+  c->prof_ev.clear();
+  c->prof_which.clear();
+  c->prof_launch.clear();
+  c->prof_cap = c->prof_used = 0;
 }
 
 extern "C" int cgd_profile_enable(cgd_ctx *c, int on) {
   int r = check(c);
   if (r) return r;
-  c->prof = on != 0;
+  prof_free(c);
   for (int i = 0; i < 3; ++i) c->prof_ms[i] = 0, c->prof_n[i] = 0;
+  c->prof = on != 0;
+  if (c->prof) {
+    const int cap = 4096;
+    c->prof_ev.resize(2 * cap);
+    for (auto &e : c->prof_ev) HIPCHK(c, hipEventCreate(&e));
+    c->prof_cap = cap;
+  }
   return CGD_OK;
 }
 
@@ -232,15 +289,54 @@ extern "C" int cgd_profile_read(cgd_ctx *c, int which, double *total_ms, long lo
   int r = check(c);
   if (r) return r;
   if (which < 0 || which > 2 || !total_ms || !launches) return fail(c, CGD_ERR_ARG, "bad profile query");
+  // fold recorded pairs into the totals
+  for (int i = 0; i < c->prof_used; ++i) {
+    HIPCHK(c, hipEventSynchronize(c->prof_ev[2 * i + 1]));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->prof_ev[2 * i], c->prof_ev[2 * i + 1]));
+    c->prof_ms[c->prof_which[i]] += ms;
+    c->prof_n[c->prof_which[i]] += c->prof_launch[i];
+  }
+  c->prof_used = 0;
+  c->prof_which.clear();
+  c->prof_launch.clear();
   *total_ms = c->prof_ms[which];
   *launches = c->prof_n[which];
   return CGD_OK;
 }
 
-static int fill(cgd_ctx *c, double *x, size_t n, hipStream_t st) {
-  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, st, x, n, 0.0);
-  HIPCHK(c, hipGetLastError());
-  return CGD_OK;
+
+// template dispatch on (N, chunk height)
+template <int N, int NZT>
+static void fcn1_t(cgd_ctx *c, dim3 g, hipStream_t st, const int *el, const double *U, double *T1) {
+  hipLaunchKernelGGL((k_fcn1<N, NZT>), g, dim3(c->block), c->lds1, st, c->ka, el, c->nchunk, U, T1);
+}
+template <int N, int NZT>
+static void fcn2_t(cgd_ctx *c, dim3 g, hipStream_t st, const int *el, const double *U, const double *T1,
+                   const double *P, double *F) {
+  hipLaunchKernelGGL((k_fcn2<N, NZT>), g, dim3(c->block), c->lds2, st, c->ka, el, c->nchunk, U, T1, P, F,
+                     c->d_f4x);
+}
+#define CGD_DISPATCH(FN, ...)                    \
+  do {                                           \
+    if (c->N == 5) {                             \
+      if (c->nzt == 8)                           \
+        FN<5, 8>(c, __VA_ARGS__);                \
+      else                                       \
+        FN<5, 16>(c, __VA_ARGS__);               \
+    } else {                                     \
+      if (c->nzt == 8)                           \
+        FN<4, 8>(c, __VA_ARGS__);                \
+      else                                       \
+        FN<4, 16>(c, __VA_ARGS__);               \
+    }                                            \
+  } while (0)
+static void launch_fcn1(cgd_ctx *c, dim3 g, hipStream_t st, const int *el, const double *U, double *T1) {
+  CGD_DISPATCH(fcn1_t, g, st, el, U, T1);
+}
+static void launch_fcn2(cgd_ctx *c, dim3 g, hipStream_t st, const int *el, const double *U, const double *T1,
+                        const double *P, double *F) {
+  CGD_DISPATCH(fcn2_t, g, st, el, U, T1, P, F);
 }
 
 extern "C" int cgd_prepare(cgd_ctx *c, const double *U, double *AuxG, void *stream) {
@@ -258,37 +354,35 @@ extern "C" int cgd_fcn(cgd_ctx *c, double *F, const double *U, double *Temp1, co
   if (r) return r;
   if (!F || !U || !Temp1 || !AuxG) return fail(c, CGD_ERR_ARG, "cgd_fcn: null array");
   hipStream_t st = S(stream);
-  const size_t n5 = 5 * (size_t)c->NumG * c->Nz;
-  if ((r = fill(c, Temp1, n5, st))) return r;
+  // every global node is written by its first element (colour order) and
+  // accumulated by the others, so Temp1 and F need no zero fill
   prof_begin(c, st);
   int launches = 0;
   for (int k = 0; k < c->ncolor; ++k) {
     const int ne = c->color_off[k + 1] - c->color_off[k];
     if (!ne) continue;
     const int *el = c->d_elist + c->color_off[k];
-    if (c->N == 5)
-      hipLaunchKernelGGL(k_fcn1<5>, dim3(ne * c->nchunk), dim3(c->block), c->lds1, st, c->ka, el, c->nchunk, U, Temp1);
-    else
-      hipLaunchKernelGGL(k_fcn1<4>, dim3(ne * c->nchunk), dim3(c->block), c->lds1, st, c->ka, el, c->nchunk, U, Temp1);
+    launch_fcn1(c, dim3(ne * c->nchunk), st, el, U, Temp1);
     HIPCHK(c, hipGetLastError());
     ++launches;
   }
   if ((r = prof_end(c, st, 0, launches))) return r;
-  if ((r = fill(c, F, n5, st))) return r;
   prof_begin(c, st);
   launches = 0;
   for (int k = 0; k < c->ncolor; ++k) {
     const int ne = c->color_off[k + 1] - c->color_off[k];
     if (!ne) continue;
     const int *el = c->d_elist + c->color_off[k];
-    for (int ch = 0; ch < c->nchunk; ++ch) {
-      if (c->N == 5)
-        hipLaunchKernelGGL(k_fcn2<5>, dim3(ne), dim3(c->block), c->lds2, st, c->ka, el, ch, U, Temp1, AuxG, F);
-      else
-        hipLaunchKernelGGL(k_fcn2<4>, dim3(ne), dim3(c->block), c->lds2, st, c->ka, el, ch, U, Temp1, AuxG, F);
-      HIPCHK(c, hipGetLastError());
-      ++launches;
-    }
+    launch_fcn2(c, dim3(ne * c->nchunk), st, el, U, Temp1, AuxG, F);
+    HIPCHK(c, hipGetLastError());
+    ++launches;
+  }
+  if (c->nchunk > 1) {
+    const size_t nfix = (size_t)c->NumG * (c->nchunk - 1);
+    hipLaunchKernelGGL(k_f4fix, dim3(grid_for(nfix, 256)), dim3(256), 0, st, c->d_nptr, c->d_ncopy, c->d_f4x, F,
+                       c->NumG, c->Nz, c->nchunk, c->nzt);
+    HIPCHK(c, hipGetLastError());
+    ++launches;
   }
   return prof_end(c, st, 1, launches);
 }
@@ -315,8 +409,12 @@ extern "C" int cgd_schur(cgd_ctx *c, double *k, double *v, const double *JRhoW, 
   if (!(fac > 0.0)) return fail(c, CGD_ERR_ARG, "cgd_schur: fac must be positive");
   hipStream_t st = S(stream);
   prof_begin(c, st);
-  hipLaunchKernelGGL(k_schur, dim3(grid_for(c->NumG, 64)), dim3(64), 0, st, c->ka, k, v, JRhoW, JWRho, JWRhoTh,
-                     JRhoThW, tri, c->d_bwork, fac, comp_tri);
+  if (c->Nz <= 64)
+    hipLaunchKernelGGL(k_schur_col, dim3((c->NumG + 3) / 4), dim3(256), 0, st, c->ka, k, v, JRhoW, JWRho, JWRhoTh,
+                       JRhoThW, tri, fac, comp_tri, c->thomas);
+  else
+    hipLaunchKernelGGL(k_schur, dim3(grid_for(c->NumG, 64)), dim3(64), 0, st, c->ka, k, v, JRhoW, JWRho, JWRhoTh,
+                       JRhoThW, tri, c->d_bwork, fac, comp_tri);
   HIPCHK(c, hipGetLastError());
   return prof_end(c, st, 2, 1);
 }
@@ -336,27 +434,46 @@ extern "C" int cgd_rw_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *
   const double Mw[3] = {m1, m2, m3};
   const double gamma = 1.0;
   const size_t n5 = 5 * (size_t)c->NumG * c->Nz;
-  const int gb = grid_for(n5, 256);
   double *k[3] = {w->k, w->k + n5, w->k + 2 * n5};
   HIPCHK(c, hipMemcpyAsync(w->Vn, V, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
   for (int is = 0; is < 3; ++is) {
-    if (is > 0) {
-      hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], A[is][0], k[1], A[is][1], is, n5);
-      HIPCHK(c, hipGetLastError());
-    }
     if ((r = cgd_prepare(c, V, w->AuxG, stream))) return r;
     if ((r = cgd_fcn(c, w->fV, V, w->Temp1, w->AuxG, stream))) return r;
     if (is == 0 && (r = cgd_jac(c, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, V, w->AuxG, stream))) return r;
-    if (is > 0) {
-      hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, w->fV, w->fV, k[0], Cc[is][0] / dt, k[1],
-                         Cc[is][1] / dt, is, n5);
+    RWStage rw{};
+    rw.stage = is;
+    for (int j = 0; j < is; ++j) {
+      rw.kprev[j] = k[j];
+      rw.cf[j] = Cc[is][j] / dt;
+    }
+    rw.Vn = w->Vn;
+    rw.V = V;
+    for (int j = 0; j <= is; ++j) rw.av[j] = (is < 2) ? A[is + 1][j] : Mw[j];
+    prof_begin(c, st);
+    if (c->Nz <= 64) {
+      hipLaunchKernelGGL(k_rw_col, dim3((c->NumG + 3) / 4), dim3(256), 0, st, c->ka, k[is], w->fV, w->JRhoW,
+                         w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma, is == 0 ? 1 : 0, c->thomas, rw);
+      HIPCHK(c, hipGetLastError());
+      if ((r = prof_end(c, st, 2, 1))) return r;
+    } else {
+      // tall columns: stage terms with a separate pass, then the drop-in solve
+      if ((r = prof_end(c, st, 2, 0))) return r;
+      const int gb = grid_for(n5, 256);
+      if (is > 0) {
+        hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, w->fV, w->fV, k[0], Cc[is][0] / dt, k[1],
+                           Cc[is][1] / dt, is, n5);
+        HIPCHK(c, hipGetLastError());
+      }
+      if ((r = cgd_schur(c, k[is], w->fV, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma, is == 0,
+                         stream)))
+        return r;
+      if (is < 2)
+        hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], A[is + 1][0], k[1], A[is + 1][1],
+                           is + 1, n5);
+      else
+        hipLaunchKernelGGL(k_axpy3, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], Mw[0], k[1], Mw[1], k[2], Mw[2], n5);
       HIPCHK(c, hipGetLastError());
     }
-    if ((r = cgd_schur(c, k[is], w->fV, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma, is == 0,
-                       stream)))
-      return r;
   }
-  hipLaunchKernelGGL(k_axpy3, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], Mw[0], k[1], Mw[1], k[2], Mw[2], n5);
-  HIPCHK(c, hipGetLastError());
   return CGD_OK;
 }

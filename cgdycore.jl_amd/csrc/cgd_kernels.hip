@@ -35,6 +35,7 @@ struct KArgs {
   const double *met1;              // [NF][5][NN][Nz]
   const double *Ms;                // [NumG][Nz]
   const double *M;                 // [2][NumG][Nz]
+  const unsigned char *first;      // [NF][NN]: 1 if this copy is the node's first in colour order
 };
 
 // metric component indices of met (dXdxI[a,b,K] -> dabK, 1-based a,b,K)
@@ -72,47 +73,48 @@ __device__ __forceinline__ void recu4(double cLL, double cL, double cR, double c
 
 // ------------------------------------------------------------------------
 // LDS tile helpers
-template <int N> struct Tiles {
+// tile[f][node][level]; NZ (levels per chunk) is a compile-time constant so
+// that every (tile, node) offset is an instruction immediate
+template <int N, int NZ> struct Tiles {
   double *t;  // tile base
-  int nzt;
-  __device__ __forceinline__ double &at(int f, int node, int l) const { return t[(f * N * N + node) * nzt + l]; }
+  __device__ __forceinline__ double &at(int f, int node, int l) const { return t[(f * N * N + node) * NZ + l]; }
 };
 
 // sum_k D[I,k] f(k,J)
-template <int N>
-__device__ __forceinline__ double dx(const double *Dm, const Tiles<N> &T, int f, int I, int J, int l) {
+template <int N, int NZ>
+__device__ __forceinline__ double dx(const double *Dm, const Tiles<N, NZ> &T, int f, int I, int J, int l) {
   double s = Dm[I * N] * T.at(f, J * N, l);
 #pragma unroll
   for (int k = 1; k < N; ++k) s += Dm[I * N + k] * T.at(f, k + J * N, l);
   return s;
 }
 // sum_k D[J,k] f(I,k)
-template <int N>
-__device__ __forceinline__ double dy(const double *Dm, const Tiles<N> &T, int f, int I, int J, int l) {
+template <int N, int NZ>
+__device__ __forceinline__ double dy(const double *Dm, const Tiles<N, NZ> &T, int f, int I, int J, int l) {
   double s = Dm[J * N] * T.at(f, I, l);
 #pragma unroll
   for (int k = 1; k < N; ++k) s += Dm[J * N + k] * T.at(f, I + k * N, l);
   return s;
 }
 // sum_k (D[I,k] fx(k,J) + D[J,k] fy(I,k)), pairwise as in the reference loops
-template <int N>
-__device__ __forceinline__ double dxy(const double *Dm, const Tiles<N> &T, int fx, int fy, int I, int J, int l) {
+template <int N, int NZ>
+__device__ __forceinline__ double dxy(const double *Dm, const Tiles<N, NZ> &T, int fx, int fy, int I, int J, int l) {
   double s = Dm[I * N] * T.at(fx, J * N, l) + Dm[J * N] * T.at(fy, I, l);
 #pragma unroll
   for (int k = 1; k < N; ++k) s += Dm[I * N + k] * T.at(fx, k + J * N, l) + Dm[J * N + k] * T.at(fy, I + k * N, l);
   return s;
 }
 // acc + sum_k (D[I,k] fx(k,J) + D[J,k] fy(I,k)), accumulated term by term
-template <int N>
-__device__ __forceinline__ double dxy_acc(double s, const double *Dm, const Tiles<N> &T, int fx, int fy, int I,
+template <int N, int NZ>
+__device__ __forceinline__ double dxy_acc(double s, const double *Dm, const Tiles<N, NZ> &T, int fx, int fy, int I,
                                           int J, int l) {
 #pragma unroll
   for (int k = 0; k < N; ++k) s += Dm[I * N + k] * T.at(fx, k + J * N, l) + Dm[J * N + k] * T.at(fy, I + k * N, l);
   return s;
 }
 // sum_k (D[I,k] fx(k,J) cx(k,J) + D[J,k] fy(I,k) cy(I,k))
-template <int N>
-__device__ __forceinline__ double dxy_prod(const double *Dm, const Tiles<N> &T, int fx, int fy, int fc, int I, int J,
+template <int N, int NZ>
+__device__ __forceinline__ double dxy_prod(const double *Dm, const Tiles<N, NZ> &T, int fx, int fy, int fc, int I, int J,
                                            int l) {
   double s = Dm[I * N] * T.at(fx, J * N, l) * T.at(fc, J * N, l) + Dm[J * N] * T.at(fy, I, l) * T.at(fc, I, l);
 #pragma unroll
@@ -120,6 +122,37 @@ __device__ __forceinline__ double dxy_prod(const double *Dm, const Tiles<N> &T, 
     s += Dm[I * N + k] * T.at(fx, k + J * N, l) * T.at(fc, k + J * N, l) +
          Dm[J * N + k] * T.at(fy, I + k * N, l) * T.at(fc, I + k * N, l);
   return s;
+}
+
+// Value of x held by the lane D positions below / above in the same 16-lane
+// DPP row (row_shr:D / row_shl:D).  Lanes whose source falls outside the
+// row keep their own x; callers replace those with a halo load.
+template <int D> __device__ __forceinline__ double dpp_from_below(double x) {
+  static_assert(D >= 1 && D <= 15, "row shift");
+  const long long b = __double_as_longlong(x);
+  const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+  const int rlo = __builtin_amdgcn_update_dpp(lo, lo, 0x110 + D, 0xf, 0xf, false);
+  const int rhi = __builtin_amdgcn_update_dpp(hi, hi, 0x110 + D, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)rhi << 32) | (unsigned int)rlo);
+}
+template <int D> __device__ __forceinline__ double dpp_from_above(double x) {
+  static_assert(D >= 1 && D <= 15, "row shift");
+  const long long b = __double_as_longlong(x);
+  const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+  const int rlo = __builtin_amdgcn_update_dpp(lo, lo, 0x100 + D, 0xf, 0xf, false);
+  const int rhi = __builtin_amdgcn_update_dpp(hi, hi, 0x100 + D, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)rhi << 32) | (unsigned int)rlo);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+// XCD b % 8).  Renumber blocks so that consecutive logical ids -- the level
+// chunks of one element, then the next element -- land on the same XCD and
+// share its L2 (a chunk's 128-byte lines are also touched by the chunks
+// above and below it).
+__device__ __forceinline__ int xcd_logical(int b, int G) {
+  const int per = G >> 3, rem = G & 7;
+  const int x = b & 7, slot = b >> 3;
+  return x < rem ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
 }
 
 template <int N> __device__ __forceinline__ void load_dmats(const KArgs &a, double *Ds, double *DWs) {
@@ -133,17 +166,19 @@ template <int N> __device__ __forceinline__ void load_dmats(const KArgs &a, doub
 // ------------------------------------------------------------------------
 // Pass 1: HyperViscKernel! (src/GPU/ViscKernel.jl:1-84).
 // Grid: one workgroup per (element of the colour, level chunk).
-template <int N>
-__global__ void __launch_bounds__(1024) k_fcn1(KArgs a, const int *__restrict__ elist, int nchunk,
+template <int N, int NZT>
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, const int *__restrict__ elist, int nchunk,
                                                const double *__restrict__ U, double *__restrict__ T1) {
   constexpr int NN = N * N;
   extern __shared__ double smem[];
   double *Ds = smem, *DWs = smem + NN;
-  Tiles<N> T{smem + 2 * NN, a.nzt};
+  Tiles<N, NZT> T{smem + 2 * NN};
   load_dmats<N>(a, Ds, DWs);
-  const int Nz = a.Nz, nzt = a.nzt;
-  const int e = elist[blockIdx.x / nchunk];
-  const int chunk = blockIdx.x - (blockIdx.x / nchunk) * nchunk;
+  const int Nz = a.Nz;
+  constexpr int nzt = NZT;
+  const int L = xcd_logical(blockIdx.x, gridDim.x);
+  const int e = elist[L / nchunk];
+  const int chunk = L - (L / nchunk) * nchunk;
   const int t = threadIdx.x;
   const int node = t / nzt, l = t - node * nzt;
   const int iz = chunk * nzt + l;
@@ -154,6 +189,7 @@ __global__ void __launch_bounds__(1024) k_fcn1(KArgs a, const int *__restrict__ 
   const size_t NNZ = (size_t)NN * Nz;
 
   double S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1, Ms = 1;
+  bool first = false;
   size_t ui = 0;
   double uC = 0, vC = 0, uD = 0, vD = 0, Th = 0;
   if (valid) {
@@ -166,6 +202,7 @@ __global__ void __launch_bounds__(1024) k_fcn1(KArgs a, const int *__restrict__ 
     S22 = a.met1[mb + 3 * NNZ];
     Js = a.met1[mb + 4 * NNZ];
     Ms = a.Ms[ui];
+    first = a.first[e * NN + node] != 0;
     const double rho = U[ui], u = U[ui + vs], v = U[ui + 2 * vs], rth = U[ui + 4 * vs];
     uC = S11 * v - S12 * u;  // Curl12
     vC = S21 * v - S22 * u;
@@ -183,12 +220,13 @@ __global__ void __launch_bounds__(1024) k_fcn1(KArgs a, const int *__restrict__ 
   __syncthreads();
   double Curl = 0, Div = 0, ThCx = 0, ThCy = 0;
   if (valid) {
-    const double Dxc = dx<N>(Ds, T, 4, I, J, l);
-    const double Dyc = dy<N>(Ds, T, 4, I, J, l);
-    Curl = dxy<N>(Ds, T, 0, 1, I, J, l) / Js;
-    Div = dxy<N>(Ds, T, 2, 3, I, J, l) / Js;
-    const double GDx = (S11 * Dxc + S21 * Dyc) / Js;  // Grad12 with J
-    const double GDy = (S12 * Dxc + S22 * Dyc) / Js;
+    const double Dxc = dx<N, NZT>(Ds, T, 4, I, J, l);
+    const double Dyc = dy<N, NZT>(Ds, T, 4, I, J, l);
+    const double iJs = 1.0 / Js;
+    Curl = dxy<N, NZT>(Ds, T, 0, 1, I, J, l) * iJs;
+    Div = dxy<N, NZT>(Ds, T, 2, 3, I, J, l) * iJs;
+    const double GDx = (S11 * Dxc + S21 * Dyc) * iJs;  // Grad12 with J
+    const double GDy = (S12 * Dxc + S22 * Dyc) * iJs;
     ThCx = S11 * GDx + S12 * GDy;  // Contra12
     ThCy = S21 * GDx + S22 * GDy;
   }
@@ -201,20 +239,32 @@ __global__ void __launch_bounds__(1024) k_fcn1(KArgs a, const int *__restrict__ 
   }
   __syncthreads();
   if (valid) {
-    const double DxCurl = dx<N>(DWs, T, 0, I, J, l);
-    const double DyCurl = dy<N>(DWs, T, 0, I, J, l);
-    const double DxDiv = dx<N>(DWs, T, 1, I, J, l);
-    const double DyDiv = dy<N>(DWs, T, 1, I, J, l);
-    const double DivTh = dxy<N>(DWs, T, 2, 3, I, J, l);
+    const double DxCurl = dx<N, NZT>(DWs, T, 0, I, J, l);
+    const double DyCurl = dy<N, NZT>(DWs, T, 0, I, J, l);
+    const double DxDiv = dx<N, NZT>(DWs, T, 1, I, J, l);
+    const double DyDiv = dy<N, NZT>(DWs, T, 1, I, J, l);
+    const double DivTh = dxy<N, NZT>(DWs, T, 2, 3, I, J, l);
     const double FuC = S12 * DxCurl + S22 * DyCurl;  // Rot12
     const double FvC = -S11 * DxCurl - S21 * DyCurl;
     const double FuD = S11 * DxDiv + S21 * DyDiv;  // Grad12
     const double FvD = S12 * DxDiv + S22 * DyDiv;
-    T1[ui] += -FuC / Ms;
-    T1[ui + vs] += -FvC / Ms;
-    T1[ui + 2 * vs] += FuD / Ms;
-    T1[ui + 3 * vs] += FvD / Ms;
-    T1[ui + 4 * vs] += DivTh / Ms;
+    // DSS: the first element (in colour order) holding the node writes,
+    // the others accumulate
+    const double iMs = 1.0 / Ms;
+    const double r1 = -FuC * iMs, r2 = -FvC * iMs, r3 = FuD * iMs, r4 = FvD * iMs, r5 = DivTh * iMs;
+    if (first) {
+      T1[ui] = r1;
+      T1[ui + vs] = r2;
+      T1[ui + 2 * vs] = r3;
+      T1[ui + 3 * vs] = r4;
+      T1[ui + 4 * vs] = r5;
+    } else {
+      T1[ui] += r1;
+      T1[ui + vs] += r2;
+      T1[ui + 2 * vs] += r3;
+      T1[ui + 3 * vs] += r4;
+      T1[ui + 4 * vs] += r5;
+    }
   }
 }
 
@@ -223,20 +273,33 @@ __global__ void __launch_bounds__(1024) k_fcn1(KArgs a, const int *__restrict__ 
 // GradFullKernel! + DivRhoTrUpwind3New2Kernel! fused into one element pass
 // (src/GPU/ViscKernel.jl:86-167, src/GPU/MomentumKernel.jl:1-178,
 // src/GPU/GradKineticKernel.jl:1-129, src/GPU/OperatorKernel.jl:637-720).
-// Launched once per (colour, chunk) so that the F4 contribution of a
-// chunk's first level to the interface below it is ordered after the
-// previous chunk of the same element.
-template <int N>
-__global__ void __launch_bounds__(1024) k_fcn2(KArgs a, const int *__restrict__ elist, int chunk,
+// Launched once per colour over (element, level chunk) workgroups; the
+// chunks of one element are independent.
+//
+// Phase 0 computes every term that couples levels (vertical momentum
+// fluxes, kinetic-energy and pressure differences, the upwind mass and
+// theta fluxes through both interfaces of the layer) from the neighbour
+// levels, so those values are dead before the LDS phases start.  Each
+// thread adds the fluxes of both its interfaces to its own level (the
+// reference adds them from the owner of the interface); F4 of interface iz
+// combines this layer's upper terms with the layer above's lower terms.
+template <int N, int NZT>
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, const int *__restrict__ elist, int nchunk,
                                                const double *__restrict__ U, const double *__restrict__ C,
-                                               const double *__restrict__ P, double *__restrict__ F) {
+                                               const double *__restrict__ P, double *__restrict__ F,
+                                               double *__restrict__ F4x) {
   constexpr int NN = N * N;
+  static_assert(NZT == 8 || NZT == 16, "a chunk must tile the 16-lane DPP rows");
   extern __shared__ double smem[];
   double *Ds = smem, *DWs = smem + NN;
-  Tiles<N> T{smem + 2 * NN, a.nzt};
+  Tiles<N, NZT> T{smem + 2 * NN};
   load_dmats<N>(a, Ds, DWs);
-  const int Nz = a.Nz, nzt = a.nzt;
-  const int e = elist[blockIdx.x];
+  const int Nz = a.Nz;
+  constexpr int nzt = NZT;
+  const int L = xcd_logical(blockIdx.x, gridDim.x);
+  const int eslot = L / nchunk;
+  const int chunk = L - eslot * nchunk;
+  const int e = elist[eslot];
   const int t = threadIdx.x;
   const int node = t / nzt, l = t - node * nzt;
   const int iz = chunk * nzt + l;
@@ -246,92 +309,184 @@ __global__ void __launch_bounds__(1024) k_fcn2(KArgs a, const int *__restrict__ 
   const size_t vs = (size_t)a.NumG * Nz;
   const size_t NNZ = (size_t)NN * Nz;
   const bool has_lo = iz > 0, has_hi = iz < Nz - 1;
+  const double Grav = a.Grav;
 
-  // ---- loads ----
-  double rho = 1, u = 0, v = 0, w = 0, rth = 1, p = 0;
-  double c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
-  double d[CGD_NMET];
-#pragma unroll
-  for (int c = 0; c < CGD_NMET; ++c) d[c] = 1.0;
-  double Ms = 1, M1 = 1, M2 = 1;
-  // level iz-1
-  double rho_m = 1, u_m = 0, v_m = 0, w_m = 0, rth_m = 1, M2_m = 1, d312_m = 0, d322_m = 0, d332_m = 1, Jc_m = 1;
-  // level iz-2 (clamped)
-  double rho_mm = 1, rth_mm = 1, Jc_mm = 1;
-  // level iz+1
-  double rho_p = 1, u_p = 0, v_p = 0, rth_p = 1, p_p = 0, M1_p = 1, J1_p = 1, Jc_p = 1;
-  double d311_p = 0, d321_p = 0, d331_p = 1;
-  // level iz+2 (clamped)
-  double rho_pp = 1, rth_pp = 1, Jc_pp = 1;
-  size_t ui = 0;
+  size_t ui = 0, mb = 0;
   if (valid) {
-    const int g = a.glob[e * NN + node];
-    ui = (size_t)g * Nz + iz;
-    const size_t mb = ((size_t)e * CGD_NMET * NN + node) * Nz + iz;
+    ui = (size_t)a.glob[e * NN + node] * Nz + iz;
+    mb = ((size_t)e * CGD_NMET * NN + node) * Nz + iz;
+  }
+#define MET(c, off) a.met[mb + (off) + (size_t)(c) * NNZ]
+  // own level: state and metric (used in every phase)
+  double rho = 1, u = 0, v = 0, w = 0, rth = 1, Ms = 1, M1 = 1, M2 = 1, p = 0;
+  double d111 = 0, d112 = 0, d121 = 0, d122 = 0, d211 = 0, d212 = 0, d221 = 0, d222 = 0;
+  double d311 = 0, d312 = 0, d321 = 0, d322 = 0, d331 = 1, d332 = 1, J1 = 1, J2 = 1, Wc = 0;
+  if (valid) {
     rho = U[ui];
     u = U[ui + vs];
     v = U[ui + 2 * vs];
     w = U[ui + 3 * vs];
     rth = U[ui + 4 * vs];
-    c1 = C[ui];
-    c2 = C[ui + vs];
-    c3 = C[ui + 2 * vs];
-    c4 = C[ui + 3 * vs];
-    c5 = C[ui + 4 * vs];
-    p = P[ui];
-#pragma unroll
-    for (int c = 0; c < CGD_NMET; ++c) d[c] = a.met[mb + c * NNZ];
     Ms = a.Ms[ui];
     M1 = a.M[ui];
     M2 = a.M[ui + vs];
-    if (has_lo) {
-      const size_t um = ui - 1, mm = mb - 1;
+    p = P[ui];
+    d111 = MET(M111, 0); d112 = MET(M112, 0); d121 = MET(M121, 0); d122 = MET(M122, 0);
+    d211 = MET(M211, 0); d212 = MET(M212, 0); d221 = MET(M221, 0); d222 = MET(M222, 0);
+    d311 = MET(M311, 0); d312 = MET(M312, 0); d321 = MET(M321, 0); d322 = MET(M322, 0);
+    d331 = MET(M331, 0); d332 = MET(M332, 0); J1 = MET(MJ1, 0); J2 = MET(MJ2, 0); Wc = MET(MCOR, 0);
+  }
+  const double Js = J1 + J2;
+  const double c_own = rth / rho;
+  const double iMs = 1.0 / Ms;
+  const bool first = valid && a.first[e * NN + node] != 0;
+
+  // ---- vertical neighbours ----
+  // A node's chunk is one 16-lane DPP row (or half of one for 8-level
+  // chunks), so the values of levels iz-1, iz-2, iz+1, iz+2 come from the
+  // neighbouring lanes; only lanes at a chunk edge load a halo level.
+  const int lr = threadIdx.x & 15;  // position in the DPP row
+  const bool edge_m1 = l < 1 || lr < 1, edge_m2 = l < 2 || lr < 2;
+  const bool edge_p1 = l + 1 >= nzt || lr + 1 > 15, edge_p2 = l + 2 >= nzt || lr + 2 > 15;
+  // level iz-1
+  double rho_m = dpp_from_below<1>(rho), u_m = dpp_from_below<1>(u), v_m = dpp_from_below<1>(v);
+  double w_m = dpp_from_below<1>(w), c_m = dpp_from_below<1>(c_own), Jc_m = dpp_from_below<1>(Js);
+  double M2_m = dpp_from_below<1>(M2), d312m = dpp_from_below<1>(d312), d322m = dpp_from_below<1>(d322);
+  double d332m = dpp_from_below<1>(d332);
+  // level iz-2 (clamped at the bottom)
+  double c_mm = dpp_from_below<2>(c_own), Jc_mm = dpp_from_below<2>(Js);
+  // level iz+1
+  double rho_p = dpp_from_above<1>(rho), u_p = dpp_from_above<1>(u), v_p = dpp_from_above<1>(v);
+  double c_p = dpp_from_above<1>(c_own), p_p = dpp_from_above<1>(p), M1_p = dpp_from_above<1>(M1);
+  double J1_p = dpp_from_above<1>(J1), Jc_p = dpp_from_above<1>(Js), d311p = dpp_from_above<1>(d311);
+  double d321p = dpp_from_above<1>(d321), d331p = dpp_from_above<1>(d331);
+  // level iz+2 (clamped at the top)
+  double c_pp = dpp_from_above<2>(c_own), Jc_pp = dpp_from_above<2>(Js);
+  if (valid) {
+    if (has_lo && edge_m1) {
+      const size_t um = ui - 1;
       rho_m = U[um];
       u_m = U[um + vs];
       v_m = U[um + 2 * vs];
       w_m = U[um + 3 * vs];
-      rth_m = U[um + 4 * vs];
+      c_m = U[um + 4 * vs] / rho_m;
+      Jc_m = MET(MJ1, -1) + MET(MJ2, -1);
       M2_m = a.M[um + vs];
-      d312_m = a.met[mm + M312 * NNZ];
-      d322_m = a.met[mm + M322 * NNZ];
-      d332_m = a.met[mm + M332 * NNZ];
-      Jc_m = a.met[mm + MJ1 * NNZ] + a.met[mm + MJ2 * NNZ];
-      const int dm2 = iz >= 2 ? 2 : 1;
-      rho_mm = U[ui - dm2];
-      rth_mm = U[ui - dm2 + 4 * vs];
-      Jc_mm = a.met[mb - dm2 + MJ1 * NNZ] + a.met[mb - dm2 + MJ2 * NNZ];
+      d312m = MET(M312, -1);
+      d322m = MET(M322, -1);
+      d332m = MET(M332, -1);
     }
-    if (has_hi) {
-      const size_t up = ui + 1, mp = mb + 1;
+    if (iz >= 2 && edge_m2) {
+      c_mm = U[ui - 2 + 4 * vs] / U[ui - 2];
+      Jc_mm = MET(MJ1, -2) + MET(MJ2, -2);
+    }
+    if (iz == 1) {
+      c_mm = c_m;
+      Jc_mm = Jc_m;
+    }
+    if (has_hi && edge_p1) {
+      const size_t up = ui + 1;
       rho_p = U[up];
       u_p = U[up + vs];
       v_p = U[up + 2 * vs];
-      rth_p = U[up + 4 * vs];
+      c_p = U[up + 4 * vs] / rho_p;
       p_p = P[up];
       M1_p = a.M[up];
-      J1_p = a.met[mp + MJ1 * NNZ];
-      Jc_p = J1_p + a.met[mp + MJ2 * NNZ];
-      d311_p = a.met[mp + M311 * NNZ];
-      d321_p = a.met[mp + M321 * NNZ];
-      d331_p = a.met[mp + M331 * NNZ];
-      const int dp2 = iz + 2 <= Nz - 1 ? 2 : 1;
-      rho_pp = U[ui + dp2];
-      rth_pp = U[ui + dp2 + 4 * vs];
-      Jc_pp = a.met[mb + dp2 + MJ1 * NNZ] + a.met[mb + dp2 + MJ2 * NNZ];
+      J1_p = MET(MJ1, 1);
+      Jc_p = J1_p + MET(MJ2, 1);
+      d311p = MET(M311, 1);
+      d321p = MET(M321, 1);
+      d331p = MET(M331, 1);
+    }
+    if (iz + 2 <= Nz - 1 && edge_p2) {
+      c_pp = U[ui + 2 + 4 * vs] / U[ui + 2];
+      Jc_pp = MET(MJ1, 2) + MET(MJ2, 2);
+    }
+    if (iz + 2 > Nz - 1) {
+      c_pp = c_p;
+      Jc_pp = Jc_p;
     }
   }
-  const double S11 = d[M111] + d[M112], S12 = d[M121] + d[M122];
-  const double S21 = d[M211] + d[M212], S22 = d[M221] + d[M222];
-  const double Js = d[MJ1] + d[MJ2];
-  // free-slip bottom (MomentumKernel.jl:35-36)
-  const double w_lo = has_lo ? w_m : -(d[M311] * u + d[M321] * v) / d[M331];
-  const double w_hi = w;
+
+  // ---- phase 0: level-coupling terms ----
+  double w_lo = 0;
+  double FU = 0, FV = 0, FW1 = 0, FW2 = 0;  // vertical vorticity-flux parts of MomentumKernel
+  double GZlo = 0, GZup = 0;                // 1/2 (KinF(iz) - KinF(iz-1)) at the two interfaces
+  double Gpup = 0;                          // -(dp/dz + g rho J) at the upper interface
+  double den_lo = 1, den_up = 1;            // rho-weighted interface masses
+  double f1 = 0, f5 = 0;
+  double K1 = 0, K2 = 0;
+  double U1 = 0, V1 = 0, W1 = 0;  // 1/2 (tempXZ[2] - tempXZ[1]) of this layer
+  if (valid) {
+    const double Kh = 0.5 * (u * u + v * v);
+    if (has_lo) {
+      w_lo = w_m;
+      K1 = Kh + 0.5 * (w_lo * w_lo);
+      K2 = Kh + 0.5 * (w * w);
+      den_lo = rho_m * M2_m + rho * M1;
+      // vorticity fluxes through the lower interface (MomentumKernel.jl:118-125)
+      const double FluxUZ = 0.5 * ((d331 * v - d321 * w_lo) - (d332m * v_m - d322m * w_lo));
+      const double FluxVZ = 0.5 * ((d331 * u - d311 * w_lo) - (d332m * u_m - d312m * w_lo));
+      const double FluxWZ = 0.5 * ((d321 * u - d311 * v) - (d322m * u_m - d312m * v_m));
+      FU += (-v * FluxWZ - w_lo * FluxVZ);
+      FV += (u * FluxWZ - w_lo * FluxUZ);
+      FW1 += (u * FluxVZ + v * FluxUZ);
+      const double K2m = 0.5 * (u_m * u_m + v_m * v_m) + 0.5 * (w_m * w_m);
+      GZlo = 0.5 * (K1 - K2m);
+      // upwind theta / mass flux through the lower interface (owner: layer iz-1)
+      const double wCon = rho_m * (d312m * u_m + d322m * v_m + d332m * w_m) + rho * (d311 * u + d321 * v + d331 * w_m);
+      const double cRR = has_hi ? c_p : c_own, JRR = has_hi ? Jc_p : Js;
+      double cFL, cFR;
+      recu4(c_mm, c_m, c_own, cRR, Jc_mm, Jc_m, Js, JRR, cFL, cFR);
+      const double Flux = 0.25 * ((fabs(wCon) + wCon) * cFL + (-fabs(wCon) + wCon) * cFR);
+      f5 += Flux * iMs;
+      f1 += (0.5 * wCon) * iMs;
+    } else {
+      // free-slip bottom (MomentumKernel.jl:35-36)
+      w_lo = -(d311 * u + d321 * v) / d331;
+      K1 = Kh + 0.5 * (w_lo * w_lo);
+      K2 = Kh + 0.5 * (w * w);
+    }
+    if (has_hi) {
+      den_up = rho_p * M1_p + rho * M2;
+      // vorticity fluxes through the upper interface (MomentumKernel.jl:126-133)
+      const double FluxUZ = 0.5 * ((d331p * v_p - d321p * w) - (d332 * v - d322 * w));
+      const double FluxVZ = 0.5 * ((d331p * u_p - d311p * w) - (d332 * u - d312 * w));
+      const double FluxWZ = 0.5 * ((d321p * u_p - d311p * v_p) - (d322 * u - d312 * v));
+      FU += (-v * FluxWZ - w * FluxVZ);
+      FV += (u * FluxWZ - w * FluxUZ);
+      FW2 += (u * FluxVZ + v * FluxUZ);
+      const double K1p = 0.5 * (u_p * u_p + v_p * v_p) + 0.5 * (w * w);
+      GZup = 0.5 * (K1p - K2);
+      // pressure gradient and gravity at the interface (GradKineticKernel.jl:111-121)
+      const double GradZ = 0.5 * (p_p - p);
+      const double Gradw = GradZ * (d332 + d331p);
+      Gpup = -(Gradw + Grav * (rho * J2 + rho_p * J1_p));
+      // upwind theta / mass flux through the upper interface (owner: this layer)
+      const double wCon = rho * (d312 * u + d322 * v + d332 * w) + rho_p * (d311p * u_p + d321p * v_p + d331p * w);
+      const double cLL = has_lo ? c_m : c_own, JLL = has_lo ? Jc_m : Js;
+      double cFL, cFR;
+      recu4(cLL, c_own, c_p, c_pp, JLL, Js, Jc_p, Jc_pp, cFL, cFR);
+      const double Flux = 0.25 * ((fabs(wCon) + wCon) * cFL + (-fabs(wCon) + wCon) * cFR);
+      f5 += -Flux * iMs;
+      f1 += -(0.5 * wCon) * iMs;
+    }
+    U1 = 0.5 * ((d332 * v - d322 * w) - (d331 * v - d321 * w_lo));
+    V1 = 0.5 * ((d332 * u - d312 * w) - (d331 * u - d311 * w_lo));
+    W1 = 0.5 * ((d322 * u - d312 * v) - (d321 * u - d311 * v));
+  }
+  const double S11 = d111 + d112, S12 = d121 + d122, S21 = d211 + d212, S22 = d221 + d222;
 
   // ---- phase 1: hyperviscosity inputs, kinetic energy, pressure ----
-  const double Kh = 0.5 * (u * u + v * v);
-  const double K1 = Kh + 0.5 * (w_lo * w_lo);
-  const double K2 = Kh + 0.5 * (w_hi * w_hi);
   if (in_tile) {
+    double c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+    if (valid) {
+      c1 = C[ui];
+      c2 = C[ui + vs];
+      c3 = C[ui + 2 * vs];
+      c4 = C[ui + 3 * vs];
+      c5 = C[ui + 4 * vs];
+    }
     T.at(0, node, l) = S11 * c2 - S12 * c1;  // Curl12(Cache1, Cache2)
     T.at(1, node, l) = S21 * c2 - S22 * c1;
     T.at(2, node, l) = S11 * c3 + S12 * c4;  // Contra12(Cache3, Cache4)
@@ -344,65 +499,57 @@ __global__ void __launch_bounds__(1024) k_fcn2(KArgs a, const int *__restrict__ 
   __syncthreads();
   double Curl = 0, Div = 0, ThCx = 0, ThCy = 0;
   double f2 = 0, f3 = 0, f4up = 0, f4lo = 0;
-  const double den_up = rho_p * M1_p + rho * M2;   // F4[iz] mass
-  const double den_lo = rho_m * M2_m + rho * M1;   // F4[iz-1] mass
   if (valid) {
-    const double Dxc = dx<N>(Ds, T, 4, I, J, l);
-    const double Dyc = dy<N>(Ds, T, 4, I, J, l);
-    Curl = dxy<N>(Ds, T, 0, 1, I, J, l) / Js;
-    Div = dxy<N>(Ds, T, 2, 3, I, J, l) / Js;
-    const double GDx = (S11 * Dxc + S21 * Dyc) / Js;
-    const double GDy = (S12 * Dxc + S22 * Dyc) / Js;
-    ThCx = rho * (S11 * GDx + S12 * GDy);  // Contra12(U1, ...)
-    ThCy = rho * (S21 * GDx + S22 * GDy);
+    {
+      const double Dxc = dx<N, NZT>(Ds, T, 4, I, J, l);
+      const double Dyc = dy<N, NZT>(Ds, T, 4, I, J, l);
+      const double iJs = 1.0 / Js;
+      Curl = dxy<N, NZT>(Ds, T, 0, 1, I, J, l) * iJs;
+      Div = dxy<N, NZT>(Ds, T, 2, 3, I, J, l) * iJs;
+      const double GDx = (S11 * Dxc + S21 * Dyc) * iJs;
+      const double GDy = (S12 * Dxc + S22 * Dyc) * iJs;
+      ThCx = rho * (S11 * GDx + S12 * GDy);  // Contra12(U1, ...)
+      ThCy = rho * (S21 * GDx + S22 * GDy);
+    }
     // GradFullKernel!
-    const double DXK1 = dx<N>(Ds, T, 5, I, J, l), DYK1 = dy<N>(Ds, T, 5, I, J, l);
-    const double DXK2 = dx<N>(Ds, T, 6, I, J, l), DYK2 = dy<N>(Ds, T, 6, I, J, l);
-    const double DXp = dx<N>(Ds, T, 7, I, J, l), DYp = dy<N>(Ds, T, 7, I, J, l);
-    double GuF1 = -(d[M111] * DXK1 + d[M211] * DYK1);
-    double GvF1 = -(d[M121] * DXK1 + d[M221] * DYK1);
-    double GuF2 = -(d[M112] * DXK2 + d[M212] * DYK2);
-    double GvF2 = -(d[M122] * DXK2 + d[M222] * DYK2);
+    const double DXK1 = dx<N, NZT>(Ds, T, 5, I, J, l), DYK1 = dy<N, NZT>(Ds, T, 5, I, J, l);
+    const double DXK2 = dx<N, NZT>(Ds, T, 6, I, J, l), DYK2 = dy<N, NZT>(Ds, T, 6, I, J, l);
+    double GuF1 = -(d111 * DXK1 + d211 * DYK1);
+    double GvF1 = -(d121 * DXK1 + d221 * DYK1);
+    double GuF2 = -(d112 * DXK2 + d212 * DYK2);
+    double GvF2 = -(d122 * DXK2 + d222 * DYK2);
+    double GwF1 = 0, GwF2 = 0;
+    if (has_lo) {
+      GuF1 += -GZlo * d311;
+      GvF1 += -GZlo * d321;
+      GwF1 += -rho * GZlo * d331;
+    }
+    if (has_hi) {
+      GuF2 += -GZup * d312;
+      GvF2 += -GZup * d322;
+      GwF2 += -rho * GZup * d332;
+    }
+    const double GradZ = 0.5 * (K2 - K1);
+    GuF2 += -GradZ * d312;
+    GuF1 += -GradZ * d311;
+    GvF2 += -GradZ * d322;
+    GvF1 += -GradZ * d321;
+    GwF2 += -rho * GradZ * d332;
+    GwF1 += -rho * GradZ * d331;
+    const double DXp = dx<N, NZT>(Ds, T, 7, I, J, l), DYp = dy<N, NZT>(Ds, T, 7, I, J, l);
     double Gu = S11 * DXp + S21 * DYp;
     double Gv = S12 * DXp + S22 * DYp;
-    double GwF1 = 0, GwF2 = 0, GradZ;
-    if (has_lo) {
-      const double K2_m = 0.5 * (u_m * u_m + v_m * v_m) + 0.5 * (w_m * w_m);
-      GradZ = 0.5 * (K1 - K2_m);
-      GuF1 += -GradZ * d[M311];
-      GvF1 += -GradZ * d[M321];
-      GwF1 += -rho * GradZ * d[M331];
-    }
-    if (has_hi) {
-      const double K1_p = 0.5 * (u_p * u_p + v_p * v_p) + 0.5 * (w * w);
-      GradZ = 0.5 * (K1_p - K2);
-      GuF2 += -GradZ * d[M312];
-      GvF2 += -GradZ * d[M322];
-      GwF2 += -rho * GradZ * d[M332];
-    }
-    GradZ = 0.5 * (K2 - K1);
-    GuF2 += -GradZ * d[M312];
-    GuF1 += -GradZ * d[M311];
-    GvF2 += -GradZ * d[M322];
-    GvF1 += -GradZ * d[M321];
-    GwF2 += -rho * GradZ * d[M332];
-    GwF1 += -rho * GradZ * d[M331];
-    GradZ = -a.Grav * rho * Js / (d[M331] + d[M332]);
-    Gu += GradZ * (d[M311] + d[M312]);
-    Gv += GradZ * (d[M321] + d[M322]);
-    f2 += (GuF1 + GuF2 - Gu / rho) / Ms;
-    f3 += (GvF1 + GvF2 - Gv / rho) / Ms;
-    if (has_hi) {
-      GradZ = 0.5 * (p_p - p);
-      const double Gradw = GradZ * (d[M332] + d331_p);
-      const double Gradp = -(Gradw + a.Grav * (rho * d[MJ2] + rho_p * J1_p));
-      f4up += (GwF2 + Gradp) / den_up;
-    }
+    const double GZg = -Grav * rho * Js / (d331 + d332);
+    Gu += GZg * (d311 + d312);
+    Gv += GZg * (d321 + d322);
+    const double irho = 1.0 / rho;
+    f2 += (GuF1 + GuF2 - Gu * irho) * iMs;
+    f3 += (GvF1 + GvF2 - Gv * irho) * iMs;
+    if (has_hi) f4up += (GwF2 + Gpup) / den_up;
     if (has_lo) f4lo += GwF1 / den_lo;
   }
   __syncthreads();
   // ---- phase 2: second hyperviscosity apply, horizontal mass / theta flux ----
-  const double c_own = rth / rho;
   if (in_tile) {
     T.at(0, node, l) = Curl;
     T.at(1, node, l) = Div;
@@ -413,48 +560,26 @@ __global__ void __launch_bounds__(1024) k_fcn2(KArgs a, const int *__restrict__ 
     T.at(6, node, l) = c_own;
   }
   __syncthreads();
-  double f1 = 0, f5 = 0;
   if (valid) {
-    const double DxCurl = dx<N>(DWs, T, 0, I, J, l);
-    const double DyCurl = dy<N>(DWs, T, 0, I, J, l);
-    const double DxDiv = dx<N>(DWs, T, 1, I, J, l);
-    const double DyDiv = dy<N>(DWs, T, 1, I, J, l);
-    const double DivTh = dxy<N>(DWs, T, 2, 3, I, J, l);
-    const double FuC = S12 * DxCurl + S22 * DyCurl;
-    const double FvC = -S11 * DxCurl - S21 * DyCurl;
-    const double FuD = S11 * DxDiv + S21 * DyDiv;
-    const double FvD = S12 * DxDiv + S22 * DyDiv;
-    f2 += -(-a.Kc * FuC + a.Kg * FuD) / Ms;
-    f3 += -(-a.Kc * FvC + a.Kg * FvD) / Ms;
-    f5 += -a.Kd * DivTh / Ms;
+    {
+      const double DxCurl = dx<N, NZT>(DWs, T, 0, I, J, l);
+      const double DyCurl = dy<N, NZT>(DWs, T, 0, I, J, l);
+      const double FuC = S12 * DxCurl + S22 * DyCurl;
+      const double FvC = -S11 * DxCurl - S21 * DyCurl;
+      const double DxDiv = dx<N, NZT>(DWs, T, 1, I, J, l);
+      const double DyDiv = dy<N, NZT>(DWs, T, 1, I, J, l);
+      const double FuD = S11 * DxDiv + S21 * DyDiv;
+      const double FvD = S12 * DxDiv + S22 * DyDiv;
+      f2 += -(-a.Kc * FuC + a.Kg * FuD) * iMs;
+      f3 += -(-a.Kc * FvC + a.Kg * FvD) * iMs;
+    }
+    const double DivTh = dxy<N, NZT>(DWs, T, 2, 3, I, J, l);
+    f5 += -a.Kd * DivTh * iMs;
     // DivRhoTrUpwind3New2Kernel!: horizontal part
-    const double DivRhoTr = dxy_prod<N>(Ds, T, 4, 5, 6, I, J, l);
-    const double DivRho = dxy<N>(Ds, T, 4, 5, I, J, l);
-    f5 += DivRhoTr / Ms;
-    f1 += DivRho / Ms;
-    // vertical part: upper interface iz and lower interface iz-1 (the
-    // reference adds both from the owner of the interface)
-    const double cm = rth_m / rho_m, cp = rth_p / rho_p;
-    if (has_hi) {
-      const double wCon = rho * (d[M312] * u + d[M322] * v + d[M332] * w) +
-                          rho_p * (d311_p * u_p + d321_p * v_p + d331_p * w);
-      const double cLL = has_lo ? cm : c_own, JLL = has_lo ? Jc_m : Js;
-      double cFL, cFR;
-      recu4(cLL, c_own, cp, rth_pp / rho_pp, JLL, Js, Jc_p, Jc_pp, cFL, cFR);
-      const double Flux = 0.25 * ((fabs(wCon) + wCon) * cFL + (-fabs(wCon) + wCon) * cFR);
-      f5 += -Flux / Ms;
-      f1 += -(0.5 * wCon) / Ms;
-    }
-    if (has_lo) {
-      const double wCon = rho_m * (d312_m * u_m + d322_m * v_m + d332_m * w_m) +
-                          rho * (d[M311] * u + d[M321] * v + d[M331] * w_m);
-      double cFL, cFR;
-      const double cRR = has_hi ? cp : c_own, JRR = has_hi ? Jc_p : Js;
-      recu4(rth_mm / rho_mm, cm, c_own, cRR, Jc_mm, Jc_m, Js, JRR, cFL, cFR);
-      const double Flux = 0.25 * ((fabs(wCon) + wCon) * cFL + (-fabs(wCon) + wCon) * cFR);
-      f5 += Flux / Ms;
-      f1 += (0.5 * wCon) / Ms;
-    }
+    const double DivRhoTr = dxy_prod<N, NZT>(Ds, T, 4, 5, 6, I, J, l);
+    const double DivRho = dxy<N, NZT>(Ds, T, 4, 5, I, J, l);
+    f5 += DivRhoTr * iMs;
+    f1 += DivRho * iMs;
   }
   __syncthreads();
   // ---- phase 3: vector-invariant momentum + Coriolis ----
@@ -462,64 +587,40 @@ __global__ void __launch_bounds__(1024) k_fcn2(KArgs a, const int *__restrict__ 
   // and likewise V, W for the bottom (K=1) and top (K=2) vertical node; the
   // horizontal sums accumulate onto the vertical term in the reference's
   // order, six LDS tiles per round.
-  const double tuZ1 = d[M331] * v - d[M321] * w_lo, tuZ2 = d[M332] * v - d[M322] * w_hi;
-  const double tvZ1 = d[M331] * u - d[M311] * w_lo, tvZ2 = d[M332] * u - d[M312] * w_hi;
-  const double twZ1 = d[M321] * u - d[M311] * v, twZ2 = d[M322] * u - d[M312] * v;
-  double U1 = 0.5 * (tuZ2 - tuZ1), U2 = U1;
-  double V1 = 0.5 * (tvZ2 - tvZ1), V2 = V1;
-  double W1 = 0.5 * (twZ2 - twZ1), W2 = W1;
+  double U2 = U1, V2 = V1, W2 = W1;
   if (in_tile) {
-    T.at(0, node, l) = -d[M121] * w_lo;  // tempuZ1, K=1
-    T.at(1, node, l) = -d[M221] * w_lo;  // tempuZ2
-    T.at(2, node, l) = -d[M111] * w_lo;  // tempvZ1
-    T.at(3, node, l) = -d[M211] * w_lo;  // tempvZ2
-    T.at(4, node, l) = d[M121] * u - d[M111] * v;  // tempwZ1
-    T.at(5, node, l) = d[M221] * u - d[M211] * v;  // tempwZ2
+    T.at(0, node, l) = -d121 * w_lo;  // tempuZ1, K=1
+    T.at(1, node, l) = -d221 * w_lo;  // tempuZ2
+    T.at(2, node, l) = -d111 * w_lo;  // tempvZ1
+    T.at(3, node, l) = -d211 * w_lo;  // tempvZ2
+    T.at(4, node, l) = d121 * u - d111 * v;  // tempwZ1
+    T.at(5, node, l) = d221 * u - d211 * v;  // tempwZ2
   }
   __syncthreads();
   if (valid) {
-    U1 = dxy_acc<N>(U1, Ds, T, 0, 1, I, J, l);
-    V1 = dxy_acc<N>(V1, Ds, T, 2, 3, I, J, l);
-    W1 = dxy_acc<N>(W1, Ds, T, 4, 5, I, J, l);
+    U1 = dxy_acc<N, NZT>(U1, Ds, T, 0, 1, I, J, l);
+    V1 = dxy_acc<N, NZT>(V1, Ds, T, 2, 3, I, J, l);
+    W1 = dxy_acc<N, NZT>(W1, Ds, T, 4, 5, I, J, l);
   }
   __syncthreads();
   if (in_tile) {
-    T.at(0, node, l) = -d[M122] * w_hi;  // K=2
-    T.at(1, node, l) = -d[M222] * w_hi;
-    T.at(2, node, l) = -d[M112] * w_hi;
-    T.at(3, node, l) = -d[M212] * w_hi;
-    T.at(4, node, l) = d[M122] * u - d[M112] * v;
-    T.at(5, node, l) = d[M222] * u - d[M212] * v;
+    T.at(0, node, l) = -d122 * w;  // K=2
+    T.at(1, node, l) = -d222 * w;
+    T.at(2, node, l) = -d112 * w;
+    T.at(3, node, l) = -d212 * w;
+    T.at(4, node, l) = d122 * u - d112 * v;
+    T.at(5, node, l) = d222 * u - d212 * v;
   }
   __syncthreads();
   if (valid) {
-    U2 = dxy_acc<N>(U2, Ds, T, 0, 1, I, J, l);
-    V2 = dxy_acc<N>(V2, Ds, T, 2, 3, I, J, l);
-    W2 = dxy_acc<N>(W2, Ds, T, 4, 5, I, J, l);
-    double FU = 0, FV = 0, FW1 = 0, FW2 = 0;
-    if (has_lo) {
-      const double tuZ2m = d332_m * v_m - d322_m * w_lo;
-      const double tvZ2m = d332_m * u_m - d312_m * w_lo;
-      const double twZ2m = d322_m * u_m - d312_m * v_m;
-      const double FluxUZ = 0.5 * (tuZ1 - tuZ2m), FluxVZ = 0.5 * (tvZ1 - tvZ2m), FluxWZ = 0.5 * (twZ1 - twZ2m);
-      FU += (-v * FluxWZ - w_lo * FluxVZ);
-      FV += (u * FluxWZ - w_lo * FluxUZ);
-      FW1 += (u * FluxVZ + v * FluxUZ);
-    }
-    if (has_hi) {
-      const double tuZ1p = d331_p * v_p - d321_p * w_hi;
-      const double tvZ1p = d331_p * u_p - d311_p * w_hi;
-      const double twZ1p = d321_p * u_p - d311_p * v_p;
-      const double FluxUZ = 0.5 * (tuZ1p - tuZ2), FluxVZ = 0.5 * (tvZ1p - tvZ2), FluxWZ = 0.5 * (twZ1p - twZ2);
-      FU += (-v * FluxWZ - w_hi * FluxVZ);
-      FV += (u * FluxWZ - w_hi * FluxUZ);
-      FW2 += (u * FluxVZ + v * FluxUZ);
-    }
-    // CoriolisShallow: d[MCOR] = 2 Omega sin(lat) at the layer midpoint
-    FU += (v * d[MCOR]) * Js;
-    FV += (-u * d[MCOR]) * Js;
-    f2 += (-v * W1 - w_lo * V1 - v * W2 - w_hi * V2 + FU) / Ms;
-    f3 += (u * W1 - w_lo * U1 + u * W2 - w_hi * U2 + FV) / Ms;
+    U2 = dxy_acc<N, NZT>(U2, Ds, T, 0, 1, I, J, l);
+    V2 = dxy_acc<N, NZT>(V2, Ds, T, 2, 3, I, J, l);
+    W2 = dxy_acc<N, NZT>(W2, Ds, T, 4, 5, I, J, l);
+    // CoriolisShallow: Wc = 2 Omega sin(lat) at the layer midpoint
+    FU += (v * Wc) * Js;
+    FV += (-u * Wc) * Js;
+    f2 += (-v * W1 - w_lo * V1 - v * W2 - w * V2 + FU) * iMs;
+    f3 += (u * W1 - w_lo * U1 + u * W2 - w * U2 + FV) * iMs;
     if (has_lo) f4lo += rho * (u * V1 + v * U1 + FW1) / den_lo;
     if (has_hi) f4up += rho * (u * V2 + v * U2 + FW2) / den_up;
   }
@@ -529,15 +630,254 @@ __global__ void __launch_bounds__(1024) k_fcn2(KArgs a, const int *__restrict__ 
   __syncthreads();
   if (valid) {
     if (has_hi && l + 1 < nzt) f4up += T.at(0, node, l + 1);
-    F[ui] += f1;
-    F[ui + vs] += f2;
-    F[ui + 2 * vs] += f3;
-    if (has_hi) F[ui + 3 * vs] += f4up;
-    F[ui + 4 * vs] += f5;
+    // DSS: the first element (in colour order) holding the node writes, the
+    // others accumulate; the w lid (iz = Nz-1) receives no tendency
+    if (first) {
+      F[ui] = f1;
+      F[ui + vs] = f2;
+      F[ui + 2 * vs] = f3;
+      F[ui + 3 * vs] = f4up;
+      F[ui + 4 * vs] = f5;
+    } else {
+      F[ui] += f1;
+      F[ui + vs] += f2;
+      F[ui + 2 * vs] += f3;
+      if (has_hi) F[ui + 3 * vs] += f4up;
+      F[ui + 4 * vs] += f5;
+    }
     // first level of a chunk above the first: its contribution to the
-    // interface below belongs to the previous chunk's last level
-    if (l == 0 && has_lo) F[ui - 1 + 3 * vs] += f4lo;
+    // interface below (the previous chunk's last level) goes to F4x and is
+    // added by k_f4fix after all colours
+    if (l == 0 && has_lo) F4x[((size_t)e * NN + node) * nchunk + chunk] = f4lo;
   }
+#undef MET
+}
+
+// F4 of the interfaces between level chunks: the lower-interface term of a
+// chunk's first level, summed over the elements sharing the node in the
+// fixed order of nodecopy (colour order), added to F after all colours.
+__global__ void k_f4fix(const int *__restrict__ nptr, const int *__restrict__ ncopy, const double *__restrict__ F4x,
+                        double *__restrict__ F, int NumG, int Nz, int nchunk, int nzt) {
+  const size_t total = (size_t)NumG * (nchunk - 1);
+  const size_t vs = (size_t)NumG * Nz;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+    const int g = (int)(q / (nchunk - 1));
+    const int ch = 1 + (int)(q - (size_t)g * (nchunk - 1));
+    const int iz = ch * nzt;
+    if (iz >= Nz) continue;
+    double s = 0.0;
+    for (int c = nptr[g]; c < nptr[g + 1]; ++c) s += F4x[(size_t)ncopy[c] * nchunk + ch];
+    F[3 * vs + (size_t)g * Nz + iz - 1] += s;
+  }
+}
+
+__device__ __forceinline__ double readlane_d(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// SchurSolveGPU! for Nz <= 64: one wavefront per column, lane = level, so
+// every load and store is a coalesced 8*Nz-byte run.  The tri build
+// (TriDiagKernel!), the w right-hand side (SchurSolveFKernel!) and the
+// back-substitution of rho, rho*theta (SchurSolveBKernel!) are pointwise
+// with neighbour lanes reached by shuffles.  The tridiagonal system is
+// solved by parallel cyclic reduction (default) or, with thomas != 0, by
+// the Thomas recurrence of triSolve! run wave-serially in the reference's
+// operation order (one useful lane per step, about 10x slower).
+// Rosenbrock stage data for the fused column kernel of cgd_rw_step.
+struct RWStage {
+  const double *kprev[2];  // k_1, k_2 (stage increments already computed)
+  double cf[2];            // c_ij / dt added to the tendency
+  const double *Vn;        // state at the start of the step
+  double *V;               // next stage state (or final state)
+  double av[3];            // V = Vn + sum_j av[j] k_j, j <= stage
+  int stage;               // 0, 1, 2
+};
+
+template <bool FUSED>
+__device__ __forceinline__ void schur_col(const KArgs &a, double *__restrict__ k, double *__restrict__ v,
+                                          const double *__restrict__ JRhoW, const double *__restrict__ JWRho,
+                                          const double *__restrict__ JWRhoTh, const double *__restrict__ JRhoThW,
+                                          double *__restrict__ tri, double fac, int comp_tri, int thomas,
+                                          const RWStage &rw) {
+  const int Nz = a.Nz, n = Nz - 1;
+  const int i = threadIdx.x & 63;
+  const size_t ig = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ig >= (size_t)a.NumG) return;  // uniform per wavefront
+  const size_t vs = (size_t)a.NumG * Nz;
+  const size_t c0 = ig * Nz;
+  const bool row = i < n, lev = i < Nz;
+  const size_t jb = (ig * (size_t)n + i) * 2, tb = (ig * (size_t)n + i) * 3;
+  double jrw0 = 0, jrw1 = 0, jwr0 = 0, jwr1 = 0, jwt0 = 0, jwt1 = 0, jtw0 = 0, jtw1 = 0;
+  if (row) {
+    jrw0 = JRhoW[jb];
+    jrw1 = JRhoW[jb + 1];
+    jwr0 = JWRho[jb];
+    jwr1 = JWRho[jb + 1];
+    jwt0 = JWRhoTh[jb];
+    jwt1 = JWRhoTh[jb + 1];
+    jtw0 = JRhoThW[jb];
+    jtw1 = JRhoThW[jb + 1];
+  }
+  double t0 = 0, t1 = 1, t2 = 0;  // tri[1], tri[2], tri[3] of row i
+  {
+    const double jwr0m = __shfl_up(jwr0, 1), jwt0m = __shfl_up(jwt0, 1);
+    const double jwr1p = __shfl_down(jwr1, 1), jwt1p = __shfl_down(jwt1, 1);
+    if (comp_tri) {
+      const double invfac = 1.0 / fac, invfac2 = invfac / fac;
+      if (row) {
+        t1 = invfac2 - jwr1 * jrw0 - jwr0 * jrw1 - jwt1 * jtw0 - jwt0 * jtw1;
+        t0 = (i == 0) ? 0.0 : -jwr0m * jrw0 - jwt0m * jtw0;
+        t2 = (i == n - 1) ? 0.0 : -jwr1p * jrw1 - jwt1p * jtw1;
+        tri[tb] = t0;
+        tri[tb + 1] = t1;
+        tri[tb + 2] = t2;
+      }
+    } else if (row) {
+      t0 = tri[tb];
+      t1 = tri[tb + 1];
+      t2 = tri[tb + 2];
+    }
+  }
+  double v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+  if (lev) {
+    v1 = v[c0 + i];
+    v2 = v[c0 + i + vs];
+    v3 = v[c0 + i + 2 * vs];
+    v4 = v[c0 + i + 3 * vs];
+    v5 = v[c0 + i + 4 * vs];
+    if (FUSED) {
+      // fV = fV + (c_ij/dt) k_j (RosenbrockSchur.jl:23-26)
+      for (int j = 0; j < rw.stage; ++j) {
+        const double *kj = rw.kprev[j];
+        v1 = v1 + rw.cf[j] * kj[c0 + i];
+        v2 = v2 + rw.cf[j] * kj[c0 + i + vs];
+        v3 = v3 + rw.cf[j] * kj[c0 + i + 2 * vs];
+        v4 = v4 + rw.cf[j] * kj[c0 + i + 3 * vs];
+        v5 = v5 + rw.cf[j] * kj[c0 + i + 4 * vs];
+      }
+    }
+  }
+  const double v1p = __shfl_down(v1, 1), v5p = __shfl_down(v5, 1);
+  if (row) {
+    v4 = v4 / fac + jwr1 * v1 + jwr0 * v1p + jwt1 * v5 + jwt0 * v5p;
+    if (!FUSED) v[c0 + i + 3 * vs] = v4;
+  }
+  double sol = 0.0;
+  if (thomas) {
+    // triSolve!: forward elimination, wave-serial (row j broadcast by readlane)
+    double x = t1, b = v4;
+    for (int j = 0; j < n - 1; ++j) {
+      const double q = readlane_d(t2, j) / readlane_d(x, j);
+      const double bj = readlane_d(b, j);
+      if (i == j + 1) {
+        x = t1 - q * t0;
+        b = b - q * bj;
+      }
+    }
+    // back substitution
+    if (i == n - 1) sol = b / x;
+    for (int j = n - 2; j >= 0; --j) {
+      const double sj1 = readlane_d(sol, j + 1);
+      const double t0j1 = readlane_d(t0, j + 1);
+      if (i == j) sol = (b - t0j1 * sj1) / x;
+    }
+  } else {
+    // parallel cyclic reduction of the same system: row i is
+    // A[i,i-1] x[i-1] + A[i,i] x[i] + A[i,i+1] x[i+1] = rhs with
+    // A[i,i-1] = tri[3,i-1], A[i,i] = tri[2,i], A[i,i+1] = tri[1,i+1]
+    // (the column-oriented storage of SchurSolve.jl); rows >= n are the
+    // identity.  log2(64) = 6 elimination sweeps, all lanes busy.
+    double ra = __shfl_up(t2, 1), rb = t1, rc = __shfl_down(t0, 1), rd = v4;
+    if (i == 0 || !row) ra = 0.0;
+    if (i >= n - 1) rc = 0.0;
+    if (!row) {
+      rb = 1.0;
+      rd = 0.0;
+    }
+#pragma unroll
+    for (int st = 1; st < 64; st <<= 1) {
+      double am = __shfl_up(ra, st), bm = __shfl_up(rb, st), cm = __shfl_up(rc, st), dm = __shfl_up(rd, st);
+      double ap = __shfl_down(ra, st), bp = __shfl_down(rb, st), cp = __shfl_down(rc, st), dp = __shfl_down(rd, st);
+      if (i < st) {
+        am = 0.0;
+        bm = 1.0;
+        cm = 0.0;
+        dm = 0.0;
+      }
+      if (i + st > 63) {
+        ap = 0.0;
+        bp = 1.0;
+        cp = 0.0;
+        dp = 0.0;
+      }
+      const double k1 = ra / bm, k2 = rc / bp;
+      rb = rb - cm * k1 - ap * k2;
+      rd = rd - dm * k1 - dp * k2;
+      ra = -am * k1;
+      rc = -cp * k2;
+    }
+    sol = rd / rb;
+  }
+  const double k4 = row ? sol : 0.0;
+  // SchurSolveBKernel!
+  const double k4m = __shfl_up(k4, 1);
+  const double jrw1m = __shfl_up(jrw1, 1), jtw1m = __shfl_up(jtw1, 1);
+  if (lev) {
+    if (i == 0) {
+      v1 += jrw0 * k4;
+      v5 += jtw0 * k4;
+    } else if (i == Nz - 1) {
+      v1 += jrw1m * k4m;
+      v5 += jtw1m * k4m;
+    } else {
+      v1 += jrw0 * k4 + jrw1m * k4m;
+      v5 += jtw0 * k4 + jtw1m * k4m;
+    }
+    const double k1 = fac * v1, k2 = fac * v2, k3 = fac * v3, k5 = fac * v5;
+    k[c0 + i] = k1;
+    k[c0 + i + vs] = k2;
+    k[c0 + i + 2 * vs] = k3;
+    k[c0 + i + 3 * vs] = k4;
+    k[c0 + i + 4 * vs] = k5;
+    if (!FUSED) {
+      v[c0 + i] = v1;
+      v[c0 + i + 4 * vs] = v5;
+    } else {
+      // next stage state V = Vn + sum_j a_{i+1,j} k_j, or the final update
+      // V = Vn + sum_j m_j k_j (RosenbrockSchur.jl:14-17,29-32)
+      const double kc[5] = {k1, k2, k3, k4, k5};
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        const size_t o = c0 + i + q * vs;
+        double r = rw.Vn[o];
+        for (int j = 0; j < rw.stage; ++j) r = r + rw.av[j] * rw.kprev[j][o];
+        r = r + rw.av[rw.stage] * kc[q];
+        rw.V[o] = r;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_schur_col(KArgs a, double *__restrict__ k, double *__restrict__ v,
+                                                   const double *__restrict__ JRhoW, const double *__restrict__ JWRho,
+                                                   const double *__restrict__ JWRhoTh,
+                                                   const double *__restrict__ JRhoThW, double *__restrict__ tri,
+                                                   double fac, int comp_tri, int thomas) {
+  const RWStage none{};
+  schur_col<false>(a, k, v, JRhoW, JWRho, JWRhoTh, JRhoThW, tri, fac, comp_tri, thomas, none);
+}
+
+// One Rosenbrock stage of cgd_rw_step after the tendency: stage terms,
+// Schur solve, next stage state.  The tendency fV is only read.
+__global__ void __launch_bounds__(256) k_rw_col(KArgs a, double *__restrict__ k, const double *__restrict__ fV,
+                                                const double *__restrict__ JRhoW, const double *__restrict__ JWRho,
+                                                const double *__restrict__ JWRhoTh,
+                                                const double *__restrict__ JRhoThW, double *__restrict__ tri,
+                                                double fac, int comp_tri, int thomas, RWStage rw) {
+  schur_col<true>(a, k, const_cast<double *>(fV), JRhoW, JWRho, JWRhoTh, JRhoThW, tri, fac, comp_tri, thomas, rw);
 }
 
 // ------------------------------------------------------------------------
@@ -727,7 +1067,3 @@ __global__ void k_msum(const double *__restrict__ M, double *__restrict__ Ms, si
     Ms[i] = M[i] + M[i + n];
 }
 
-template __global__ void k_fcn1<4>(KArgs, const int *, int, const double *, double *);
-template __global__ void k_fcn1<5>(KArgs, const int *, int, const double *, double *);
-template __global__ void k_fcn2<4>(KArgs, const int *, int, const double *, const double *, const double *, double *);
-template __global__ void k_fcn2<5>(KArgs, const int *, int, const double *, const double *, const double *, double *);

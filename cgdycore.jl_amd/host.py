@@ -271,6 +271,11 @@ def _ctx(FE, Metric, Phys, Global) -> NativeContext:
     return Global._ctx
 
 
+def init_context(FE, Metric, Phys, Global) -> NativeContext:
+    """Create the native context now (otherwise the first Fcn/Jac call does)."""
+    return _ctx(FE, Metric, Phys, Global)
+
+
 def _check_state(t: torch.Tensor, Global: GlobalStruct, nv: int, name: str):
     if t.dtype != torch.float64 or not t.is_cuda:
         raise CGDError(f"{name}: float64 CUDA tensor expected")

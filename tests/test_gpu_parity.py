@@ -25,23 +25,39 @@ def torch_cuda():
     return torch
 
 
-def _setup(pkg, oracle_mod, ne, nz, p=4, seed=0):
+def perturbed(U0, seed):
+    """Jet state plus seeded wind noise, so that divergence, curl and w are
+    O(1) fields (for the pure zonal jet several caches are roundoff)."""
+    rng = np.random.default_rng(seed)
+    U = U0.copy()
+    U[1] += 5.0 * rng.standard_normal(U[1].shape)
+    U[2] += 5.0 * rng.standard_normal(U[2].shape)
+    U[3] += 0.5 * rng.standard_normal(U[3].shape)
+    U[3][:, -1] = 0.0
+    return U
+
+
+def _setup(pkg, oracle_mod, ne, nz, p=4, seed=0, noisy=False):
     D = pkg.discretization
     host = pkg.host
     disc = D.build(ne, p, nz, 30000.0)
     phys = D.Phys()
     U0 = pkg.state.jet_state(disc, seed=seed)
+    if noisy:
+        U0 = perturbed(U0, 100 + seed)
     FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda")
     orc = oracle_mod.Oracle(disc, phys, (Global.Model.HyperDCurl, Global.Model.HyperDGrad, Global.Model.HyperDDiv))
     return disc, phys, U0, (FE, Metric, Global, Cache, JCache), orc
 
 
-@pytest.mark.parametrize("ne,nz", [(3, 8), (4, 10), (6, 10), (4, 64), (5, 41), (3, 3)])
-def test_fcn_prepare_parity(pkg, oracle_mod, torch_cuda, ne, nz):
+@pytest.mark.parametrize("ne,nz,noisy", [(3, 8, False), (4, 10, False), (6, 10, False), (4, 64, False),
+                                         (5, 41, False), (3, 3, False), (4, 10, True), (4, 64, True),
+                                         (3, 33, True)])
+def test_fcn_prepare_parity(pkg, oracle_mod, torch_cuda, ne, nz, noisy):
     torch = torch_cuda
     host = pkg.host
-    disc, phys, U0, (FE, Metric, Global, Cache, JCache), orc = _setup(pkg, oracle_mod, ne, nz)
-    print(f"fcn parity ne={ne} nz={nz}", flush=True)
+    disc, phys, U0, (FE, Metric, Global, Cache, JCache), orc = _setup(pkg, oracle_mod, ne, nz, noisy=noisy)
+    print(f"fcn parity ne={ne} nz={nz} noisy={noisy}", flush=True)
     U = torch.as_tensor(U0, device="cuda")
     F = torch.full_like(U, 7.0)  # FcnGPU! must zero-and-overwrite
     eq = host.CompressibleShallow()
@@ -53,14 +69,23 @@ def test_fcn_prepare_parity(pkg, oracle_mod, torch_cuda, ne, nz):
     Ag = Cache.AuxG.cpu().numpy()
     for c in range(3):
         assert rel_l2(Ag[c], A[c]) <= TOL, f"AuxG[{c}]"
-    T1g = Cache.Temp1.cpu().numpy()
-    for v in range(5):
-        assert rel_l2(T1g[v], T1o[v]) <= TOL, f"Temp1[{v}] {rel_l2(T1g[v], T1o[v]):.3e}"
+    # the hyperviscosity cache (rot of curl, grad of div, div grad theta) is
+    # roundoff for the pure zonal jet; it is compared on the noisy state
+    if noisy:
+        T1g = Cache.Temp1.cpu().numpy()
+        for v in range(5):
+            e = rel_l2(T1g[v], T1o[v])
+            assert e <= TOL, f"Temp1[{v}] {e:.3e}"
     Fg = F.cpu().numpy()
     for v in range(5):
         e = rel_l2(Fg[v], Fo[v])
         print(f"  F{v + 1} rel L2 {e:.3e}")
-        assert e <= TOL, f"F{v + 1} rel L2 {e:.3e}"
+        # For the unperturbed zonal jet the continuity tendency F1 is the
+        # truncation-sized remainder of flux terms ~1e4 times larger, so its
+        # own norm is not a meaningful scale for roundoff; it is held to the
+        # bar on the perturbed states and through the stepped state.
+        tol = TOL if (noisy or v != 0) else 1e-11
+        assert e <= tol, f"F{v + 1} rel L2 {e:.3e}"
     assert np.all(Fg[3][:, -1] == 0.0)  # the w lid receives no tendency
 
 
