@@ -58,6 +58,54 @@ static int grid_for(size_t n, int block) {
   return (int)std::min<size_t>(std::max<size_t>(g, 1), 65536);
 }
 
+// Greedy colouring in element order: each element takes the smallest colour
+// not used by an element sharing one of its nodes.  Returns the number of
+// colours, or -1 if more than 64 would be needed.
+static int color_elements(const std::vector<int> &g32, int NN, int NF, int NumG, std::vector<int> &color) {
+  std::vector<int> nptr(NumG + 1, 0);
+  for (int v : g32) nptr[v + 1]++;
+  for (int i = 0; i < NumG; ++i) nptr[i + 1] += nptr[i];
+  std::vector<int> nel(nptr[NumG]);
+  {
+    std::vector<int> fillp(nptr.begin(), nptr.end() - 1);
+    for (int e = 0; e < NF; ++e)
+      for (int i = 0; i < NN; ++i) nel[fillp[g32[(size_t)e * NN + i]]++] = e;
+  }
+  color.assign(NF, -1);
+  int ncol = 0;
+  std::vector<int> mark(64, -1);
+  for (int e = 0; e < NF; ++e) {
+    for (int i = 0; i < NN; ++i) {
+      const int gn = g32[(size_t)e * NN + i];
+      for (int q = nptr[gn]; q < nptr[gn + 1]; ++q) {
+        const int o = nel[q];
+        if (o != e && color[o] >= 0) mark[color[o]] = e;
+      }
+    }
+    int col = 0;
+    while (col < 64 && mark[col] == e) ++col;
+    if (col >= 64) return -1;
+    color[e] = col;
+    ncol = std::max(ncol, col + 1);
+  }
+  return ncol;
+}
+
+extern "C" int cgd_color_elements(const int64_t *glob, int NN, int NF, int NumG, int *color, int *ncolor) {
+  if (!glob || !color || !ncolor || NN < 1 || NF < 1 || NumG < 1) return CGD_ERR_ARG;
+  std::vector<int> g32((size_t)NN * NF);
+  for (size_t i = 0; i < g32.size(); ++i) {
+    if (glob[i] < 1 || glob[i] > NumG) return CGD_ERR_ARG;
+    g32[i] = (int)(glob[i] - 1);
+  }
+  std::vector<int> col;
+  const int n = color_elements(g32, NN, NF, NumG, col);
+  if (n < 0) return CGD_ERR_UNSUPPORTED;
+  std::copy(col.begin(), col.end(), color);
+  *ncolor = n;
+  return CGD_OK;
+}
+
 extern "C" const char *cgd_last_error(const cgd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
@@ -118,32 +166,9 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
     g32[i] = (int)(hglob[i] - 1);
   }
   // greedy element colouring: no two elements of one colour share a node
-  std::vector<int> nptr(d->NumG + 1, 0);
-  for (int v : g32) nptr[v + 1]++;
-  for (int i = 0; i < d->NumG; ++i) nptr[i + 1] += nptr[i];
-  std::vector<int> nel(nptr[d->NumG]);
-  {
-    std::vector<int> fillp(nptr.begin(), nptr.end() - 1);
-    for (int e = 0; e < NF; ++e)
-      for (int i = 0; i < NN; ++i) nel[fillp[g32[(size_t)e * NN + i]]++] = e;
-  }
   std::vector<int> color(NF, -1);
-  int ncol = 0;
-  std::vector<int> mark(64, -1);
-  for (int e = 0; e < NF; ++e) {
-    for (int i = 0; i < NN; ++i) {
-      int gn = g32[(size_t)e * NN + i];
-      for (int q = nptr[gn]; q < nptr[gn + 1]; ++q) {
-        int o = nel[q];
-        if (o != e && color[o] >= 0) mark[color[o]] = e;
-      }
-    }
-    int col = 0;
-    while (col < 64 && mark[col] == e) ++col;
-    if (col >= 64) return fail(c, CGD_ERR_ARG, "element colouring needs more than 64 colours");
-    color[e] = col;
-    ncol = std::max(ncol, col + 1);
-  }
+  const int ncol = color_elements(g32, NN, NF, d->NumG, color);
+  if (ncol < 0) return fail(c, CGD_ERR_ARG, "element colouring needs more than 64 colours");
   c->ncolor = ncol;
   c->color_off.assign(ncol + 1, 0);
   for (int e = 0; e < NF; ++e) c->color_off[color[e] + 1]++;
@@ -225,16 +250,11 @@ static void prof_free(cgd_ctx *c);
 
 extern "C" int cgd_ctx_destroy(cgd_ctx *c) {
   if (!c) return CGD_ERR_ARG;
-  (void)hipFree(c->d_glob);
-  (void)hipFree(c->d_elist);
-  (void)hipFree(c->d_met);
-  (void)hipFree(c->d_met1);
-  (void)hipFree(c->d_ms);
-  (void)hipFree(c->d_bwork);
-  (void)hipFree(c->d_f4x);
-  (void)hipFree(c->d_nptr);
-  (void)hipFree(c->d_first);
-  (void)hipFree(c->d_ncopy);
+  // a context that failed validation holds no device memory: tear it down
+  // without touching the HIP runtime
+  for (void *p : {(void *)c->d_glob, (void *)c->d_elist, (void *)c->d_met, (void *)c->d_met1, (void *)c->d_ms,
+                  (void *)c->d_bwork, (void *)c->d_f4x, (void *)c->d_nptr, (void *)c->d_first, (void *)c->d_ncopy})
+    if (p) (void)hipFree(p);
   prof_free(c);
   delete c;
   return CGD_OK;

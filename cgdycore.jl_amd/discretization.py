@@ -63,7 +63,8 @@ def _legendre(n, x):
         return p0, np.zeros_like(x)
     for k in range(2, n + 1):
         p0, p1 = p1, ((2 * k - 1) * x * p1 - (k - 1) * p0) / k
-    dp = n * (x * p1 - p0) / (x * x - 1.0)
+    with np.errstate(divide="ignore", invalid="ignore"):  # x = +-1 gives 0/0; callers use interior x
+        dp = n * (x * p1 - p0) / (x * x - 1.0)
     return p1, dp
 
 

@@ -84,13 +84,26 @@ def native():
     L.cgd_rw_step.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), _vp]
     L.cgd_profile_enable.argtypes = [_vp, C.c_int]
     L.cgd_profile_read.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
+    L.cgd_color_elements.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp]
     _LIB = L
     return L
 
 
 def exported_symbols():
     return ["cgd_ctx_create", "cgd_ctx_destroy", "cgd_last_error", "cgd_prepare", "cgd_fcn", "cgd_jac",
-            "cgd_schur", "cgd_rw_step", "cgd_profile_enable", "cgd_profile_read"]
+            "cgd_schur", "cgd_rw_step", "cgd_profile_enable", "cgd_profile_read", "cgd_color_elements"]
+
+
+def color_elements(glob: np.ndarray, num_g: int):
+    """Element colouring of the DSS schedule (host only, no GPU needed)."""
+    g = np.ascontiguousarray(glob, dtype=np.int64)
+    nf, nn = g.shape
+    col = np.zeros(nf, dtype=np.int32)
+    nc = C.c_int()
+    rc = native().cgd_color_elements(g.ctypes.data, nn, nf, num_g, col.ctypes.data, C.byref(nc))
+    if rc != 0:
+        raise CGDError(f"cgd_color_elements failed (code {rc})")
+    return col, nc.value
 
 
 def _stream():

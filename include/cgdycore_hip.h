@@ -100,6 +100,14 @@ typedef struct {
 } cgd_rw_work;
 int cgd_rw_step(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, void *stream);
 
+/* Host-only helper (no GPU needed): the element colouring that schedules the
+ * DSS.  glob is a HOST int64 (N*N, NF) 1-based array; color receives NF
+ * colours (0-based), ncolor their count.  Elements of one colour share no
+ * global node; nodes accumulate element contributions in colour order.  No
+ * reference counterpart (the reference scatters with atomics,
+ * src/GPU/ViscKernel.jl:78-82). */
+int cgd_color_elements(const int64_t *glob, int NN, int NF, int NumG, int *color, int *ncolor);
+
 /* Per-kernel timing of the tendency passes with HIP events on the launch
  * stream (used by the benchmark; off by default).  which: 0 = pass 1
  * (hyperviscosity), 1 = pass 2 (fused tendency), 2 = column solve. */

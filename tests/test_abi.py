@@ -1,0 +1,123 @@
+"""The C ABI without a GPU: the library loads, exports every entry point the
+header declares, and rejects descriptors outside the implemented path
+before it touches the device."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+import torch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cgdycore_hip.h")
+
+
+@pytest.fixture(scope="module")
+def lib(pkg):
+    path = pkg.lib_path()
+    if not os.path.exists(path):
+        subprocess.run(["make", "-C", os.path.dirname(path)], check=True, capture_output=True)
+    return pkg.host.native()
+
+
+def header_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|void)\s*\*?\s*(cgd_\w+)\s*\(", src, flags=re.M)))
+
+
+def test_header_symbols_exported(lib, pkg):
+    names = header_functions()
+    assert "cgd_fcn" in names and "cgd_rw_step" in names and len(names) >= 11
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in the header but not exported"
+    assert sorted(pkg.host.exported_symbols()) == names
+    # the exported dynamic symbols are C names (no C++ mangling)
+    nm = subprocess.run(["nm", "-D", "--defined-only", pkg.lib_path()], capture_output=True, text=True, check=True)
+    exported = {ln.split()[-1] for ln in nm.stdout.splitlines() if " T " in ln}
+    assert set(names) <= exported
+
+
+def test_built_for_gfx950(pkg):
+    out = subprocess.run(["strings", pkg.lib_path()], capture_output=True, text=True, check=True).stdout
+    assert "gfx950" in out
+
+
+def _desc(pkg, **over):
+    d = pkg.host._Desc()
+    d.N, d.Nz, d.NF, d.NumG, d.NumV, d.NumTr, d.precision = 5, 8, 24, 386, 5, 0, 0
+    d.Grav, d.Omega, d.Rd, d.Cpd, d.p0, d.kappa = 9.80616, 7.29e-5, 287.0, 1004.0, 1e5, 287.0 / 1004.0
+    # non-null stand-ins: validation rejects before dereferencing any of them
+    for f in ("Glob", "DS", "DW", "dXdxI", "J", "X", "M", "dz"):
+        setattr(d, f, 16)
+    for k, v in over.items():
+        setattr(d, k, v)
+    return d
+
+
+@pytest.mark.parametrize(
+    "over,code,msg",
+    [
+        (dict(N=3), -2, "OrdPoly"),
+        (dict(Nz=2), -1, "Nz"),
+        (dict(NumV=6), -2, "NumV"),
+        (dict(NumTr=2), -2, "tracers"),
+        (dict(precision=7), -2, "precision"),
+        (dict(HorLimit=1), -2, "dry CompressibleShallow"),
+        (dict(Microphysics=1), -2, "dry CompressibleShallow"),
+        (dict(HyperDDivW=1e14), -2, "HyperDDivW"),
+        (dict(dz=None), -1, "null static input"),
+        (dict(NF=0), -1, "positive"),
+    ],
+)
+def test_descriptor_validation(lib, pkg, over, code, msg):
+    d = _desc(pkg, **over)
+    h = C.c_void_p()
+    rc = lib.cgd_ctx_create(C.byref(d), C.byref(h), None)
+    try:
+        assert rc == code
+        assert msg in lib.cgd_last_error(h).decode()
+        # every call on a failed context reports the state error
+        assert lib.cgd_fcn(h, None, None, None, None, None) == -4
+        assert lib.cgd_rw_step(h, None, 1.0, None, None) == -4
+        assert "context creation failed" in lib.cgd_last_error(h).decode()
+    finally:
+        assert lib.cgd_ctx_destroy(h) == 0
+
+
+def test_null_arguments(lib):
+    assert lib.cgd_ctx_create(None, None, None) == -1
+    assert lib.cgd_ctx_destroy(None) == -1
+    assert lib.cgd_last_error(None) == b"null context"
+    assert lib.cgd_color_elements(None, 25, 1, 1, None, None) == -1
+
+
+def test_colouring_rejects_bad_numbering(pkg, lib):
+    g = np.array([[1, 2, 3, 4]], dtype=np.int64)
+    with pytest.raises(pkg.host.CGDError):
+        pkg.host.color_elements(g, 3)  # node 4 > NumG
+    with pytest.raises(pkg.host.CGDError):
+        pkg.host.color_elements(g - 1, 4)  # 0 is not a 1-based index
+
+
+def test_host_rejects_cpu_tensors(pkg):
+    """The host mirror refuses host tensors instead of computing on the CPU."""
+    h = pkg.host
+    with pytest.raises(h.CGDError):
+        h._check_state(torch.zeros(5, 10, 8, dtype=torch.float64), h.GlobalStruct(
+            Model=None, Nz=8, NumG=10, NF=1, N=5), 5, "U")
+
+
+def test_ssp_knoth_values(pkg):
+    a, c, m, g = pkg.host.ssp_knoth()
+    assert a[1, 0] == 1 and a[2, 0] == a[2, 1] == 0.25 and g == 1.0
+    assert c[2, 0] == c[2, 1] == -0.75 and c[1, 0] == 0.0
+    assert np.allclose(m, [2 / 3, 2 / 3, 2 / 3], rtol=0, atol=1e-16)
+
+
+def test_hyperdiffusion_scaling(pkg):
+    f = pkg.host.hyperdiffusion_for
+    assert f(30) == 5e14
+    assert abs(f(60) / f(30) - 2.0 ** -3.2) < 1e-15

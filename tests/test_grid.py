@@ -1,0 +1,85 @@
+"""Cubed-sphere numbering (setup restatement) against the reference fixture
+and the invariants that pin it at p = 4."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "globind_ne30_p3.json")
+
+
+def test_glob_matches_reference_fixture(pkg):
+    """Glob at ne=30, p=3 is byte-identical to TestKernels/GlobInd."""
+    ref = json.load(open(GOLDEN))
+    g = pkg.grid.cubed_sphere(30)
+    glob, num_g = pkg.grid.numbering_cg_quad(g, 3)
+    assert glob.shape == (5400, 16)
+    assert num_g == ref["num_g"] == 48602
+    assert list(glob[0]) == ref["first_element"]
+    assert list(glob[-1]) == ref["last_element"]
+    assert hashlib.sha256(glob.astype("<i8").tobytes()).hexdigest() == ref["sha256"]
+
+
+@pytest.mark.parametrize("ne", [2, 3, 6, 9])
+def test_numbering_invariants_p4(pkg, ne):
+    p = 4
+    g = pkg.grid.cubed_sphere(ne)
+    glob, num_g = pkg.grid.numbering_cg_quad(g, p)
+    nf = 6 * ne * ne
+    assert glob.shape == (nf, (p + 1) ** 2)
+    # Euler: V = 6 ne^2 + 2 vertices, E = 12 ne^2 edges
+    assert g.num_nodes == 6 * ne * ne + 2 and g.num_edges == 12 * ne * ne
+    assert num_g == 6 * ne * ne * p * p + 2
+    assert sorted(np.unique(glob)) == list(range(1, num_g + 1))
+    for row in glob:
+        assert len(set(row)) == len(row)
+    _, counts = np.unique(glob, return_counts=True)
+    hist = dict(zip(*np.unique(counts, return_counts=True)))
+    assert hist.get(1) == 6 * ne * ne * (p - 1) ** 2  # face interiors
+    assert hist.get(2) == 12 * ne * ne * (p - 1)  # edge interiors
+    assert hist.get(3) == 8  # cube corners
+    assert hist.get(4) == 6 * ne * ne - 6  # other vertices
+
+
+@pytest.mark.parametrize("ne", [3, 6])
+def test_shared_nodes_coincide(pkg, ne):
+    """Every copy of a global node sits at the same point of the sphere."""
+    D = pkg.discretization
+    disc = D.build(ne, 4, 3, 30000.0)
+    X = disc.X[:, 0, :, 0, :]  # (NF, 3, N^2) at the bottom node of layer 1
+    pts = np.transpose(X, (0, 2, 1)).reshape(-1, 3)
+    idx = disc.glob.reshape(-1) - 1
+    ref = np.zeros((disc.num_g, 3))
+    ref[idx] = pts
+    err = np.abs(pts - ref[idx]).max() / disc.rad
+    assert err < 1e-12
+
+
+def test_element_corners_counterclockwise(pkg):
+    """Corner order N1..N4 runs counter-clockwise seen from outside, the
+    orientation the reference enforces (Face.jl:134-156)."""
+    g = pkg.grid.cubed_sphere(4)
+    P = g.face_corners()
+    n = np.cross(P[:, 1] - P[:, 0], P[:, 3] - P[:, 0])
+    mid = P.mean(axis=1)
+    assert np.all(np.einsum("ij,ij->i", n, mid) > 0)
+
+
+@pytest.mark.parametrize("ne", [2, 6, 30])
+def test_colouring_schedules_dss(pkg, ne):
+    """Elements of one colour share no global node (the DSS writes of a colour
+    launch never collide) and every element has a colour."""
+    host = pkg.host
+    try:
+        host.native()
+    except Exception as e:  # library not built
+        pytest.skip(str(e))
+    g = pkg.grid.cubed_sphere(ne)
+    glob, num_g = pkg.grid.numbering_cg_quad(g, 4)
+    col, nc = host.color_elements(glob, num_g)
+    assert col.min() >= 0 and col.max() == nc - 1 and nc <= 8
+    for c in range(nc):
+        nodes = glob[col == c].ravel()
+        assert len(np.unique(nodes)) == nodes.size, f"colour {c} shares a node"
