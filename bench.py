@@ -47,6 +47,21 @@ def survey_bytes(ne, p, nz, wt=8, wm=8, wc=8):
     return dict(step=step, pass1=pass1, pass2=pass2, Ne=Ne, Ng=Ng)
 
 
+def measured_traffic(kernel, workload, algorithmic):
+    """HBM bytes per tendency evaluation of ``kernel`` from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary of the same workload (each counter
+    scaled by its reading for a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{workload}.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("algorithmic_bytes_per_eval") != algorithmic:
+        return None
+    return t["traffic_bytes_per_eval"]
+
+
 def cpu_info():
     model = ""
     try:
@@ -71,7 +86,8 @@ def main():
     ap.add_argument("--dt", type=float, default=None,
                     help="default 100 s at ne<=30 and 3000/ne s above (horizontal acoustic CFL of p=4)")
     ap.add_argument("--hyperd", type=float, default=None, help="hyperviscosity coefficient (m^4/s)")
-    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle RW steps for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="oracle RW steps for the CPU baseline, about 5 s each at ne=30 (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -139,8 +155,11 @@ def main():
     n_eval = 3 * args.steps
     p2_eval_s = p2_ms / 1e3 / n_eval
     achieved = sb["pass2"] / p2_eval_s / 1e9
+    workload = f"rw_step_fp64_ne{args.ne}_p{args.p}_nz{args.nz}"
+    traffic = measured_traffic("k_fcn2", workload, sb["pass2"])
     roofline = {"bound": "hbm", "kernel": "k_fcn2 (fused tendency pass)", "achieved": round(achieved, 1),
-                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                "traffic": traffic,
                 "algorithmic_bytes_per_eval": sb["pass2"], "launches_per_eval": p2_launch // max(n_eval, 1),
                 "ms_per_eval": round(p2_eval_s * 1e3, 4),
                 "step_roofline_frac": round(sb["step"] / (wall / args.steps) / 1e9 / PEAK_HBM_GBS, 4),
@@ -162,7 +181,7 @@ def main():
            "value": round(mpts, 2), "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64", "data": "synthetic: seeded baroclinic jet (seed 0) on a flat cubed sphere",
-           "config": {"workload": f"rw_step_fp64_ne{args.ne}_p{args.p}_nz{args.nz}", "ne": args.ne, "p": args.p,
+           "config": {"workload": workload, "ne": args.ne, "p": args.p,
                       "nz": args.nz, "dt": args.dt, "hyperd": Global.Model.HyperDDiv, "points": Ne,
                       "precision": "fp64", "finite": finite},
            "roofline": roofline, "cpu_baseline": cpu}

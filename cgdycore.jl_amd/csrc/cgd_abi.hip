@@ -128,6 +128,9 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   if (!d->Glob || !d->DS || !d->DW || !d->dXdxI || !d->J || !d->X || !d->M || !d->dz)
     return fail(c, CGD_ERR_ARG, "null static input pointer");
   if ((size_t)d->NF * d->N * d->N * d->Nz > (size_t)1 << 40) return fail(c, CGD_ERR_ARG, "grid too large");
+  // the element kernels index a field with 32-bit offsets
+  if ((size_t)d->NumG * d->Nz >= (size_t)1 << 28 || (size_t)d->N * d->N * d->Nz >= (size_t)1 << 28)
+    return fail(c, CGD_ERR_ARG, "NumG*Nz must be below 2^28");
   hipStream_t st = S(stream);
   c->N = d->N;
   c->NN = d->N * d->N;
@@ -239,7 +242,7 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   a.glob = c->d_glob;
   a.met = c->d_met;
   a.met1 = c->d_met1;
-  a.Ms = c->d_ms;
+  a.iMs = c->d_ms;
   a.M = d->M;
   a.first = c->d_first;
   c->ok = true;

@@ -9,10 +9,11 @@
 //   DSS mass         Ms[g][iz] = M1 + M2, M[K][g][iz]
 //
 // Element kernels: one thread per (node, level) point of one element and
-// one chunk of <= 40 levels; thread t -> node = t / nzt, level = t % nzt.
+// one chunk of NZT (8 or 16) levels; thread t -> node = t / NZT,
+// level = t % NZT, so a chunk column fills whole 16-lane DPP rows.
 // Horizontal derivatives exchange values through LDS tiles
-// tile[f][node][level]; vertical neighbours are read directly (the same
-// cache lines the neighbouring lanes load).  The DSS is a non-atomic
+// tile[f][node][level]; vertical neighbours come from the neighbouring
+// lanes by DPP row shifts, with one global load at each chunk edge.  The DSS is a non-atomic
 // read-modify-write: one launch per element colour (no two elements of a
 // colour share a global node), so every global node accumulates its element
 // contributions in a fixed order.
@@ -33,7 +34,7 @@ struct KArgs {
   const int *glob;                 // [NF][NN], 0-based
   const double *met;               // [NF][17][NN][Nz]
   const double *met1;              // [NF][5][NN][Nz]
-  const double *Ms;                // [NumG][Nz]
+  const double *iMs;               // [NumG][Nz]: 1 / (M1 + M2)
   const double *M;                 // [2][NumG][Nz]
   const unsigned char *first;      // [NF][NN]: 1 if this copy is the node's first in colour order
 };
@@ -58,17 +59,52 @@ __device__ __forceinline__ double eos_dpdrhoth(double Rd, double p0, double kapp
 __device__ __forceinline__ double jmin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double jmax(double a, double b) { return (b > a) ? b : a; }
 
-// RecU4 (src/GPU/OperatorKernel.jl:1044-1059)
+// 1/x for normal, finite x: the hardware estimate (v_rcp_f64) refined by two
+// Newton-Raphson steps r <- r + r (1 - x r), each at least doubling the
+// correct bits; the result is within an ulp or two of the rounded quotient
+// (the IEEE division sequence costs about twice as many instructions)
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Element i of a field at a uniform base: the byte offset is formed in 32
+// bits so the access is a single saddr-form global load/store (scalar base
+// + 32-bit vector offset).  Callers keep i * sizeof(T) below 2^31.
+template <typename T> __device__ __forceinline__ T &at(T *base, unsigned i) {
+  return *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + i * (unsigned)sizeof(T));
+}
+template <typename T> __device__ __forceinline__ const T &at(const T *base, unsigned i) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(T));
+}
+
+// RecU4 (src/GPU/OperatorKernel.jl:1044-1059): the J-weighted limited
+// reconstruction of c at the face between cL and cR.  The weights
+//   kR = JL/(JL+JR) * (JLL+JL)/(JLL+JL+JR),  kL = JL/(JLL+JL) * JR/(JLL+JL+JR)
+// share one reciprocal of the product of their three denominators (and
+// likewise for the right-biased pair).
 __device__ __forceinline__ void recu4(double cLL, double cL, double cR, double cRR, double JLL, double JL,
                                       double JR, double JRR, double &cFL, double &cFR) {
-  double kR = (JL / (JL + JR)) * ((JLL + JL) / (JLL + JL + JR));
-  double kL = (JL / (JLL + JL)) * (JR / (JLL + JL + JR));
-  double r = (cL - cLL) / (cR - cL + 1.e-20);
-  cFL = cL + jmax(0.0, jmin(r, jmin(kL * r + kR, 1.0))) * (cR - cL);
-  kL = (JR / (JR + JL)) * ((JRR + JR) / (JL + JR + JRR));
-  kR = (JR / (JRR + JR)) * (JL / (JL + JR + JRR));
-  r = (cR - cRR) / (cL - cR + 1.e-20);
-  cFR = cR + jmax(0.0, jmin(r, jmin(kR * r + kL, 1.0))) * (cL - cR);
+  const double A = JL + JR;
+  {
+    const double B = JLL + JL, C = B + JR;
+    const double q = rcp(A * B * C);
+    const double kR = (JL * B) * B * q;
+    const double kL = (JL * JR) * A * q;
+    const double r = (cL - cLL) * rcp(cR - cL + 1.e-20);
+    cFL = cL + jmax(0.0, jmin(r, jmin(kL * r + kR, 1.0))) * (cR - cL);
+  }
+  {
+    const double B = JRR + JR, C = A + JRR;
+    const double q = rcp(A * B * C);
+    const double kL = (JR * B) * B * q;
+    const double kR = (JR * JL) * A * q;
+    const double r = (cR - cRR) * rcp(cL - cR + 1.e-20);
+    cFR = cR + jmax(0.0, jmin(r, jmin(kR * r + kL, 1.0))) * (cL - cR);
+  }
 }
 
 // ------------------------------------------------------------------------
@@ -188,27 +224,30 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
   const size_t vs = (size_t)a.NumG * Nz;
   const size_t NNZ = (size_t)NN * Nz;
 
-  double S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1, Ms = 1;
+  // uniform bases + 32-bit lane offsets (the host checks NumG*Nz < 2^28)
+  const double *metE = a.met1 + (size_t)e * CGD_NMET1 * NNZ;
+  double S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1, iMs = 1;
   bool first = false;
-  size_t ui = 0;
+  unsigned ui = 0;
   double uC = 0, vC = 0, uD = 0, vD = 0, Th = 0;
   if (valid) {
-    const int g = a.glob[e * NN + node];
-    ui = (size_t)g * Nz + iz;
-    const size_t mb = ((size_t)e * CGD_NMET1 * NN + node) * Nz + iz;
-    S11 = a.met1[mb];
-    S12 = a.met1[mb + NNZ];
-    S21 = a.met1[mb + 2 * NNZ];
-    S22 = a.met1[mb + 3 * NNZ];
-    Js = a.met1[mb + 4 * NNZ];
-    Ms = a.Ms[ui];
+    ui = (unsigned)a.glob[e * NN + node] * (unsigned)Nz + iz;
+    __builtin_assume(ui < (1u << 28));
+    const unsigned mo = (unsigned)(node * Nz + iz);
+    __builtin_assume(mo < (1u << 28));
+    S11 = at(metE, mo);
+    S12 = at(metE + NNZ, mo);
+    S21 = at(metE + 2 * NNZ, mo);
+    S22 = at(metE + 3 * NNZ, mo);
+    Js = at(metE + 4 * NNZ, mo);
+    iMs = at(a.iMs, ui);
     first = a.first[e * NN + node] != 0;
-    const double rho = U[ui], u = U[ui + vs], v = U[ui + 2 * vs], rth = U[ui + 4 * vs];
+    const double rho = at(U, ui), u = at(U + vs, ui), v = at(U + 2 * vs, ui), rth = at(U + 4 * vs, ui);
     uC = S11 * v - S12 * u;  // Curl12
     vC = S21 * v - S22 * u;
     uD = S11 * u + S12 * v;  // Contra12
     vD = S21 * u + S22 * v;
-    Th = rth / rho;
+    Th = rth * rcp(rho);
   }
   if (in_tile) {
     T.at(0, node, l) = uC;
@@ -222,7 +261,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
   if (valid) {
     const double Dxc = dx<N, NZT>(Ds, T, 4, I, J, l);
     const double Dyc = dy<N, NZT>(Ds, T, 4, I, J, l);
-    const double iJs = 1.0 / Js;
+    const double iJs = rcp(Js);
     Curl = dxy<N, NZT>(Ds, T, 0, 1, I, J, l) * iJs;
     Div = dxy<N, NZT>(Ds, T, 2, 3, I, J, l) * iJs;
     const double GDx = (S11 * Dxc + S21 * Dyc) * iJs;  // Grad12 with J
@@ -250,20 +289,20 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
     const double FvD = S12 * DxDiv + S22 * DyDiv;
     // DSS: the first element (in colour order) holding the node writes,
     // the others accumulate
-    const double iMs = 1.0 / Ms;
     const double r1 = -FuC * iMs, r2 = -FvC * iMs, r3 = FuD * iMs, r4 = FvD * iMs, r5 = DivTh * iMs;
+    double *T1a = T1, *T1b = T1 + vs, *T1c = T1 + 2 * vs, *T1d = T1 + 3 * vs, *T1e = T1 + 4 * vs;
     if (first) {
-      T1[ui] = r1;
-      T1[ui + vs] = r2;
-      T1[ui + 2 * vs] = r3;
-      T1[ui + 3 * vs] = r4;
-      T1[ui + 4 * vs] = r5;
+      at(T1a, ui) = r1;
+      at(T1b, ui) = r2;
+      at(T1c, ui) = r3;
+      at(T1d, ui) = r4;
+      at(T1e, ui) = r5;
     } else {
-      T1[ui] += r1;
-      T1[ui + vs] += r2;
-      T1[ui + 2 * vs] += r3;
-      T1[ui + 3 * vs] += r4;
-      T1[ui + 4 * vs] += r5;
+      at(T1a, ui) += r1;
+      at(T1b, ui) += r2;
+      at(T1c, ui) += r3;
+      at(T1d, ui) += r4;
+      at(T1e, ui) += r5;
     }
   }
 }
@@ -311,34 +350,43 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   const bool has_lo = iz > 0, has_hi = iz < Nz - 1;
   const double Grav = a.Grav;
 
-  size_t ui = 0, mb = 0;
+  // Addresses: a uniform base per field (scalar registers) plus a 32-bit
+  // per-lane offset, so every load and store is one global access in its
+  // saddr form with no 64-bit vector address arithmetic.  The host checks
+  // NumG*Nz < 2^28.
+  const double *Ur = U, *Uu = U + vs, *Uv = U + 2 * vs, *Uw = U + 3 * vs, *Ut = U + 4 * vs;
+  const double *metE = a.met + (size_t)e * CGD_NMET * NNZ;
+  unsigned ui = 0, mo = 0;
   if (valid) {
-    ui = (size_t)a.glob[e * NN + node] * Nz + iz;
-    mb = ((size_t)e * CGD_NMET * NN + node) * Nz + iz;
+    ui = (unsigned)a.glob[e * NN + node] * (unsigned)Nz + iz;
+    mo = (unsigned)(node * Nz + iz);
   }
-#define MET(c, off) a.met[mb + (off) + (size_t)(c) * NNZ]
+  __builtin_assume(ui < (1u << 28));
+  __builtin_assume(mo < (1u << 28));
+#define MET(c, off) at(metE + (size_t)(c) * NNZ, mo + (off))
   // own level: state and metric (used in every phase)
-  double rho = 1, u = 0, v = 0, w = 0, rth = 1, Ms = 1, M1 = 1, M2 = 1, p = 0;
+  double rho = 1, u = 0, v = 0, w = 0, rth = 1, iMs = 1, M1 = 1, M2 = 1, p = 0;
   double d111 = 0, d112 = 0, d121 = 0, d122 = 0, d211 = 0, d212 = 0, d221 = 0, d222 = 0;
   double d311 = 0, d312 = 0, d321 = 0, d322 = 0, d331 = 1, d332 = 1, J1 = 1, J2 = 1, Wc = 0;
+  const double *M1b = a.M, *M2b = a.M + vs;
   if (valid) {
-    rho = U[ui];
-    u = U[ui + vs];
-    v = U[ui + 2 * vs];
-    w = U[ui + 3 * vs];
-    rth = U[ui + 4 * vs];
-    Ms = a.Ms[ui];
-    M1 = a.M[ui];
-    M2 = a.M[ui + vs];
-    p = P[ui];
+    rho = at(Ur, ui);
+    u = at(Uu, ui);
+    v = at(Uv, ui);
+    w = at(Uw, ui);
+    rth = at(Ut, ui);
+    iMs = at(a.iMs, ui);
+    M1 = at(M1b, ui);
+    M2 = at(M2b, ui);
+    p = at(P, ui);
     d111 = MET(M111, 0); d112 = MET(M112, 0); d121 = MET(M121, 0); d122 = MET(M122, 0);
     d211 = MET(M211, 0); d212 = MET(M212, 0); d221 = MET(M221, 0); d222 = MET(M222, 0);
     d311 = MET(M311, 0); d312 = MET(M312, 0); d321 = MET(M321, 0); d322 = MET(M322, 0);
     d331 = MET(M331, 0); d332 = MET(M332, 0); J1 = MET(MJ1, 0); J2 = MET(MJ2, 0); Wc = MET(MCOR, 0);
   }
   const double Js = J1 + J2;
-  const double c_own = rth / rho;
-  const double iMs = 1.0 / Ms;
+  const double irho = rcp(rho);
+  const double c_own = rth * irho;
   const bool first = valid && a.first[e * NN + node] != 0;
 
   // ---- vertical neighbours ----
@@ -364,20 +412,20 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   double c_pp = dpp_from_above<2>(c_own), Jc_pp = dpp_from_above<2>(Js);
   if (valid) {
     if (has_lo && edge_m1) {
-      const size_t um = ui - 1;
-      rho_m = U[um];
-      u_m = U[um + vs];
-      v_m = U[um + 2 * vs];
-      w_m = U[um + 3 * vs];
-      c_m = U[um + 4 * vs] / rho_m;
+      const unsigned um = ui - 1;
+      rho_m = at(Ur, um);
+      u_m = at(Uu, um);
+      v_m = at(Uv, um);
+      w_m = at(Uw, um);
+      c_m = at(Ut, um) * rcp(rho_m);
       Jc_m = MET(MJ1, -1) + MET(MJ2, -1);
-      M2_m = a.M[um + vs];
+      M2_m = at(M2b, um);
       d312m = MET(M312, -1);
       d322m = MET(M322, -1);
       d332m = MET(M332, -1);
     }
     if (iz >= 2 && edge_m2) {
-      c_mm = U[ui - 2 + 4 * vs] / U[ui - 2];
+      c_mm = at(Ut, ui - 2) * rcp(at(Ur, ui - 2));
       Jc_mm = MET(MJ1, -2) + MET(MJ2, -2);
     }
     if (iz == 1) {
@@ -385,13 +433,13 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
       Jc_mm = Jc_m;
     }
     if (has_hi && edge_p1) {
-      const size_t up = ui + 1;
-      rho_p = U[up];
-      u_p = U[up + vs];
-      v_p = U[up + 2 * vs];
-      c_p = U[up + 4 * vs] / rho_p;
-      p_p = P[up];
-      M1_p = a.M[up];
+      const unsigned up = ui + 1;
+      rho_p = at(Ur, up);
+      u_p = at(Uu, up);
+      v_p = at(Uv, up);
+      c_p = at(Ut, up) * rcp(rho_p);
+      p_p = at(P, up);
+      M1_p = at(M1b, up);
       J1_p = MET(MJ1, 1);
       Jc_p = J1_p + MET(MJ2, 1);
       d311p = MET(M311, 1);
@@ -399,7 +447,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
       d331p = MET(M331, 1);
     }
     if (iz + 2 <= Nz - 1 && edge_p2) {
-      c_pp = U[ui + 2 + 4 * vs] / U[ui + 2];
+      c_pp = at(Ut, ui + 2) * rcp(at(Ur, ui + 2));
       Jc_pp = MET(MJ1, 2) + MET(MJ2, 2);
     }
     if (iz + 2 > Nz - 1) {
@@ -413,7 +461,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   double FU = 0, FV = 0, FW1 = 0, FW2 = 0;  // vertical vorticity-flux parts of MomentumKernel
   double GZlo = 0, GZup = 0;                // 1/2 (KinF(iz) - KinF(iz-1)) at the two interfaces
   double Gpup = 0;                          // -(dp/dz + g rho J) at the upper interface
-  double den_lo = 1, den_up = 1;            // rho-weighted interface masses
+  double iden_lo = 1, iden_up = 1;          // 1 / rho-weighted interface masses
   double f1 = 0, f5 = 0;
   double K1 = 0, K2 = 0;
   double U1 = 0, V1 = 0, W1 = 0;  // 1/2 (tempXZ[2] - tempXZ[1]) of this layer
@@ -423,7 +471,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
       w_lo = w_m;
       K1 = Kh + 0.5 * (w_lo * w_lo);
       K2 = Kh + 0.5 * (w * w);
-      den_lo = rho_m * M2_m + rho * M1;
+      iden_lo = rcp(rho_m * M2_m + rho * M1);
       // vorticity fluxes through the lower interface (MomentumKernel.jl:118-125)
       const double FluxUZ = 0.5 * ((d331 * v - d321 * w_lo) - (d332m * v_m - d322m * w_lo));
       const double FluxVZ = 0.5 * ((d331 * u - d311 * w_lo) - (d332m * u_m - d312m * w_lo));
@@ -443,12 +491,12 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
       f1 += (0.5 * wCon) * iMs;
     } else {
       // free-slip bottom (MomentumKernel.jl:35-36)
-      w_lo = -(d311 * u + d321 * v) / d331;
+      w_lo = -(d311 * u + d321 * v) * rcp(d331);
       K1 = Kh + 0.5 * (w_lo * w_lo);
       K2 = Kh + 0.5 * (w * w);
     }
     if (has_hi) {
-      den_up = rho_p * M1_p + rho * M2;
+      iden_up = rcp(rho_p * M1_p + rho * M2);
       // vorticity fluxes through the upper interface (MomentumKernel.jl:126-133)
       const double FluxUZ = 0.5 * ((d331p * v_p - d321p * w) - (d332 * v - d322 * w));
       const double FluxVZ = 0.5 * ((d331p * u_p - d311p * w) - (d332 * u - d312 * w));
@@ -481,11 +529,11 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   if (in_tile) {
     double c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
     if (valid) {
-      c1 = C[ui];
-      c2 = C[ui + vs];
-      c3 = C[ui + 2 * vs];
-      c4 = C[ui + 3 * vs];
-      c5 = C[ui + 4 * vs];
+      c1 = at(C, ui);
+      c2 = at(C + vs, ui);
+      c3 = at(C + 2 * vs, ui);
+      c4 = at(C + 3 * vs, ui);
+      c5 = at(C + 4 * vs, ui);
     }
     T.at(0, node, l) = S11 * c2 - S12 * c1;  // Curl12(Cache1, Cache2)
     T.at(1, node, l) = S21 * c2 - S22 * c1;
@@ -503,7 +551,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     {
       const double Dxc = dx<N, NZT>(Ds, T, 4, I, J, l);
       const double Dyc = dy<N, NZT>(Ds, T, 4, I, J, l);
-      const double iJs = 1.0 / Js;
+      const double iJs = rcp(Js);
       Curl = dxy<N, NZT>(Ds, T, 0, 1, I, J, l) * iJs;
       Div = dxy<N, NZT>(Ds, T, 2, 3, I, J, l) * iJs;
       const double GDx = (S11 * Dxc + S21 * Dyc) * iJs;
@@ -539,14 +587,13 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     const double DXp = dx<N, NZT>(Ds, T, 7, I, J, l), DYp = dy<N, NZT>(Ds, T, 7, I, J, l);
     double Gu = S11 * DXp + S21 * DYp;
     double Gv = S12 * DXp + S22 * DYp;
-    const double GZg = -Grav * rho * Js / (d331 + d332);
+    const double GZg = -Grav * rho * Js * rcp(d331 + d332);
     Gu += GZg * (d311 + d312);
     Gv += GZg * (d321 + d322);
-    const double irho = 1.0 / rho;
     f2 += (GuF1 + GuF2 - Gu * irho) * iMs;
     f3 += (GvF1 + GvF2 - Gv * irho) * iMs;
-    if (has_hi) f4up += (GwF2 + Gpup) / den_up;
-    if (has_lo) f4lo += GwF1 / den_lo;
+    if (has_hi) f4up += (GwF2 + Gpup) * iden_up;
+    if (has_lo) f4lo += GwF1 * iden_lo;
   }
   __syncthreads();
   // ---- phase 2: second hyperviscosity apply, horizontal mass / theta flux ----
@@ -621,8 +668,8 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     FV += (-u * Wc) * Js;
     f2 += (-v * W1 - w_lo * V1 - v * W2 - w * V2 + FU) * iMs;
     f3 += (u * W1 - w_lo * U1 + u * W2 - w * U2 + FV) * iMs;
-    if (has_lo) f4lo += rho * (u * V1 + v * U1 + FW1) / den_lo;
-    if (has_hi) f4up += rho * (u * V2 + v * U2 + FW2) / den_up;
+    if (has_lo) f4lo += rho * (u * V1 + v * U1 + FW1) * iden_lo;
+    if (has_hi) f4up += rho * (u * V2 + v * U2 + FW2) * iden_up;
   }
   __syncthreads();
   // ---- phase 4: F4 from the layer above (same element, same node) ----
@@ -632,18 +679,19 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     if (has_hi && l + 1 < nzt) f4up += T.at(0, node, l + 1);
     // DSS: the first element (in colour order) holding the node writes, the
     // others accumulate; the w lid (iz = Nz-1) receives no tendency
+    double *Fr = F, *Fu = F + vs, *Fv = F + 2 * vs, *Fw = F + 3 * vs, *Ft = F + 4 * vs;
     if (first) {
-      F[ui] = f1;
-      F[ui + vs] = f2;
-      F[ui + 2 * vs] = f3;
-      F[ui + 3 * vs] = f4up;
-      F[ui + 4 * vs] = f5;
+      at(Fr, ui) = f1;
+      at(Fu, ui) = f2;
+      at(Fv, ui) = f3;
+      at(Fw, ui) = f4up;
+      at(Ft, ui) = f5;
     } else {
-      F[ui] += f1;
-      F[ui + vs] += f2;
-      F[ui + 2 * vs] += f3;
-      if (has_hi) F[ui + 3 * vs] += f4up;
-      F[ui + 4 * vs] += f5;
+      at(Fr, ui) += f1;
+      at(Fu, ui) += f2;
+      at(Fv, ui) += f3;
+      if (has_hi) at(Fw, ui) += f4up;
+      at(Ft, ui) += f5;
     }
     // first level of a chunk above the first: its contribution to the
     // interface below (the previous chunk's last level) goes to F4x and is
@@ -1062,8 +1110,9 @@ __global__ void k_relayout_metric(const double *__restrict__ dXdxI, const double
   }
 }
 
-__global__ void k_msum(const double *__restrict__ M, double *__restrict__ Ms, size_t n) {
+// 1 / (M1 + M2): the DSS weight every element kernel multiplies by
+__global__ void k_msum(const double *__restrict__ M, double *__restrict__ iMs, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    Ms[i] = M[i] + M[i + n];
+    iMs[i] = 1.0 / (M[i] + M[i + n]);
 }
 

@@ -1,0 +1,111 @@
+"""Short text summaries of rocprofv3 SQLite results (``run_results.db``).
+
+    python cgdycore.jl_amd/rocprof_summary.py kernels DB --command CMD --out FILE
+    python cgdycore.jl_amd/rocprof_summary.py traffic --kernel k_fcn2 --per-eval 7 \
+        --fetch DB --write DB --cal-fetch DB --cal-write DB --copy-bytes B \
+        --workload W --algorithmic BYTES --command-fetch CMD --command-write CMD --out FILE.json
+
+``kernels`` tabulates calls, total and average duration per kernel from a
+``--kernel-trace --stats`` run.  ``traffic`` turns the per-dispatch
+FETCH_SIZE / WRITE_SIZE counters (KiB) of two ``--pmc`` runs into HBM bytes
+per tendency evaluation, each scaled by what the same counter reads for a
+plain device copy of known size (the calibration runs).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sqlite3
+import statistics
+
+
+def _rows(db, sql):
+    con = sqlite3.connect(db)
+    try:
+        return con.execute(sql).fetchall()
+    finally:
+        con.close()
+
+
+def kernel_table(db):
+    """[(name, calls, total_us, avg_us, pct)] sorted by total time."""
+    rows = _rows(db, "select name, total_calls, total_duration, average, percentage from top_kernels")
+    # the top_kernels view reports microseconds
+    return [(n, int(c), float(t), float(a), p) for n, c, t, a, p in rows]
+
+
+def short_name(name, width=60):
+    n = name.split("(")[0].replace("void ", "")
+    return n if len(n) <= width else n[: width - 3] + "..."
+
+
+def write_kernels(db, command, out):
+    tab = kernel_table(db)
+    lines = ["# rocprofv3 kernel trace summary", f"# command: {command}", "",
+             f"{'kernel':62s} {'calls':>6s} {'total_us':>11s} {'avg_us':>9s} {'pct':>6s}"]
+    for n, c, t, a, p in tab:
+        lines.append(f"{short_name(n):62s} {c:6d} {t:11.1f} {a:9.2f} {p:6.2f}")
+    open(out, "w").write("\n".join(lines) + "\n")
+    return tab
+
+
+def counter_per_dispatch(db, counter, kernel_substr):
+    rows = _rows(db, "select kernel_name, value from counters_collection where counter_name = '%s'" % counter)
+    return [v for n, v in rows if kernel_substr in n]
+
+
+def copy_factor(db, counter, copy_bytes):
+    """Counter KiB of the largest copy dispatches / the bytes the copy moves one way."""
+    vals = counter_per_dispatch(db, counter, "copyBuffer")
+    if not vals:
+        raise SystemExit(f"no copy kernel dispatches in {db}")
+    top = sorted(vals)[-3:]
+    return statistics.median(top) * 1024.0 / copy_bytes
+
+
+def write_traffic(a):
+    ff = copy_factor(a.cal_fetch, "FETCH_SIZE", a.copy_bytes)
+    wf = copy_factor(a.cal_write, "WRITE_SIZE", a.copy_bytes)
+    fv = counter_per_dispatch(a.fetch, "FETCH_SIZE", a.kernel)
+    wv = counter_per_dispatch(a.write, "WRITE_SIZE", a.kernel)
+    if not fv or not wv:
+        raise SystemExit(f"no dispatches of {a.kernel}")
+    ev_f = len(fv) / a.per_eval
+    ev_w = len(wv) / a.per_eval
+    fetch = sum(fv) * 1024.0 / ff / ev_f
+    write = sum(wv) * 1024.0 / wf / ev_w
+    res = {"workload": a.workload, "kernel": a.kernel, "dispatches_per_eval": a.per_eval,
+           "evals_profiled": ev_f, "fetch_factor": round(ff, 4), "write_factor": round(wf, 4),
+           "calibration_copy_bytes": a.copy_bytes, "fetch_bytes_per_eval": round(fetch),
+           "write_bytes_per_eval": round(write), "traffic_bytes_per_eval": round(fetch + write),
+           "algorithmic_bytes_per_eval": a.algorithmic,
+           "traffic_over_algorithmic": round((fetch + write) / a.algorithmic, 4) if a.algorithmic else None,
+           "command_fetch": a.command_fetch, "command_write": a.command_write}
+    open(a.out, "w").write(json.dumps(res, indent=1) + "\n")
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    k = sub.add_parser("kernels")
+    k.add_argument("db")
+    k.add_argument("--command", required=True)
+    k.add_argument("--out", required=True)
+    t = sub.add_parser("traffic")
+    for f in ("fetch", "write", "cal_fetch", "cal_write", "kernel", "workload", "out", "command_fetch",
+              "command_write"):
+        t.add_argument("--" + f.replace("_", "-"), dest=f, required=True)
+    t.add_argument("--copy-bytes", type=int, required=True)
+    t.add_argument("--per-eval", type=int, required=True)
+    t.add_argument("--algorithmic", type=int, default=0)
+    a = ap.parse_args(argv)
+    if a.cmd == "kernels":
+        for row in write_kernels(a.db, a.command, a.out)[:8]:
+            print(f"{short_name(row[0]):62s} {row[1]:6d} {row[3]:9.2f}")
+    else:
+        print(json.dumps(write_traffic(a)))
+
+
+if __name__ == "__main__":
+    main()
