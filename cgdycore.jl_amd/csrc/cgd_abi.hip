@@ -58,10 +58,9 @@ static int grid_for(size_t n, int block) {
   return (int)std::min<size_t>(std::max<size_t>(g, 1), 65536);
 }
 
-// Greedy colouring in element order: each element takes the smallest colour
-// not used by an element sharing one of its nodes.  Returns the number of
-// colours, or -1 if more than 64 would be needed.
-static int color_elements(const std::vector<int> &g32, int NN, int NF, int NumG, std::vector<int> &color) {
+// Element adjacency: elements sharing at least one global node.
+static void element_adjacency(const std::vector<int> &g32, int NN, int NF, int NumG, std::vector<int> &aptr,
+                              std::vector<int> &adj) {
   std::vector<int> nptr(NumG + 1, 0);
   for (int v : g32) nptr[v + 1]++;
   for (int i = 0; i < NumG; ++i) nptr[i + 1] += nptr[i];
@@ -71,17 +70,34 @@ static int color_elements(const std::vector<int> &g32, int NN, int NF, int NumG,
     for (int e = 0; e < NF; ++e)
       for (int i = 0; i < NN; ++i) nel[fillp[g32[(size_t)e * NN + i]]++] = e;
   }
-  color.assign(NF, -1);
-  int ncol = 0;
-  std::vector<int> mark(64, -1);
+  aptr.assign(NF + 1, 0);
+  adj.clear();
+  std::vector<int> mark(NF, -1);
   for (int e = 0; e < NF; ++e) {
     for (int i = 0; i < NN; ++i) {
       const int gn = g32[(size_t)e * NN + i];
       for (int q = nptr[gn]; q < nptr[gn + 1]; ++q) {
         const int o = nel[q];
-        if (o != e && color[o] >= 0) mark[color[o]] = e;
+        if (o != e && mark[o] != e) {
+          mark[o] = e;
+          adj.push_back(o);
+        }
       }
     }
+    aptr[e + 1] = (int)adj.size();
+  }
+}
+
+// Greedy colouring in element order: each element takes the smallest colour
+// not used by an adjacent element.  Returns the number of colours, or -1 if
+// more than 64 would be needed.
+static int color_greedy(int NF, const std::vector<int> &aptr, const std::vector<int> &adj, std::vector<int> &color) {
+  color.assign(NF, -1);
+  int ncol = 0;
+  std::vector<int> mark(64, -1);
+  for (int e = 0; e < NF; ++e) {
+    for (int q = aptr[e]; q < aptr[e + 1]; ++q)
+      if (color[adj[q]] >= 0) mark[color[adj[q]]] = e;
     int col = 0;
     while (col < 64 && mark[col] == e) ++col;
     if (col >= 64) return -1;
@@ -89,6 +105,96 @@ static int color_elements(const std::vector<int> &g32, int NN, int NF, int NumG,
     ncol = std::max(ncol, col + 1);
   }
   return ncol;
+}
+
+// Four colours by depth-first search with forward checking: the element with
+// the fewest admissible colours is coloured next (ties: lowest index), its
+// colour is removed from its neighbours' domains, and an emptied domain
+// backtracks.  Four colours are the minimum (four elements meet at a vertex
+// away from the cube corners); on the cubed sphere such colourings exist and
+// are found with almost no backtracking, giving four launches of equal size
+// instead of a greedy colouring's long tail of small colours.  Returns false
+// when the step budget runs out.
+static bool color_four(int NF, const std::vector<int> &aptr, const std::vector<int> &adj, std::vector<int> &color) {
+  const long long budget = 64LL * NF + 100000;
+  std::vector<unsigned char> dom(NF, 15);
+  color.assign(NF, -1);
+  struct Frame {
+    int e;
+    unsigned char rem;
+    size_t trail;
+  };
+  std::vector<Frame> stack;
+  std::vector<std::pair<int, unsigned char>> trail;
+  // candidates by domain size: a lazy min-structure over (popcount, index)
+  std::vector<int> forced;  // elements whose domain shrank to one colour
+  int scan = 0;             // all elements below scan are coloured or queued
+  auto popc = [](unsigned v) { return __builtin_popcount(v); };
+  auto pick = [&]() -> int {
+    while (!forced.empty()) {
+      const int e = forced.back();
+      forced.pop_back();
+      if (color[e] < 0) return e;
+    }
+    int best = -1, bs = 99;
+    while (scan < NF && color[scan] >= 0) ++scan;
+    for (int e = scan; e < NF; ++e)
+      if (color[e] < 0 && popc(dom[e]) < bs) {
+        bs = popc(dom[e]);
+        best = e;
+        if (bs <= 2) break;
+      }
+    return best;
+  };
+  long long steps = 0;
+  for (;;) {
+    const int e0 = pick();
+    if (e0 < 0) return true;
+    if (++steps > budget) return false;
+    stack.push_back({e0, dom[e0], trail.size()});
+    bool placed = false;
+    while (!stack.empty()) {
+      Frame &f = stack.back();
+      while (trail.size() > f.trail) {
+        dom[trail.back().first] = trail.back().second;
+        trail.pop_back();
+      }
+      color[f.e] = -1;
+      if (f.rem == 0) {
+        stack.pop_back();
+        forced.clear();
+        scan = 0;
+        continue;
+      }
+      const int c = __builtin_ctz(f.rem);
+      f.rem &= (unsigned char)~(1u << c);
+      color[f.e] = c;
+      bool fail = false;
+      for (int q = aptr[f.e]; q < aptr[f.e + 1] && !fail; ++q) {
+        const int b = adj[q];
+        if (color[b] < 0 && (dom[b] >> c & 1)) {
+          trail.push_back({b, dom[b]});
+          dom[b] &= (unsigned char)~(1u << c);
+          if (dom[b] == 0) fail = true;
+          else if (popc(dom[b]) == 1) forced.push_back(b);
+        }
+      }
+      if (!fail) {
+        placed = true;
+        break;
+      }
+      if (++steps > budget) return false;
+    }
+    if (!placed) return false;
+  }
+}
+
+// DSS schedule: four balanced colours when the search succeeds, else greedy.
+static int color_elements(const std::vector<int> &g32, int NN, int NF, int NumG, std::vector<int> &color) {
+  std::vector<int> aptr, adj;
+  element_adjacency(g32, NN, NF, NumG, aptr, adj);
+  if (color_four(NF, aptr, adj, color)) return 4;
+  return color_greedy(NF, aptr, adj, color);
 }
 
 extern "C" int cgd_color_elements(const int64_t *glob, int NN, int NF, int NumG, int *color, int *ncolor) {

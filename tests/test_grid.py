@@ -67,10 +67,11 @@ def test_element_corners_counterclockwise(pkg):
     assert np.all(np.einsum("ij,ij->i", n, mid) > 0)
 
 
-@pytest.mark.parametrize("ne", [2, 6, 30])
+@pytest.mark.parametrize("ne", [2, 3, 6, 7, 30])
 def test_colouring_schedules_dss(pkg, ne):
     """Elements of one colour share no global node (the DSS writes of a colour
-    launch never collide) and every element has a colour."""
+    launch never collide), every element has a colour, and the schedule uses
+    the minimum of four colours of (nearly) equal size."""
     host = pkg.host
     try:
         host.native()
@@ -79,7 +80,9 @@ def test_colouring_schedules_dss(pkg, ne):
     g = pkg.grid.cubed_sphere(ne)
     glob, num_g = pkg.grid.numbering_cg_quad(g, 4)
     col, nc = host.color_elements(glob, num_g)
-    assert col.min() >= 0 and col.max() == nc - 1 and nc <= 8
+    assert col.min() >= 0 and col.max() == nc - 1 and nc == 4
+    sizes = np.bincount(col)
+    assert sizes.max() - sizes.min() <= 2
     for c in range(nc):
         nodes = glob[col == c].ravel()
         assert len(np.unique(nodes)) == nodes.size, f"colour {c} shares a node"
