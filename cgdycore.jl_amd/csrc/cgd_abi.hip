@@ -21,17 +21,16 @@ struct cgd_ctx {
   int thomas = 0;  // column solve: 0 = cyclic reduction, 1 = serial Thomas (CGD_THOMAS=1)
   size_t lds1 = 0, lds2 = 0;
   int ncolor = 0;
-  std::vector<int> color_off;  // host offsets into d_elist per colour
+  std::vector<int> color_off;  // first element slot of each colour (elements are stored in colour order)
   int *d_glob = nullptr;       // [NF][NN] 0-based
-  int *d_elist = nullptr;      // elements grouped by colour
+  int *d_slot = nullptr;       // element -> slot (colour order), set-up only
   double *d_met = nullptr, *d_met1 = nullptr, *d_ms = nullptr, *d_bwork = nullptr;
   double *d_f4x = nullptr;     // [NF][NN][nchunk] chunk-bottom F4 terms
   int *d_nptr = nullptr, *d_ncopy = nullptr;  // global node -> element copies (colour order)
-  unsigned char *d_first = nullptr;           // [NF][NN] first copy of its node in colour order
   bool ok = false;
   std::string err;
   // per-pass timing (events on the launch stream)
-  bool prof = false;
+  int prof = 0;  // bit w: time pass w (0 = pass 1, 1 = pass 2, 2 = column)
   double prof_ms[3] = {0, 0, 0};
   long long prof_n[3] = {0, 0, 0};
   std::vector<hipEvent_t> prof_ev;
@@ -244,15 +243,14 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   c->NF = d->NF;
   c->NumG = d->NumG;
   const int NN = c->NN, Nz = c->Nz, NF = c->NF;
-  // level chunk height (a template constant, 8 or 16 levels: a chunk tiles
-  // the 16-lane DPP rows).  The fused tendency kernel needs ~250 VGPRs, so
-  // 8-level chunks (4 waves per workgroup) keep two workgroups per CU; this
-  // measured faster than 16 at ne=30, nz=64.  CGD_NZT=16 overrides.
+  // level chunk height NZT (a template constant): a workgroup is one
+  // element x NZT levels, N*N*NZT lanes rounded up to whole wavefronts.
+  // CGD_NZT overrides the default (5, 8, 10 or 16 at N = 5; 8 or 16 at N = 4).
   {
     int nzt = 8;
     if (const char *ev = getenv("CGD_NZT")) {
       const int o = atoi(ev);
-      if (o == 8 || o == 16) nzt = o;
+      if (o == 8 || o == 16 || (c->N == 5 && (o == 5 || o == 10))) nzt = o;
     }
     c->nzt = nzt;
     c->nchunk = (Nz + nzt - 1) / nzt;
@@ -287,7 +285,10 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
     std::vector<int> fp(c->color_off.begin(), c->color_off.end() - 1);
     for (int e = 0; e < NF; ++e) elist[fp[color[e]]++] = e;
   }
-  // global node -> element copies (e*NN + node) in colour order
+  // elements are stored in colour order: slot q holds element elist[q]
+  std::vector<int> slot(NF);
+  for (int q = 0; q < NF; ++q) slot[elist[q]] = q;
+  // global node -> element copies (slot*NN + node) in colour order
   std::vector<int> cptr(d->NumG + 1, 0), ccopy((size_t)NF * NN);
   for (int v : g32) cptr[v + 1]++;
   for (int i = 0; i < d->NumG; ++i) cptr[i + 1] += cptr[i];
@@ -295,31 +296,32 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
     std::vector<int> fp(cptr.begin(), cptr.end() - 1);
     for (int q = 0; q < NF; ++q) {
       const int e = elist[q];
-      for (int i = 0; i < NN; ++i) ccopy[fp[g32[(size_t)e * NN + i]]++] = e * NN + i;
+      for (int i = 0; i < NN; ++i) ccopy[fp[g32[(size_t)e * NN + i]]++] = q * NN + i;
     }
   }
-  std::vector<unsigned char> firstc((size_t)NF * NN, 0);
+  // numbering by slot, bit 31 marking the first copy of each node
+  std::vector<int> gslot((size_t)NF * NN);
+  for (int q = 0; q < NF; ++q)
+    for (int i = 0; i < NN; ++i) gslot[(size_t)q * NN + i] = g32[(size_t)elist[q] * NN + i];
   for (int gn = 0; gn < d->NumG; ++gn)
-    if (cptr[gn + 1] > cptr[gn]) firstc[ccopy[cptr[gn]]] = 1;
+    if (cptr[gn + 1] > cptr[gn]) gslot[ccopy[cptr[gn]]] |= (int)0x80000000u;
   // device buffers (set-up only; nothing is allocated on the step path)
   const size_t npt = (size_t)NF * NN * Nz;
-  HIPCHK(c, hipMalloc(&c->d_glob, g32.size() * sizeof(int)));
-  HIPCHK(c, hipMalloc(&c->d_elist, (size_t)NF * sizeof(int)));
+  HIPCHK(c, hipMalloc(&c->d_glob, gslot.size() * sizeof(int)));
+  HIPCHK(c, hipMalloc(&c->d_slot, (size_t)NF * sizeof(int)));
   HIPCHK(c, hipMalloc(&c->d_met, npt * CGD_NMET * sizeof(double)));
   HIPCHK(c, hipMalloc(&c->d_met1, npt * CGD_NMET1 * sizeof(double)));
   HIPCHK(c, hipMalloc(&c->d_ms, (size_t)d->NumG * Nz * sizeof(double)));
   HIPCHK(c, hipMalloc(&c->d_bwork, (size_t)d->NumG * (Nz - 1) * sizeof(double)));
   HIPCHK(c, hipMalloc(&c->d_f4x, (size_t)NF * NN * c->nchunk * sizeof(double)));
-  HIPCHK(c, hipMalloc(&c->d_first, firstc.size()));
-  HIPCHK(c, hipMemcpyAsync(c->d_first, firstc.data(), firstc.size(), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMalloc(&c->d_nptr, cptr.size() * sizeof(int)));
   HIPCHK(c, hipMalloc(&c->d_ncopy, ccopy.size() * sizeof(int)));
   HIPCHK(c, hipMemcpyAsync(c->d_nptr, cptr.data(), cptr.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_ncopy, ccopy.data(), ccopy.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipMemcpyAsync(c->d_glob, g32.data(), g32.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  HIPCHK(c, hipMemcpyAsync(c->d_elist, elist.data(), elist.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_relayout_metric, dim3(grid_for(npt, 256)), dim3(256), 0, st, d->dXdxI, d->J, d->X, c->d_met,
-                     c->d_met1, NN, Nz, NF, d->Omega);
+  HIPCHK(c, hipMemcpyAsync(c->d_glob, gslot.data(), gslot.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  HIPCHK(c, hipMemcpyAsync(c->d_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_relayout_metric, dim3(grid_for(npt, 256)), dim3(256), 0, st, d->dXdxI, d->J, d->X, c->d_slot,
+                     c->d_met, c->d_met1, NN, Nz, NF, d->Omega);
   HIPCHK(c, hipGetLastError());
   const size_t ng = (size_t)d->NumG * Nz;
   hipLaunchKernelGGL(k_msum, dim3(grid_for(ng, 256)), dim3(256), 0, st, d->M, c->d_ms, ng);
@@ -350,7 +352,6 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   a.met1 = c->d_met1;
   a.iMs = c->d_ms;
   a.M = d->M;
-  a.first = c->d_first;
   c->ok = true;
   return CGD_OK;
 }
@@ -361,8 +362,8 @@ extern "C" int cgd_ctx_destroy(cgd_ctx *c) {
   if (!c) return CGD_ERR_ARG;
   // a context that failed validation holds no device memory: tear it down
   // without touching the HIP runtime
-  for (void *p : {(void *)c->d_glob, (void *)c->d_elist, (void *)c->d_met, (void *)c->d_met1, (void *)c->d_ms,
-                  (void *)c->d_bwork, (void *)c->d_f4x, (void *)c->d_nptr, (void *)c->d_first, (void *)c->d_ncopy})
+  for (void *p : {(void *)c->d_glob, (void *)c->d_slot, (void *)c->d_met, (void *)c->d_met1, (void *)c->d_ms,
+                  (void *)c->d_bwork, (void *)c->d_f4x, (void *)c->d_nptr, (void *)c->d_ncopy})
     if (p) (void)hipFree(p);
   prof_free(c);
   delete c;
@@ -378,12 +379,12 @@ static int check(cgd_ctx *c) {
 // Timing of one pass (a group of launches): an event pair is recorded on the
 // launch stream; nothing synchronises until cgd_profile_read.  The events
 // are created by cgd_profile_enable (never on the step path).
-static void prof_begin(cgd_ctx *c, hipStream_t st) {
-  if (!c->prof || c->prof_used >= c->prof_cap) return;
+static void prof_begin(cgd_ctx *c, hipStream_t st, int which) {
+  if (!(c->prof >> which & 1) || c->prof_used >= c->prof_cap) return;
   (void)hipEventRecord(c->prof_ev[2 * c->prof_used], st);
 }
 static int prof_end(cgd_ctx *c, hipStream_t st, int which, int launches) {
-  if (!c->prof || c->prof_used >= c->prof_cap) return CGD_OK;
+  if (!(c->prof >> which & 1) || c->prof_used >= c->prof_cap) return CGD_OK;
   HIPCHK(c, hipEventRecord(c->prof_ev[2 * c->prof_used + 1], st));
   c->prof_which.push_back(which);
   c->prof_launch.push_back(launches);
@@ -404,7 +405,7 @@ extern "C" int cgd_profile_enable(cgd_ctx *c, int on) {
   if (r) return r;
   prof_free(c);
   for (int i = 0; i < 3; ++i) c->prof_ms[i] = 0, c->prof_n[i] = 0;
-  c->prof = on != 0;
+  c->prof = on & 7;
   if (c->prof) {
     const int cap = 4096;
     c->prof_ev.resize(2 * cap);
@@ -437,35 +438,37 @@ extern "C" int cgd_profile_read(cgd_ctx *c, int which, double *total_ms, long lo
 
 // template dispatch on (N, chunk height)
 template <int N, int NZT>
-static void fcn1_t(cgd_ctx *c, dim3 g, hipStream_t st, const int *el, const double *U, double *T1) {
-  hipLaunchKernelGGL((k_fcn1<N, NZT>), g, dim3(c->block), c->lds1, st, c->ka, el, c->nchunk, U, T1);
+static void fcn1_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, double *T1) {
+  hipLaunchKernelGGL((k_fcn1<N, NZT>), g, dim3(c->block), c->lds1, st, c->ka, slot0, c->nchunk, U, T1);
 }
 template <int N, int NZT>
-static void fcn2_t(cgd_ctx *c, dim3 g, hipStream_t st, const int *el, const double *U, const double *T1,
+static void fcn2_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, const double *T1,
                    const double *P, double *F) {
-  hipLaunchKernelGGL((k_fcn2<N, NZT>), g, dim3(c->block), c->lds2, st, c->ka, el, c->nchunk, U, T1, P, F,
+  hipLaunchKernelGGL((k_fcn2<N, NZT>), g, dim3(c->block), c->lds2, st, c->ka, slot0, c->nchunk, U, T1, P, F,
                      c->d_f4x);
 }
 #define CGD_DISPATCH(FN, ...)                    \
   do {                                           \
     if (c->N == 5) {                             \
-      if (c->nzt == 8)                           \
-        FN<5, 8>(c, __VA_ARGS__);                \
-      else                                       \
-        FN<5, 16>(c, __VA_ARGS__);               \
+      switch (c->nzt) {                          \
+        case 5: FN<5, 5>(c, __VA_ARGS__); break;   \
+        case 10: FN<5, 10>(c, __VA_ARGS__); break; \
+        case 16: FN<5, 16>(c, __VA_ARGS__); break; \
+        default: FN<5, 8>(c, __VA_ARGS__);         \
+      }                                          \
     } else {                                     \
-      if (c->nzt == 8)                           \
-        FN<4, 8>(c, __VA_ARGS__);                \
-      else                                       \
+      if (c->nzt == 16)                          \
         FN<4, 16>(c, __VA_ARGS__);               \
+      else                                       \
+        FN<4, 8>(c, __VA_ARGS__);                \
     }                                            \
   } while (0)
-static void launch_fcn1(cgd_ctx *c, dim3 g, hipStream_t st, const int *el, const double *U, double *T1) {
-  CGD_DISPATCH(fcn1_t, g, st, el, U, T1);
+static void launch_fcn1(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, double *T1) {
+  CGD_DISPATCH(fcn1_t, g, st, slot0, U, T1);
 }
-static void launch_fcn2(cgd_ctx *c, dim3 g, hipStream_t st, const int *el, const double *U, const double *T1,
+static void launch_fcn2(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, const double *T1,
                         const double *P, double *F) {
-  CGD_DISPATCH(fcn2_t, g, st, el, U, T1, P, F);
+  CGD_DISPATCH(fcn2_t, g, st, slot0, U, T1, P, F);
 }
 
 extern "C" int cgd_prepare(cgd_ctx *c, const double *U, double *AuxG, void *stream) {
@@ -478,35 +481,34 @@ extern "C" int cgd_prepare(cgd_ctx *c, const double *U, double *AuxG, void *stre
   return CGD_OK;
 }
 
-extern "C" int cgd_fcn(cgd_ctx *c, double *F, const double *U, double *Temp1, const double *AuxG, void *stream) {
-  int r = check(c);
-  if (r) return r;
-  if (!F || !U || !Temp1 || !AuxG) return fail(c, CGD_ERR_ARG, "cgd_fcn: null array");
-  hipStream_t st = S(stream);
+// Both tendency passes; with f4fix the chunk-interface w terms are added by
+// k_f4fix (cgd_fcn), without it the caller consumes the side sums itself
+// (the fused stage kernel of cgd_rw_step).
+static int run_fcn(cgd_ctx *c, double *F, const double *U, double *Temp1, const double *AuxG, hipStream_t st,
+                   bool f4fix) {
+  int r;
   // every global node is written by its first element (colour order) and
   // accumulated by the others, so Temp1 and F need no zero fill
-  prof_begin(c, st);
+  prof_begin(c, st, 0);
   int launches = 0;
   for (int k = 0; k < c->ncolor; ++k) {
     const int ne = c->color_off[k + 1] - c->color_off[k];
     if (!ne) continue;
-    const int *el = c->d_elist + c->color_off[k];
-    launch_fcn1(c, dim3(ne * c->nchunk), st, el, U, Temp1);
+    launch_fcn1(c, dim3(ne * c->nchunk), st, c->color_off[k], U, Temp1);
     HIPCHK(c, hipGetLastError());
     ++launches;
   }
   if ((r = prof_end(c, st, 0, launches))) return r;
-  prof_begin(c, st);
+  prof_begin(c, st, 1);
   launches = 0;
   for (int k = 0; k < c->ncolor; ++k) {
     const int ne = c->color_off[k + 1] - c->color_off[k];
     if (!ne) continue;
-    const int *el = c->d_elist + c->color_off[k];
-    launch_fcn2(c, dim3(ne * c->nchunk), st, el, U, Temp1, AuxG, F);
+    launch_fcn2(c, dim3(ne * c->nchunk), st, c->color_off[k], U, Temp1, AuxG, F);
     HIPCHK(c, hipGetLastError());
     ++launches;
   }
-  if (c->nchunk > 1) {
+  if (f4fix && c->nchunk > 1) {
     const size_t nfix = (size_t)c->NumG * (c->nchunk - 1);
     hipLaunchKernelGGL(k_f4fix, dim3(grid_for(nfix, 256)), dim3(256), 0, st, c->d_nptr, c->d_ncopy, c->d_f4x, F,
                        c->NumG, c->Nz, c->nchunk, c->nzt);
@@ -514,6 +516,13 @@ extern "C" int cgd_fcn(cgd_ctx *c, double *F, const double *U, double *Temp1, co
     ++launches;
   }
   return prof_end(c, st, 1, launches);
+}
+
+extern "C" int cgd_fcn(cgd_ctx *c, double *F, const double *U, double *Temp1, const double *AuxG, void *stream) {
+  int r = check(c);
+  if (r) return r;
+  if (!F || !U || !Temp1 || !AuxG) return fail(c, CGD_ERR_ARG, "cgd_fcn: null array");
+  return run_fcn(c, F, U, Temp1, AuxG, S(stream), true);
 }
 
 extern "C" int cgd_jac(cgd_ctx *c, double *JRhoW, double *JWRho, double *JWRhoTh, double *JRhoThW, const double *U,
@@ -537,7 +546,7 @@ extern "C" int cgd_schur(cgd_ctx *c, double *k, double *v, const double *JRhoW, 
   if (!k || !v || !JRhoW || !JWRho || !JWRhoTh || !JRhoThW || !tri) return fail(c, CGD_ERR_ARG, "cgd_schur: null array");
   if (!(fac > 0.0)) return fail(c, CGD_ERR_ARG, "cgd_schur: fac must be positive");
   hipStream_t st = S(stream);
-  prof_begin(c, st);
+  prof_begin(c, st, 2);
   if (c->Nz <= 64)
     hipLaunchKernelGGL(k_schur_col, dim3((c->NumG + 3) / 4), dim3(256), 0, st, c->ka, k, v, JRhoW, JWRho, JWRhoTh,
                        JRhoThW, tri, fac, comp_tri, c->thomas);
@@ -562,47 +571,64 @@ extern "C" int cgd_rw_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *
   const double m3 = 2.0 / 3.0, m2 = 1.0 / 6.0 - (-0.75) * m3, m1 = 1.0 / 6.0 - (-0.75) * m3;
   const double Mw[3] = {m1, m2, m3};
   const double gamma = 1.0;
-  const size_t n5 = 5 * (size_t)c->NumG * c->Nz;
+  const size_t ng = (size_t)c->NumG * c->Nz, n5 = 5 * ng;
   double *k[3] = {w->k, w->k + n5, w->k + 2 * n5};
+  if (c->Nz <= 64) {
+    // fused: Vn = V with the first pressure, then per stage the two
+    // tendency passes and one column kernel (Jacobian, tri, stage terms,
+    // Schur solve, next state and its pressure)
+    hipLaunchKernelGGL(k_rw_begin, dim3(grid_for(ng, 256)), dim3(256), 0, st, c->ka, V, w->Vn, w->AuxG, ng);
+    HIPCHK(c, hipGetLastError());
+    for (int is = 0; is < 3; ++is) {
+      if ((r = run_fcn(c, w->fV, V, w->Temp1, w->AuxG, st, false))) return r;
+      RWStage rw{};
+      rw.fV = w->fV;
+      rw.stage = is;
+      for (int j = 0; j < is; ++j) {
+        rw.kprev[j] = k[j];
+        rw.cf[j] = Cc[is][j] / dt;
+      }
+      for (int j = 0; j <= is; ++j) rw.av[j] = (is < 2) ? A[is + 1][j] : Mw[j];
+      rw.Vn = w->Vn;
+      rw.dz = c->desc.dz;
+      rw.V = V;
+      rw.k = is < 2 ? k[is] : nullptr;  // k_3 is consumed in registers
+      rw.P = is < 2 ? w->AuxG : nullptr;
+      rw.F4x = c->d_f4x;
+      rw.nptr = c->d_nptr;
+      rw.ncopy = c->d_ncopy;
+      rw.nchunk = c->nchunk;
+      rw.nzt = c->nzt;
+      rw.fac = dt * gamma;
+      rw.thomas = c->thomas;
+      prof_begin(c, st, 2);
+      hipLaunchKernelGGL(k_rw_stage, dim3((c->NumG + 3) / 4), dim3(256), 0, st, c->ka, rw);
+      HIPCHK(c, hipGetLastError());
+      if ((r = prof_end(c, st, 2, 1))) return r;
+    }
+    return CGD_OK;
+  }
+  // tall columns: the reference's sequence of calls, stage terms by k_axpy
   HIPCHK(c, hipMemcpyAsync(w->Vn, V, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
   for (int is = 0; is < 3; ++is) {
     if ((r = cgd_prepare(c, V, w->AuxG, stream))) return r;
     if ((r = cgd_fcn(c, w->fV, V, w->Temp1, w->AuxG, stream))) return r;
     if (is == 0 && (r = cgd_jac(c, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, V, w->AuxG, stream))) return r;
-    RWStage rw{};
-    rw.stage = is;
-    for (int j = 0; j < is; ++j) {
-      rw.kprev[j] = k[j];
-      rw.cf[j] = Cc[is][j] / dt;
-    }
-    rw.Vn = w->Vn;
-    rw.V = V;
-    for (int j = 0; j <= is; ++j) rw.av[j] = (is < 2) ? A[is + 1][j] : Mw[j];
-    prof_begin(c, st);
-    if (c->Nz <= 64) {
-      hipLaunchKernelGGL(k_rw_col, dim3((c->NumG + 3) / 4), dim3(256), 0, st, c->ka, k[is], w->fV, w->JRhoW,
-                         w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma, is == 0 ? 1 : 0, c->thomas, rw);
-      HIPCHK(c, hipGetLastError());
-      if ((r = prof_end(c, st, 2, 1))) return r;
-    } else {
-      // tall columns: stage terms with a separate pass, then the drop-in solve
-      if ((r = prof_end(c, st, 2, 0))) return r;
-      const int gb = grid_for(n5, 256);
-      if (is > 0) {
-        hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, w->fV, w->fV, k[0], Cc[is][0] / dt, k[1],
-                           Cc[is][1] / dt, is, n5);
-        HIPCHK(c, hipGetLastError());
-      }
-      if ((r = cgd_schur(c, k[is], w->fV, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma, is == 0,
-                         stream)))
-        return r;
-      if (is < 2)
-        hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], A[is + 1][0], k[1], A[is + 1][1],
-                           is + 1, n5);
-      else
-        hipLaunchKernelGGL(k_axpy3, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], Mw[0], k[1], Mw[1], k[2], Mw[2], n5);
+    const int gb = grid_for(n5, 256);
+    if (is > 0) {
+      hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, w->fV, w->fV, k[0], Cc[is][0] / dt, k[1],
+                         Cc[is][1] / dt, is, n5);
       HIPCHK(c, hipGetLastError());
     }
+    if ((r = cgd_schur(c, k[is], w->fV, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma, is == 0,
+                       stream)))
+      return r;
+    if (is < 2)
+      hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], A[is + 1][0], k[1], A[is + 1][1],
+                         is + 1, n5);
+    else
+      hipLaunchKernelGGL(k_axpy3, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], Mw[0], k[1], Mw[1], k[2], Mw[2], n5);
+    HIPCHK(c, hipGetLastError());
   }
   return CGD_OK;
 }

@@ -31,12 +31,11 @@ struct KArgs {
   double Kc, Kg, Kd;
   double D[CGD_NMAX * CGD_NMAX];   // D[i*N + k] = DS[i,k]
   double DWm[CGD_NMAX * CGD_NMAX]; // DW[i*N + k]
-  const int *glob;                 // [NF][NN], 0-based
-  const double *met;               // [NF][17][NN][Nz]
-  const double *met1;              // [NF][5][NN][Nz]
+  const int *glob;                 // [slot][NN]: 0-based node | bit 31 set on the node's first copy
+  const double *met;               // [slot][17][NN][Nz]
+  const double *met1;              // [slot][5][NN][Nz]
   const double *iMs;               // [NumG][Nz]: 1 / (M1 + M2)
   const double *M;                 // [2][NumG][Nz]
-  const unsigned char *first;      // [NF][NN]: 1 if this copy is the node's first in colour order
 };
 
 // metric component indices of met (dXdxI[a,b,K] -> dabK, 1-based a,b,K)
@@ -203,7 +202,7 @@ template <int N> __device__ __forceinline__ void load_dmats(const KArgs &a, doub
 // Pass 1: HyperViscKernel! (src/GPU/ViscKernel.jl:1-84).
 // Grid: one workgroup per (element of the colour, level chunk).
 template <int N, int NZT>
-__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, const int *__restrict__ elist, int nchunk,
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, int slot0, int nchunk,
                                                const double *__restrict__ U, double *__restrict__ T1) {
   constexpr int NN = N * N;
   extern __shared__ double smem[];
@@ -213,7 +212,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
   const int Nz = a.Nz;
   constexpr int nzt = NZT;
   const int L = xcd_logical(blockIdx.x, gridDim.x);
-  const int e = elist[L / nchunk];
+  const int e = slot0 + L / nchunk;  // element slot (elements are stored in colour order)
   const int chunk = L - (L / nchunk) * nchunk;
   const int t = threadIdx.x;
   const int node = t / nzt, l = t - node * nzt;
@@ -229,9 +228,12 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
   double S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1, iMs = 1;
   bool first = false;
   unsigned ui = 0;
+  double o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0;
   double uC = 0, vC = 0, uD = 0, vD = 0, Th = 0;
   if (valid) {
-    ui = (unsigned)a.glob[e * NN + node] * (unsigned)Nz + iz;
+    const int gv = a.glob[e * NN + node];
+    first = gv < 0;
+    ui = (unsigned)(gv & 0x7fffffff) * (unsigned)Nz + iz;
     __builtin_assume(ui < (1u << 28));
     const unsigned mo = (unsigned)(node * Nz + iz);
     __builtin_assume(mo < (1u << 28));
@@ -241,7 +243,16 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
     S22 = at(metE + 3 * NNZ, mo);
     Js = at(metE + 4 * NNZ, mo);
     iMs = at(a.iMs, ui);
-    first = a.first[e * NN + node] != 0;
+    // the DSS accumulators of this node: no other element of the colour
+    // touches them, so they are read now, with the state, not after the
+    // derivative phases
+    if (!first) {
+      o1 = at(T1, ui);
+      o2 = at(T1 + vs, ui);
+      o3 = at(T1 + 2 * vs, ui);
+      o4 = at(T1 + 3 * vs, ui);
+      o5 = at(T1 + 4 * vs, ui);
+    }
     const double rho = at(U, ui), u = at(U + vs, ui), v = at(U + 2 * vs, ui), rth = at(U + 4 * vs, ui);
     uC = S11 * v - S12 * u;  // Curl12
     vC = S21 * v - S22 * u;
@@ -291,19 +302,11 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
     // the others accumulate
     const double r1 = -FuC * iMs, r2 = -FvC * iMs, r3 = FuD * iMs, r4 = FvD * iMs, r5 = DivTh * iMs;
     double *T1a = T1, *T1b = T1 + vs, *T1c = T1 + 2 * vs, *T1d = T1 + 3 * vs, *T1e = T1 + 4 * vs;
-    if (first) {
-      at(T1a, ui) = r1;
-      at(T1b, ui) = r2;
-      at(T1c, ui) = r3;
-      at(T1d, ui) = r4;
-      at(T1e, ui) = r5;
-    } else {
-      at(T1a, ui) += r1;
-      at(T1b, ui) += r2;
-      at(T1c, ui) += r3;
-      at(T1d, ui) += r4;
-      at(T1e, ui) += r5;
-    }
+    at(T1a, ui) = first ? r1 : o1 + r1;
+    at(T1b, ui) = first ? r2 : o2 + r2;
+    at(T1c, ui) = first ? r3 : o3 + r3;
+    at(T1d, ui) = first ? r4 : o4 + r4;
+    at(T1e, ui) = first ? r5 : o5 + r5;
   }
 }
 
@@ -323,12 +326,12 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
 // reference adds them from the owner of the interface); F4 of interface iz
 // combines this layer's upper terms with the layer above's lower terms.
 template <int N, int NZT>
-__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, const int *__restrict__ elist, int nchunk,
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, int slot0, int nchunk,
                                                const double *__restrict__ U, const double *__restrict__ C,
                                                const double *__restrict__ P, double *__restrict__ F,
                                                double *__restrict__ F4x) {
   constexpr int NN = N * N;
-  static_assert(NZT == 8 || NZT == 16, "a chunk must tile the 16-lane DPP rows");
+  static_assert(NZT >= 3 && NZT <= 16, "chunk heights 3..16 (the DPP neighbour reach is 2)");
   extern __shared__ double smem[];
   double *Ds = smem, *DWs = smem + NN;
   Tiles<N, NZT> T{smem + 2 * NN};
@@ -338,7 +341,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   const int L = xcd_logical(blockIdx.x, gridDim.x);
   const int eslot = L / nchunk;
   const int chunk = L - eslot * nchunk;
-  const int e = elist[eslot];
+  const int e = slot0 + eslot;  // element slot (elements are stored in colour order)
   const int t = threadIdx.x;
   const int node = t / nzt, l = t - node * nzt;
   const int iz = chunk * nzt + l;
@@ -357,8 +360,11 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   const double *Ur = U, *Uu = U + vs, *Uv = U + 2 * vs, *Uw = U + 3 * vs, *Ut = U + 4 * vs;
   const double *metE = a.met + (size_t)e * CGD_NMET * NNZ;
   unsigned ui = 0, mo = 0;
+  bool first = false;
   if (valid) {
-    ui = (unsigned)a.glob[e * NN + node] * (unsigned)Nz + iz;
+    const int gv = a.glob[e * NN + node];
+    first = gv < 0;
+    ui = (unsigned)(gv & 0x7fffffff) * (unsigned)Nz + iz;
     mo = (unsigned)(node * Nz + iz);
   }
   __builtin_assume(ui < (1u << 28));
@@ -387,7 +393,16 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   const double Js = J1 + J2;
   const double irho = rcp(rho);
   const double c_own = rth * irho;
-  const bool first = valid && a.first[e * NN + node] != 0;
+  // the DSS accumulators of this node (untouched by the other elements of
+  // the colour), read with the state rather than after the LDS phases
+  double o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0;
+  if (valid && !first) {
+    o1 = at(F, ui);
+    o2 = at(F + vs, ui);
+    o3 = at(F + 2 * vs, ui);
+    o4 = at(F + 3 * vs, ui);
+    o5 = at(F + 4 * vs, ui);
+  }
 
   // ---- vertical neighbours ----
   // A node's chunk is one 16-lane DPP row (or half of one for 8-level
@@ -680,19 +695,11 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     // DSS: the first element (in colour order) holding the node writes, the
     // others accumulate; the w lid (iz = Nz-1) receives no tendency
     double *Fr = F, *Fu = F + vs, *Fv = F + 2 * vs, *Fw = F + 3 * vs, *Ft = F + 4 * vs;
-    if (first) {
-      at(Fr, ui) = f1;
-      at(Fu, ui) = f2;
-      at(Fv, ui) = f3;
-      at(Fw, ui) = f4up;
-      at(Ft, ui) = f5;
-    } else {
-      at(Fr, ui) += f1;
-      at(Fu, ui) += f2;
-      at(Fv, ui) += f3;
-      if (has_hi) at(Fw, ui) += f4up;
-      at(Ft, ui) += f5;
-    }
+    at(Fr, ui) = first ? f1 : o1 + f1;
+    at(Fu, ui) = first ? f2 : o2 + f2;
+    at(Fv, ui) = first ? f3 : o3 + f3;
+    if (first || has_hi) at(Fw, ui) = first ? f4up : o4 + f4up;
+    at(Ft, ui) = first ? f5 : o5 + f5;
     // first level of a chunk above the first: its contribution to the
     // interface below (the previous chunk's last level) goes to F4x and is
     // added by k_f4fix after all colours
@@ -726,97 +733,17 @@ __device__ __forceinline__ double readlane_d(double x, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// SchurSolveGPU! for Nz <= 64: one wavefront per column, lane = level, so
-// every load and store is a coalesced 8*Nz-byte run.  The tri build
-// (TriDiagKernel!), the w right-hand side (SchurSolveFKernel!) and the
-// back-substitution of rho, rho*theta (SchurSolveBKernel!) are pointwise
-// with neighbour lanes reached by shuffles.  The tridiagonal system is
-// solved by parallel cyclic reduction (default) or, with thomas != 0, by
-// the Thomas recurrence of triSolve! run wave-serially in the reference's
-// operation order (one useful lane per step, about 10x slower).
-// Rosenbrock stage data for the fused column kernel of cgd_rw_step.
-struct RWStage {
-  const double *kprev[2];  // k_1, k_2 (stage increments already computed)
-  double cf[2];            // c_ij / dt added to the tendency
-  const double *Vn;        // state at the start of the step
-  double *V;               // next stage state (or final state)
-  double av[3];            // V = Vn + sum_j av[j] k_j, j <= stage
-  int stage;               // 0, 1, 2
-};
-
-template <bool FUSED>
-__device__ __forceinline__ void schur_col(const KArgs &a, double *__restrict__ k, double *__restrict__ v,
-                                          const double *__restrict__ JRhoW, const double *__restrict__ JWRho,
-                                          const double *__restrict__ JWRhoTh, const double *__restrict__ JRhoThW,
-                                          double *__restrict__ tri, double fac, int comp_tri, int thomas,
-                                          const RWStage &rw) {
-  const int Nz = a.Nz, n = Nz - 1;
-  const int i = threadIdx.x & 63;
-  const size_t ig = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (ig >= (size_t)a.NumG) return;  // uniform per wavefront
-  const size_t vs = (size_t)a.NumG * Nz;
-  const size_t c0 = ig * Nz;
-  const bool row = i < n, lev = i < Nz;
-  const size_t jb = (ig * (size_t)n + i) * 2, tb = (ig * (size_t)n + i) * 3;
-  double jrw0 = 0, jrw1 = 0, jwr0 = 0, jwr1 = 0, jwt0 = 0, jwt1 = 0, jtw0 = 0, jtw1 = 0;
-  if (row) {
-    jrw0 = JRhoW[jb];
-    jrw1 = JRhoW[jb + 1];
-    jwr0 = JWRho[jb];
-    jwr1 = JWRho[jb + 1];
-    jwt0 = JWRhoTh[jb];
-    jwt1 = JWRhoTh[jb + 1];
-    jtw0 = JRhoThW[jb];
-    jtw1 = JRhoThW[jb + 1];
-  }
-  double t0 = 0, t1 = 1, t2 = 0;  // tri[1], tri[2], tri[3] of row i
-  {
-    const double jwr0m = __shfl_up(jwr0, 1), jwt0m = __shfl_up(jwt0, 1);
-    const double jwr1p = __shfl_down(jwr1, 1), jwt1p = __shfl_down(jwt1, 1);
-    if (comp_tri) {
-      const double invfac = 1.0 / fac, invfac2 = invfac / fac;
-      if (row) {
-        t1 = invfac2 - jwr1 * jrw0 - jwr0 * jrw1 - jwt1 * jtw0 - jwt0 * jtw1;
-        t0 = (i == 0) ? 0.0 : -jwr0m * jrw0 - jwt0m * jtw0;
-        t2 = (i == n - 1) ? 0.0 : -jwr1p * jrw1 - jwt1p * jtw1;
-        tri[tb] = t0;
-        tri[tb + 1] = t1;
-        tri[tb + 2] = t2;
-      }
-    } else if (row) {
-      t0 = tri[tb];
-      t1 = tri[tb + 1];
-      t2 = tri[tb + 2];
-    }
-  }
-  double v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
-  if (lev) {
-    v1 = v[c0 + i];
-    v2 = v[c0 + i + vs];
-    v3 = v[c0 + i + 2 * vs];
-    v4 = v[c0 + i + 3 * vs];
-    v5 = v[c0 + i + 4 * vs];
-    if (FUSED) {
-      // fV = fV + (c_ij/dt) k_j (RosenbrockSchur.jl:23-26)
-      for (int j = 0; j < rw.stage; ++j) {
-        const double *kj = rw.kprev[j];
-        v1 = v1 + rw.cf[j] * kj[c0 + i];
-        v2 = v2 + rw.cf[j] * kj[c0 + i + vs];
-        v3 = v3 + rw.cf[j] * kj[c0 + i + 2 * vs];
-        v4 = v4 + rw.cf[j] * kj[c0 + i + 3 * vs];
-        v5 = v5 + rw.cf[j] * kj[c0 + i + 4 * vs];
-      }
-    }
-  }
-  const double v1p = __shfl_down(v1, 1), v5p = __shfl_down(v5, 1);
-  if (row) {
-    v4 = v4 / fac + jwr1 * v1 + jwr0 * v1p + jwt1 * v5 + jwt0 * v5p;
-    if (!FUSED) v[c0 + i + 3 * vs] = v4;
-  }
+// Solution of the Schur tridiagonal system of one column, one row per lane
+// (rows >= n are the identity).  Row i of the matrix is
+// A[i,i-1] x[i-1] + A[i,i] x[i] + A[i,i+1] x[i+1] = rhs with A[i,i-1] =
+// tri[3,i-1], A[i,i] = tri[2,i], A[i,i+1] = tri[1,i+1] (the column-oriented
+// storage of SchurSolve.jl); lane i holds t0, t1, t2 = tri[1..3, i].
+__device__ __forceinline__ double col_solve(double t0, double t1, double t2, double rhs, int i, int n, int thomas) {
+  const bool row = i < n;
   double sol = 0.0;
   if (thomas) {
     // triSolve!: forward elimination, wave-serial (row j broadcast by readlane)
-    double x = t1, b = v4;
+    double x = t1, b = rhs;
     for (int j = 0; j < n - 1; ++j) {
       const double q = readlane_d(t2, j) / readlane_d(x, j);
       const double bj = readlane_d(b, j);
@@ -833,12 +760,8 @@ __device__ __forceinline__ void schur_col(const KArgs &a, double *__restrict__ k
       if (i == j) sol = (b - t0j1 * sj1) / x;
     }
   } else {
-    // parallel cyclic reduction of the same system: row i is
-    // A[i,i-1] x[i-1] + A[i,i] x[i] + A[i,i+1] x[i+1] = rhs with
-    // A[i,i-1] = tri[3,i-1], A[i,i] = tri[2,i], A[i,i+1] = tri[1,i+1]
-    // (the column-oriented storage of SchurSolve.jl); rows >= n are the
-    // identity.  log2(64) = 6 elimination sweeps, all lanes busy.
-    double ra = __shfl_up(t2, 1), rb = t1, rc = __shfl_down(t0, 1), rd = v4;
+    // parallel cyclic reduction: log2(64) = 6 elimination sweeps, all lanes busy
+    double ra = __shfl_up(t2, 1), rb = t1, rc = __shfl_down(t0, 1), rd = rhs;
     if (i == 0 || !row) ra = 0.0;
     if (i >= n - 1) rc = 0.0;
     if (!row) {
@@ -861,15 +784,99 @@ __device__ __forceinline__ void schur_col(const KArgs &a, double *__restrict__ k
         cp = 0.0;
         dp = 0.0;
       }
-      const double k1 = ra / bm, k2 = rc / bp;
+      const double k1 = ra * rcp(bm), k2 = rc * rcp(bp);
       rb = rb - cm * k1 - ap * k2;
       rd = rd - dm * k1 - dp * k2;
       ra = -am * k1;
       rc = -cp * k2;
     }
-    sol = rd / rb;
+    sol = rd * rcp(rb);
   }
-  const double k4 = row ? sol : 0.0;
+  return row ? sol : 0.0;
+}
+
+// TriDiagKernel! (SchurSolve.jl:45-74) for row i from the J blocks of
+// rows i-1, i, i+1 (neighbour rows reached by shuffles)
+__device__ __forceinline__ void col_tri(double jrw0, double jrw1, double jwr0, double jwr1, double jwt0,
+                                        double jwt1, double jtw0, double jtw1, double fac, int i, int n,
+                                        double &t0, double &t1, double &t2) {
+  const double jwr0m = __shfl_up(jwr0, 1), jwt0m = __shfl_up(jwt0, 1);
+  const double jwr1p = __shfl_down(jwr1, 1), jwt1p = __shfl_down(jwt1, 1);
+  t0 = 0;
+  t1 = 1;
+  t2 = 0;
+  if (i < n) {
+    const double invfac = 1.0 / fac, invfac2 = invfac / fac;
+    t1 = invfac2 - jwr1 * jrw0 - jwr0 * jrw1 - jwt1 * jtw0 - jwt0 * jtw1;
+    t0 = (i == 0) ? 0.0 : -jwr0m * jrw0 - jwt0m * jtw0;
+    t2 = (i == n - 1) ? 0.0 : -jwr1p * jrw1 - jwt1p * jtw1;
+  }
+}
+
+// SchurSolveGPU! for Nz <= 64: one wavefront per column, lane = level, so
+// every load and store is a coalesced 8*Nz-byte run.  The tri build
+// (TriDiagKernel!), the w right-hand side (SchurSolveFKernel!) and the
+// back-substitution of rho, rho*theta (SchurSolveBKernel!) are pointwise
+// with neighbour lanes reached by shuffles; the tridiagonal system is
+// solved by parallel cyclic reduction (default) or, with thomas != 0, by
+// the Thomas recurrence of triSolve! run wave-serially in the reference's
+// operation order (one useful lane per step, about 2.5x slower).
+__global__ void __launch_bounds__(256) k_schur_col(KArgs a, double *__restrict__ k, double *__restrict__ v,
+                                                   const double *__restrict__ JRhoW, const double *__restrict__ JWRho,
+                                                   const double *__restrict__ JWRhoTh,
+                                                   const double *__restrict__ JRhoThW, double *__restrict__ tri,
+                                                   double fac, int comp_tri, int thomas) {
+  const int Nz = a.Nz, n = Nz - 1;
+  const int i = threadIdx.x & 63;
+  const size_t ig = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ig >= (size_t)a.NumG) return;  // uniform per wavefront
+  const size_t vs = (size_t)a.NumG * Nz;
+  const size_t c0 = ig * Nz;
+  const bool row = i < n, lev = i < Nz;
+  const size_t jb = (ig * (size_t)n + i) * 2, tb = (ig * (size_t)n + i) * 3;
+  double jrw0 = 0, jrw1 = 0, jwr0 = 0, jwr1 = 0, jwt0 = 0, jwt1 = 0, jtw0 = 0, jtw1 = 0;
+  if (row) {
+    jrw0 = JRhoW[jb];
+    jrw1 = JRhoW[jb + 1];
+    jwr0 = JWRho[jb];
+    jwr1 = JWRho[jb + 1];
+    jwt0 = JWRhoTh[jb];
+    jwt1 = JWRhoTh[jb + 1];
+    jtw0 = JRhoThW[jb];
+    jtw1 = JRhoThW[jb + 1];
+  }
+  double t0, t1, t2;
+  if (comp_tri) {
+    col_tri(jrw0, jrw1, jwr0, jwr1, jwt0, jwt1, jtw0, jtw1, fac, i, n, t0, t1, t2);
+    if (row) {
+      tri[tb] = t0;
+      tri[tb + 1] = t1;
+      tri[tb + 2] = t2;
+    }
+  } else {
+    t0 = 0;
+    t1 = 1;
+    t2 = 0;
+    if (row) {
+      t0 = tri[tb];
+      t1 = tri[tb + 1];
+      t2 = tri[tb + 2];
+    }
+  }
+  double v1 = 0, v2 = 0, v3 = 0, v4 = 0, v5 = 0;
+  if (lev) {
+    v1 = v[c0 + i];
+    v2 = v[c0 + i + vs];
+    v3 = v[c0 + i + 2 * vs];
+    v4 = v[c0 + i + 3 * vs];
+    v5 = v[c0 + i + 4 * vs];
+  }
+  const double v1p = __shfl_down(v1, 1), v5p = __shfl_down(v5, 1);
+  if (row) {
+    v4 = v4 / fac + jwr1 * v1 + jwr0 * v1p + jwt1 * v5 + jwt0 * v5p;
+    v[c0 + i + 3 * vs] = v4;
+  }
+  const double k4 = col_solve(t0, t1, t2, v4, i, n, thomas);
   // SchurSolveBKernel!
   const double k4m = __shfl_up(k4, 1);
   const double jrw1m = __shfl_up(jrw1, 1), jtw1m = __shfl_up(jtw1, 1);
@@ -884,48 +891,14 @@ __device__ __forceinline__ void schur_col(const KArgs &a, double *__restrict__ k
       v1 += jrw0 * k4 + jrw1m * k4m;
       v5 += jtw0 * k4 + jtw1m * k4m;
     }
-    const double k1 = fac * v1, k2 = fac * v2, k3 = fac * v3, k5 = fac * v5;
-    k[c0 + i] = k1;
-    k[c0 + i + vs] = k2;
-    k[c0 + i + 2 * vs] = k3;
+    k[c0 + i] = fac * v1;
+    k[c0 + i + vs] = fac * v2;
+    k[c0 + i + 2 * vs] = fac * v3;
     k[c0 + i + 3 * vs] = k4;
-    k[c0 + i + 4 * vs] = k5;
-    if (!FUSED) {
-      v[c0 + i] = v1;
-      v[c0 + i + 4 * vs] = v5;
-    } else {
-      // next stage state V = Vn + sum_j a_{i+1,j} k_j, or the final update
-      // V = Vn + sum_j m_j k_j (RosenbrockSchur.jl:14-17,29-32)
-      const double kc[5] = {k1, k2, k3, k4, k5};
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        const size_t o = c0 + i + q * vs;
-        double r = rw.Vn[o];
-        for (int j = 0; j < rw.stage; ++j) r = r + rw.av[j] * rw.kprev[j][o];
-        r = r + rw.av[rw.stage] * kc[q];
-        rw.V[o] = r;
-      }
-    }
+    k[c0 + i + 4 * vs] = fac * v5;
+    v[c0 + i] = v1;
+    v[c0 + i + 4 * vs] = v5;
   }
-}
-
-__global__ void __launch_bounds__(256) k_schur_col(KArgs a, double *__restrict__ k, double *__restrict__ v,
-                                                   const double *__restrict__ JRhoW, const double *__restrict__ JWRho,
-                                                   const double *__restrict__ JWRhoTh,
-                                                   const double *__restrict__ JRhoThW, double *__restrict__ tri,
-                                                   double fac, int comp_tri, int thomas) {
-  const RWStage none{};
-  schur_col<false>(a, k, v, JRhoW, JWRho, JWRhoTh, JRhoThW, tri, fac, comp_tri, thomas, none);
-}
-
-// One Rosenbrock stage of cgd_rw_step after the tendency: stage terms,
-// Schur solve, next stage state.  The tendency fV is only read.
-__global__ void __launch_bounds__(256) k_rw_col(KArgs a, double *__restrict__ k, const double *__restrict__ fV,
-                                                const double *__restrict__ JRhoW, const double *__restrict__ JWRho,
-                                                const double *__restrict__ JWRhoTh,
-                                                const double *__restrict__ JRhoThW, double *__restrict__ tri,
-                                                double fac, int comp_tri, int thomas, RWStage rw) {
-  schur_col<true>(a, k, const_cast<double *>(fV), JRhoW, JWRho, JWRhoTh, JRhoThW, tri, fac, comp_tri, thomas, rw);
 }
 
 // ------------------------------------------------------------------------
@@ -947,6 +920,32 @@ __global__ void k_fill(double *__restrict__ x, size_t n, double val) {
 }
 
 // JacSchurKernel! (src/DyCore/JacSchur.jl:29-75); p = AuxG[:,:,4]
+// JacSchurKernel! (src/DyCore/JacSchur.jl:29-75) for the interface between
+// levels L (below) and R (above): the two entries of each block.
+struct JacBlocks {
+  double rw0, rw1, wr0, wr1, wt0, wt1, tw0, tw1;  // JRhoW, JWRho, JWRhoTh, JRhoThW
+};
+__device__ __forceinline__ JacBlocks jac_blocks(const KArgs &a, double RhoL, double RhoR, double RhoThL,
+                                                double RhoThR, double dPdThL, double dPdThR, double dzL,
+                                                double dzR) {
+  // the quotients of the reference as products with four reciprocals
+  JacBlocks J;
+  const double iS = rcp(dzL + dzR), iL = rcp(dzL), iR = rcp(dzR);
+  const double RhoF = (RhoL * dzL + RhoR * dzR) * iS;
+  const double iRhoF = rcp(RhoF);
+  J.rw0 = -RhoF * iL;
+  J.rw1 = RhoF * iR;
+  J.wt0 = -dPdThR * iRhoF * (2.0 * iS);
+  J.wt1 = dPdThL * iRhoF * (2.0 * iS);
+  J.wr0 = -a.Grav * dzR * iRhoF * iS;
+  J.wr1 = -a.Grav * dzL * iRhoF * iS;
+  const double RhoThF = (RhoThL * dzL + RhoThR * dzR) * iS;
+  J.tw0 = -RhoThF * iL;
+  J.tw1 = RhoThF * iR;
+  return J;
+}
+
+// JacSchurGPU!: p = AuxG[:,:,4] is added to rho*theta (zero for State = Dry)
 __global__ void k_jac(KArgs a, const double *__restrict__ U, const double *__restrict__ P4,
                       const double *__restrict__ dz, double *__restrict__ JRhoW, double *__restrict__ JWRho,
                       double *__restrict__ JWRhoTh, double *__restrict__ JRhoThW) {
@@ -957,22 +956,138 @@ __global__ void k_jac(KArgs a, const double *__restrict__ U, const double *__res
     const size_t ig = q / n;
     const int iz = (int)(q - ig * n);
     const size_t i0 = ig * Nz + iz;
-    const double RhoL = U[i0], RhoR = U[i0 + 1];
     const double RhoThL = U[i0 + 4 * vs] + P4[i0], RhoThR = U[i0 + 1 + 4 * vs] + P4[i0 + 1];
-    const double dzL = dz[i0], dzR = dz[i0 + 1];
-    const double RhoF = (RhoL * dzL + RhoR * dzR) / (dzL + dzR);
+    const JacBlocks J = jac_blocks(a, U[i0], U[i0 + 1], RhoThL, RhoThR, eos_dpdrhoth(a.Rd, a.p0, a.kappa, RhoThL),
+                                   eos_dpdrhoth(a.Rd, a.p0, a.kappa, RhoThR), dz[i0], dz[i0 + 1]);
     const size_t o = 2 * q;
-    JRhoW[o] = -RhoF / dzL;
-    JRhoW[o + 1] = RhoF / dzR;
-    const double dPdThL = eos_dpdrhoth(a.Rd, a.p0, a.kappa, RhoThL);
-    const double dPdThR = eos_dpdrhoth(a.Rd, a.p0, a.kappa, RhoThR);
-    JWRhoTh[o] = -dPdThR / RhoF / (0.5 * (dzL + dzR));
-    JWRhoTh[o + 1] = dPdThL / RhoF / (0.5 * (dzL + dzR));
-    JWRho[o] = -a.Grav * dzR / RhoF / (dzL + dzR);
-    JWRho[o + 1] = -a.Grav * dzL / RhoF / (dzL + dzR);
-    const double RhoThF = (RhoThL * dzL + RhoThR * dzR) / (dzL + dzR);
-    JRhoThW[o] = -RhoThF / dzL;
-    JRhoThW[o + 1] = RhoThF / dzR;
+    JRhoW[o] = J.rw0;
+    JRhoW[o + 1] = J.rw1;
+    JWRhoTh[o] = J.wt0;
+    JWRhoTh[o + 1] = J.wt1;
+    JWRho[o] = J.wr0;
+    JWRho[o + 1] = J.wr1;
+    JRhoThW[o] = J.tw0;
+    JRhoThW[o + 1] = J.tw1;
+  }
+}
+
+// One Rosenbrock-W stage of cgd_rw_step after the tendency
+// (RosenbrockSchur.jl:12-32), for Nz <= 64: one wavefront per column,
+// lane = level.  Nothing that can be recomputed is stored and re-read:
+//  - the frozen Jacobian (JacSchurGPU! of the stage-1 state, which is Vn)
+//    and the Schur tri matrix are rebuilt from Vn in every stage, with the
+//    same arithmetic as k_jac and the tri build of k_schur_col;
+//  - the w tendency of the interfaces between level chunks (the side sums
+//    of k_fcn2) is added here, in k_f4fix's order;
+//  - fV + sum_j (c_ij/dt) k_j, the Schur solve, and the next stage state
+//    V = Vn + sum_j a_{i+1,j} k_j (or the final V = Vn + sum_j m_j k_j)
+//    follow in registers, and the pressure of the next stage's
+//    FcnPrepareGPU! is written with it.
+// AuxG[:,:,4] (zero for State = Dry) is not read.
+struct RWStage {
+  const double *fV;        // this stage's tendency
+  const double *kprev[2];  // k_1, k_2 (earlier stages)
+  double cf[2];            // c_ij / dt
+  double av[3];            // V = Vn + sum_j av[j] k_j, j <= stage
+  const double *Vn;        // state at the start of the step
+  const double *dz;        // (Nz, NumG)
+  double *V;               // next stage state, or the final state
+  double *k;               // this stage's increment (null: not stored)
+  double *P;               // AuxG[:,:,1] for the next stage (null: not written)
+  const double *F4x;       // chunk-interface w terms of k_fcn2
+  const int *nptr, *ncopy; // node -> element copies, colour order
+  int nchunk, nzt;
+  double fac;              // dt * gamma
+  int stage, thomas;
+};
+
+__global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage rw) {
+  const int Nz = a.Nz, n = Nz - 1;
+  const int i = threadIdx.x & 63;
+  const size_t ig = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ig >= (size_t)a.NumG) return;  // uniform per wavefront
+  const size_t vs = (size_t)a.NumG * Nz;
+  const size_t c0 = ig * Nz;
+  const bool row = i < n, lev = i < Nz;
+  const double fac = rw.fac;
+  double vn[5] = {0, 0, 0, 0, 0};
+  double dzi = 1.0;
+  if (lev) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) vn[q] = rw.Vn[c0 + i + q * vs];
+    dzi = rw.dz[c0 + i];
+  }
+  // frozen Jacobian at interface i (levels i, i+1) and the tri matrix
+  const double dpd = lev ? eos_dpdrhoth(a.Rd, a.p0, a.kappa, vn[4]) : 0.0;
+  const double rhoR = __shfl_down(vn[0], 1), rthR = __shfl_down(vn[4], 1);
+  const double dpdR = __shfl_down(dpd, 1), dzR = __shfl_down(dzi, 1);
+  JacBlocks J{0, 0, 0, 0, 0, 0, 0, 0};
+  if (row) J = jac_blocks(a, vn[0], rhoR, vn[4], rthR, dpd, dpdR, dzi, dzR);
+  double t0, t1, t2;
+  col_tri(J.rw0, J.rw1, J.wr0, J.wr1, J.wt0, J.wt1, J.tw0, J.tw1, fac, i, n, t0, t1, t2);
+  // tendency, chunk-interface w terms, stage terms
+  double v[5] = {0, 0, 0, 0, 0};
+  double kp[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
+  if (lev) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) v[q] = rw.fV[c0 + i + q * vs];
+    if (rw.nchunk > 1 && (i + 1) % rw.nzt == 0 && i + 1 < Nz) {
+      const int ch = (i + 1) / rw.nzt;
+      double s = 0.0;
+      for (int c = rw.nptr[ig]; c < rw.nptr[ig + 1]; ++c) s += rw.F4x[(size_t)rw.ncopy[c] * rw.nchunk + ch];
+      v[3] += s;
+    }
+    for (int j = 0; j < rw.stage; ++j) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        kp[j][q] = rw.kprev[j][c0 + i + q * vs];
+        v[q] = v[q] + rw.cf[j] * kp[j][q];
+      }
+    }
+  }
+  // SchurSolveFKernel!, the tridiagonal solve, SchurSolveBKernel!
+  const double v1p = __shfl_down(v[0], 1), v5p = __shfl_down(v[4], 1);
+  double rhs = 0.0;
+  if (row) rhs = v[3] / fac + J.wr1 * v[0] + J.wr0 * v1p + J.wt1 * v[4] + J.wt0 * v5p;
+  const double k4 = col_solve(t0, t1, t2, rhs, i, n, rw.thomas);
+  const double k4m = __shfl_up(k4, 1);
+  const double jrw1m = __shfl_up(J.rw1, 1), jtw1m = __shfl_up(J.tw1, 1);
+  if (!lev) return;
+  if (i == 0) {
+    v[0] += J.rw0 * k4;
+    v[4] += J.tw0 * k4;
+  } else if (i == Nz - 1) {
+    v[0] += jrw1m * k4m;
+    v[4] += jtw1m * k4m;
+  } else {
+    v[0] += J.rw0 * k4 + jrw1m * k4m;
+    v[4] += J.tw0 * k4 + jtw1m * k4m;
+  }
+  const double kc[5] = {fac * v[0], fac * v[1], fac * v[2], k4, fac * v[4]};
+  if (rw.k) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) rw.k[c0 + i + q * vs] = kc[q];
+  }
+  double rt = 0.0;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    double r = vn[q];
+    for (int j = 0; j < rw.stage; ++j) r = r + rw.av[j] * kp[j][q];
+    r = r + rw.av[rw.stage] * kc[q];
+    rw.V[c0 + i + q * vs] = r;
+    if (q == 4) rt = r;
+  }
+  if (rw.P) rw.P[c0 + i] = eos_p(a, rt);
+}
+
+// Start of cgd_rw_step: Vn = V and the pressure of the first stage's
+// FcnPrepareGPU! in one pass over V.
+__global__ void k_rw_begin(KArgs a, const double *__restrict__ V, double *__restrict__ Vn, double *__restrict__ P,
+                           size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) Vn[i + q * n] = V[i + q * n];
+    P[i] = eos_p(a, V[i + 4 * n]);
   }
 }
 
@@ -1071,10 +1186,11 @@ __global__ void k_axpy3(double *__restrict__ y, const double *__restrict__ x0, c
 
 // ------------------------------------------------------------------------
 // Context set-up: level-fastest metric for the element kernels.
-// met[e][c][ID][iz] from dXdxI (3,3,2,NN,Nz,NF), J (NN,2,Nz,NF), X (NN,2,3,Nz,NF)
+// met[slot][c][ID][iz] from dXdxI (3,3,2,NN,Nz,NF), J (NN,2,Nz,NF), X (NN,2,3,Nz,NF)
 __global__ void k_relayout_metric(const double *__restrict__ dXdxI, const double *__restrict__ Jm,
-                                  const double *__restrict__ X, double *__restrict__ met, double *__restrict__ met1,
-                                  int NN, int Nz, int NF, double Omega) {
+                                  const double *__restrict__ X, const int *__restrict__ slot,
+                                  double *__restrict__ met, double *__restrict__ met1, int NN, int Nz, int NF,
+                                  double Omega) {
   const size_t total = (size_t)NF * NN * Nz;
   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
     const int iz = (int)(q % Nz);
@@ -1090,7 +1206,8 @@ __global__ void k_relayout_metric(const double *__restrict__ dXdxI, const double
                                        Jm[(size_t)ID + (size_t)NN * (0 + 2 * ((size_t)iz + (size_t)Nz * e))],
                                        Jm[(size_t)ID + (size_t)NN * (1 + 2 * ((size_t)iz + (size_t)Nz * e))]};
     const size_t NNZ = (size_t)NN * Nz;
-    const size_t mb = (e * CGD_NMET * NN + ID) * (size_t)Nz + iz;
+    const size_t es = (size_t)slot[e];  // stored in colour order
+    const size_t mb = (es * CGD_NMET * NN + ID) * (size_t)Nz + iz;
     for (int c = 0; c < CGD_NMET - 1; ++c) met[mb + c * NNZ] = vals[c];
     // CoriolisShallow at the layer midpoint (Coriolis.jl:16-28, MomentumKernel.jl:156-159)
 #define XA(K, c) X[(size_t)ID + (size_t)NN * ((K) + 2 * ((c) + 3 * ((size_t)iz + (size_t)Nz * e)))]
@@ -1099,7 +1216,7 @@ __global__ void k_relayout_metric(const double *__restrict__ dXdxI, const double
     const double z = 0.5 * (XA(0, 2) + XA(1, 2));
     const double sinlat = z / sqrt(x * x + y * y + z * z);
     met[mb + MCOR * NNZ] = 2.0 * Omega * sinlat;
-    const size_t m1 = (e * CGD_NMET1 * NN + ID) * (size_t)Nz + iz;
+    const size_t m1 = (es * CGD_NMET1 * NN + ID) * (size_t)Nz + iz;
     met1[m1] = DXI(1, 1, 1) + DXI(1, 1, 2);
     met1[m1 + NNZ] = DXI(1, 2, 1) + DXI(1, 2, 2);
     met1[m1 + 2 * NNZ] = DXI(2, 1, 1) + DXI(2, 1, 2);

@@ -268,8 +268,9 @@ class NativeContext:
         except Exception:
             pass
 
-    def profile(self, on=True):
-        self.check(self.L.cgd_profile_enable(self.h, int(on)), "cgd_profile_enable")
+    def profile(self, mask=7):
+        """Time the passes in ``mask`` (bit 0 pass 1, bit 1 pass 2, bit 2 column; 0 = off)."""
+        self.check(self.L.cgd_profile_enable(self.h, int(mask)), "cgd_profile_enable")
 
     def profile_read(self, which):
         ms = C.c_double()

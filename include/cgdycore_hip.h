@@ -92,8 +92,14 @@ int cgd_schur(cgd_ctx *ctx, double *k, double *v, const double *JRhoW, const dou
 
 /* RosenbrockSchur!(V,dt,...) with SSP-Knoth -- src/Integration/RosenbrockSchur.jl:1-33
  * and src/Integration/RosenbrockMethod.jl:24-48.  The work buffers are the
- * reference's Cache.Vn, Cache.fV, Cache.k, Cache.AuxG (4th component must be
- * zero), Cache.Temp1 and the JCache blocks; all are caller owned. */
+ * reference's Cache.Vn, Cache.fV, Cache.k, Cache.AuxG, Cache.Temp1 and the
+ * JCache blocks; all are caller owned.  V is advanced by one step exactly as
+ * the reference's sequence of calls would; the work buffers are scratch.
+ * For Nz <= 64 the stage kernel recomputes the frozen Jacobian from Vn
+ * instead of storing it, so JRhoW..JRhoThW, tri, fV and k[:,:,:,3] are left
+ * unwritten, AuxG[:,:,1] holds the pressure of the last stage state and
+ * AuxG[:,:,2:4] are neither read nor written (AuxG[:,:,4] must be zero, as
+ * for State = Dry). */
 typedef struct {
   double *Vn, *fV, *k, *AuxG, *Temp1;
   double *JRhoW, *JWRho, *JWRhoTh, *JRhoThW, *tri;
@@ -110,7 +116,10 @@ int cgd_color_elements(const int64_t *glob, int NN, int NF, int NumG, int *color
 
 /* Per-kernel timing of the tendency passes with HIP events on the launch
  * stream (used by the benchmark; off by default).  which: 0 = pass 1
- * (hyperviscosity), 1 = pass 2 (fused tendency), 2 = column solve. */
+ * (hyperviscosity), 1 = pass 2 (fused tendency), 2 = column solve.
+ * cgd_profile_enable takes a bit mask of the passes to time (bit which;
+ * 0 turns timing off).  Every recorded event is a stream barrier with a
+ * cache write-back, so time only what is read. */
 int cgd_profile_enable(cgd_ctx *ctx, int on);
 int cgd_profile_read(cgd_ctx *ctx, int which, double *total_ms, long long *launches);
 

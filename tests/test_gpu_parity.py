@@ -121,7 +121,7 @@ def test_jac_schur_parity(pkg, oracle_mod, torch_cuda, ne, nz):
     assert np.all(kg[3][:, -1] == 0.0)
 
 
-@pytest.mark.parametrize("ne,nz,dt,nsteps", [(3, 8, 300.0, 2), (6, 10, 300.0, 3)])
+@pytest.mark.parametrize("ne,nz,dt,nsteps", [(3, 8, 300.0, 2), (6, 10, 300.0, 3), (4, 64, 300.0, 2)])
 def test_rw_step_parity(pkg, oracle_mod, torch_cuda, ne, nz, dt, nsteps):
     torch = torch_cuda
     host = pkg.host
@@ -141,13 +141,15 @@ def test_rw_step_parity(pkg, oracle_mod, torch_cuda, ne, nz, dt, nsteps):
         assert e <= TOL, f"V{c + 1} rel L2 {e:.3e}"
 
 
-def test_rw_composed_equals_fused(pkg, torch_cuda):
+@pytest.mark.parametrize("ne,nz", [(3, 8), (4, 20)])
+def test_rw_composed_equals_fused(pkg, torch_cuda, ne, nz):
     """RosenbrockSchur with non-native callables (stage-by-stage composition in
-    Python) equals the single cgd_rw_step call."""
+    Python) equals the single cgd_rw_step call; nz = 20 spans three level
+    chunks, so the chunk-interface w terms go through the fused stage kernel."""
     torch = torch_cuda
     host = pkg.host
     D = pkg.discretization
-    disc = D.build(3, 4, 8, 30000.0)
+    disc = D.build(ne, 4, nz, 30000.0)
     phys = D.Phys()
     U0 = pkg.state.jet_state(disc, seed=3)
     FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda")
