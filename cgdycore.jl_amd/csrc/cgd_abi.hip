@@ -25,6 +25,8 @@ struct cgd_ctx {
   int *d_glob = nullptr;       // [NF][NN] 0-based
   int *d_slot = nullptr;       // element -> slot (colour order), set-up only
   double *d_met = nullptr, *d_met1 = nullptr, *d_ms = nullptr, *d_bwork = nullptr;
+  double *d_metn = nullptr, *d_met1n = nullptr;  // compact metric layout: per element node
+  int cm = 0;                                    // 1: compact metric layout
   double *d_f4x = nullptr;     // [NF][NN][nchunk] chunk-bottom F4 terms
   int *d_nptr = nullptr, *d_ncopy = nullptr;  // global node -> element copies (colour order)
   bool ok = false;
@@ -213,6 +215,15 @@ extern "C" int cgd_color_elements(const int64_t *glob, int NN, int NF, int NumG,
 
 extern "C" const char *cgd_last_error(const cgd_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+extern "C" int cgd_ctx_info(const cgd_ctx *c, int *ncolor, int *chunk_levels, int *compact_metric) {
+  if (!c) return CGD_ERR_ARG;
+  if (!c->ok) return CGD_ERR_STATE;
+  if (ncolor) *ncolor = c->ncolor;
+  if (chunk_levels) *chunk_levels = c->nzt;
+  if (compact_metric) *compact_metric = c->cm;
+  return CGD_OK;
+}
+
 extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   if (!out) return CGD_ERR_ARG;
   *out = nullptr;
@@ -245,20 +256,21 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   const int NN = c->NN, Nz = c->Nz, NF = c->NF;
   // level chunk height NZT (a template constant): a workgroup is one
   // element x NZT levels, N*N*NZT lanes rounded up to whole wavefronts.
-  // CGD_NZT overrides the default (5, 8, 10 or 16 at N = 5; 8 or 16 at N = 4).
+  // 8 levels measured fastest at ne=30, nz=64 (5 and 10 fill the
+  // wavefronts better but straddle cache lines); CGD_NZT=16 overrides.
   {
     int nzt = 8;
     if (const char *ev = getenv("CGD_NZT")) {
       const int o = atoi(ev);
-      if (o == 8 || o == 16 || (c->N == 5 && (o == 5 || o == 10))) nzt = o;
+      if (o == 16 && c->N == 5) nzt = o;
     }
     c->nzt = nzt;
     c->nchunk = (Nz + nzt - 1) / nzt;
     if (const char *ev = getenv("CGD_THOMAS")) c->thomas = atoi(ev) != 0;
   }
   c->block = ((NN * c->nzt + 63) / 64) * 64;
-  c->lds1 = sizeof(double) * (2 * NN + 5 * NN * c->nzt);
-  c->lds2 = sizeof(double) * (2 * NN + 8 * NN * c->nzt);
+  c->lds1 = sizeof(double) * (2 * NN + 9 * NN * c->nzt);   // D, DW, tile sets of 5 and 4
+  c->lds2 = sizeof(double) * (2 * NN + 16 * NN * c->nzt);  // D, DW, two sets of 8 tiles
 
   // global numbering -> 0-based int32, range check
   std::vector<int64_t> hglob((size_t)NN * NF);
@@ -326,6 +338,43 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   const size_t ng = (size_t)d->NumG * Nz;
   hipLaunchKernelGGL(k_msum, dim3(grid_for(ng, 256)), dim3(256), 0, st, d->M, c->d_ms, ng);
   HIPCHK(c, hipGetLastError());
+  // compact metric layout when every component but the terrain row has the
+  // same bits at all levels (CGD_METRIC=full keeps the full layout)
+  {
+    const char *ev = getenv("CGD_METRIC");
+    const bool allow = !(ev && strcmp(ev, "full") == 0);
+    unsigned *d_mask = nullptr;
+    unsigned hmask[2] = {~0u, ~0u};
+    if (allow) {
+      HIPCHK(c, hipMalloc(&d_mask, 2 * sizeof(unsigned)));
+      HIPCHK(c, hipMemsetAsync(d_mask, 0, 2 * sizeof(unsigned), st));
+      hipLaunchKernelGGL(k_met_invariant, dim3(1024), dim3(256), 0, st, c->d_met, CGD_NMET, NF, NN, Nz, d_mask);
+      hipLaunchKernelGGL(k_met_invariant, dim3(1024), dim3(256), 0, st, c->d_met1, CGD_NMET1, NF, NN, Nz,
+                         d_mask + 1);
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipMemcpyAsync(hmask, d_mask, sizeof(hmask), hipMemcpyDeviceToHost, st));
+      HIPCHK(c, hipStreamSynchronize(st));
+      HIPCHK(c, hipFree(d_mask));
+    }
+    unsigned per_point = 0;
+    for (int q = 0; q < CGD_NMET; ++q)
+      if (met_per_point(q)) per_point |= 1u << q;
+    if (allow && (hmask[0] & ~per_point) == 0 && hmask[1] == 0) {
+      double *metp = nullptr;
+      HIPCHK(c, hipMalloc(&metp, npt * CGD_NMETP * sizeof(double)));
+      HIPCHK(c, hipMalloc(&c->d_metn, (size_t)NF * NN * CGD_NMETN * sizeof(double)));
+      HIPCHK(c, hipMalloc(&c->d_met1n, (size_t)NF * NN * CGD_NMET1 * sizeof(double)));
+      hipLaunchKernelGGL(k_met_compact, dim3(grid_for(npt, 256)), dim3(256), 0, st, c->d_met, c->d_met1, metp,
+                         c->d_metn, c->d_met1n, NF, NN, Nz);
+      HIPCHK(c, hipGetLastError());
+      HIPCHK(c, hipStreamSynchronize(st));
+      HIPCHK(c, hipFree(c->d_met));
+      HIPCHK(c, hipFree(c->d_met1));
+      c->d_met = metp;
+      c->d_met1 = nullptr;
+      c->cm = 1;
+    }
+  }
   HIPCHK(c, hipStreamSynchronize(st));
 
   KArgs &a = c->ka;
@@ -338,6 +387,9 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   a.Cpd = d->Cpd;
   a.p0 = d->p0;
   a.kappa = d->kappa;
+  a.Rd_p0 = d->Rd / d->p0;
+  a.eos_y = 1.0 / (1.0 - d->kappa);
+  a.dpd_y = d->kappa / (1.0 - d->kappa);
   a.Kc = d->HyperDCurl;
   a.Kg = d->HyperDGrad;
   a.Kd = d->HyperDDiv;
@@ -349,6 +401,8 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
     }
   a.glob = c->d_glob;
   a.met = c->d_met;
+  a.metn = c->d_metn;
+  a.met1n = c->d_met1n;
   a.met1 = c->d_met1;
   a.iMs = c->d_ms;
   a.M = d->M;
@@ -362,7 +416,8 @@ extern "C" int cgd_ctx_destroy(cgd_ctx *c) {
   if (!c) return CGD_ERR_ARG;
   // a context that failed validation holds no device memory: tear it down
   // without touching the HIP runtime
-  for (void *p : {(void *)c->d_glob, (void *)c->d_slot, (void *)c->d_met, (void *)c->d_met1, (void *)c->d_ms,
+  for (void *p : {(void *)c->d_glob, (void *)c->d_slot, (void *)c->d_met, (void *)c->d_met1, (void *)c->d_metn,
+                  (void *)c->d_met1n, (void *)c->d_ms,
                   (void *)c->d_bwork, (void *)c->d_f4x, (void *)c->d_nptr, (void *)c->d_ncopy})
     if (p) (void)hipFree(p);
   prof_free(c);
@@ -436,32 +491,29 @@ extern "C" int cgd_profile_read(cgd_ctx *c, int which, double *total_ms, long lo
 }
 
 
-// template dispatch on (N, chunk height)
-template <int N, int NZT>
+// template dispatch on (N, chunk height, metric layout)
+template <int N, int NZT, bool CM>
 static void fcn1_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, double *T1) {
-  hipLaunchKernelGGL((k_fcn1<N, NZT>), g, dim3(c->block), c->lds1, st, c->ka, slot0, c->nchunk, U, T1);
+  hipLaunchKernelGGL((k_fcn1<N, NZT, CM>), g, dim3(c->block), c->lds1, st, c->ka, slot0, c->nchunk, U, T1);
 }
-template <int N, int NZT>
+template <int N, int NZT, bool CM>
 static void fcn2_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, const double *T1,
                    const double *P, double *F) {
-  hipLaunchKernelGGL((k_fcn2<N, NZT>), g, dim3(c->block), c->lds2, st, c->ka, slot0, c->nchunk, U, T1, P, F,
+  hipLaunchKernelGGL((k_fcn2<N, NZT, CM>), g, dim3(c->block), c->lds2, st, c->ka, slot0, c->nchunk, U, T1, P, F,
                      c->d_f4x);
 }
-#define CGD_DISPATCH(FN, ...)                    \
-  do {                                           \
-    if (c->N == 5) {                             \
-      switch (c->nzt) {                          \
-        case 5: FN<5, 5>(c, __VA_ARGS__); break;   \
-        case 10: FN<5, 10>(c, __VA_ARGS__); break; \
-        case 16: FN<5, 16>(c, __VA_ARGS__); break; \
-        default: FN<5, 8>(c, __VA_ARGS__);         \
-      }                                          \
-    } else {                                     \
-      if (c->nzt == 16)                          \
-        FN<4, 16>(c, __VA_ARGS__);               \
-      else                                       \
-        FN<4, 8>(c, __VA_ARGS__);                \
-    }                                            \
+#define CGD_DISPATCH(FN, ...)                                      \
+  do {                                                             \
+    if (c->N == 5 && c->nzt == 16) {                               \
+      if (c->cm) FN<5, 16, true>(c, __VA_ARGS__);                  \
+      else FN<5, 16, false>(c, __VA_ARGS__);                       \
+    } else if (c->N == 5) {                                        \
+      if (c->cm) FN<5, 8, true>(c, __VA_ARGS__);                   \
+      else FN<5, 8, false>(c, __VA_ARGS__);                        \
+    } else {                                                       \
+      if (c->cm) FN<4, 8, true>(c, __VA_ARGS__);                   \
+      else FN<4, 8, false>(c, __VA_ARGS__);                        \
+    }                                                              \
   } while (0)
 static void launch_fcn1(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, double *T1) {
   CGD_DISPATCH(fcn1_t, g, st, slot0, U, T1);
@@ -547,9 +599,10 @@ extern "C" int cgd_schur(cgd_ctx *c, double *k, double *v, const double *JRhoW, 
   if (!(fac > 0.0)) return fail(c, CGD_ERR_ARG, "cgd_schur: fac must be positive");
   hipStream_t st = S(stream);
   prof_begin(c, st, 2);
+  const double invfac = 1.0 / fac, invfac2 = invfac / fac;
   if (c->Nz <= 64)
     hipLaunchKernelGGL(k_schur_col, dim3((c->NumG + 3) / 4), dim3(256), 0, st, c->ka, k, v, JRhoW, JWRho, JWRhoTh,
-                       JRhoThW, tri, fac, comp_tri, c->thomas);
+                       JRhoThW, tri, fac, invfac2, comp_tri, c->thomas);
   else
     hipLaunchKernelGGL(k_schur, dim3(grid_for(c->NumG, 64)), dim3(64), 0, st, c->ka, k, v, JRhoW, JWRho, JWRhoTh,
                        JRhoThW, tri, c->d_bwork, fac, comp_tri);
@@ -583,7 +636,6 @@ extern "C" int cgd_rw_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *
       if ((r = run_fcn(c, w->fV, V, w->Temp1, w->AuxG, st, false))) return r;
       RWStage rw{};
       rw.fV = w->fV;
-      rw.stage = is;
       for (int j = 0; j < is; ++j) {
         rw.kprev[j] = k[j];
         rw.cf[j] = Cc[is][j] / dt;
@@ -600,9 +652,14 @@ extern "C" int cgd_rw_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *
       rw.nchunk = c->nchunk;
       rw.nzt = c->nzt;
       rw.fac = dt * gamma;
+      rw.invfac = 1.0 / rw.fac;
+      rw.invfac2 = rw.invfac / rw.fac;
       rw.thomas = c->thomas;
       prof_begin(c, st, 2);
-      hipLaunchKernelGGL(k_rw_stage, dim3((c->NumG + 3) / 4), dim3(256), 0, st, c->ka, rw);
+      const dim3 gcol((c->NumG + 3) / 4);
+      if (is == 0) hipLaunchKernelGGL(k_rw_stage<0>, gcol, dim3(256), 0, st, c->ka, rw);
+      else if (is == 1) hipLaunchKernelGGL(k_rw_stage<1>, gcol, dim3(256), 0, st, c->ka, rw);
+      else hipLaunchKernelGGL(k_rw_stage<2>, gcol, dim3(256), 0, st, c->ka, rw);
       HIPCHK(c, hipGetLastError());
       if ((r = prof_end(c, st, 2, 1))) return r;
     }

@@ -24,16 +24,21 @@
 #define CGD_NMAX 5
 #define CGD_NMET 17  // pass-2 metric components
 #define CGD_NMET1 5  // pass-1 metric components
+#define CGD_NMETP 5  // compact layout: components stored per point (terrain row dXdxI[3,1:2,K], Coriolis)
+#define CGD_NMETN 12 // compact layout: components stored once per element node
 
 struct KArgs {
   int Nz, NumG, nzt;
   double Grav, Omega, Rd, Cpd, p0, kappa;
+  double Rd_p0, eos_y, dpd_y;      // Rd/p0, 1/(1-kappa), kappa/(1-kappa)
   double Kc, Kg, Kd;
   double D[CGD_NMAX * CGD_NMAX];   // D[i*N + k] = DS[i,k]
   double DWm[CGD_NMAX * CGD_NMAX]; // DW[i*N + k]
   const int *glob;                 // [slot][NN]: 0-based node | bit 31 set on the node's first copy
-  const double *met;               // [slot][17][NN][Nz]
-  const double *met1;              // [slot][5][NN][Nz]
+  const double *met;               // [slot][17][NN][Nz], or [slot][4][NN][Nz] in the compact layout
+  const double *met1;              // [slot][5][NN][Nz] (full layout only)
+  const double *metn;              // compact layout: [slot][13][NN]
+  const double *met1n;             // compact layout: [slot][5][NN]
   const double *iMs;               // [NumG][Nz]: 1 / (M1 + M2)
   const double *M;                 // [2][NumG][Nz]
 };
@@ -44,14 +49,27 @@ enum {
   M311, M312, M321, M322, M331, M332, MJ1, MJ2, MCOR
 };
 
+// Compact metric layout, chosen at context creation when every component
+// except the terrain row dXdxI[3,1:2,K] and the Coriolis parameter (z/|X|
+// of level-scaled positions, invariant only up to rounding) has the same
+// bits at all levels of each element node (shallow atmosphere, flat
+// terrain, uniform layers): those 12 components are stored once per
+// element node, the other 5 per point.  Index of a component in either table:
+__host__ __device__ constexpr bool met_per_point(int c) { return (c >= M311 && c <= M322) || c == MCOR; }
+__host__ __device__ constexpr int met_pidx(int c) { return c == MCOR ? 4 : c - M311; }
+__host__ __device__ constexpr int met_nidx(int c) { return c < M311 ? c : c - 4; }
+
 // ------------------------------------------------------------------------
 // Equation of state (src/Models/Equation.jl:25-38,79)
+// p = p0 (Rd rho*theta / p0)^(1/(1-kappa)) evaluated as exp(y log x) like
+// fast_powGPU; dp/d(rho*theta) = Rd (Rd rho*theta / p0)^(kappa/(1-kappa))
+// in the same form.  The constants Rd/p0, 1/(1-kappa), kappa/(1-kappa)
+// are formed once on the host.
 __device__ __forceinline__ double eos_p(const KArgs &a, double rhoth) {
-  const double y = 1.0 / (1.0 - a.kappa);
-  return a.p0 * exp(y * log(a.Rd * rhoth / a.p0));
+  return a.p0 * exp(a.eos_y * log(a.Rd_p0 * rhoth));
 }
-__device__ __forceinline__ double eos_dpdrhoth(double Rd, double p0, double kappa, double rhoth) {
-  return Rd * pow(Rd * rhoth / p0, kappa / (1.0 - kappa));
+__device__ __forceinline__ double eos_dpdrhoth(const KArgs &a, double rhoth) {
+  return a.Rd * exp(a.dpd_y * log(a.Rd_p0 * rhoth));
 }
 
 // Julia min/max on non-NaN values
@@ -201,13 +219,15 @@ template <int N> __device__ __forceinline__ void load_dmats(const KArgs &a, doub
 // ------------------------------------------------------------------------
 // Pass 1: HyperViscKernel! (src/GPU/ViscKernel.jl:1-84).
 // Grid: one workgroup per (element of the colour, level chunk).
-template <int N, int NZT>
+template <int N, int NZT, bool CM>
 __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, int slot0, int nchunk,
                                                const double *__restrict__ U, double *__restrict__ T1) {
   constexpr int NN = N * N;
   extern __shared__ double smem[];
   double *Ds = smem, *DWs = smem + NN;
-  Tiles<N, NZT> T{smem + 2 * NN};
+  // two tile sets used alternately, so a phase's tiles are never rewritten
+  // before every lane has passed the next phase's barrier
+  Tiles<N, NZT> T{smem + 2 * NN}, TB{smem + 2 * NN + 5 * NN * NZT};
   load_dmats<N>(a, Ds, DWs);
   const int Nz = a.Nz;
   constexpr int nzt = NZT;
@@ -237,11 +257,21 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
     __builtin_assume(ui < (1u << 28));
     const unsigned mo = (unsigned)(node * Nz + iz);
     __builtin_assume(mo < (1u << 28));
-    S11 = at(metE, mo);
-    S12 = at(metE + NNZ, mo);
-    S21 = at(metE + 2 * NNZ, mo);
-    S22 = at(metE + 3 * NNZ, mo);
-    Js = at(metE + 4 * NNZ, mo);
+    if constexpr (CM) {
+      const double *mN = a.met1n + (size_t)e * CGD_NMET1 * NN;
+      const unsigned nd = (unsigned)node;
+      S11 = at(mN, nd);
+      S12 = at(mN + NN, nd);
+      S21 = at(mN + 2 * NN, nd);
+      S22 = at(mN + 3 * NN, nd);
+      Js = at(mN + 4 * NN, nd);
+    } else {
+      S11 = at(metE, mo);
+      S12 = at(metE + NNZ, mo);
+      S21 = at(metE + 2 * NNZ, mo);
+      S22 = at(metE + 3 * NNZ, mo);
+      Js = at(metE + 4 * NNZ, mo);
+    }
     iMs = at(a.iMs, ui);
     // the DSS accumulators of this node: no other element of the colour
     // touches them, so they are read now, with the state, not after the
@@ -280,20 +310,19 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
     ThCx = S11 * GDx + S12 * GDy;  // Contra12
     ThCy = S21 * GDx + S22 * GDy;
   }
-  __syncthreads();
   if (in_tile) {
-    T.at(0, node, l) = Curl;
-    T.at(1, node, l) = Div;
-    T.at(2, node, l) = ThCx;
-    T.at(3, node, l) = ThCy;
+    TB.at(0, node, l) = Curl;
+    TB.at(1, node, l) = Div;
+    TB.at(2, node, l) = ThCx;
+    TB.at(3, node, l) = ThCy;
   }
   __syncthreads();
   if (valid) {
-    const double DxCurl = dx<N, NZT>(DWs, T, 0, I, J, l);
-    const double DyCurl = dy<N, NZT>(DWs, T, 0, I, J, l);
-    const double DxDiv = dx<N, NZT>(DWs, T, 1, I, J, l);
-    const double DyDiv = dy<N, NZT>(DWs, T, 1, I, J, l);
-    const double DivTh = dxy<N, NZT>(DWs, T, 2, 3, I, J, l);
+    const double DxCurl = dx<N, NZT>(DWs, TB, 0, I, J, l);
+    const double DyCurl = dy<N, NZT>(DWs, TB, 0, I, J, l);
+    const double DxDiv = dx<N, NZT>(DWs, TB, 1, I, J, l);
+    const double DyDiv = dy<N, NZT>(DWs, TB, 1, I, J, l);
+    const double DivTh = dxy<N, NZT>(DWs, TB, 2, 3, I, J, l);
     const double FuC = S12 * DxCurl + S22 * DyCurl;  // Rot12
     const double FvC = -S11 * DxCurl - S21 * DyCurl;
     const double FuD = S11 * DxDiv + S21 * DyDiv;  // Grad12
@@ -325,7 +354,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
 // thread adds the fluxes of both its interfaces to its own level (the
 // reference adds them from the owner of the interface); F4 of interface iz
 // combines this layer's upper terms with the layer above's lower terms.
-template <int N, int NZT>
+template <int N, int NZT, bool CM>
 __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, int slot0, int nchunk,
                                                const double *__restrict__ U, const double *__restrict__ C,
                                                const double *__restrict__ P, double *__restrict__ F,
@@ -334,7 +363,10 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   static_assert(NZT >= 3 && NZT <= 16, "chunk heights 3..16 (the DPP neighbour reach is 2)");
   extern __shared__ double smem[];
   double *Ds = smem, *DWs = smem + NN;
-  Tiles<N, NZT> T{smem + 2 * NN};
+  // two tile sets used alternately (phases 1, 3a, 4 in T; 2, 3b in TB), so
+  // a phase's tiles are never rewritten before every lane has passed the
+  // next phase's barrier: one barrier per phase
+  Tiles<N, NZT> T{smem + 2 * NN}, TB{smem + 2 * NN + 8 * NN * NZT};
   load_dmats<N>(a, Ds, DWs);
   const int Nz = a.Nz;
   constexpr int nzt = NZT;
@@ -358,7 +390,8 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   // saddr form with no 64-bit vector address arithmetic.  The host checks
   // NumG*Nz < 2^28.
   const double *Ur = U, *Uu = U + vs, *Uv = U + 2 * vs, *Uw = U + 3 * vs, *Ut = U + 4 * vs;
-  const double *metE = a.met + (size_t)e * CGD_NMET * NNZ;
+  const double *metE = a.met + (size_t)e * (CM ? CGD_NMETP : CGD_NMET) * NNZ;
+  const double *metN = CM ? a.metn + (size_t)e * CGD_NMETN * NN : nullptr;
   unsigned ui = 0, mo = 0;
   bool first = false;
   if (valid) {
@@ -369,7 +402,9 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   }
   __builtin_assume(ui < (1u << 28));
   __builtin_assume(mo < (1u << 28));
-#define MET(c, off) at(metE + (size_t)(c) * NNZ, mo + (off))
+#define MET(c, off) at(metE + (size_t)(CM ? met_pidx(c) : (c)) * NNZ, mo + (off))
+#define METN(c) at(metN + (size_t)met_nidx(c) * NN, (unsigned)node)
+#define MET0(c) (CM && !met_per_point(c) ? METN(c) : MET(c, 0))
   // own level: state and metric (used in every phase)
   double rho = 1, u = 0, v = 0, w = 0, rth = 1, iMs = 1, M1 = 1, M2 = 1, p = 0;
   double d111 = 0, d112 = 0, d121 = 0, d122 = 0, d211 = 0, d212 = 0, d221 = 0, d222 = 0;
@@ -385,10 +420,10 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     M1 = at(M1b, ui);
     M2 = at(M2b, ui);
     p = at(P, ui);
-    d111 = MET(M111, 0); d112 = MET(M112, 0); d121 = MET(M121, 0); d122 = MET(M122, 0);
-    d211 = MET(M211, 0); d212 = MET(M212, 0); d221 = MET(M221, 0); d222 = MET(M222, 0);
-    d311 = MET(M311, 0); d312 = MET(M312, 0); d321 = MET(M321, 0); d322 = MET(M322, 0);
-    d331 = MET(M331, 0); d332 = MET(M332, 0); J1 = MET(MJ1, 0); J2 = MET(MJ2, 0); Wc = MET(MCOR, 0);
+    d111 = MET0(M111); d112 = MET0(M112); d121 = MET0(M121); d122 = MET0(M122);
+    d211 = MET0(M211); d212 = MET0(M212); d221 = MET0(M221); d222 = MET0(M222);
+    d311 = MET0(M311); d312 = MET0(M312); d321 = MET0(M321); d322 = MET0(M322);
+    d331 = MET0(M331); d332 = MET0(M332); J1 = MET0(MJ1); J2 = MET0(MJ2); Wc = MET0(MCOR);
   }
   const double Js = J1 + J2;
   const double irho = rcp(rho);
@@ -411,20 +446,23 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   const int lr = threadIdx.x & 15;  // position in the DPP row
   const bool edge_m1 = l < 1 || lr < 1, edge_m2 = l < 2 || lr < 2;
   const bool edge_p1 = l + 1 >= nzt || lr + 1 > 15, edge_p2 = l + 2 >= nzt || lr + 2 > 15;
+  // (in the compact metric layout the level-invariant components of the
+  // neighbour levels are this level's own)
   // level iz-1
   double rho_m = dpp_from_below<1>(rho), u_m = dpp_from_below<1>(u), v_m = dpp_from_below<1>(v);
-  double w_m = dpp_from_below<1>(w), c_m = dpp_from_below<1>(c_own), Jc_m = dpp_from_below<1>(Js);
+  double w_m = dpp_from_below<1>(w), c_m = dpp_from_below<1>(c_own), Jc_m = CM ? Js : dpp_from_below<1>(Js);
   double M2_m = dpp_from_below<1>(M2), d312m = dpp_from_below<1>(d312), d322m = dpp_from_below<1>(d322);
-  double d332m = dpp_from_below<1>(d332);
+  double d332m = CM ? d332 : dpp_from_below<1>(d332);
   // level iz-2 (clamped at the bottom)
-  double c_mm = dpp_from_below<2>(c_own), Jc_mm = dpp_from_below<2>(Js);
+  double c_mm = dpp_from_below<2>(c_own), Jc_mm = CM ? Js : dpp_from_below<2>(Js);
   // level iz+1
   double rho_p = dpp_from_above<1>(rho), u_p = dpp_from_above<1>(u), v_p = dpp_from_above<1>(v);
   double c_p = dpp_from_above<1>(c_own), p_p = dpp_from_above<1>(p), M1_p = dpp_from_above<1>(M1);
-  double J1_p = dpp_from_above<1>(J1), Jc_p = dpp_from_above<1>(Js), d311p = dpp_from_above<1>(d311);
-  double d321p = dpp_from_above<1>(d321), d331p = dpp_from_above<1>(d331);
+  double J1_p = CM ? J1 : dpp_from_above<1>(J1), Jc_p = CM ? Js : dpp_from_above<1>(Js);
+  double d311p = dpp_from_above<1>(d311), d321p = dpp_from_above<1>(d321);
+  double d331p = CM ? d331 : dpp_from_above<1>(d331);
   // level iz+2 (clamped at the top)
-  double c_pp = dpp_from_above<2>(c_own), Jc_pp = dpp_from_above<2>(Js);
+  double c_pp = dpp_from_above<2>(c_own), Jc_pp = CM ? Js : dpp_from_above<2>(Js);
   if (valid) {
     if (has_lo && edge_m1) {
       const unsigned um = ui - 1;
@@ -433,15 +471,17 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
       v_m = at(Uv, um);
       w_m = at(Uw, um);
       c_m = at(Ut, um) * rcp(rho_m);
-      Jc_m = MET(MJ1, -1) + MET(MJ2, -1);
       M2_m = at(M2b, um);
       d312m = MET(M312, -1);
       d322m = MET(M322, -1);
-      d332m = MET(M332, -1);
+      if constexpr (!CM) {
+        Jc_m = MET(MJ1, -1) + MET(MJ2, -1);
+        d332m = MET(M332, -1);
+      }
     }
     if (iz >= 2 && edge_m2) {
       c_mm = at(Ut, ui - 2) * rcp(at(Ur, ui - 2));
-      Jc_mm = MET(MJ1, -2) + MET(MJ2, -2);
+      if constexpr (!CM) Jc_mm = MET(MJ1, -2) + MET(MJ2, -2);
     }
     if (iz == 1) {
       c_mm = c_m;
@@ -455,15 +495,17 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
       c_p = at(Ut, up) * rcp(rho_p);
       p_p = at(P, up);
       M1_p = at(M1b, up);
-      J1_p = MET(MJ1, 1);
-      Jc_p = J1_p + MET(MJ2, 1);
       d311p = MET(M311, 1);
       d321p = MET(M321, 1);
-      d331p = MET(M331, 1);
+      if constexpr (!CM) {
+        J1_p = MET(MJ1, 1);
+        Jc_p = J1_p + MET(MJ2, 1);
+        d331p = MET(M331, 1);
+      }
     }
     if (iz + 2 <= Nz - 1 && edge_p2) {
       c_pp = at(Ut, ui + 2) * rcp(at(Ur, ui + 2));
-      Jc_pp = MET(MJ1, 2) + MET(MJ2, 2);
+      if constexpr (!CM) Jc_pp = MET(MJ1, 2) + MET(MJ2, 2);
     }
     if (iz + 2 > Nz - 1) {
       c_pp = c_p;
@@ -610,40 +652,38 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     if (has_hi) f4up += (GwF2 + Gpup) * iden_up;
     if (has_lo) f4lo += GwF1 * iden_lo;
   }
-  __syncthreads();
   // ---- phase 2: second hyperviscosity apply, horizontal mass / theta flux ----
   if (in_tile) {
-    T.at(0, node, l) = Curl;
-    T.at(1, node, l) = Div;
-    T.at(2, node, l) = ThCx;
-    T.at(3, node, l) = ThCy;
-    T.at(4, node, l) = (-rho) * (S11 * u + S12 * v);  // Contra12(-Rho, u, v)
-    T.at(5, node, l) = (-rho) * (S21 * u + S22 * v);
-    T.at(6, node, l) = c_own;
+    TB.at(0, node, l) = Curl;
+    TB.at(1, node, l) = Div;
+    TB.at(2, node, l) = ThCx;
+    TB.at(3, node, l) = ThCy;
+    TB.at(4, node, l) = (-rho) * (S11 * u + S12 * v);  // Contra12(-Rho, u, v)
+    TB.at(5, node, l) = (-rho) * (S21 * u + S22 * v);
+    TB.at(6, node, l) = c_own;
   }
   __syncthreads();
   if (valid) {
     {
-      const double DxCurl = dx<N, NZT>(DWs, T, 0, I, J, l);
-      const double DyCurl = dy<N, NZT>(DWs, T, 0, I, J, l);
+      const double DxCurl = dx<N, NZT>(DWs, TB, 0, I, J, l);
+      const double DyCurl = dy<N, NZT>(DWs, TB, 0, I, J, l);
       const double FuC = S12 * DxCurl + S22 * DyCurl;
       const double FvC = -S11 * DxCurl - S21 * DyCurl;
-      const double DxDiv = dx<N, NZT>(DWs, T, 1, I, J, l);
-      const double DyDiv = dy<N, NZT>(DWs, T, 1, I, J, l);
+      const double DxDiv = dx<N, NZT>(DWs, TB, 1, I, J, l);
+      const double DyDiv = dy<N, NZT>(DWs, TB, 1, I, J, l);
       const double FuD = S11 * DxDiv + S21 * DyDiv;
       const double FvD = S12 * DxDiv + S22 * DyDiv;
       f2 += -(-a.Kc * FuC + a.Kg * FuD) * iMs;
       f3 += -(-a.Kc * FvC + a.Kg * FvD) * iMs;
     }
-    const double DivTh = dxy<N, NZT>(DWs, T, 2, 3, I, J, l);
+    const double DivTh = dxy<N, NZT>(DWs, TB, 2, 3, I, J, l);
     f5 += -a.Kd * DivTh * iMs;
     // DivRhoTrUpwind3New2Kernel!: horizontal part
-    const double DivRhoTr = dxy_prod<N, NZT>(Ds, T, 4, 5, 6, I, J, l);
-    const double DivRho = dxy<N, NZT>(Ds, T, 4, 5, I, J, l);
+    const double DivRhoTr = dxy_prod<N, NZT>(Ds, TB, 4, 5, 6, I, J, l);
+    const double DivRho = dxy<N, NZT>(Ds, TB, 4, 5, I, J, l);
     f5 += DivRhoTr * iMs;
     f1 += DivRho * iMs;
   }
-  __syncthreads();
   // ---- phase 3: vector-invariant momentum + Coriolis ----
   // U1 = 1/2 (tempuZ[2]-tempuZ[1]) + sum_k (D[I,k] tempuZ1(k,J) + D[J,k] tempuZ2(I,k))
   // and likewise V, W for the bottom (K=1) and top (K=2) vertical node; the
@@ -664,20 +704,19 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     V1 = dxy_acc<N, NZT>(V1, Ds, T, 2, 3, I, J, l);
     W1 = dxy_acc<N, NZT>(W1, Ds, T, 4, 5, I, J, l);
   }
-  __syncthreads();
   if (in_tile) {
-    T.at(0, node, l) = -d122 * w;  // K=2
-    T.at(1, node, l) = -d222 * w;
-    T.at(2, node, l) = -d112 * w;
-    T.at(3, node, l) = -d212 * w;
-    T.at(4, node, l) = d122 * u - d112 * v;
-    T.at(5, node, l) = d222 * u - d212 * v;
+    TB.at(0, node, l) = -d122 * w;  // K=2
+    TB.at(1, node, l) = -d222 * w;
+    TB.at(2, node, l) = -d112 * w;
+    TB.at(3, node, l) = -d212 * w;
+    TB.at(4, node, l) = d122 * u - d112 * v;
+    TB.at(5, node, l) = d222 * u - d212 * v;
   }
   __syncthreads();
   if (valid) {
-    U2 = dxy_acc<N, NZT>(U2, Ds, T, 0, 1, I, J, l);
-    V2 = dxy_acc<N, NZT>(V2, Ds, T, 2, 3, I, J, l);
-    W2 = dxy_acc<N, NZT>(W2, Ds, T, 4, 5, I, J, l);
+    U2 = dxy_acc<N, NZT>(U2, Ds, TB, 0, 1, I, J, l);
+    V2 = dxy_acc<N, NZT>(V2, Ds, TB, 2, 3, I, J, l);
+    W2 = dxy_acc<N, NZT>(W2, Ds, TB, 4, 5, I, J, l);
     // CoriolisShallow: Wc = 2 Omega sin(lat) at the layer midpoint
     FU += (v * Wc) * Js;
     FV += (-u * Wc) * Js;
@@ -686,7 +725,6 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     if (has_lo) f4lo += rho * (u * V1 + v * U1 + FW1) * iden_lo;
     if (has_hi) f4up += rho * (u * V2 + v * U2 + FW2) * iden_up;
   }
-  __syncthreads();
   // ---- phase 4: F4 from the layer above (same element, same node) ----
   if (in_tile) T.at(0, node, l) = f4lo;
   __syncthreads();
@@ -706,6 +744,8 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     if (l == 0 && has_lo) F4x[((size_t)e * NN + node) * nchunk + chunk] = f4lo;
   }
 #undef MET
+#undef METN
+#undef MET0
 }
 
 // F4 of the interfaces between level chunks: the lower-interface term of a
@@ -797,8 +837,9 @@ __device__ __forceinline__ double col_solve(double t0, double t1, double t2, dou
 
 // TriDiagKernel! (SchurSolve.jl:45-74) for row i from the J blocks of
 // rows i-1, i, i+1 (neighbour rows reached by shuffles)
+// (invfac2 = (1/fac)/fac, formed once on the host)
 __device__ __forceinline__ void col_tri(double jrw0, double jrw1, double jwr0, double jwr1, double jwt0,
-                                        double jwt1, double jtw0, double jtw1, double fac, int i, int n,
+                                        double jwt1, double jtw0, double jtw1, double invfac2, int i, int n,
                                         double &t0, double &t1, double &t2) {
   const double jwr0m = __shfl_up(jwr0, 1), jwt0m = __shfl_up(jwt0, 1);
   const double jwr1p = __shfl_down(jwr1, 1), jwt1p = __shfl_down(jwt1, 1);
@@ -806,7 +847,6 @@ __device__ __forceinline__ void col_tri(double jrw0, double jrw1, double jwr0, d
   t1 = 1;
   t2 = 0;
   if (i < n) {
-    const double invfac = 1.0 / fac, invfac2 = invfac / fac;
     t1 = invfac2 - jwr1 * jrw0 - jwr0 * jrw1 - jwt1 * jtw0 - jwt0 * jtw1;
     t0 = (i == 0) ? 0.0 : -jwr0m * jrw0 - jwt0m * jtw0;
     t2 = (i == n - 1) ? 0.0 : -jwr1p * jrw1 - jwt1p * jtw1;
@@ -825,7 +865,7 @@ __global__ void __launch_bounds__(256) k_schur_col(KArgs a, double *__restrict__
                                                    const double *__restrict__ JRhoW, const double *__restrict__ JWRho,
                                                    const double *__restrict__ JWRhoTh,
                                                    const double *__restrict__ JRhoThW, double *__restrict__ tri,
-                                                   double fac, int comp_tri, int thomas) {
+                                                   double fac, double invfac2, int comp_tri, int thomas) {
   const int Nz = a.Nz, n = Nz - 1;
   const int i = threadIdx.x & 63;
   const size_t ig = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -847,7 +887,7 @@ __global__ void __launch_bounds__(256) k_schur_col(KArgs a, double *__restrict__
   }
   double t0, t1, t2;
   if (comp_tri) {
-    col_tri(jrw0, jrw1, jwr0, jwr1, jwt0, jwt1, jtw0, jtw1, fac, i, n, t0, t1, t2);
+    col_tri(jrw0, jrw1, jwr0, jwr1, jwt0, jwt1, jtw0, jtw1, invfac2, i, n, t0, t1, t2);
     if (row) {
       tri[tb] = t0;
       tri[tb + 1] = t1;
@@ -957,8 +997,8 @@ __global__ void k_jac(KArgs a, const double *__restrict__ U, const double *__res
     const int iz = (int)(q - ig * n);
     const size_t i0 = ig * Nz + iz;
     const double RhoThL = U[i0 + 4 * vs] + P4[i0], RhoThR = U[i0 + 1 + 4 * vs] + P4[i0 + 1];
-    const JacBlocks J = jac_blocks(a, U[i0], U[i0 + 1], RhoThL, RhoThR, eos_dpdrhoth(a.Rd, a.p0, a.kappa, RhoThL),
-                                   eos_dpdrhoth(a.Rd, a.p0, a.kappa, RhoThR), dz[i0], dz[i0 + 1]);
+    const JacBlocks J = jac_blocks(a, U[i0], U[i0 + 1], RhoThL, RhoThR, eos_dpdrhoth(a, RhoThL),
+                                   eos_dpdrhoth(a, RhoThR), dz[i0], dz[i0 + 1]);
     const size_t o = 2 * q;
     JRhoW[o] = J.rw0;
     JRhoW[o + 1] = J.rw1;
@@ -998,9 +1038,11 @@ struct RWStage {
   const int *nptr, *ncopy; // node -> element copies, colour order
   int nchunk, nzt;
   double fac;              // dt * gamma
-  int stage, thomas;
+  double invfac, invfac2;  // 1/fac and (1/fac)/fac
+  int thomas;
 };
 
+template <int STAGE>
 __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage rw) {
   const int Nz = a.Nz, n = Nz - 1;
   const int i = threadIdx.x & 63;
@@ -1018,13 +1060,13 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage rw) {
     dzi = rw.dz[c0 + i];
   }
   // frozen Jacobian at interface i (levels i, i+1) and the tri matrix
-  const double dpd = lev ? eos_dpdrhoth(a.Rd, a.p0, a.kappa, vn[4]) : 0.0;
+  const double dpd = lev ? eos_dpdrhoth(a, vn[4]) : 0.0;
   const double rhoR = __shfl_down(vn[0], 1), rthR = __shfl_down(vn[4], 1);
   const double dpdR = __shfl_down(dpd, 1), dzR = __shfl_down(dzi, 1);
   JacBlocks J{0, 0, 0, 0, 0, 0, 0, 0};
   if (row) J = jac_blocks(a, vn[0], rhoR, vn[4], rthR, dpd, dpdR, dzi, dzR);
   double t0, t1, t2;
-  col_tri(J.rw0, J.rw1, J.wr0, J.wr1, J.wt0, J.wt1, J.tw0, J.tw1, fac, i, n, t0, t1, t2);
+  col_tri(J.rw0, J.rw1, J.wr0, J.wr1, J.wt0, J.wt1, J.tw0, J.tw1, rw.invfac2, i, n, t0, t1, t2);
   // tendency, chunk-interface w terms, stage terms
   double v[5] = {0, 0, 0, 0, 0};
   double kp[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
@@ -1037,7 +1079,8 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage rw) {
       for (int c = rw.nptr[ig]; c < rw.nptr[ig + 1]; ++c) s += rw.F4x[(size_t)rw.ncopy[c] * rw.nchunk + ch];
       v[3] += s;
     }
-    for (int j = 0; j < rw.stage; ++j) {
+#pragma unroll
+    for (int j = 0; j < STAGE; ++j) {
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
         kp[j][q] = rw.kprev[j][c0 + i + q * vs];
@@ -1048,7 +1091,7 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage rw) {
   // SchurSolveFKernel!, the tridiagonal solve, SchurSolveBKernel!
   const double v1p = __shfl_down(v[0], 1), v5p = __shfl_down(v[4], 1);
   double rhs = 0.0;
-  if (row) rhs = v[3] / fac + J.wr1 * v[0] + J.wr0 * v1p + J.wt1 * v[4] + J.wt0 * v5p;
+  if (row) rhs = v[3] * rw.invfac + J.wr1 * v[0] + J.wr0 * v1p + J.wt1 * v[4] + J.wt0 * v5p;
   const double k4 = col_solve(t0, t1, t2, rhs, i, n, rw.thomas);
   const double k4m = __shfl_up(k4, 1);
   const double jrw1m = __shfl_up(J.rw1, 1), jtw1m = __shfl_up(J.tw1, 1);
@@ -1072,8 +1115,9 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage rw) {
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
     double r = vn[q];
-    for (int j = 0; j < rw.stage; ++j) r = r + rw.av[j] * kp[j][q];
-    r = r + rw.av[rw.stage] * kc[q];
+#pragma unroll
+    for (int j = 0; j < STAGE; ++j) r = r + rw.av[j] * kp[j][q];
+    r = r + rw.av[STAGE] * kc[q];
     rw.V[c0 + i + q * vs] = r;
     if (q == 4) rt = r;
   }
@@ -1224,6 +1268,48 @@ __global__ void k_relayout_metric(const double *__restrict__ dXdxI, const double
     met1[m1 + 4 * NNZ] = vals[14] + vals[15];
 #undef DXI
 #undef XA
+  }
+}
+
+// Bit mask of the metric components (met[slot][c][node][iz], ncomp
+// components) whose value differs between levels of some element node.
+__global__ void k_met_invariant(const double *__restrict__ met, int ncomp, int NF, int NN, int Nz,
+                                unsigned *__restrict__ mask) {
+  unsigned m = 0;
+  const size_t total = (size_t)NF * ncomp * NN;  // (slot, c, node) columns
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+    const int c = (int)((q / NN) % ncomp);
+    const double *col = met + q * Nz;
+    const long long b0 = __double_as_longlong(col[0]);
+    for (int iz = 1; iz < Nz; ++iz)
+      if (__double_as_longlong(col[iz]) != b0) {
+        m |= 1u << c;
+        break;
+      }
+  }
+  if (m) atomicOr(mask, m);
+}
+
+// Full -> compact metric layout (met_pidx / met_nidx), pass-1 table per node.
+__global__ void k_met_compact(const double *__restrict__ met, const double *__restrict__ met1,
+                              double *__restrict__ metp, double *__restrict__ metn, double *__restrict__ met1n,
+                              int NF, int NN, int Nz) {
+  const size_t total = (size_t)NF * NN * Nz;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(q % Nz);
+    const size_t r = q / Nz;
+    const int node = (int)(r % NN);
+    const size_t e = r / NN;
+    for (int c = 0; c < CGD_NMET; ++c) {
+      const double x = met[((e * CGD_NMET + c) * NN + node) * Nz + iz];
+      if (met_per_point(c))
+        metp[((e * CGD_NMETP + met_pidx(c)) * NN + node) * Nz + iz] = x;
+      else if (iz == 0)
+        metn[(e * CGD_NMETN + met_nidx(c)) * NN + node] = x;
+    }
+    if (iz == 0)
+      for (int c = 0; c < CGD_NMET1; ++c)
+        met1n[(e * CGD_NMET1 + c) * NN + node] = met1[((e * CGD_NMET1 + c) * NN + node) * Nz];
   }
 }
 

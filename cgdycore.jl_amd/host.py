@@ -85,13 +85,15 @@ def native():
     L.cgd_profile_enable.argtypes = [_vp, C.c_int]
     L.cgd_profile_read.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.cgd_color_elements.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp]
+    L.cgd_ctx_info.argtypes = [_vp, _vp, _vp, _vp]
     _LIB = L
     return L
 
 
 def exported_symbols():
     return ["cgd_ctx_create", "cgd_ctx_destroy", "cgd_last_error", "cgd_prepare", "cgd_fcn", "cgd_jac",
-            "cgd_schur", "cgd_rw_step", "cgd_profile_enable", "cgd_profile_read", "cgd_color_elements"]
+            "cgd_schur", "cgd_rw_step", "cgd_profile_enable", "cgd_profile_read", "cgd_color_elements",
+            "cgd_ctx_info"]
 
 
 def color_elements(glob: np.ndarray, num_g: int):
@@ -267,6 +269,12 @@ class NativeContext:
             self.close()
         except Exception:
             pass
+
+    def info(self):
+        """(number of colours, chunk height in levels, compact metric layout?)"""
+        nc, nz, cm = C.c_int(), C.c_int(), C.c_int()
+        self.check(self.L.cgd_ctx_info(self.h, C.byref(nc), C.byref(nz), C.byref(cm)), "cgd_ctx_info")
+        return nc.value, nz.value, bool(cm.value)
 
     def profile(self, mask=7):
         """Time the passes in ``mask`` (bit 0 pass 1, bit 1 pass 2, bit 2 column; 0 = off)."""

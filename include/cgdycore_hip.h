@@ -106,6 +106,13 @@ typedef struct {
 } cgd_rw_work;
 int cgd_rw_step(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, void *stream);
 
+/* The schedule and layout cgd_ctx_create chose: the number of element
+ * colours (DSS launches per pass), the level chunk height of the element
+ * kernels, and whether the metric is stored in the compact layout (every
+ * component except the terrain row dXdxI[3,1:2,:] identical across levels,
+ * so stored once per element node).  No reference counterpart. */
+int cgd_ctx_info(const cgd_ctx *ctx, int *ncolor, int *chunk_levels, int *compact_metric);
+
 /* Host-only helper (no GPU needed): the element colouring that schedules the
  * DSS.  glob is a HOST int64 (N*N, NF) 1-based array; color receives NF
  * colours (0-based), ncolor their count.  Elements of one colour share no

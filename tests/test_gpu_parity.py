@@ -3,6 +3,8 @@
 Bar: relative L2 error per prognostic variable <= 1e-12 for the FP64 path
 (the tolerance of the plain FP64 drop-in), on the seeded jet state.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -87,6 +89,45 @@ def test_fcn_prepare_parity(pkg, oracle_mod, torch_cuda, ne, nz, noisy):
         tol = TOL if (noisy or v != 0) else 1e-11
         assert e <= tol, f"F{v + 1} rel L2 {e:.3e}"
     assert np.all(Fg[3][:, -1] == 0.0)  # the w lid receives no tendency
+
+
+@pytest.mark.parametrize("ne,nz,varying", [(3, 10, True), (4, 64, True), (4, 64, False)])
+def test_fcn_parity_metric_layouts(pkg, oracle_mod, torch_cuda, ne, nz, varying):
+    """The flat, uniform grid's metric is level-invariant except for the
+    terrain row, and the context stores it compactly; a metric whose
+    horizontal block varies with height (as over terrain or on a stretched
+    grid; here a seeded level-dependent scaling) keeps the full per-point
+    layout.  Both agree with the oracle on the same inputs."""
+    torch = torch_cuda
+    host = pkg.host
+    D = pkg.discretization
+    disc = D.build(ne, 4, nz, 30000.0)
+    if varying:
+        scale = 1.0 + 1e-3 * np.sin(0.7 * np.arange(nz))
+        disc.dXdxI = disc.dXdxI * scale[None, :, None, None, None, None]
+    phys = D.Phys()
+    U0 = perturbed(pkg.state.jet_state(disc, seed=1), 7)
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda")
+    orc = oracle_mod.Oracle(disc, phys, (Global.Model.HyperDCurl, Global.Model.HyperDGrad, Global.Model.HyperDDiv))
+    print(f"metric layout parity ne={ne} nz={nz} varying={varying}", flush=True)
+    U = torch.as_tensor(U0, device="cuda")
+    F = torch.empty_like(U)
+    eq = host.CompressibleShallow()
+    host.FcnPrepareGPU(U, FE, Metric, phys, Cache, None, Global, None, eq)
+    host.FcnGPU(F, U, FE, Metric, phys, Cache, None, Global, None, eq)
+    torch.cuda.synchronize()
+    ncolor, chunk, compact = Global._ctx.info()
+    assert ncolor == 4 and chunk == 8
+    # CGD_METRIC=full forces the full layout for every grid
+    assert compact == (not varying and os.environ.get("CGD_METRIC") != "full")
+    A = orc.prepare(U0.copy())
+    Fo, T1o = orc.fcn(U0.copy(), A)
+    Fg, T1g = F.cpu().numpy(), Cache.Temp1.cpu().numpy()
+    for v in range(5):
+        assert rel_l2(T1g[v], T1o[v]) <= TOL, f"Temp1[{v}]"
+        e = rel_l2(Fg[v], Fo[v])
+        print(f"  F{v + 1} rel L2 {e:.3e}")
+        assert e <= TOL, f"F{v + 1} rel L2 {e:.3e}"
 
 
 @pytest.mark.parametrize("ne,nz", [(3, 8), (4, 64)])
