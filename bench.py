@@ -63,6 +63,47 @@ def measured_traffic(kernel, workload, algorithmic):
     return t["traffic_bytes_per_eval"]
 
 
+def mixed_precision(host, disc, phys, U0, args, fp64_ms, V_fp64):
+    """The same steps from the same initial state with the tendency and DSS
+    in FP32 (FP64 EOS, w-interface balance, horizontal pressure gradient and
+    column solve): throughput, and relative L2 / Linf error per prognostic
+    variable against the FP64 state after the same number of steps."""
+    import torch
+
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, hyperd=args.hyperd, device="cuda:0",
+                                                          precision="mixed")
+    eq = host.CompressibleShallow()
+    host.init_context(FE, Metric, phys, Global)
+    V = torch.as_tensor(U0, device="cuda:0").clone()
+
+    def step():
+        host.RosenbrockSchur(V, args.dt, host.FcnGPU, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys,
+                             Cache, JCache, None, Global, None, eq)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = wall / args.steps * 1e3
+    names = ("rho", "u", "v", "w", "rhotheta")
+    rel_l2, rel_linf = {}, {}
+    for c, n in enumerate(names):
+        d = (V[c] - V_fp64[c]).double()
+        ref = V_fp64[c].double()
+        rel_l2[n] = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(ref))
+        rel_linf[n] = float(d.abs().max() / ref.abs().max())
+    Global._ctx.close()
+    return {"value": round(disc.element_points * args.steps / wall / 1e6, 2), "unit": "Mpts/s",
+            "ms_per_step": round(ms, 4), "speedup_vs_fp64": round(fp64_ms / ms, 3),
+            "steps_compared": args.warmup + args.steps, "rel_l2_vs_fp64": rel_l2, "rel_linf_vs_fp64": rel_linf,
+            "precision": "FP32 tendency and DSS; FP64 EOS, w-interface balance, horizontal pressure gradient, "
+                         "column solve and stage arithmetic"}
+
+
 def cpu_info():
     model = ""
     try:
@@ -89,6 +130,7 @@ def main():
     ap.add_argument("--hyperd", type=float, default=None, help="hyperviscosity coefficient (m^4/s)")
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="oracle RW steps for the CPU baseline, about 5 s each at ne=30 (0: skip)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -145,6 +187,7 @@ def main():
     wall = time.perf_counter() - t0
     p2_ms, p2_launch = ctx.profile_read(1)
     finite = bool(torch.isfinite(V).all().item())
+    V_fp64 = V.clone()  # warmup + steps from the initial state, for the mixed-precision comparison
     # per-pass breakdown from a few extra steps with every pass bracketed
     nb = min(args.steps, 5)
     ctx.profile(ALL_PASSES)
@@ -174,6 +217,11 @@ def main():
                 "step_roofline_frac": round(sb["step"] / (wall / args.steps) / 1e9 / PEAK_HBM_GBS, 4),
                 "pass_ms_per_step": {k: round(v[0] / nb, 4) for k, v in passes.items()}}
 
+    mixed = None
+    if not args.no_mixed:
+        mixed = mixed_precision(host, disc, phys, U0, args, ms_per_step, V_fp64)
+        print(f"[bench] mixed precision: {mixed}", file=sys.stderr, flush=True)
+
     cpu = None
     if args.cpu_steps > 0:
         O = ge.load_oracle()
@@ -193,7 +241,7 @@ def main():
            "config": {"workload": workload, "ne": args.ne, "p": args.p,
                       "nz": args.nz, "dt": args.dt, "hyperd": Global.Model.HyperDDiv, "points": Ne,
                       "precision": "fp64", "finite": finite},
-           "roofline": roofline, "cpu_baseline": cpu}
+           "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed}
     print(json.dumps(out), flush=True)
 
 

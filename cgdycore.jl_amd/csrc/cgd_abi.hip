@@ -19,7 +19,6 @@ struct cgd_ctx {
   int N = 0, NN = 0, Nz = 0, NF = 0, NumG = 0;
   int nzt = 0, nchunk = 0, block = 0;
   int thomas = 0;  // column solve: 0 = cyclic reduction, 1 = serial Thomas (CGD_THOMAS=1)
-  size_t lds1 = 0, lds2 = 0;
   int ncolor = 0;
   std::vector<int> color_off;  // first element slot of each colour (elements are stored in colour order)
   int *d_glob = nullptr;       // [NF][NN] 0-based
@@ -27,6 +26,8 @@ struct cgd_ctx {
   double *d_met = nullptr, *d_met1 = nullptr, *d_ms = nullptr, *d_bwork = nullptr;
   double *d_metn = nullptr, *d_met1n = nullptr;  // compact metric layout: per element node
   int cm = 0;                                    // 1: compact metric layout
+  int mixed = 0;                                 // 1: mixed-precision cgd_rw_step
+  float *d_ims32 = nullptr;                      // 1/(M1+M2) in FP32 (mixed precision)
   double *d_f4x = nullptr;     // [NF][NN][nchunk] chunk-bottom F4 terms
   int *d_nptr = nullptr, *d_ncopy = nullptr;  // global node -> element copies (colour order)
   bool ok = false;
@@ -39,6 +40,16 @@ struct cgd_ctx {
   std::vector<int> prof_which, prof_launch;
   int prof_cap = 0, prof_used = 0;
 };
+
+// LDS bytes of the element kernels for tendency type R: pass 1 holds D, DW
+// and tile sets of 5 and 4; pass 2 an FP64 D and pressure tile, then D, DW
+// and two sets of 7 tiles
+template <typename R> static size_t lds_fcn1(const cgd_ctx *c) {
+  return sizeof(R) * (2 * c->NN + 9 * c->NN * c->nzt);
+}
+template <typename R> static size_t lds_fcn2(const cgd_ctx *c) {
+  return sizeof(double) * (c->NN + c->NN * c->nzt) + sizeof(R) * (2 * c->NN + 14 * c->NN * c->nzt);
+}
 
 static int fail(cgd_ctx *c, int code, const std::string &msg) {
   if (c) c->err = msg;
@@ -236,7 +247,9 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   if (d->NF < 1 || d->NumG < 1) return fail(c, CGD_ERR_ARG, "NF and NumG must be positive");
   if (d->NumV != 5) return fail(c, CGD_ERR_UNSUPPORTED, "NumV must be 5 (rho, u, v, w, rho*theta)");
   if (d->NumTr != 0) return fail(c, CGD_ERR_UNSUPPORTED, "tracers (NumTr > 0) are not implemented");
-  if (d->precision != CGD_FP64) return fail(c, CGD_ERR_UNSUPPORTED, "unknown precision mode");
+  if (d->precision != CGD_FP64 && d->precision != CGD_MIXED)
+    return fail(c, CGD_ERR_UNSUPPORTED, "unknown precision mode");
+  c->mixed = d->precision == CGD_MIXED;
   if (d->HorLimit || d->Turbulence || d->VerticalDiffusion || d->SurfaceFlux || d->Forcing || d->Microphysics ||
       d->Damping || d->EDMF)
     return fail(c, CGD_ERR_UNSUPPORTED, "only the dry CompressibleShallow path without physics hooks is implemented");
@@ -269,8 +282,6 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
     if (const char *ev = getenv("CGD_THOMAS")) c->thomas = atoi(ev) != 0;
   }
   c->block = ((NN * c->nzt + 63) / 64) * 64;
-  c->lds1 = sizeof(double) * (2 * NN + 9 * NN * c->nzt);   // D, DW, tile sets of 5 and 4
-  c->lds2 = sizeof(double) * (2 * NN + 16 * NN * c->nzt);  // D, DW, two sets of 8 tiles
 
   // global numbering -> 0-based int32, range check
   std::vector<int64_t> hglob((size_t)NN * NF);
@@ -375,6 +386,11 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
       c->cm = 1;
     }
   }
+  if (c->mixed) {
+    HIPCHK(c, hipMalloc(&c->d_ims32, ng * sizeof(float)));
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(ng, 256)), dim3(256), 0, st, c->d_ms, c->d_ims32, ng);
+    HIPCHK(c, hipGetLastError());
+  }
   HIPCHK(c, hipStreamSynchronize(st));
 
   KArgs &a = c->ka;
@@ -405,6 +421,7 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   a.met1n = c->d_met1n;
   a.met1 = c->d_met1;
   a.iMs = c->d_ms;
+  a.iMs32 = c->d_ims32;
   a.M = d->M;
   c->ok = true;
   return CGD_OK;
@@ -417,7 +434,7 @@ extern "C" int cgd_ctx_destroy(cgd_ctx *c) {
   // a context that failed validation holds no device memory: tear it down
   // without touching the HIP runtime
   for (void *p : {(void *)c->d_glob, (void *)c->d_slot, (void *)c->d_met, (void *)c->d_met1, (void *)c->d_metn,
-                  (void *)c->d_met1n, (void *)c->d_ms,
+                  (void *)c->d_met1n, (void *)c->d_ms, (void *)c->d_ims32,
                   (void *)c->d_bwork, (void *)c->d_f4x, (void *)c->d_nptr, (void *)c->d_ncopy})
     if (p) (void)hipFree(p);
   prof_free(c);
@@ -491,36 +508,40 @@ extern "C" int cgd_profile_read(cgd_ctx *c, int which, double *total_ms, long lo
 }
 
 
-// template dispatch on (N, chunk height, metric layout)
-template <int N, int NZT, bool CM>
-static void fcn1_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, double *T1) {
-  hipLaunchKernelGGL((k_fcn1<N, NZT, CM>), g, dim3(c->block), c->lds1, st, c->ka, slot0, c->nchunk, U, T1);
+// template dispatch on (N, chunk height, metric layout); R is the tendency
+// type (double, or float in the mixed-precision mode)
+template <int N, int NZT, bool CM, typename R>
+static void fcn1_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const R *UR, R *T1) {
+  hipLaunchKernelGGL((k_fcn1<N, NZT, CM, R>), g, dim3(c->block), lds_fcn1<R>(c), st, c->ka, slot0, c->nchunk, UR,
+                     T1);
 }
-template <int N, int NZT, bool CM>
-static void fcn2_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, const double *T1,
-                   const double *P, double *F) {
-  hipLaunchKernelGGL((k_fcn2<N, NZT, CM>), g, dim3(c->block), c->lds2, st, c->ka, slot0, c->nchunk, U, T1, P, F,
-                     c->d_f4x);
+template <int N, int NZT, bool CM, typename R>
+static void fcn2_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, const R *UR, const R *T1,
+                   const double *P, R *F) {
+  hipLaunchKernelGGL((k_fcn2<N, NZT, CM, R>), g, dim3(c->block), lds_fcn2<R>(c), st, c->ka, slot0, c->nchunk, U,
+                     UR, T1, P, F, reinterpret_cast<R *>(c->d_f4x));
 }
-#define CGD_DISPATCH(FN, ...)                                      \
+#define CGD_DISPATCH(FN, R, ...)                                   \
   do {                                                             \
     if (c->N == 5 && c->nzt == 16) {                               \
-      if (c->cm) FN<5, 16, true>(c, __VA_ARGS__);                  \
-      else FN<5, 16, false>(c, __VA_ARGS__);                       \
+      if (c->cm) FN<5, 16, true, R>(c, __VA_ARGS__);               \
+      else FN<5, 16, false, R>(c, __VA_ARGS__);                    \
     } else if (c->N == 5) {                                        \
-      if (c->cm) FN<5, 8, true>(c, __VA_ARGS__);                   \
-      else FN<5, 8, false>(c, __VA_ARGS__);                        \
+      if (c->cm) FN<5, 8, true, R>(c, __VA_ARGS__);                \
+      else FN<5, 8, false, R>(c, __VA_ARGS__);                     \
     } else {                                                       \
-      if (c->cm) FN<4, 8, true>(c, __VA_ARGS__);                   \
-      else FN<4, 8, false>(c, __VA_ARGS__);                        \
+      if (c->cm) FN<4, 8, true, R>(c, __VA_ARGS__);                \
+      else FN<4, 8, false, R>(c, __VA_ARGS__);                     \
     }                                                              \
   } while (0)
-static void launch_fcn1(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, double *T1) {
-  CGD_DISPATCH(fcn1_t, g, st, slot0, U, T1);
+template <typename R>
+static void launch_fcn1(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const R *UR, R *T1) {
+  CGD_DISPATCH(fcn1_t, R, g, st, slot0, UR, T1);
 }
-static void launch_fcn2(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, const double *T1,
-                        const double *P, double *F) {
-  CGD_DISPATCH(fcn2_t, g, st, slot0, U, T1, P, F);
+template <typename R>
+static void launch_fcn2(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, const R *UR, const R *T1,
+                        const double *P, R *F) {
+  CGD_DISPATCH(fcn2_t, R, g, st, slot0, U, UR, T1, P, F);
 }
 
 extern "C" int cgd_prepare(cgd_ctx *c, const double *U, double *AuxG, void *stream) {
@@ -535,8 +556,10 @@ extern "C" int cgd_prepare(cgd_ctx *c, const double *U, double *AuxG, void *stre
 
 // Both tendency passes; with f4fix the chunk-interface w terms are added by
 // k_f4fix (cgd_fcn), without it the caller consumes the side sums itself
-// (the fused stage kernel of cgd_rw_step).
-static int run_fcn(cgd_ctx *c, double *F, const double *U, double *Temp1, const double *AuxG, hipStream_t st,
+// (the fused stage kernel of cgd_rw_step).  U is the FP64 state; UR the
+// state the R-typed terms read (U itself, or its FP32 shadow).
+template <typename R>
+static int run_fcn(cgd_ctx *c, R *F, const double *U, const R *UR, R *Temp1, const double *AuxG, hipStream_t st,
                    bool f4fix) {
   int r;
   // every global node is written by its first element (colour order) and
@@ -546,7 +569,7 @@ static int run_fcn(cgd_ctx *c, double *F, const double *U, double *Temp1, const 
   for (int k = 0; k < c->ncolor; ++k) {
     const int ne = c->color_off[k + 1] - c->color_off[k];
     if (!ne) continue;
-    launch_fcn1(c, dim3(ne * c->nchunk), st, c->color_off[k], U, Temp1);
+    launch_fcn1<R>(c, dim3(ne * c->nchunk), st, c->color_off[k], UR, Temp1);
     HIPCHK(c, hipGetLastError());
     ++launches;
   }
@@ -556,16 +579,18 @@ static int run_fcn(cgd_ctx *c, double *F, const double *U, double *Temp1, const 
   for (int k = 0; k < c->ncolor; ++k) {
     const int ne = c->color_off[k + 1] - c->color_off[k];
     if (!ne) continue;
-    launch_fcn2(c, dim3(ne * c->nchunk), st, c->color_off[k], U, Temp1, AuxG, F);
+    launch_fcn2<R>(c, dim3(ne * c->nchunk), st, c->color_off[k], U, UR, Temp1, AuxG, F);
     HIPCHK(c, hipGetLastError());
     ++launches;
   }
-  if (f4fix && c->nchunk > 1) {
-    const size_t nfix = (size_t)c->NumG * (c->nchunk - 1);
-    hipLaunchKernelGGL(k_f4fix, dim3(grid_for(nfix, 256)), dim3(256), 0, st, c->d_nptr, c->d_ncopy, c->d_f4x, F,
-                       c->NumG, c->Nz, c->nchunk, c->nzt);
-    HIPCHK(c, hipGetLastError());
-    ++launches;
+  if constexpr (std::is_same<R, double>::value) {
+    if (f4fix && c->nchunk > 1) {
+      const size_t nfix = (size_t)c->NumG * (c->nchunk - 1);
+      hipLaunchKernelGGL(k_f4fix, dim3(grid_for(nfix, 256)), dim3(256), 0, st, c->d_nptr, c->d_ncopy, c->d_f4x, F,
+                         c->NumG, c->Nz, c->nchunk, c->nzt);
+      HIPCHK(c, hipGetLastError());
+      ++launches;
+    }
   }
   return prof_end(c, st, 1, launches);
 }
@@ -574,7 +599,7 @@ extern "C" int cgd_fcn(cgd_ctx *c, double *F, const double *U, double *Temp1, co
   int r = check(c);
   if (r) return r;
   if (!F || !U || !Temp1 || !AuxG) return fail(c, CGD_ERR_ARG, "cgd_fcn: null array");
-  return run_fcn(c, F, U, Temp1, AuxG, S(stream), true);
+  return run_fcn<double>(c, F, U, U, Temp1, AuxG, S(stream), true);
 }
 
 extern "C" int cgd_jac(cgd_ctx *c, double *JRhoW, double *JWRho, double *JWRhoTh, double *JRhoThW, const double *U,
@@ -610,6 +635,59 @@ extern "C" int cgd_schur(cgd_ctx *c, double *k, double *v, const double *JRhoW, 
   return prof_end(c, st, 2, 1);
 }
 
+// Fused step for Nz <= 64: the first pressure, then per stage the two
+// tendency passes and one column kernel (Jacobian, tri, stage terms, Schur
+// solve, next state and its pressure; the first also stores Vn = V).  With
+// R = float (mixed precision) the FP32 tendency and its FP32 state shadow
+// share the fV work buffer and the FP32 cache sits in Temp1.
+template <typename R>
+static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, hipStream_t st, double *const k[3],
+                         const double A[3][3], const double Cc[3][3], const double Mw[3], double gamma) {
+  int r;
+  const size_t ng = (size_t)c->NumG * c->Nz, n5 = 5 * ng;
+  const bool mixed = std::is_same<R, float>::value;
+  R *F = reinterpret_cast<R *>(w->fV);
+  R *T1 = reinterpret_cast<R *>(w->Temp1);
+  float *V32 = mixed ? reinterpret_cast<float *>(w->fV) + n5 : nullptr;
+  const R *UR = mixed ? reinterpret_cast<const R *>(V32) : reinterpret_cast<const R *>(V);
+  hipLaunchKernelGGL(k_rw_begin, dim3(grid_for(ng, 256)), dim3(256), 0, st, c->ka, V, w->AuxG, V32, ng);
+  HIPCHK(c, hipGetLastError());
+  for (int is = 0; is < 3; ++is) {
+    if ((r = run_fcn<R>(c, F, V, UR, T1, w->AuxG, st, false))) return r;
+    RWStage<R> rw{};
+    rw.fV = F;
+    rw.V32 = is < 2 ? V32 : nullptr;
+    for (int j = 0; j < is; ++j) {
+      rw.kprev[j] = k[j];
+      rw.cf[j] = Cc[is][j] / dt;
+    }
+    for (int j = 0; j <= is; ++j) rw.av[j] = (is < 2) ? A[is + 1][j] : Mw[j];
+    rw.Vn = w->Vn;
+    rw.Vnw = w->Vn;
+    rw.dz = c->desc.dz;
+    rw.V = V;
+    rw.k = is < 2 ? k[is] : nullptr;  // k_3 is consumed in registers
+    rw.P = is < 2 ? w->AuxG : nullptr;
+    rw.F4x = reinterpret_cast<const R *>(c->d_f4x);
+    rw.nptr = c->d_nptr;
+    rw.ncopy = c->d_ncopy;
+    rw.nchunk = c->nchunk;
+    rw.nzt = c->nzt;
+    rw.fac = dt * gamma;
+    rw.invfac = 1.0 / rw.fac;
+    rw.invfac2 = rw.invfac / rw.fac;
+    rw.thomas = c->thomas;
+    prof_begin(c, st, 2);
+    const dim3 gcol((c->NumG + 3) / 4);
+    if (is == 0) hipLaunchKernelGGL((k_rw_stage<0, R>), gcol, dim3(256), 0, st, c->ka, rw);
+    else if (is == 1) hipLaunchKernelGGL((k_rw_stage<1, R>), gcol, dim3(256), 0, st, c->ka, rw);
+    else hipLaunchKernelGGL((k_rw_stage<2, R>), gcol, dim3(256), 0, st, c->ka, rw);
+    HIPCHK(c, hipGetLastError());
+    if ((r = prof_end(c, st, 2, 1))) return r;
+  }
+  return CGD_OK;
+}
+
 extern "C" int cgd_rw_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, void *stream) {
   int r = check(c);
   if (r) return r;
@@ -627,43 +705,8 @@ extern "C" int cgd_rw_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *
   const size_t ng = (size_t)c->NumG * c->Nz, n5 = 5 * ng;
   double *k[3] = {w->k, w->k + n5, w->k + 2 * n5};
   if (c->Nz <= 64) {
-    // fused: Vn = V with the first pressure, then per stage the two
-    // tendency passes and one column kernel (Jacobian, tri, stage terms,
-    // Schur solve, next state and its pressure)
-    hipLaunchKernelGGL(k_rw_begin, dim3(grid_for(ng, 256)), dim3(256), 0, st, c->ka, V, w->Vn, w->AuxG, ng);
-    HIPCHK(c, hipGetLastError());
-    for (int is = 0; is < 3; ++is) {
-      if ((r = run_fcn(c, w->fV, V, w->Temp1, w->AuxG, st, false))) return r;
-      RWStage rw{};
-      rw.fV = w->fV;
-      for (int j = 0; j < is; ++j) {
-        rw.kprev[j] = k[j];
-        rw.cf[j] = Cc[is][j] / dt;
-      }
-      for (int j = 0; j <= is; ++j) rw.av[j] = (is < 2) ? A[is + 1][j] : Mw[j];
-      rw.Vn = w->Vn;
-      rw.dz = c->desc.dz;
-      rw.V = V;
-      rw.k = is < 2 ? k[is] : nullptr;  // k_3 is consumed in registers
-      rw.P = is < 2 ? w->AuxG : nullptr;
-      rw.F4x = c->d_f4x;
-      rw.nptr = c->d_nptr;
-      rw.ncopy = c->d_ncopy;
-      rw.nchunk = c->nchunk;
-      rw.nzt = c->nzt;
-      rw.fac = dt * gamma;
-      rw.invfac = 1.0 / rw.fac;
-      rw.invfac2 = rw.invfac / rw.fac;
-      rw.thomas = c->thomas;
-      prof_begin(c, st, 2);
-      const dim3 gcol((c->NumG + 3) / 4);
-      if (is == 0) hipLaunchKernelGGL(k_rw_stage<0>, gcol, dim3(256), 0, st, c->ka, rw);
-      else if (is == 1) hipLaunchKernelGGL(k_rw_stage<1>, gcol, dim3(256), 0, st, c->ka, rw);
-      else hipLaunchKernelGGL(k_rw_stage<2>, gcol, dim3(256), 0, st, c->ka, rw);
-      HIPCHK(c, hipGetLastError());
-      if ((r = prof_end(c, st, 2, 1))) return r;
-    }
-    return CGD_OK;
+    if (c->mixed) return rw_step_fused<float>(c, V, dt, w, st, k, A, Cc, Mw, gamma);
+    return rw_step_fused<double>(c, V, dt, w, st, k, A, Cc, Mw, gamma);
   }
   // tall columns: the reference's sequence of calls, stage terms by k_axpy
   HIPCHK(c, hipMemcpyAsync(w->Vn, V, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));

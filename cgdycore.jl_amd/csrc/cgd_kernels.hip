@@ -21,6 +21,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 #define CGD_NMAX 5
 #define CGD_NMET 17  // pass-2 metric components
 #define CGD_NMET1 5  // pass-1 metric components
@@ -40,6 +42,7 @@ struct KArgs {
   const double *metn;              // compact layout: [slot][13][NN]
   const double *met1n;             // compact layout: [slot][5][NN]
   const double *iMs;               // [NumG][Nz]: 1 / (M1 + M2)
+  const float *iMs32;              // the same in FP32 (mixed-precision mode)
   const double *M;                 // [2][NumG][Nz]
 };
 
@@ -73,8 +76,8 @@ __device__ __forceinline__ double eos_dpdrhoth(const KArgs &a, double rhoth) {
 }
 
 // Julia min/max on non-NaN values
-__device__ __forceinline__ double jmin(double a, double b) { return (b < a) ? b : a; }
-__device__ __forceinline__ double jmax(double a, double b) { return (b > a) ? b : a; }
+template <typename R> __device__ __forceinline__ R jmin(R a, R b) { return (b < a) ? b : a; }
+template <typename R> __device__ __forceinline__ R jmax(R a, R b) { return (b > a) ? b : a; }
 
 // 1/x for normal, finite x: the hardware estimate (v_rcp_f64) refined by two
 // Newton-Raphson steps r <- r + r (1 - x r), each at least doubling the
@@ -86,6 +89,14 @@ __device__ __forceinline__ double rcp(double x) {
   r = fma(r, e, r);
   e = fma(-x, r, 1.0);
   return fma(r, e, r);
+}
+__device__ __forceinline__ double rabs(double x) { return fabs(x); }
+__device__ __forceinline__ float rabs(float x) { return fabsf(x); }
+
+// FP32: v_rcp_f32 (1 ulp) and one Newton-Raphson step
+__device__ __forceinline__ float rcp(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return fmaf(r, fmaf(-x, r, 1.0f), r);
 }
 
 // Element i of a field at a uniform base: the byte offset is formed in 32
@@ -103,73 +114,75 @@ template <typename T> __device__ __forceinline__ const T &at(const T *base, unsi
 //   kR = JL/(JL+JR) * (JLL+JL)/(JLL+JL+JR),  kL = JL/(JLL+JL) * JR/(JLL+JL+JR)
 // share one reciprocal of the product of their three denominators (and
 // likewise for the right-biased pair).
-__device__ __forceinline__ void recu4(double cLL, double cL, double cR, double cRR, double JLL, double JL,
-                                      double JR, double JRR, double &cFL, double &cFR) {
-  const double A = JL + JR;
+template <typename R>
+__device__ __forceinline__ void recu4(R cLL, R cL, R cR, R cRR, R JLL, R JL, R JR, R JRR, R &cFL, R &cFR) {
+  const R A = JL + JR;
+  const R tiny = R(1.e-20), zero = R(0), one = R(1);
   {
-    const double B = JLL + JL, C = B + JR;
-    const double q = rcp(A * B * C);
-    const double kR = (JL * B) * B * q;
-    const double kL = (JL * JR) * A * q;
-    const double r = (cL - cLL) * rcp(cR - cL + 1.e-20);
-    cFL = cL + jmax(0.0, jmin(r, jmin(kL * r + kR, 1.0))) * (cR - cL);
+    const R B = JLL + JL, C = B + JR;
+    const R q = rcp(A * B * C);
+    const R kR = (JL * B) * B * q;
+    const R kL = (JL * JR) * A * q;
+    const R r = (cL - cLL) * rcp(cR - cL + tiny);
+    cFL = cL + jmax(zero, jmin(r, jmin(kL * r + kR, one))) * (cR - cL);
   }
   {
-    const double B = JRR + JR, C = A + JRR;
-    const double q = rcp(A * B * C);
-    const double kL = (JR * B) * B * q;
-    const double kR = (JR * JL) * A * q;
-    const double r = (cR - cRR) * rcp(cL - cR + 1.e-20);
-    cFR = cR + jmax(0.0, jmin(r, jmin(kR * r + kL, 1.0))) * (cL - cR);
+    const R B = JRR + JR, C = A + JRR;
+    const R q = rcp(A * B * C);
+    const R kL = (JR * B) * B * q;
+    const R kR = (JR * JL) * A * q;
+    const R r = (cR - cRR) * rcp(cL - cR + tiny);
+    cFR = cR + jmax(zero, jmin(r, jmin(kR * r + kL, one))) * (cL - cR);
   }
 }
 
 // ------------------------------------------------------------------------
 // LDS tile helpers
-// tile[f][node][level]; NZ (levels per chunk) is a compile-time constant so
-// that every (tile, node) offset is an instruction immediate
-template <int N, int NZ> struct Tiles {
-  double *t;  // tile base
-  __device__ __forceinline__ double &at(int f, int node, int l) const { return t[(f * N * N + node) * NZ + l]; }
+// tile[f][node][level] of values of type R; NZ (levels per chunk) is a
+// compile-time constant so that every (tile, node) offset is an instruction
+// immediate
+template <int N, int NZ, typename R = double> struct Tiles {
+  R *t;  // tile base
+  __device__ __forceinline__ R &at(int f, int node, int l) const { return t[(f * N * N + node) * NZ + l]; }
 };
 
 // sum_k D[I,k] f(k,J)
-template <int N, int NZ>
-__device__ __forceinline__ double dx(const double *Dm, const Tiles<N, NZ> &T, int f, int I, int J, int l) {
-  double s = Dm[I * N] * T.at(f, J * N, l);
+template <int N, int NZ, typename R>
+__device__ __forceinline__ R dx(const R *Dm, const Tiles<N, NZ, R> &T, int f, int I, int J, int l) {
+  R s = Dm[I * N] * T.at(f, J * N, l);
 #pragma unroll
   for (int k = 1; k < N; ++k) s += Dm[I * N + k] * T.at(f, k + J * N, l);
   return s;
 }
 // sum_k D[J,k] f(I,k)
-template <int N, int NZ>
-__device__ __forceinline__ double dy(const double *Dm, const Tiles<N, NZ> &T, int f, int I, int J, int l) {
-  double s = Dm[J * N] * T.at(f, I, l);
+template <int N, int NZ, typename R>
+__device__ __forceinline__ R dy(const R *Dm, const Tiles<N, NZ, R> &T, int f, int I, int J, int l) {
+  R s = Dm[J * N] * T.at(f, I, l);
 #pragma unroll
   for (int k = 1; k < N; ++k) s += Dm[J * N + k] * T.at(f, I + k * N, l);
   return s;
 }
 // sum_k (D[I,k] fx(k,J) + D[J,k] fy(I,k)), pairwise as in the reference loops
-template <int N, int NZ>
-__device__ __forceinline__ double dxy(const double *Dm, const Tiles<N, NZ> &T, int fx, int fy, int I, int J, int l) {
-  double s = Dm[I * N] * T.at(fx, J * N, l) + Dm[J * N] * T.at(fy, I, l);
+template <int N, int NZ, typename R>
+__device__ __forceinline__ R dxy(const R *Dm, const Tiles<N, NZ, R> &T, int fx, int fy, int I, int J, int l) {
+  R s = Dm[I * N] * T.at(fx, J * N, l) + Dm[J * N] * T.at(fy, I, l);
 #pragma unroll
   for (int k = 1; k < N; ++k) s += Dm[I * N + k] * T.at(fx, k + J * N, l) + Dm[J * N + k] * T.at(fy, I + k * N, l);
   return s;
 }
 // acc + sum_k (D[I,k] fx(k,J) + D[J,k] fy(I,k)), accumulated term by term
-template <int N, int NZ>
-__device__ __forceinline__ double dxy_acc(double s, const double *Dm, const Tiles<N, NZ> &T, int fx, int fy, int I,
-                                          int J, int l) {
+template <int N, int NZ, typename R>
+__device__ __forceinline__ R dxy_acc(R s, const R *Dm, const Tiles<N, NZ, R> &T, int fx, int fy, int I, int J,
+                                     int l) {
 #pragma unroll
   for (int k = 0; k < N; ++k) s += Dm[I * N + k] * T.at(fx, k + J * N, l) + Dm[J * N + k] * T.at(fy, I + k * N, l);
   return s;
 }
 // sum_k (D[I,k] fx(k,J) cx(k,J) + D[J,k] fy(I,k) cy(I,k))
-template <int N, int NZ>
-__device__ __forceinline__ double dxy_prod(const double *Dm, const Tiles<N, NZ> &T, int fx, int fy, int fc, int I, int J,
-                                           int l) {
-  double s = Dm[I * N] * T.at(fx, J * N, l) * T.at(fc, J * N, l) + Dm[J * N] * T.at(fy, I, l) * T.at(fc, I, l);
+template <int N, int NZ, typename R>
+__device__ __forceinline__ R dxy_prod(const R *Dm, const Tiles<N, NZ, R> &T, int fx, int fy, int fc, int I, int J,
+                                      int l) {
+  R s = Dm[I * N] * T.at(fx, J * N, l) * T.at(fc, J * N, l) + Dm[J * N] * T.at(fy, I, l) * T.at(fc, I, l);
 #pragma unroll
   for (int k = 1; k < N; ++k)
     s += Dm[I * N + k] * T.at(fx, k + J * N, l) * T.at(fc, k + J * N, l) +
@@ -196,6 +209,16 @@ template <int D> __device__ __forceinline__ double dpp_from_above(double x) {
   const int rhi = __builtin_amdgcn_update_dpp(hi, hi, 0x100 + D, 0xf, 0xf, false);
   return __longlong_as_double(((long long)rhi << 32) | (unsigned int)rlo);
 }
+template <int D> __device__ __forceinline__ float dpp_from_below(float x) {
+  static_assert(D >= 1 && D <= 15, "row shift");
+  const int b = __float_as_int(x);
+  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, 0x110 + D, 0xf, 0xf, false));
+}
+template <int D> __device__ __forceinline__ float dpp_from_above(float x) {
+  static_assert(D >= 1 && D <= 15, "row shift");
+  const int b = __float_as_int(x);
+  return __int_as_float(__builtin_amdgcn_update_dpp(b, b, 0x100 + D, 0xf, 0xf, false));
+}
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
 // XCD b % 8).  Renumber blocks so that consecutive logical ids -- the level
@@ -208,26 +231,29 @@ __device__ __forceinline__ int xcd_logical(int b, int G) {
   return x < rem ? x * (per + 1) + slot : rem * (per + 1) + (x - rem) * per + slot;
 }
 
-template <int N> __device__ __forceinline__ void load_dmats(const KArgs &a, double *Ds, double *DWs) {
+template <int N, typename R> __device__ __forceinline__ void load_dmats(const KArgs &a, R *Ds, R *DWs) {
   const int t = threadIdx.x;
   if (t < N * N) {
-    Ds[t] = a.D[t];
-    DWs[t] = a.DWm[t];
+    Ds[t] = R(a.D[t]);
+    DWs[t] = R(a.DWm[t]);
   }
 }
 
 // ------------------------------------------------------------------------
 // Pass 1: HyperViscKernel! (src/GPU/ViscKernel.jl:1-84).
 // Grid: one workgroup per (element of the colour, level chunk).
-template <int N, int NZT, bool CM>
+// R is the arithmetic and storage type of the state it reads (U: the state,
+// or its FP32 shadow), of Temp1 and of the LDS tiles: double, or float in
+// the mixed-precision mode.
+template <int N, int NZT, bool CM, typename R>
 __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, int slot0, int nchunk,
-                                               const double *__restrict__ U, double *__restrict__ T1) {
+                                               const R *__restrict__ U, R *__restrict__ T1) {
   constexpr int NN = N * N;
   extern __shared__ double smem[];
-  double *Ds = smem, *DWs = smem + NN;
+  R *Ds = reinterpret_cast<R *>(smem), *DWs = Ds + NN;
   // two tile sets used alternately, so a phase's tiles are never rewritten
   // before every lane has passed the next phase's barrier
-  Tiles<N, NZT> T{smem + 2 * NN}, TB{smem + 2 * NN + 5 * NN * NZT};
+  Tiles<N, NZT, R> T{Ds + 2 * NN}, TB{Ds + 2 * NN + 5 * NN * NZT};
   load_dmats<N>(a, Ds, DWs);
   const int Nz = a.Nz;
   constexpr int nzt = NZT;
@@ -245,11 +271,13 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
 
   // uniform bases + 32-bit lane offsets (the host checks NumG*Nz < 2^28)
   const double *metE = a.met1 + (size_t)e * CGD_NMET1 * NNZ;
-  double S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1, iMs = 1;
+  const R *iMsR = std::is_same<R, double>::value ? reinterpret_cast<const R *>(a.iMs)
+                                                  : reinterpret_cast<const R *>(a.iMs32);
+  R S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1, iMs = 1;
   bool first = false;
   unsigned ui = 0;
-  double o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0;
-  double uC = 0, vC = 0, uD = 0, vD = 0, Th = 0;
+  R o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0;
+  R uC = 0, vC = 0, uD = 0, vD = 0, Th = 0;
   if (valid) {
     const int gv = a.glob[e * NN + node];
     first = gv < 0;
@@ -260,19 +288,19 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
     if constexpr (CM) {
       const double *mN = a.met1n + (size_t)e * CGD_NMET1 * NN;
       const unsigned nd = (unsigned)node;
-      S11 = at(mN, nd);
-      S12 = at(mN + NN, nd);
-      S21 = at(mN + 2 * NN, nd);
-      S22 = at(mN + 3 * NN, nd);
-      Js = at(mN + 4 * NN, nd);
+      S11 = R(at(mN, nd));
+      S12 = R(at(mN + NN, nd));
+      S21 = R(at(mN + 2 * NN, nd));
+      S22 = R(at(mN + 3 * NN, nd));
+      Js = R(at(mN + 4 * NN, nd));
     } else {
-      S11 = at(metE, mo);
-      S12 = at(metE + NNZ, mo);
-      S21 = at(metE + 2 * NNZ, mo);
-      S22 = at(metE + 3 * NNZ, mo);
-      Js = at(metE + 4 * NNZ, mo);
+      S11 = R(at(metE, mo));
+      S12 = R(at(metE + NNZ, mo));
+      S21 = R(at(metE + 2 * NNZ, mo));
+      S22 = R(at(metE + 3 * NNZ, mo));
+      Js = R(at(metE + 4 * NNZ, mo));
     }
-    iMs = at(a.iMs, ui);
+    iMs = at(iMsR, ui);
     // the DSS accumulators of this node: no other element of the colour
     // touches them, so they are read now, with the state, not after the
     // derivative phases
@@ -283,7 +311,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
       o4 = at(T1 + 3 * vs, ui);
       o5 = at(T1 + 4 * vs, ui);
     }
-    const double rho = at(U, ui), u = at(U + vs, ui), v = at(U + 2 * vs, ui), rth = at(U + 4 * vs, ui);
+    const R rho = at(U, ui), u = at(U + vs, ui), v = at(U + 2 * vs, ui), rth = at(U + 4 * vs, ui);
     uC = S11 * v - S12 * u;  // Curl12
     vC = S21 * v - S22 * u;
     uD = S11 * u + S12 * v;  // Contra12
@@ -298,15 +326,15 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
     T.at(4, node, l) = Th;
   }
   __syncthreads();
-  double Curl = 0, Div = 0, ThCx = 0, ThCy = 0;
+  R Curl = 0, Div = 0, ThCx = 0, ThCy = 0;
   if (valid) {
-    const double Dxc = dx<N, NZT>(Ds, T, 4, I, J, l);
-    const double Dyc = dy<N, NZT>(Ds, T, 4, I, J, l);
-    const double iJs = rcp(Js);
-    Curl = dxy<N, NZT>(Ds, T, 0, 1, I, J, l) * iJs;
-    Div = dxy<N, NZT>(Ds, T, 2, 3, I, J, l) * iJs;
-    const double GDx = (S11 * Dxc + S21 * Dyc) * iJs;  // Grad12 with J
-    const double GDy = (S12 * Dxc + S22 * Dyc) * iJs;
+    const R Dxc = dx<N, NZT, R>(Ds, T, 4, I, J, l);
+    const R Dyc = dy<N, NZT, R>(Ds, T, 4, I, J, l);
+    const R iJs = rcp(Js);
+    Curl = dxy<N, NZT, R>(Ds, T, 0, 1, I, J, l) * iJs;
+    Div = dxy<N, NZT, R>(Ds, T, 2, 3, I, J, l) * iJs;
+    const R GDx = (S11 * Dxc + S21 * Dyc) * iJs;  // Grad12 with J
+    const R GDy = (S12 * Dxc + S22 * Dyc) * iJs;
     ThCx = S11 * GDx + S12 * GDy;  // Contra12
     ThCy = S21 * GDx + S22 * GDy;
   }
@@ -318,19 +346,19 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
   }
   __syncthreads();
   if (valid) {
-    const double DxCurl = dx<N, NZT>(DWs, TB, 0, I, J, l);
-    const double DyCurl = dy<N, NZT>(DWs, TB, 0, I, J, l);
-    const double DxDiv = dx<N, NZT>(DWs, TB, 1, I, J, l);
-    const double DyDiv = dy<N, NZT>(DWs, TB, 1, I, J, l);
-    const double DivTh = dxy<N, NZT>(DWs, TB, 2, 3, I, J, l);
-    const double FuC = S12 * DxCurl + S22 * DyCurl;  // Rot12
-    const double FvC = -S11 * DxCurl - S21 * DyCurl;
-    const double FuD = S11 * DxDiv + S21 * DyDiv;  // Grad12
-    const double FvD = S12 * DxDiv + S22 * DyDiv;
+    const R DxCurl = dx<N, NZT, R>(DWs, TB, 0, I, J, l);
+    const R DyCurl = dy<N, NZT, R>(DWs, TB, 0, I, J, l);
+    const R DxDiv = dx<N, NZT, R>(DWs, TB, 1, I, J, l);
+    const R DyDiv = dy<N, NZT, R>(DWs, TB, 1, I, J, l);
+    const R DivTh = dxy<N, NZT, R>(DWs, TB, 2, 3, I, J, l);
+    const R FuC = S12 * DxCurl + S22 * DyCurl;  // Rot12
+    const R FvC = -S11 * DxCurl - S21 * DyCurl;
+    const R FuD = S11 * DxDiv + S21 * DyDiv;  // Grad12
+    const R FvD = S12 * DxDiv + S22 * DyDiv;
     // DSS: the first element (in colour order) holding the node writes,
     // the others accumulate
-    const double r1 = -FuC * iMs, r2 = -FvC * iMs, r3 = FuD * iMs, r4 = FvD * iMs, r5 = DivTh * iMs;
-    double *T1a = T1, *T1b = T1 + vs, *T1c = T1 + 2 * vs, *T1d = T1 + 3 * vs, *T1e = T1 + 4 * vs;
+    const R r1 = -FuC * iMs, r2 = -FvC * iMs, r3 = FuD * iMs, r4 = FvD * iMs, r5 = DivTh * iMs;
+    R *T1a = T1, *T1b = T1 + vs, *T1c = T1 + 2 * vs, *T1d = T1 + 3 * vs, *T1e = T1 + 4 * vs;
     at(T1a, ui) = first ? r1 : o1 + r1;
     at(T1b, ui) = first ? r2 : o2 + r2;
     at(T1c, ui) = first ? r3 : o3 + r3;
@@ -354,20 +382,31 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
 // thread adds the fluxes of both its interfaces to its own level (the
 // reference adds them from the owner of the interface); F4 of interface iz
 // combines this layer's upper terms with the layer above's lower terms.
-template <int N, int NZT, bool CM>
+// R is the arithmetic and storage type of the tendency (F, the chunk side
+// sums F4x), of the hyperviscosity cache C = Temp1, of the winds and rho*theta
+// it reads (UR: the state, or its FP32 shadow) and of the LDS tiles: double,
+// or float in the mixed-precision mode.  FP64 in both modes: rho (from the
+// FP64 state U), the pressure P, the w-interface balance of pressure
+// gradient and gravity (rho, p, J and dXdxI[3,3] in FP64) and the
+// horizontal pressure gradient (an FP64 tile and derivative).
+template <int N, int NZT, bool CM, typename R>
 __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, int slot0, int nchunk,
-                                               const double *__restrict__ U, const double *__restrict__ C,
-                                               const double *__restrict__ P, double *__restrict__ F,
-                                               double *__restrict__ F4x) {
+                                               const double *__restrict__ U, const R *__restrict__ UR,
+                                               const R *__restrict__ C, const double *__restrict__ P,
+                                               R *__restrict__ F, R *__restrict__ F4x) {
   constexpr int NN = N * N;
   static_assert(NZT >= 3 && NZT <= 16, "chunk heights 3..16 (the DPP neighbour reach is 2)");
   extern __shared__ double smem[];
-  double *Ds = smem, *DWs = smem + NN;
-  // two tile sets used alternately (phases 1, 3a, 4 in T; 2, 3b in TB), so
-  // a phase's tiles are never rewritten before every lane has passed the
-  // next phase's barrier: one barrier per phase
-  Tiles<N, NZT> T{smem + 2 * NN}, TB{smem + 2 * NN + 8 * NN * NZT};
+  // LDS: FP64 D matrix and pressure tile, then the R-typed D, DW and two
+  // sets of 7 tiles used alternately (phases 1, 3a, 4 in T; 2, 3b in TB),
+  // so a phase's tiles are never rewritten before every lane has passed
+  // the next phase's barrier: one barrier per phase
+  double *Dd = smem;
+  Tiles<N, NZT, double> TP{smem + NN};
+  R *Ds = reinterpret_cast<R *>(smem + NN + NN * NZT), *DWs = Ds + NN;
+  Tiles<N, NZT, R> T{Ds + 2 * NN}, TB{Ds + 2 * NN + 7 * NN * NZT};
   load_dmats<N>(a, Ds, DWs);
+  if (threadIdx.x < NN) Dd[threadIdx.x] = a.D[threadIdx.x];
   const int Nz = a.Nz;
   constexpr int nzt = NZT;
   const int L = xcd_logical(blockIdx.x, gridDim.x);
@@ -389,7 +428,10 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   // per-lane offset, so every load and store is one global access in its
   // saddr form with no 64-bit vector address arithmetic.  The host checks
   // NumG*Nz < 2^28.
-  const double *Ur = U, *Uu = U + vs, *Uv = U + 2 * vs, *Uw = U + 3 * vs, *Ut = U + 4 * vs;
+  const double *Ur = U;
+  const R *Uu = UR + vs, *Uv = UR + 2 * vs, *Uw = UR + 3 * vs, *Ut = UR + 4 * vs;
+  const R *iMsR = std::is_same<R, double>::value ? reinterpret_cast<const R *>(a.iMs)
+                                                  : reinterpret_cast<const R *>(a.iMs32);
   const double *metE = a.met + (size_t)e * (CM ? CGD_NMETP : CGD_NMET) * NNZ;
   const double *metN = CM ? a.metn + (size_t)e * CGD_NMETN * NN : nullptr;
   unsigned ui = 0, mo = 0;
@@ -405,10 +447,12 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
 #define MET(c, off) at(metE + (size_t)(CM ? met_pidx(c) : (c)) * NNZ, mo + (off))
 #define METN(c) at(metN + (size_t)met_nidx(c) * NN, (unsigned)node)
 #define MET0(c) (CM && !met_per_point(c) ? METN(c) : MET(c, 0))
-  // own level: state and metric (used in every phase)
-  double rho = 1, u = 0, v = 0, w = 0, rth = 1, iMs = 1, M1 = 1, M2 = 1, p = 0;
-  double d111 = 0, d112 = 0, d121 = 0, d122 = 0, d211 = 0, d212 = 0, d221 = 0, d222 = 0;
-  double d311 = 0, d312 = 0, d321 = 0, d322 = 0, d331 = 1, d332 = 1, J1 = 1, J2 = 1, Wc = 0;
+  // own level: state and metric (used in every phase); the FP64 copies
+  // (rho, p, J1, J2, d331, d332) feed the w-interface balance
+  double rho = 1, p = 0, J1 = 1, J2 = 1, d331 = 1, d332 = 1;
+  R u = 0, v = 0, w = 0, rth = 1, iMs = 1, M1 = 1, M2 = 1;
+  R d111 = 0, d112 = 0, d121 = 0, d122 = 0, d211 = 0, d212 = 0, d221 = 0, d222 = 0;
+  R d311 = 0, d312 = 0, d321 = 0, d322 = 0, Wc = 0;
   const double *M1b = a.M, *M2b = a.M + vs;
   if (valid) {
     rho = at(Ur, ui);
@@ -416,21 +460,24 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     v = at(Uv, ui);
     w = at(Uw, ui);
     rth = at(Ut, ui);
-    iMs = at(a.iMs, ui);
-    M1 = at(M1b, ui);
-    M2 = at(M2b, ui);
+    iMs = at(iMsR, ui);
+    M1 = R(at(M1b, ui));
+    M2 = R(at(M2b, ui));
     p = at(P, ui);
-    d111 = MET0(M111); d112 = MET0(M112); d121 = MET0(M121); d122 = MET0(M122);
-    d211 = MET0(M211); d212 = MET0(M212); d221 = MET0(M221); d222 = MET0(M222);
-    d311 = MET0(M311); d312 = MET0(M312); d321 = MET0(M321); d322 = MET0(M322);
-    d331 = MET0(M331); d332 = MET0(M332); J1 = MET0(MJ1); J2 = MET0(MJ2); Wc = MET0(MCOR);
+    d111 = R(MET0(M111)); d112 = R(MET0(M112)); d121 = R(MET0(M121)); d122 = R(MET0(M122));
+    d211 = R(MET0(M211)); d212 = R(MET0(M212)); d221 = R(MET0(M221)); d222 = R(MET0(M222));
+    d311 = R(MET0(M311)); d312 = R(MET0(M312)); d321 = R(MET0(M321)); d322 = R(MET0(M322));
+    d331 = MET0(M331); d332 = MET0(M332); J1 = MET0(MJ1); J2 = MET0(MJ2); Wc = R(MET0(MCOR));
   }
-  const double Js = J1 + J2;
-  const double irho = rcp(rho);
-  const double c_own = rth * irho;
+  const R rho_r = R(rho), d331_r = R(d331), d332_r = R(d332);
+  const R Js = R(J1 + J2);
+  const R irho = R(rcp(rho));
+  const R c_own = rth * irho;
+  const R h = R(0.5), q4 = R(0.25);
+  const R Kc = R(a.Kc), Kg = R(a.Kg), Kd = R(a.Kd);
   // the DSS accumulators of this node (untouched by the other elements of
   // the colour), read with the state rather than after the LDS phases
-  double o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0;
+  R o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0;
   if (valid && !first) {
     o1 = at(F, ui);
     o2 = at(F + vs, ui);
@@ -449,20 +496,23 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   // (in the compact metric layout the level-invariant components of the
   // neighbour levels are this level's own)
   // level iz-1
-  double rho_m = dpp_from_below<1>(rho), u_m = dpp_from_below<1>(u), v_m = dpp_from_below<1>(v);
-  double w_m = dpp_from_below<1>(w), c_m = dpp_from_below<1>(c_own), Jc_m = CM ? Js : dpp_from_below<1>(Js);
-  double M2_m = dpp_from_below<1>(M2), d312m = dpp_from_below<1>(d312), d322m = dpp_from_below<1>(d322);
-  double d332m = CM ? d332 : dpp_from_below<1>(d332);
+  double rho_m = dpp_from_below<1>(rho);
+  R u_m = dpp_from_below<1>(u), v_m = dpp_from_below<1>(v);
+  R w_m = dpp_from_below<1>(w), c_m = dpp_from_below<1>(c_own), Jc_m = CM ? Js : dpp_from_below<1>(Js);
+  R M2_m = dpp_from_below<1>(M2), d312m = dpp_from_below<1>(d312), d322m = dpp_from_below<1>(d322);
+  R d332m = CM ? d332_r : dpp_from_below<1>(d332_r);
   // level iz-2 (clamped at the bottom)
-  double c_mm = dpp_from_below<2>(c_own), Jc_mm = CM ? Js : dpp_from_below<2>(Js);
+  R c_mm = dpp_from_below<2>(c_own), Jc_mm = CM ? Js : dpp_from_below<2>(Js);
   // level iz+1
-  double rho_p = dpp_from_above<1>(rho), u_p = dpp_from_above<1>(u), v_p = dpp_from_above<1>(v);
-  double c_p = dpp_from_above<1>(c_own), p_p = dpp_from_above<1>(p), M1_p = dpp_from_above<1>(M1);
-  double J1_p = CM ? J1 : dpp_from_above<1>(J1), Jc_p = CM ? Js : dpp_from_above<1>(Js);
-  double d311p = dpp_from_above<1>(d311), d321p = dpp_from_above<1>(d321);
+  double rho_p = dpp_from_above<1>(rho), p_p = dpp_from_above<1>(p);
+  R u_p = dpp_from_above<1>(u), v_p = dpp_from_above<1>(v);
+  R c_p = dpp_from_above<1>(c_own), M1_p = dpp_from_above<1>(M1);
+  double J1_p = CM ? J1 : dpp_from_above<1>(J1);
+  R Jc_p = CM ? Js : dpp_from_above<1>(Js);
+  R d311p = dpp_from_above<1>(d311), d321p = dpp_from_above<1>(d321);
   double d331p = CM ? d331 : dpp_from_above<1>(d331);
   // level iz+2 (clamped at the top)
-  double c_pp = dpp_from_above<2>(c_own), Jc_pp = CM ? Js : dpp_from_above<2>(Js);
+  R c_pp = dpp_from_above<2>(c_own), Jc_pp = CM ? Js : dpp_from_above<2>(Js);
   if (valid) {
     if (has_lo && edge_m1) {
       const unsigned um = ui - 1;
@@ -470,18 +520,18 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
       u_m = at(Uu, um);
       v_m = at(Uv, um);
       w_m = at(Uw, um);
-      c_m = at(Ut, um) * rcp(rho_m);
-      M2_m = at(M2b, um);
-      d312m = MET(M312, -1);
-      d322m = MET(M322, -1);
+      c_m = at(Ut, um) * R(rcp(rho_m));
+      M2_m = R(at(M2b, um));
+      d312m = R(MET(M312, -1));
+      d322m = R(MET(M322, -1));
       if constexpr (!CM) {
-        Jc_m = MET(MJ1, -1) + MET(MJ2, -1);
-        d332m = MET(M332, -1);
+        Jc_m = R(MET(MJ1, -1) + MET(MJ2, -1));
+        d332m = R(MET(M332, -1));
       }
     }
     if (iz >= 2 && edge_m2) {
-      c_mm = at(Ut, ui - 2) * rcp(at(Ur, ui - 2));
-      if constexpr (!CM) Jc_mm = MET(MJ1, -2) + MET(MJ2, -2);
+      c_mm = at(Ut, ui - 2) * R(rcp(at(Ur, ui - 2)));
+      if constexpr (!CM) Jc_mm = R(MET(MJ1, -2) + MET(MJ2, -2));
     }
     if (iz == 1) {
       c_mm = c_m;
@@ -492,20 +542,20 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
       rho_p = at(Ur, up);
       u_p = at(Uu, up);
       v_p = at(Uv, up);
-      c_p = at(Ut, up) * rcp(rho_p);
+      c_p = at(Ut, up) * R(rcp(rho_p));
       p_p = at(P, up);
-      M1_p = at(M1b, up);
-      d311p = MET(M311, 1);
-      d321p = MET(M321, 1);
+      M1_p = R(at(M1b, up));
+      d311p = R(MET(M311, 1));
+      d321p = R(MET(M321, 1));
       if constexpr (!CM) {
         J1_p = MET(MJ1, 1);
-        Jc_p = J1_p + MET(MJ2, 1);
+        Jc_p = R(J1_p + MET(MJ2, 1));
         d331p = MET(M331, 1);
       }
     }
     if (iz + 2 <= Nz - 1 && edge_p2) {
-      c_pp = at(Ut, ui + 2) * rcp(at(Ur, ui + 2));
-      if constexpr (!CM) Jc_pp = MET(MJ1, 2) + MET(MJ2, 2);
+      c_pp = at(Ut, ui + 2) * R(rcp(at(Ur, ui + 2)));
+      if constexpr (!CM) Jc_pp = R(MET(MJ1, 2) + MET(MJ2, 2));
     }
     if (iz + 2 > Nz - 1) {
       c_pp = c_p;
@@ -513,78 +563,80 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     }
   }
 
+  const R rho_m_r = R(rho_m), rho_p_r = R(rho_p), d331p_r = R(d331p);
+
   // ---- phase 0: level-coupling terms ----
-  double w_lo = 0;
-  double FU = 0, FV = 0, FW1 = 0, FW2 = 0;  // vertical vorticity-flux parts of MomentumKernel
-  double GZlo = 0, GZup = 0;                // 1/2 (KinF(iz) - KinF(iz-1)) at the two interfaces
-  double Gpup = 0;                          // -(dp/dz + g rho J) at the upper interface
-  double iden_lo = 1, iden_up = 1;          // 1 / rho-weighted interface masses
-  double f1 = 0, f5 = 0;
-  double K1 = 0, K2 = 0;
-  double U1 = 0, V1 = 0, W1 = 0;  // 1/2 (tempXZ[2] - tempXZ[1]) of this layer
+  R w_lo = 0;
+  R FU = 0, FV = 0, FW1 = 0, FW2 = 0;  // vertical vorticity-flux parts of MomentumKernel
+  R GZlo = 0, GZup = 0;                // 1/2 (KinF(iz) - KinF(iz-1)) at the two interfaces
+  double Gpup = 0;                     // -(dp/dz + g rho J) at the upper interface (FP64)
+  R iden_lo = 1, iden_up = 1;          // 1 / rho-weighted interface masses
+  R f1 = 0, f5 = 0;
+  R K1 = 0, K2 = 0;
+  R U1 = 0, V1 = 0, W1 = 0;  // 1/2 (tempXZ[2] - tempXZ[1]) of this layer
   if (valid) {
-    const double Kh = 0.5 * (u * u + v * v);
+    const R Kh = h * (u * u + v * v);
     if (has_lo) {
       w_lo = w_m;
-      K1 = Kh + 0.5 * (w_lo * w_lo);
-      K2 = Kh + 0.5 * (w * w);
-      iden_lo = rcp(rho_m * M2_m + rho * M1);
+      K1 = Kh + h * (w_lo * w_lo);
+      K2 = Kh + h * (w * w);
+      iden_lo = rcp(rho_m_r * M2_m + rho_r * M1);
       // vorticity fluxes through the lower interface (MomentumKernel.jl:118-125)
-      const double FluxUZ = 0.5 * ((d331 * v - d321 * w_lo) - (d332m * v_m - d322m * w_lo));
-      const double FluxVZ = 0.5 * ((d331 * u - d311 * w_lo) - (d332m * u_m - d312m * w_lo));
-      const double FluxWZ = 0.5 * ((d321 * u - d311 * v) - (d322m * u_m - d312m * v_m));
+      const R FluxUZ = h * ((d331_r * v - d321 * w_lo) - (d332m * v_m - d322m * w_lo));
+      const R FluxVZ = h * ((d331_r * u - d311 * w_lo) - (d332m * u_m - d312m * w_lo));
+      const R FluxWZ = h * ((d321 * u - d311 * v) - (d322m * u_m - d312m * v_m));
       FU += (-v * FluxWZ - w_lo * FluxVZ);
       FV += (u * FluxWZ - w_lo * FluxUZ);
       FW1 += (u * FluxVZ + v * FluxUZ);
-      const double K2m = 0.5 * (u_m * u_m + v_m * v_m) + 0.5 * (w_m * w_m);
-      GZlo = 0.5 * (K1 - K2m);
+      const R K2m = h * (u_m * u_m + v_m * v_m) + h * (w_m * w_m);
+      GZlo = h * (K1 - K2m);
       // upwind theta / mass flux through the lower interface (owner: layer iz-1)
-      const double wCon = rho_m * (d312m * u_m + d322m * v_m + d332m * w_m) + rho * (d311 * u + d321 * v + d331 * w_m);
-      const double cRR = has_hi ? c_p : c_own, JRR = has_hi ? Jc_p : Js;
-      double cFL, cFR;
+      const R wCon = rho_m_r * (d312m * u_m + d322m * v_m + d332m * w_m) + rho_r * (d311 * u + d321 * v + d331_r * w_m);
+      const R cRR = has_hi ? c_p : c_own, JRR = has_hi ? Jc_p : Js;
+      R cFL, cFR;
       recu4(c_mm, c_m, c_own, cRR, Jc_mm, Jc_m, Js, JRR, cFL, cFR);
-      const double Flux = 0.25 * ((fabs(wCon) + wCon) * cFL + (-fabs(wCon) + wCon) * cFR);
+      const R Flux = q4 * ((rabs(wCon) + wCon) * cFL + (-rabs(wCon) + wCon) * cFR);
       f5 += Flux * iMs;
-      f1 += (0.5 * wCon) * iMs;
+      f1 += (h * wCon) * iMs;
     } else {
       // free-slip bottom (MomentumKernel.jl:35-36)
-      w_lo = -(d311 * u + d321 * v) * rcp(d331);
-      K1 = Kh + 0.5 * (w_lo * w_lo);
-      K2 = Kh + 0.5 * (w * w);
+      w_lo = -(d311 * u + d321 * v) * rcp(d331_r);
+      K1 = Kh + h * (w_lo * w_lo);
+      K2 = Kh + h * (w * w);
     }
     if (has_hi) {
-      iden_up = rcp(rho_p * M1_p + rho * M2);
+      iden_up = rcp(rho_p_r * M1_p + rho_r * M2);
       // vorticity fluxes through the upper interface (MomentumKernel.jl:126-133)
-      const double FluxUZ = 0.5 * ((d331p * v_p - d321p * w) - (d332 * v - d322 * w));
-      const double FluxVZ = 0.5 * ((d331p * u_p - d311p * w) - (d332 * u - d312 * w));
-      const double FluxWZ = 0.5 * ((d321p * u_p - d311p * v_p) - (d322 * u - d312 * v));
+      const R FluxUZ = h * ((d331p_r * v_p - d321p * w) - (d332_r * v - d322 * w));
+      const R FluxVZ = h * ((d331p_r * u_p - d311p * w) - (d332_r * u - d312 * w));
+      const R FluxWZ = h * ((d321p * u_p - d311p * v_p) - (d322 * u - d312 * v));
       FU += (-v * FluxWZ - w * FluxVZ);
       FV += (u * FluxWZ - w * FluxUZ);
       FW2 += (u * FluxVZ + v * FluxUZ);
-      const double K1p = 0.5 * (u_p * u_p + v_p * v_p) + 0.5 * (w * w);
-      GZup = 0.5 * (K1p - K2);
+      const R K1p = h * (u_p * u_p + v_p * v_p) + h * (w * w);
+      GZup = h * (K1p - K2);
       // pressure gradient and gravity at the interface (GradKineticKernel.jl:111-121)
       const double GradZ = 0.5 * (p_p - p);
       const double Gradw = GradZ * (d332 + d331p);
       Gpup = -(Gradw + Grav * (rho * J2 + rho_p * J1_p));
       // upwind theta / mass flux through the upper interface (owner: this layer)
-      const double wCon = rho * (d312 * u + d322 * v + d332 * w) + rho_p * (d311p * u_p + d321p * v_p + d331p * w);
-      const double cLL = has_lo ? c_m : c_own, JLL = has_lo ? Jc_m : Js;
-      double cFL, cFR;
+      const R wCon = rho_r * (d312 * u + d322 * v + d332_r * w) + rho_p_r * (d311p * u_p + d321p * v_p + d331p_r * w);
+      const R cLL = has_lo ? c_m : c_own, JLL = has_lo ? Jc_m : Js;
+      R cFL, cFR;
       recu4(cLL, c_own, c_p, c_pp, JLL, Js, Jc_p, Jc_pp, cFL, cFR);
-      const double Flux = 0.25 * ((fabs(wCon) + wCon) * cFL + (-fabs(wCon) + wCon) * cFR);
+      const R Flux = q4 * ((rabs(wCon) + wCon) * cFL + (-rabs(wCon) + wCon) * cFR);
       f5 += -Flux * iMs;
-      f1 += -(0.5 * wCon) * iMs;
+      f1 += -(h * wCon) * iMs;
     }
-    U1 = 0.5 * ((d332 * v - d322 * w) - (d331 * v - d321 * w_lo));
-    V1 = 0.5 * ((d332 * u - d312 * w) - (d331 * u - d311 * w_lo));
-    W1 = 0.5 * ((d322 * u - d312 * v) - (d321 * u - d311 * v));
+    U1 = h * ((d332_r * v - d322 * w) - (d331_r * v - d321 * w_lo));
+    V1 = h * ((d332_r * u - d312 * w) - (d331_r * u - d311 * w_lo));
+    W1 = h * ((d322 * u - d312 * v) - (d321 * u - d311 * v));
   }
-  const double S11 = d111 + d112, S12 = d121 + d122, S21 = d211 + d212, S22 = d221 + d222;
+  const R S11 = d111 + d112, S12 = d121 + d122, S21 = d211 + d212, S22 = d221 + d222;
 
   // ---- phase 1: hyperviscosity inputs, kinetic energy, pressure ----
   if (in_tile) {
-    double c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+    R c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
     if (valid) {
       c1 = at(C, ui);
       c2 = at(C + vs, ui);
@@ -599,57 +651,58 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     T.at(4, node, l) = c5;
     T.at(5, node, l) = K1;
     T.at(6, node, l) = K2;
-    T.at(7, node, l) = p;
+    TP.at(0, node, l) = p;
   }
   __syncthreads();
-  double Curl = 0, Div = 0, ThCx = 0, ThCy = 0;
-  double f2 = 0, f3 = 0, f4up = 0, f4lo = 0;
+  R Curl = 0, Div = 0, ThCx = 0, ThCy = 0;
+  R f2 = 0, f3 = 0, f4up = 0, f4lo = 0;
   if (valid) {
     {
-      const double Dxc = dx<N, NZT>(Ds, T, 4, I, J, l);
-      const double Dyc = dy<N, NZT>(Ds, T, 4, I, J, l);
-      const double iJs = rcp(Js);
-      Curl = dxy<N, NZT>(Ds, T, 0, 1, I, J, l) * iJs;
-      Div = dxy<N, NZT>(Ds, T, 2, 3, I, J, l) * iJs;
-      const double GDx = (S11 * Dxc + S21 * Dyc) * iJs;
-      const double GDy = (S12 * Dxc + S22 * Dyc) * iJs;
-      ThCx = rho * (S11 * GDx + S12 * GDy);  // Contra12(U1, ...)
-      ThCy = rho * (S21 * GDx + S22 * GDy);
+      const R Dxc = dx<N, NZT, R>(Ds, T, 4, I, J, l);
+      const R Dyc = dy<N, NZT, R>(Ds, T, 4, I, J, l);
+      const R iJs = rcp(Js);
+      Curl = dxy<N, NZT, R>(Ds, T, 0, 1, I, J, l) * iJs;
+      Div = dxy<N, NZT, R>(Ds, T, 2, 3, I, J, l) * iJs;
+      const R GDx = (S11 * Dxc + S21 * Dyc) * iJs;
+      const R GDy = (S12 * Dxc + S22 * Dyc) * iJs;
+      ThCx = rho_r * (S11 * GDx + S12 * GDy);  // Contra12(U1, ...)
+      ThCy = rho_r * (S21 * GDx + S22 * GDy);
     }
     // GradFullKernel!
-    const double DXK1 = dx<N, NZT>(Ds, T, 5, I, J, l), DYK1 = dy<N, NZT>(Ds, T, 5, I, J, l);
-    const double DXK2 = dx<N, NZT>(Ds, T, 6, I, J, l), DYK2 = dy<N, NZT>(Ds, T, 6, I, J, l);
-    double GuF1 = -(d111 * DXK1 + d211 * DYK1);
-    double GvF1 = -(d121 * DXK1 + d221 * DYK1);
-    double GuF2 = -(d112 * DXK2 + d212 * DYK2);
-    double GvF2 = -(d122 * DXK2 + d222 * DYK2);
-    double GwF1 = 0, GwF2 = 0;
+    const R DXK1 = dx<N, NZT, R>(Ds, T, 5, I, J, l), DYK1 = dy<N, NZT, R>(Ds, T, 5, I, J, l);
+    const R DXK2 = dx<N, NZT, R>(Ds, T, 6, I, J, l), DYK2 = dy<N, NZT, R>(Ds, T, 6, I, J, l);
+    R GuF1 = -(d111 * DXK1 + d211 * DYK1);
+    R GvF1 = -(d121 * DXK1 + d221 * DYK1);
+    R GuF2 = -(d112 * DXK2 + d212 * DYK2);
+    R GvF2 = -(d122 * DXK2 + d222 * DYK2);
+    R GwF1 = 0, GwF2 = 0;
     if (has_lo) {
       GuF1 += -GZlo * d311;
       GvF1 += -GZlo * d321;
-      GwF1 += -rho * GZlo * d331;
+      GwF1 += -rho_r * GZlo * d331_r;
     }
     if (has_hi) {
       GuF2 += -GZup * d312;
       GvF2 += -GZup * d322;
-      GwF2 += -rho * GZup * d332;
+      GwF2 += -rho_r * GZup * d332_r;
     }
-    const double GradZ = 0.5 * (K2 - K1);
+    const R GradZ = h * (K2 - K1);
     GuF2 += -GradZ * d312;
     GuF1 += -GradZ * d311;
     GvF2 += -GradZ * d322;
     GvF1 += -GradZ * d321;
-    GwF2 += -rho * GradZ * d332;
-    GwF1 += -rho * GradZ * d331;
-    const double DXp = dx<N, NZT>(Ds, T, 7, I, J, l), DYp = dy<N, NZT>(Ds, T, 7, I, J, l);
+    GwF2 += -rho_r * GradZ * d332_r;
+    GwF1 += -rho_r * GradZ * d331_r;
+    // horizontal pressure gradient from the FP64 pressure tile
+    const double DXp = dx<N, NZT, double>(Dd, TP, 0, I, J, l), DYp = dy<N, NZT, double>(Dd, TP, 0, I, J, l);
     double Gu = S11 * DXp + S21 * DYp;
     double Gv = S12 * DXp + S22 * DYp;
-    const double GZg = -Grav * rho * Js * rcp(d331 + d332);
+    const R GZg = R(-Grav * rho * double(Js) * rcp(d331 + d332));
     Gu += GZg * (d311 + d312);
     Gv += GZg * (d321 + d322);
-    f2 += (GuF1 + GuF2 - Gu * irho) * iMs;
-    f3 += (GvF1 + GvF2 - Gv * irho) * iMs;
-    if (has_hi) f4up += (GwF2 + Gpup) * iden_up;
+    f2 += (GuF1 + GuF2 - R(Gu) * irho) * iMs;
+    f3 += (GvF1 + GvF2 - R(Gv) * irho) * iMs;
+    if (has_hi) f4up += R((GwF2 + Gpup) * iden_up);
     if (has_lo) f4lo += GwF1 * iden_lo;
   }
   // ---- phase 2: second hyperviscosity apply, horizontal mass / theta flux ----
@@ -658,29 +711,29 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     TB.at(1, node, l) = Div;
     TB.at(2, node, l) = ThCx;
     TB.at(3, node, l) = ThCy;
-    TB.at(4, node, l) = (-rho) * (S11 * u + S12 * v);  // Contra12(-Rho, u, v)
-    TB.at(5, node, l) = (-rho) * (S21 * u + S22 * v);
+    TB.at(4, node, l) = (-rho_r) * (S11 * u + S12 * v);  // Contra12(-Rho, u, v)
+    TB.at(5, node, l) = (-rho_r) * (S21 * u + S22 * v);
     TB.at(6, node, l) = c_own;
   }
   __syncthreads();
   if (valid) {
     {
-      const double DxCurl = dx<N, NZT>(DWs, TB, 0, I, J, l);
-      const double DyCurl = dy<N, NZT>(DWs, TB, 0, I, J, l);
-      const double FuC = S12 * DxCurl + S22 * DyCurl;
-      const double FvC = -S11 * DxCurl - S21 * DyCurl;
-      const double DxDiv = dx<N, NZT>(DWs, TB, 1, I, J, l);
-      const double DyDiv = dy<N, NZT>(DWs, TB, 1, I, J, l);
-      const double FuD = S11 * DxDiv + S21 * DyDiv;
-      const double FvD = S12 * DxDiv + S22 * DyDiv;
-      f2 += -(-a.Kc * FuC + a.Kg * FuD) * iMs;
-      f3 += -(-a.Kc * FvC + a.Kg * FvD) * iMs;
+      const R DxCurl = dx<N, NZT, R>(DWs, TB, 0, I, J, l);
+      const R DyCurl = dy<N, NZT, R>(DWs, TB, 0, I, J, l);
+      const R FuC = S12 * DxCurl + S22 * DyCurl;
+      const R FvC = -S11 * DxCurl - S21 * DyCurl;
+      const R DxDiv = dx<N, NZT, R>(DWs, TB, 1, I, J, l);
+      const R DyDiv = dy<N, NZT, R>(DWs, TB, 1, I, J, l);
+      const R FuD = S11 * DxDiv + S21 * DyDiv;
+      const R FvD = S12 * DxDiv + S22 * DyDiv;
+      f2 += -(-Kc * FuC + Kg * FuD) * iMs;
+      f3 += -(-Kc * FvC + Kg * FvD) * iMs;
     }
-    const double DivTh = dxy<N, NZT>(DWs, TB, 2, 3, I, J, l);
-    f5 += -a.Kd * DivTh * iMs;
+    const R DivTh = dxy<N, NZT, R>(DWs, TB, 2, 3, I, J, l);
+    f5 += -Kd * DivTh * iMs;
     // DivRhoTrUpwind3New2Kernel!: horizontal part
-    const double DivRhoTr = dxy_prod<N, NZT>(Ds, TB, 4, 5, 6, I, J, l);
-    const double DivRho = dxy<N, NZT>(Ds, TB, 4, 5, I, J, l);
+    const R DivRhoTr = dxy_prod<N, NZT, R>(Ds, TB, 4, 5, 6, I, J, l);
+    const R DivRho = dxy<N, NZT, R>(Ds, TB, 4, 5, I, J, l);
     f5 += DivRhoTr * iMs;
     f1 += DivRho * iMs;
   }
@@ -689,7 +742,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   // and likewise V, W for the bottom (K=1) and top (K=2) vertical node; the
   // horizontal sums accumulate onto the vertical term in the reference's
   // order, six LDS tiles per round.
-  double U2 = U1, V2 = V1, W2 = W1;
+  R U2 = U1, V2 = V1, W2 = W1;
   if (in_tile) {
     T.at(0, node, l) = -d121 * w_lo;  // tempuZ1, K=1
     T.at(1, node, l) = -d221 * w_lo;  // tempuZ2
@@ -700,9 +753,9 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   }
   __syncthreads();
   if (valid) {
-    U1 = dxy_acc<N, NZT>(U1, Ds, T, 0, 1, I, J, l);
-    V1 = dxy_acc<N, NZT>(V1, Ds, T, 2, 3, I, J, l);
-    W1 = dxy_acc<N, NZT>(W1, Ds, T, 4, 5, I, J, l);
+    U1 = dxy_acc<N, NZT, R>(U1, Ds, T, 0, 1, I, J, l);
+    V1 = dxy_acc<N, NZT, R>(V1, Ds, T, 2, 3, I, J, l);
+    W1 = dxy_acc<N, NZT, R>(W1, Ds, T, 4, 5, I, J, l);
   }
   if (in_tile) {
     TB.at(0, node, l) = -d122 * w;  // K=2
@@ -714,16 +767,16 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   }
   __syncthreads();
   if (valid) {
-    U2 = dxy_acc<N, NZT>(U2, Ds, TB, 0, 1, I, J, l);
-    V2 = dxy_acc<N, NZT>(V2, Ds, TB, 2, 3, I, J, l);
-    W2 = dxy_acc<N, NZT>(W2, Ds, TB, 4, 5, I, J, l);
+    U2 = dxy_acc<N, NZT, R>(U2, Ds, TB, 0, 1, I, J, l);
+    V2 = dxy_acc<N, NZT, R>(V2, Ds, TB, 2, 3, I, J, l);
+    W2 = dxy_acc<N, NZT, R>(W2, Ds, TB, 4, 5, I, J, l);
     // CoriolisShallow: Wc = 2 Omega sin(lat) at the layer midpoint
     FU += (v * Wc) * Js;
     FV += (-u * Wc) * Js;
     f2 += (-v * W1 - w_lo * V1 - v * W2 - w * V2 + FU) * iMs;
     f3 += (u * W1 - w_lo * U1 + u * W2 - w * U2 + FV) * iMs;
-    if (has_lo) f4lo += rho * (u * V1 + v * U1 + FW1) * iden_lo;
-    if (has_hi) f4up += rho * (u * V2 + v * U2 + FW2) * iden_up;
+    if (has_lo) f4lo += rho_r * (u * V1 + v * U1 + FW1) * iden_lo;
+    if (has_hi) f4up += rho_r * (u * V2 + v * U2 + FW2) * iden_up;
   }
   // ---- phase 4: F4 from the layer above (same element, same node) ----
   if (in_tile) T.at(0, node, l) = f4lo;
@@ -732,7 +785,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     if (has_hi && l + 1 < nzt) f4up += T.at(0, node, l + 1);
     // DSS: the first element (in colour order) holding the node writes, the
     // others accumulate; the w lid (iz = Nz-1) receives no tendency
-    double *Fr = F, *Fu = F + vs, *Fv = F + 2 * vs, *Fw = F + 3 * vs, *Ft = F + 4 * vs;
+    R *Fr = F, *Fu = F + vs, *Fv = F + 2 * vs, *Fw = F + 3 * vs, *Ft = F + 4 * vs;
     at(Fr, ui) = first ? f1 : o1 + f1;
     at(Fu, ui) = first ? f2 : o2 + f2;
     at(Fv, ui) = first ? f3 : o3 + f3;
@@ -1024,17 +1077,19 @@ __global__ void k_jac(KArgs a, const double *__restrict__ U, const double *__res
 //    follow in registers, and the pressure of the next stage's
 //    FcnPrepareGPU! is written with it.
 // AuxG[:,:,4] (zero for State = Dry) is not read.
-struct RWStage {
-  const double *fV;        // this stage's tendency
+template <typename R> struct RWStage {
+  const R *fV;             // this stage's tendency (FP32 in the mixed-precision mode)
+  float *V32;              // FP32 shadow of the next stage state (mixed-precision mode; else null)
   const double *kprev[2];  // k_1, k_2 (earlier stages)
   double cf[2];            // c_ij / dt
   double av[3];            // V = Vn + sum_j av[j] k_j, j <= stage
-  const double *Vn;        // state at the start of the step
+  const double *Vn;        // state at the start of the step (stages 2, 3)
+  double *Vnw;             // where stage 1 stores it
   const double *dz;        // (Nz, NumG)
   double *V;               // next stage state, or the final state
   double *k;               // this stage's increment (null: not stored)
   double *P;               // AuxG[:,:,1] for the next stage (null: not written)
-  const double *F4x;       // chunk-interface w terms of k_fcn2
+  const R *F4x;            // chunk-interface w terms of k_fcn2
   const int *nptr, *ncopy; // node -> element copies, colour order
   int nchunk, nzt;
   double fac;              // dt * gamma
@@ -1042,8 +1097,8 @@ struct RWStage {
   int thomas;
 };
 
-template <int STAGE>
-__global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage rw) {
+template <int STAGE, typename R>
+__global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage<R> rw) {
   const int Nz = a.Nz, n = Nz - 1;
   const int i = threadIdx.x & 63;
   const size_t ig = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -1052,11 +1107,19 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage rw) {
   const size_t c0 = ig * Nz;
   const bool row = i < n, lev = i < Nz;
   const double fac = rw.fac;
+  // the state at the start of the step: at stage 1 it is V itself, which
+  // this kernel also copies to Vn (each lane reads its values before it
+  // overwrites V with the next stage state)
   double vn[5] = {0, 0, 0, 0, 0};
   double dzi = 1.0;
   if (lev) {
+    const double *src = STAGE == 0 ? rw.V : rw.Vn;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) vn[q] = rw.Vn[c0 + i + q * vs];
+    for (int q = 0; q < 5; ++q) vn[q] = src[c0 + i + q * vs];
+    if constexpr (STAGE == 0) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) rw.Vnw[c0 + i + q * vs] = vn[q];
+    }
     dzi = rw.dz[c0 + i];
   }
   // frozen Jacobian at interface i (levels i, i+1) and the tri matrix
@@ -1072,12 +1135,12 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage rw) {
   double kp[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
   if (lev) {
 #pragma unroll
-    for (int q = 0; q < 5; ++q) v[q] = rw.fV[c0 + i + q * vs];
+    for (int q = 0; q < 5; ++q) v[q] = double(rw.fV[c0 + i + q * vs]);
     if (rw.nchunk > 1 && (i + 1) % rw.nzt == 0 && i + 1 < Nz) {
       const int ch = (i + 1) / rw.nzt;
-      double s = 0.0;
+      R s = 0;
       for (int c = rw.nptr[ig]; c < rw.nptr[ig + 1]; ++c) s += rw.F4x[(size_t)rw.ncopy[c] * rw.nchunk + ch];
-      v[3] += s;
+      v[3] = double(R(v[3]) + s);
     }
 #pragma unroll
     for (int j = 0; j < STAGE; ++j) {
@@ -1119,20 +1182,28 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage rw) {
     for (int j = 0; j < STAGE; ++j) r = r + rw.av[j] * kp[j][q];
     r = r + rw.av[STAGE] * kc[q];
     rw.V[c0 + i + q * vs] = r;
+    if (rw.V32) rw.V32[c0 + i + q * vs] = float(r);
     if (q == 4) rt = r;
   }
   if (rw.P) rw.P[c0 + i] = eos_p(a, rt);
 }
 
-// Start of cgd_rw_step: Vn = V and the pressure of the first stage's
-// FcnPrepareGPU! in one pass over V.
-__global__ void k_rw_begin(KArgs a, const double *__restrict__ V, double *__restrict__ Vn, double *__restrict__ P,
-                           size_t n) {
+// Start of cgd_rw_step: the pressure of the first stage's FcnPrepareGPU!
+// (the copy Vn = V is made by the first stage kernel), and in the
+// mixed-precision mode the FP32 shadow of V.
+__global__ void k_rw_begin(KArgs a, const double *__restrict__ V, double *__restrict__ P,
+                           float *__restrict__ V32, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-#pragma unroll
-    for (int q = 0; q < 5; ++q) Vn[i + q * n] = V[i + q * n];
     P[i] = eos_p(a, V[i + 4 * n]);
+    if (V32)
+      for (int q = 0; q < 5; ++q) V32[i + q * n] = float(V[i + q * n]);
   }
+}
+
+// FP32 copy of a table (set-up)
+__global__ void k_to_f32(const double *__restrict__ x, float *__restrict__ y, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = float(x[i]);
 }
 
 // SchurSolveGPU! (src/Integration/SchurSolve.jl:45-177): tri build,

@@ -202,7 +202,14 @@ class GlobalStruct:
     NumG: int
     NF: int
     N: int
+    # "fp64", or "mixed": RosenbrockSchur computes the tendency and DSS in FP32
+    # (FP64 for the EOS, the w-interface balance, the horizontal pressure
+    # gradient and the column solve); the separate Fcn/Jac/Schur calls stay FP64
+    Precision: str = "fp64"
     _ctx: object = field(default=None, repr=False)
+
+
+PRECISIONS = {"fp64": 0, "mixed": 1}  # cgd_desc.precision: CGD_FP64, CGD_MIXED
 
 
 def ssp_knoth():
@@ -233,7 +240,9 @@ class NativeContext:
         M = Global.Model
         d = _Desc()
         d.N, d.Nz, d.NF, d.NumG = Global.N, Global.Nz, Global.NF, Global.NumG
-        d.NumV, d.NumTr, d.precision = M.NumV, M.NumTr, 0
+        if Global.Precision not in PRECISIONS:
+            raise CGDError(f"unknown precision {Global.Precision!r} (expected one of {sorted(PRECISIONS)})")
+        d.NumV, d.NumTr, d.precision = M.NumV, M.NumTr, PRECISIONS[Global.Precision]
         d.HorLimit = int(M.HorLimit)
         d.Turbulence, d.VerticalDiffusion = int(M.Turbulence), int(M.VerticalDiffusion)
         d.SurfaceFlux, d.Forcing, d.Microphysics = int(M.SurfaceFlux), int(M.Forcing), int(M.Microphysics)
@@ -403,7 +412,7 @@ def hyperdiffusion_for(ne: int) -> float:
 
 
 def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None = None, device="cuda",
-                 nT: int = 5):
+                 nT: int = 5, precision: str = "fp64"):
     """Upload the static inputs and allocate ``Cache``/``JCache``."""
     hd = hyperdiffusion_for(disc.ne) if hyperd is None else hyperd
     dev = torch.device(device)
@@ -415,7 +424,7 @@ def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None
     Metric = MetricStruct(dXdxI=torch.as_tensor(disc.dXdxI, **f64), J=torch.as_tensor(disc.J, **f64),
                           X=torch.as_tensor(disc.X, **f64), dz=torch.as_tensor(disc.dz, **f64))
     Model = ModelStruct(HyperDCurl=hd, HyperDGrad=hd, HyperDDiv=hd)
-    Global = GlobalStruct(Model=Model, Nz=disc.nz, NumG=disc.num_g, NF=disc.nf, N=disc.N)
+    Global = GlobalStruct(Model=Model, Nz=disc.nz, NumG=disc.num_g, NF=disc.nf, N=disc.N, Precision=precision)
     n = (disc.num_g, disc.nz)
     Cache = CacheStruct(AuxG=torch.zeros((4,) + n, **f64), Temp1=torch.zeros((nT,) + n, **f64),
                         k=torch.zeros((3, 5) + n, **f64), fV=torch.zeros((5,) + n, **f64),

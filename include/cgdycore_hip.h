@@ -36,7 +36,15 @@ enum {
   CGD_ERR_STATE = -4        /* context not usable (e.g. creation failed) */
 };
 
-enum { CGD_FP64 = 0 };
+/* precision modes (cgd_desc.precision).  CGD_MIXED applies to cgd_rw_step
+ * only: the tendency and its DSS are computed and accumulated in FP32; the
+ * equation of state, the w-interface balance of pressure gradient and
+ * gravity, the horizontal pressure gradient, the column Jacobian and Schur
+ * solve and the Rosenbrock stage arithmetic stay FP64, as does the state V.
+ * The fV and Temp1 work buffers then hold FP32 data (the FP32 tendency and
+ * an FP32 copy of the stage state in fV, the FP32 cache in Temp1).  The
+ * other entry points compute in FP64 in either mode. */
+enum { CGD_FP64 = 0, CGD_MIXED = 1 };
 
 typedef struct {
   /* dimensions (reference names) */
@@ -46,7 +54,7 @@ typedef struct {
   int NumG;   /* global CG nodes */
   int NumV;   /* must be 5: rho, u, v, w, rho*theta */
   int NumTr;  /* must be 0 */
-  int precision; /* CGD_FP64 */
+  int precision; /* CGD_FP64 or CGD_MIXED */
   /* model switches that select FcnGPU! branches; only the BASELINE set is
    * implemented: HorLimit=0, KoeffDivW=0, no EDMF/TKE/physics hooks */
   int HorLimit;

@@ -208,3 +208,54 @@ def test_rw_composed_equals_fused(pkg, torch_cuda, ne, nz):
     torch.cuda.synchronize()
     for c in range(5):
         assert rel_l2(V2[c].cpu().numpy(), V1[c].cpu().numpy()) <= 1e-14
+
+
+@pytest.mark.parametrize("ne,nz,nsteps", [(4, 64, 3), (6, 10, 3)])
+def test_mixed_precision_step(pkg, oracle_mod, torch_cuda, ne, nz, nsteps):
+    """Precision "mixed": the Rosenbrock-W step computes the tendency and its
+    DSS in FP32 (FP64 for the EOS, the w-interface balance, the horizontal
+    pressure gradient and the column solve); the separate FcnGPU call of a
+    mixed context is still the FP64 path (1e-12).
+
+    The stepped state's relative L2 error against the FP64 oracle is held to
+    the bounds below, an order above what this configuration shows (rho
+    2.3e-5, u 1.1e-5, v 1.2e-4, w 2.7e-3, rho*theta 1.0e-6 after three
+    300 s steps at ne=4).  They are not the 1e-5 the FP64 comparison aims
+    for: the jet is in hydrostatic and geostrophic balance, so every
+    tendency is a small residual of large cancelling terms, and FP32
+    rounding of those terms is large against it (v and w are also small
+    fields, so their relative errors are larger still)."""
+    torch = torch_cuda
+    host = pkg.host
+    D = pkg.discretization
+    disc = D.build(ne, 4, nz, 30000.0)
+    phys = D.Phys()
+    U0 = pkg.state.jet_state(disc, seed=0)
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda", precision="mixed")
+    orc = oracle_mod.Oracle(disc, phys, (Global.Model.HyperDCurl, Global.Model.HyperDGrad, Global.Model.HyperDDiv))
+    print(f"mixed precision ne={ne} nz={nz} steps={nsteps}", flush=True)
+    # drop-in tendency of a mixed context: FP64
+    U = torch.as_tensor(U0, device="cuda")
+    F = torch.empty_like(U)
+    eq = host.CompressibleShallow()
+    host.FcnPrepareGPU(U, FE, Metric, phys, Cache, None, Global, None, eq)
+    host.FcnGPU(F, U, FE, Metric, phys, Cache, None, Global, None, eq)
+    torch.cuda.synchronize()
+    Fo, _ = orc.fcn(U0.copy(), orc.prepare(U0.copy()))
+    Fg = F.cpu().numpy()
+    for v in (1, 2, 3, 4):
+        assert rel_l2(Fg[v], Fo[v]) <= TOL, f"FP64 F{v + 1} of a mixed context"
+    # mixed-precision steps
+    V = torch.as_tensor(U0, device="cuda").clone()
+    host.TimeStepper(V, 300.0, nsteps, FE, Metric, phys, Cache, JCache, Global)
+    torch.cuda.synchronize()
+    Vo = U0.copy()
+    orc.rw_step(Vo, 300.0, nsteps)
+    Vg = V.cpu().numpy()
+    assert np.isfinite(Vg).all()
+    bound = (2e-4, 1e-4, 1e-3, 3e-2, 1e-5)
+    for c in range(5):
+        e = rel_l2(Vg[c], Vo[c])
+        de = rel_l2(Vg[c] - U0[c], Vo[c] - U0[c])
+        print(f"  V{c + 1} rel L2 {e:.3e} (increment {de:.3e})")
+        assert e <= bound[c], f"V{c + 1} rel L2 {e:.3e}"
