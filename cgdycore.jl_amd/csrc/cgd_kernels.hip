@@ -574,6 +574,10 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   R f1 = 0, f5 = 0;
   R K1 = 0, K2 = 0;
   R U1 = 0, V1 = 0, W1 = 0;  // 1/2 (tempXZ[2] - tempXZ[1]) of this layer
+  // RecU4's weights depend on ratios of the layer volumes J only; in FP32
+  // their products of three volumes (~1e45 m^9) would overflow, so there
+  // the volumes are scaled by 1/J of this level first (FP64 uses them as is)
+  const R jsc = std::is_same<R, double>::value ? R(1) : rcp(Js);
   if (valid) {
     const R Kh = h * (u * u + v * v);
     if (has_lo) {
@@ -594,7 +598,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
       const R wCon = rho_m_r * (d312m * u_m + d322m * v_m + d332m * w_m) + rho_r * (d311 * u + d321 * v + d331_r * w_m);
       const R cRR = has_hi ? c_p : c_own, JRR = has_hi ? Jc_p : Js;
       R cFL, cFR;
-      recu4(c_mm, c_m, c_own, cRR, Jc_mm, Jc_m, Js, JRR, cFL, cFR);
+      recu4(c_mm, c_m, c_own, cRR, Jc_mm * jsc, Jc_m * jsc, Js * jsc, JRR * jsc, cFL, cFR);
       const R Flux = q4 * ((rabs(wCon) + wCon) * cFL + (-rabs(wCon) + wCon) * cFR);
       f5 += Flux * iMs;
       f1 += (h * wCon) * iMs;
@@ -623,7 +627,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
       const R wCon = rho_r * (d312 * u + d322 * v + d332_r * w) + rho_p_r * (d311p * u_p + d321p * v_p + d331p_r * w);
       const R cLL = has_lo ? c_m : c_own, JLL = has_lo ? Jc_m : Js;
       R cFL, cFR;
-      recu4(cLL, c_own, c_p, c_pp, JLL, Js, Jc_p, Jc_pp, cFL, cFR);
+      recu4(cLL, c_own, c_p, c_pp, JLL * jsc, Js * jsc, Jc_p * jsc, Jc_pp * jsc, cFL, cFR);
       const R Flux = q4 * ((rabs(wCon) + wCon) * cFL + (-rabs(wCon) + wCon) * cFR);
       f5 += -Flux * iMs;
       f1 += -(h * wCon) * iMs;

@@ -214,17 +214,12 @@ def test_rw_composed_equals_fused(pkg, torch_cuda, ne, nz):
 def test_mixed_precision_step(pkg, oracle_mod, torch_cuda, ne, nz, nsteps):
     """Precision "mixed": the Rosenbrock-W step computes the tendency and its
     DSS in FP32 (FP64 for the EOS, the w-interface balance, the horizontal
-    pressure gradient and the column solve); the separate FcnGPU call of a
-    mixed context is still the FP64 path (1e-12).
-
-    The stepped state's relative L2 error against the FP64 oracle is held to
-    the bounds below, an order above what this configuration shows (rho
-    2.3e-5, u 1.1e-5, v 1.2e-4, w 2.7e-3, rho*theta 1.0e-6 after three
-    300 s steps at ne=4).  They are not the 1e-5 the FP64 comparison aims
-    for: the jet is in hydrostatic and geostrophic balance, so every
-    tendency is a small residual of large cancelling terms, and FP32
-    rounding of those terms is large against it (v and w are also small
-    fields, so their relative errors are larger still)."""
+    pressure gradient and the column solve).  The stepped state stays within
+    1e-5 relative L2 of the FP64 oracle for rho, u, v and rho*theta (about
+    1e-8 to 3e-7 here), and w, which starts at zero and stays orders of
+    magnitude smaller than the winds, within 1e-4 of its own norm (2e-5
+    here); the separate FcnGPU call of a mixed context is still the FP64
+    path (1e-12)."""
     torch = torch_cuda
     host = pkg.host
     D = pkg.discretization
@@ -253,9 +248,10 @@ def test_mixed_precision_step(pkg, oracle_mod, torch_cuda, ne, nz, nsteps):
     orc.rw_step(Vo, 300.0, nsteps)
     Vg = V.cpu().numpy()
     assert np.isfinite(Vg).all()
-    bound = (2e-4, 1e-4, 1e-3, 3e-2, 1e-5)
+    errs = []
     for c in range(5):
         e = rel_l2(Vg[c], Vo[c])
         de = rel_l2(Vg[c] - U0[c], Vo[c] - U0[c])
         print(f"  V{c + 1} rel L2 {e:.3e} (increment {de:.3e})")
-        assert e <= bound[c], f"V{c + 1} rel L2 {e:.3e}"
+        errs.append(e)
+    assert max(errs[0], errs[1], errs[2], errs[4]) <= 1e-5 and errs[3] <= 1e-4, errs
