@@ -28,6 +28,8 @@ struct cgd_ctx {
   int cm = 0;                                    // 1: compact metric layout
   int mixed = 0;                                 // 1: mixed-precision cgd_rw_step
   float *d_ims32 = nullptr;                      // 1/(M1+M2) in FP32 (mixed precision)
+  float *d_m32 = nullptr;                        // M in FP32 (mixed precision)
+  float *d_metp32 = nullptr;                     // compact per-point metric in FP32 (mixed precision)
   double *d_f4x = nullptr;     // [NF][NN][nchunk] chunk-bottom F4 terms
   int *d_nptr = nullptr, *d_ncopy = nullptr;  // global node -> element copies (colour order)
   bool ok = false;
@@ -389,6 +391,13 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   if (c->mixed) {
     HIPCHK(c, hipMalloc(&c->d_ims32, ng * sizeof(float)));
     hipLaunchKernelGGL(k_to_f32, dim3(grid_for(ng, 256)), dim3(256), 0, st, c->d_ms, c->d_ims32, ng);
+    HIPCHK(c, hipMalloc(&c->d_m32, 2 * ng * sizeof(float)));
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(2 * ng, 256)), dim3(256), 0, st, d->M, c->d_m32, 2 * ng);
+    if (c->cm) {
+      const size_t nmp = npt * CGD_NMETP;
+      HIPCHK(c, hipMalloc(&c->d_metp32, nmp * sizeof(float)));
+      hipLaunchKernelGGL(k_to_f32, dim3(grid_for(nmp, 256)), dim3(256), 0, st, c->d_met, c->d_metp32, nmp);
+    }
     HIPCHK(c, hipGetLastError());
   }
   HIPCHK(c, hipStreamSynchronize(st));
@@ -422,6 +431,8 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   a.met1 = c->d_met1;
   a.iMs = c->d_ms;
   a.iMs32 = c->d_ims32;
+  a.M32 = c->d_m32;
+  a.metp32 = c->d_metp32;
   a.M = d->M;
   c->ok = true;
   return CGD_OK;
@@ -434,7 +445,7 @@ extern "C" int cgd_ctx_destroy(cgd_ctx *c) {
   // a context that failed validation holds no device memory: tear it down
   // without touching the HIP runtime
   for (void *p : {(void *)c->d_glob, (void *)c->d_slot, (void *)c->d_met, (void *)c->d_met1, (void *)c->d_metn,
-                  (void *)c->d_met1n, (void *)c->d_ms, (void *)c->d_ims32,
+                  (void *)c->d_met1n, (void *)c->d_ms, (void *)c->d_ims32, (void *)c->d_m32, (void *)c->d_metp32,
                   (void *)c->d_bwork, (void *)c->d_f4x, (void *)c->d_nptr, (void *)c->d_ncopy})
     if (p) (void)hipFree(p);
   prof_free(c);
@@ -657,8 +668,11 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
     RWStage<R> rw{};
     rw.fV = F;
     rw.V32 = is < 2 ? V32 : nullptr;
+    // the stage increments are stored as R (FP32 in the mixed mode: they
+    // are ~dt*F, so their rounding is ~1e-7 of an increment)
+    R *kR[3] = {reinterpret_cast<R *>(k[0]), reinterpret_cast<R *>(w->k) + n5, reinterpret_cast<R *>(w->k) + 2 * n5};
     for (int j = 0; j < is; ++j) {
-      rw.kprev[j] = k[j];
+      rw.kprev[j] = kR[j];
       rw.cf[j] = Cc[is][j] / dt;
     }
     for (int j = 0; j <= is; ++j) rw.av[j] = (is < 2) ? A[is + 1][j] : Mw[j];
@@ -666,7 +680,7 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
     rw.Vnw = w->Vn;
     rw.dz = c->desc.dz;
     rw.V = V;
-    rw.k = is < 2 ? k[is] : nullptr;  // k_3 is consumed in registers
+    rw.k = is < 2 ? kR[is] : nullptr;  // k_3 is consumed in registers
     rw.P = is < 2 ? w->AuxG : nullptr;
     rw.F4x = reinterpret_cast<const R *>(c->d_f4x);
     rw.nptr = c->d_nptr;

@@ -43,6 +43,8 @@ struct KArgs {
   const double *met1n;             // compact layout: [slot][5][NN]
   const double *iMs;               // [NumG][Nz]: 1 / (M1 + M2)
   const float *iMs32;              // the same in FP32 (mixed-precision mode)
+  const float *M32;                // M in FP32 (mixed-precision mode)
+  const float *metp32;             // compact layout's per-point components in FP32 (mixed precision)
   const double *M;                 // [2][NumG][Nz]
 };
 
@@ -432,7 +434,13 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   const R *Uu = UR + vs, *Uv = UR + 2 * vs, *Uw = UR + 3 * vs, *Ut = UR + 4 * vs;
   const R *iMsR = std::is_same<R, double>::value ? reinterpret_cast<const R *>(a.iMs)
                                                   : reinterpret_cast<const R *>(a.iMs32);
-  const double *metE = a.met + (size_t)e * (CM ? CGD_NMETP : CGD_NMET) * NNZ;
+  constexpr bool F32 = !std::is_same<R, double>::value;
+  // per-point metric: the compact layout's components are used in R terms
+  // only, so the mixed mode reads an FP32 copy; the full layout stays FP64
+  // (it also carries dXdxI[3,3] and J for the FP64 w-interface balance)
+  using MP = typename std::conditional<CM && F32, float, double>::type;
+  const MP *metE = (CM && F32) ? reinterpret_cast<const MP *>(a.metp32) + (size_t)e * CGD_NMETP * NNZ
+                               : reinterpret_cast<const MP *>(a.met) + (size_t)e * (CM ? CGD_NMETP : CGD_NMET) * NNZ;
   const double *metN = CM ? a.metn + (size_t)e * CGD_NMETN * NN : nullptr;
   unsigned ui = 0, mo = 0;
   bool first = false;
@@ -453,7 +461,9 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   R u = 0, v = 0, w = 0, rth = 1, iMs = 1, M1 = 1, M2 = 1;
   R d111 = 0, d112 = 0, d121 = 0, d122 = 0, d211 = 0, d212 = 0, d221 = 0, d222 = 0;
   R d311 = 0, d312 = 0, d321 = 0, d322 = 0, Wc = 0;
-  const double *M1b = a.M, *M2b = a.M + vs;
+  using MM = typename std::conditional<F32, float, double>::type;
+  const MM *M1b = F32 ? reinterpret_cast<const MM *>(a.M32) : reinterpret_cast<const MM *>(a.M);
+  const MM *M2b = M1b + vs;
   if (valid) {
     rho = at(Ur, ui);
     u = at(Uu, ui);
@@ -1084,14 +1094,14 @@ __global__ void k_jac(KArgs a, const double *__restrict__ U, const double *__res
 template <typename R> struct RWStage {
   const R *fV;             // this stage's tendency (FP32 in the mixed-precision mode)
   float *V32;              // FP32 shadow of the next stage state (mixed-precision mode; else null)
-  const double *kprev[2];  // k_1, k_2 (earlier stages)
+  const R *kprev[2];       // k_1, k_2 (earlier stages; FP32 storage in the mixed mode)
   double cf[2];            // c_ij / dt
   double av[3];            // V = Vn + sum_j av[j] k_j, j <= stage
   const double *Vn;        // state at the start of the step (stages 2, 3)
   double *Vnw;             // where stage 1 stores it
   const double *dz;        // (Nz, NumG)
   double *V;               // next stage state, or the final state
-  double *k;               // this stage's increment (null: not stored)
+  R *k;                    // this stage's increment (null: not stored)
   double *P;               // AuxG[:,:,1] for the next stage (null: not written)
   const R *F4x;            // chunk-interface w terms of k_fcn2
   const int *nptr, *ncopy; // node -> element copies, colour order
@@ -1150,7 +1160,7 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage<R> rw) {
     for (int j = 0; j < STAGE; ++j) {
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
-        kp[j][q] = rw.kprev[j][c0 + i + q * vs];
+        kp[j][q] = double(rw.kprev[j][c0 + i + q * vs]);
         v[q] = v[q] + rw.cf[j] * kp[j][q];
       }
     }
@@ -1176,7 +1186,7 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage<R> rw) {
   const double kc[5] = {fac * v[0], fac * v[1], fac * v[2], k4, fac * v[4]};
   if (rw.k) {
 #pragma unroll
-    for (int q = 0; q < 5; ++q) rw.k[c0 + i + q * vs] = kc[q];
+    for (int q = 0; q < 5; ++q) rw.k[c0 + i + q * vs] = R(kc[q]);
   }
   double rt = 0.0;
 #pragma unroll

@@ -41,9 +41,10 @@ enum {
  * equation of state, the w-interface balance of pressure gradient and
  * gravity, the horizontal pressure gradient, the column Jacobian and Schur
  * solve and the Rosenbrock stage arithmetic stay FP64, as does the state V.
- * The fV and Temp1 work buffers then hold FP32 data (the FP32 tendency and
- * an FP32 copy of the stage state in fV, the FP32 cache in Temp1).  The
- * other entry points compute in FP64 in either mode. */
+ * The fV, Temp1 and k work buffers then hold FP32 data (the FP32 tendency
+ * and an FP32 copy of the stage state in fV, the FP32 cache in Temp1, the
+ * stage increments, solved in FP64, stored in FP32 in k).  The other entry
+ * points compute in FP64 in either mode. */
 enum { CGD_FP64 = 0, CGD_MIXED = 1 };
 
 typedef struct {
