@@ -44,13 +44,14 @@ struct cgd_ctx {
 };
 
 // LDS bytes of the element kernels for tendency type R: pass 1 holds D, DW
-// and tile sets of 5 and 4; pass 2 an FP64 D and pressure tile, then D, DW
-// and two sets of 7 tiles
+// and tile sets of 5 and 4; pass 2 an FP64 D and pressure tile, then D, DW,
+// two sets of 7 tiles and the per-node horizontal metric
 template <typename R> static size_t lds_fcn1(const cgd_ctx *c) {
   return sizeof(R) * (2 * c->NN + 9 * c->NN * c->nzt);
 }
 template <typename R> static size_t lds_fcn2(const cgd_ctx *c) {
-  return sizeof(double) * (c->NN + c->NN * c->nzt) + sizeof(R) * (2 * c->NN + 14 * c->NN * c->nzt);
+  // + a 12-entry per-node table of the horizontal metric (compact layout)
+  return sizeof(double) * (c->NN + c->NN * c->nzt) + sizeof(R) * (2 * c->NN + 14 * c->NN * c->nzt + 12 * c->NN);
 }
 
 static int fail(cgd_ctx *c, int code, const std::string &msg) {

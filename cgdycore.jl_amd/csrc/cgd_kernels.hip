@@ -400,13 +400,19 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   static_assert(NZT >= 3 && NZT <= 16, "chunk heights 3..16 (the DPP neighbour reach is 2)");
   extern __shared__ double smem[];
   // LDS: FP64 D matrix and pressure tile, then the R-typed D, DW and two
-  // sets of 7 tiles used alternately (phases 1, 3a, 4 in T; 2, 3b in TB),
+  // sets of 7 tiles used alternately (phases 1a, 2, 4 in T; 1b, 3 in TB),
   // so a phase's tiles are never rewritten before every lane has passed
   // the next phase's barrier: one barrier per phase
   double *Dd = smem;
   Tiles<N, NZT, double> TP{smem + NN};
   R *Ds = reinterpret_cast<R *>(smem + NN + NN * NZT), *DWs = Ds + NN;
   Tiles<N, NZT, R> T{Ds + 2 * NN}, TB{Ds + 2 * NN + 7 * NN * NZT};
+  // compact metric layout in FP32: the horizontal components d[1:2][1:2]K
+  // and their sums S_ab are per-node constants first used in phase 1, so
+  // they live in an LDS table (written now, read after phase 1's barrier)
+  // instead of registers for the whole kernel (5 waves per SIMD instead of
+  // 4); in FP64 the extra LDS reads cost more than the registers save
+  R *mt = TB.t + 7 * NN * NZT;
   load_dmats<N>(a, Ds, DWs);
   if (threadIdx.x < NN) Dd[threadIdx.x] = a.D[threadIdx.x];
   const int Nz = a.Nz;
@@ -474,10 +480,25 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     M1 = R(at(M1b, ui));
     M2 = R(at(M2b, ui));
     p = at(P, ui);
-    d111 = R(MET0(M111)); d112 = R(MET0(M112)); d121 = R(MET0(M121)); d122 = R(MET0(M122));
-    d211 = R(MET0(M211)); d212 = R(MET0(M212)); d221 = R(MET0(M221)); d222 = R(MET0(M222));
+    if constexpr (!(CM && F32)) {
+      d111 = R(MET0(M111)); d112 = R(MET0(M112)); d121 = R(MET0(M121)); d122 = R(MET0(M122));
+      d211 = R(MET0(M211)); d212 = R(MET0(M212)); d221 = R(MET0(M221)); d222 = R(MET0(M222));
+    }
     d311 = R(MET0(M311)); d312 = R(MET0(M312)); d321 = R(MET0(M321)); d322 = R(MET0(M322));
     d331 = MET0(M331); d332 = MET0(M332); J1 = MET0(MJ1); J2 = MET0(MJ2); Wc = R(MET0(MCOR));
+  }
+  if constexpr (CM && F32) {
+    if (in_tile && l == 0) {
+      R hm[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) hm[c] = R(METN(M111 + c));
+#pragma unroll
+      for (int c = 0; c < 8; ++c) mt[c * NN + node] = hm[c];
+      mt[8 * NN + node] = hm[0] + hm[1];   // S11 = d111 + d112
+      mt[9 * NN + node] = hm[2] + hm[3];   // S12
+      mt[10 * NN + node] = hm[4] + hm[5];  // S21
+      mt[11 * NN + node] = hm[6] + hm[7];  // S22
+    }
   }
   const R rho_r = R(rho), d331_r = R(d331), d332_r = R(d332);
   const R Js = R(J1 + J2);
@@ -646,9 +667,66 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     V1 = h * ((d332_r * u - d312 * w) - (d331_r * u - d311 * w_lo));
     W1 = h * ((d322 * u - d312 * v) - (d321 * u - d311 * v));
   }
-  const R S11 = d111 + d112, S12 = d121 + d122, S21 = d211 + d212, S22 = d221 + d222;
+  const R S11r = d111 + d112, S12r = d121 + d122, S21r = d211 + d212, S22r = d221 + d222;
+#define HM(c, reg) (CM && F32 ? mt[(c) * NN + node] : (reg))
+#define D111 HM(0, d111)
+#define D112 HM(1, d112)
+#define D121 HM(2, d121)
+#define D122 HM(3, d122)
+#define D211 HM(4, d211)
+#define D212 HM(5, d212)
+#define D221 HM(6, d221)
+#define D222 HM(7, d222)
+#define S11 HM(8, S11r)
+#define S12 HM(9, S12r)
+#define S21 HM(10, S21r)
+#define S22 HM(11, S22r)
 
-  // ---- phase 1: hyperviscosity inputs, kinetic energy, pressure ----
+  R f2 = 0, f3 = 0, f4up = 0, f4lo = 0;
+  // ---- phase 1: vector-invariant momentum + Coriolis ----
+  // (first after phase 0, so the vertical vorticity-flux parts FU, FV, FW,
+  // U1, V1, W1 retire before the hyperviscosity and gradient phases)
+  // U1 = 1/2 (tempuZ[2]-tempuZ[1]) + sum_k (D[I,k] tempuZ1(k,J) + D[J,k] tempuZ2(I,k))
+  // and likewise V, W for the bottom (K=1) and top (K=2) vertical node; the
+  // horizontal sums accumulate onto the vertical term in the reference's
+  // order, six LDS tiles per round.
+  R U2 = U1, V2 = V1, W2 = W1;
+  if (in_tile) {
+    T.at(0, node, l) = -D121 * w_lo;  // tempuZ1, K=1
+    T.at(1, node, l) = -D221 * w_lo;  // tempuZ2
+    T.at(2, node, l) = -D111 * w_lo;  // tempvZ1
+    T.at(3, node, l) = -D211 * w_lo;  // tempvZ2
+    T.at(4, node, l) = D121 * u - D111 * v;  // tempwZ1
+    T.at(5, node, l) = D221 * u - D211 * v;  // tempwZ2
+  }
+  __syncthreads();
+  if (valid) {
+    U1 = dxy_acc<N, NZT, R>(U1, Ds, T, 0, 1, I, J, l);
+    V1 = dxy_acc<N, NZT, R>(V1, Ds, T, 2, 3, I, J, l);
+    W1 = dxy_acc<N, NZT, R>(W1, Ds, T, 4, 5, I, J, l);
+  }
+  if (in_tile) {
+    TB.at(0, node, l) = -D122 * w;  // K=2
+    TB.at(1, node, l) = -D222 * w;
+    TB.at(2, node, l) = -D112 * w;
+    TB.at(3, node, l) = -D212 * w;
+    TB.at(4, node, l) = D122 * u - D112 * v;
+    TB.at(5, node, l) = D222 * u - D212 * v;
+  }
+  __syncthreads();
+  if (valid) {
+    U2 = dxy_acc<N, NZT, R>(U2, Ds, TB, 0, 1, I, J, l);
+    V2 = dxy_acc<N, NZT, R>(V2, Ds, TB, 2, 3, I, J, l);
+    W2 = dxy_acc<N, NZT, R>(W2, Ds, TB, 4, 5, I, J, l);
+    // CoriolisShallow: Wc = 2 Omega sin(lat) at the layer midpoint
+    FU += (v * Wc) * Js;
+    FV += (-u * Wc) * Js;
+    f2 += (-v * W1 - w_lo * V1 - v * W2 - w * V2 + FU) * iMs;
+    f3 += (u * W1 - w_lo * U1 + u * W2 - w * U2 + FV) * iMs;
+    if (has_lo) f4lo += rho_r * (u * V1 + v * U1 + FW1) * iden_lo;
+    if (has_hi) f4up += rho_r * (u * V2 + v * U2 + FW2) * iden_up;
+  }
+  // ---- phase 2: hyperviscosity inputs, kinetic energy, pressure ----
   if (in_tile) {
     R c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
     if (valid) {
@@ -669,7 +747,6 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   }
   __syncthreads();
   R Curl = 0, Div = 0, ThCx = 0, ThCy = 0;
-  R f2 = 0, f3 = 0, f4up = 0, f4lo = 0;
   if (valid) {
     {
       const R Dxc = dx<N, NZT, R>(Ds, T, 4, I, J, l);
@@ -685,10 +762,10 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     // GradFullKernel!
     const R DXK1 = dx<N, NZT, R>(Ds, T, 5, I, J, l), DYK1 = dy<N, NZT, R>(Ds, T, 5, I, J, l);
     const R DXK2 = dx<N, NZT, R>(Ds, T, 6, I, J, l), DYK2 = dy<N, NZT, R>(Ds, T, 6, I, J, l);
-    R GuF1 = -(d111 * DXK1 + d211 * DYK1);
-    R GvF1 = -(d121 * DXK1 + d221 * DYK1);
-    R GuF2 = -(d112 * DXK2 + d212 * DYK2);
-    R GvF2 = -(d122 * DXK2 + d222 * DYK2);
+    R GuF1 = -(D111 * DXK1 + D211 * DYK1);
+    R GvF1 = -(D121 * DXK1 + D221 * DYK1);
+    R GuF2 = -(D112 * DXK2 + D212 * DYK2);
+    R GvF2 = -(D122 * DXK2 + D222 * DYK2);
     R GwF1 = 0, GwF2 = 0;
     if (has_lo) {
       GuF1 += -GZlo * d311;
@@ -719,7 +796,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     if (has_hi) f4up += R((GwF2 + Gpup) * iden_up);
     if (has_lo) f4lo += GwF1 * iden_lo;
   }
-  // ---- phase 2: second hyperviscosity apply, horizontal mass / theta flux ----
+  // ---- phase 3: second hyperviscosity apply, horizontal mass / theta flux ----
   if (in_tile) {
     TB.at(0, node, l) = Curl;
     TB.at(1, node, l) = Div;
@@ -751,47 +828,6 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     f5 += DivRhoTr * iMs;
     f1 += DivRho * iMs;
   }
-  // ---- phase 3: vector-invariant momentum + Coriolis ----
-  // U1 = 1/2 (tempuZ[2]-tempuZ[1]) + sum_k (D[I,k] tempuZ1(k,J) + D[J,k] tempuZ2(I,k))
-  // and likewise V, W for the bottom (K=1) and top (K=2) vertical node; the
-  // horizontal sums accumulate onto the vertical term in the reference's
-  // order, six LDS tiles per round.
-  R U2 = U1, V2 = V1, W2 = W1;
-  if (in_tile) {
-    T.at(0, node, l) = -d121 * w_lo;  // tempuZ1, K=1
-    T.at(1, node, l) = -d221 * w_lo;  // tempuZ2
-    T.at(2, node, l) = -d111 * w_lo;  // tempvZ1
-    T.at(3, node, l) = -d211 * w_lo;  // tempvZ2
-    T.at(4, node, l) = d121 * u - d111 * v;  // tempwZ1
-    T.at(5, node, l) = d221 * u - d211 * v;  // tempwZ2
-  }
-  __syncthreads();
-  if (valid) {
-    U1 = dxy_acc<N, NZT, R>(U1, Ds, T, 0, 1, I, J, l);
-    V1 = dxy_acc<N, NZT, R>(V1, Ds, T, 2, 3, I, J, l);
-    W1 = dxy_acc<N, NZT, R>(W1, Ds, T, 4, 5, I, J, l);
-  }
-  if (in_tile) {
-    TB.at(0, node, l) = -d122 * w;  // K=2
-    TB.at(1, node, l) = -d222 * w;
-    TB.at(2, node, l) = -d112 * w;
-    TB.at(3, node, l) = -d212 * w;
-    TB.at(4, node, l) = d122 * u - d112 * v;
-    TB.at(5, node, l) = d222 * u - d212 * v;
-  }
-  __syncthreads();
-  if (valid) {
-    U2 = dxy_acc<N, NZT, R>(U2, Ds, TB, 0, 1, I, J, l);
-    V2 = dxy_acc<N, NZT, R>(V2, Ds, TB, 2, 3, I, J, l);
-    W2 = dxy_acc<N, NZT, R>(W2, Ds, TB, 4, 5, I, J, l);
-    // CoriolisShallow: Wc = 2 Omega sin(lat) at the layer midpoint
-    FU += (v * Wc) * Js;
-    FV += (-u * Wc) * Js;
-    f2 += (-v * W1 - w_lo * V1 - v * W2 - w * V2 + FU) * iMs;
-    f3 += (u * W1 - w_lo * U1 + u * W2 - w * U2 + FV) * iMs;
-    if (has_lo) f4lo += rho_r * (u * V1 + v * U1 + FW1) * iden_lo;
-    if (has_hi) f4up += rho_r * (u * V2 + v * U2 + FW2) * iden_up;
-  }
   // ---- phase 4: F4 from the layer above (same element, same node) ----
   if (in_tile) T.at(0, node, l) = f4lo;
   __syncthreads();
@@ -813,6 +849,19 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
 #undef MET
 #undef METN
 #undef MET0
+#undef HM
+#undef D111
+#undef D112
+#undef D121
+#undef D122
+#undef D211
+#undef D212
+#undef D221
+#undef D222
+#undef S11
+#undef S12
+#undef S21
+#undef S22
 }
 
 // F4 of the interfaces between level chunks: the lower-interface term of a
