@@ -662,7 +662,7 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
   R *T1 = reinterpret_cast<R *>(w->Temp1);
   float *V32 = mixed ? reinterpret_cast<float *>(w->fV) + n5 : nullptr;
   const R *UR = mixed ? reinterpret_cast<const R *>(V32) : reinterpret_cast<const R *>(V);
-  hipLaunchKernelGGL(k_rw_begin, dim3(grid_for(ng, 256)), dim3(256), 0, st, c->ka, V, w->AuxG, V32, ng);
+  hipLaunchKernelGGL(k_rw_begin, dim3(grid_for((ng + 3) / 4, 256)), dim3(256), 0, st, c->ka, V, w->AuxG, V32, ng);
   HIPCHK(c, hipGetLastError());
   for (int is = 0; is < 3; ++is) {
     if ((r = run_fcn<R>(c, F, V, UR, T1, w->AuxG, st, false))) return r;

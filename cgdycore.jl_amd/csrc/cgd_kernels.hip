@@ -1254,9 +1254,24 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage<R> rw) {
 // Start of cgd_rw_step: the pressure of the first stage's FcnPrepareGPU!
 // (the copy Vn = V is made by the first stage kernel), and in the
 // mixed-precision mode the FP32 shadow of V.
+// Each thread takes four points a grid-stride apart, so their loads and
+// EOS chains (exp, log) overlap.
 __global__ void k_rw_begin(KArgs a, const double *__restrict__ V, double *__restrict__ P,
                            float *__restrict__ V32, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    double rt[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rt[u] = V[i + u * stride + 4 * n];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) P[i + u * stride] = eos_p(a, rt[u]);
+    if (V32)
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        for (int q = 0; q < 5; ++q) V32[i + u * stride + q * n] = float(V[i + u * stride + q * n]);
+  }
+  for (; i < n; i += stride) {
     P[i] = eos_p(a, V[i + 4 * n]);
     if (V32)
       for (int q = 0; q < 5; ++q) V32[i + q * n] = float(V[i + q * n]);
