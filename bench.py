@@ -104,6 +104,85 @@ def mixed_precision(host, disc, phys, U0, args, fp64_ms, V_fp64):
                          "column solve and stage arithmetic"}
 
 
+def ensemble(pkg, args):
+    """The batched-ensemble workload: args.ens_members members (theta-noise
+    seeds 0..n-1) on cubed spheres ne = 8, 12, 16 (member m on
+    (8, 12, 16)[m mod 3]), p = 4, nz = 32, dt = args.ens_dt, one batch
+    context, launch shapes from the autotuner, each step replayed from a
+    captured HIP graph.  Reports the mixed-precision batch, the FP64 batch,
+    and the FP64 members stepped one context at a time (the sum of the
+    per-grid step times, each grid timed on its own context), with the
+    mixed batch's error against the FP64 batch after the same steps."""
+    import torch
+
+    host = pkg.host
+    D = pkg.discretization
+    phys = D.Phys()
+    nes, seeds = pkg.state.ensemble_members(args.ens_members)
+    nz, dt, K, W = 32, args.ens_dt, args.ens_steps, 1
+    grids = {ne: D.build(ne, 4, nz, 30000.0) for ne in sorted(set(nes))}
+    discs = [grids[ne] for ne in nes]
+    U0 = pkg.state.ensemble_state(discs, seeds)
+    points = sum(d.element_points for d in discs)
+
+    def timed(fn, nsteps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / nsteps * 1e3
+
+    out = {"members": len(nes), "grids": {str(ne): nes.count(ne) for ne in sorted(set(nes))}, "p": 4, "nz": nz,
+           "dt": dt, "steps": K, "points": points}
+    finals = {}
+    for prec in ("fp64", "mixed"):
+        ens = host.ensemble_setup(discs, phys, precision=prec, device="cuda:0")
+        ctx = ens.Global._ctx
+        V = torch.as_tensor(U0, device="cuda:0").clone()
+        shape, tune_ms = ctx.autotune(V, dt, ens.Cache, ens.JCache, reps=3)
+        ctx.graph_create(V, dt, ens.Cache, ens.JCache, nsteps=1)
+        for _ in range(W):
+            ctx.graph_launch()
+        ms = timed(ctx.graph_launch, K)
+        finals[prec] = V.clone()
+        out[prec] = {"value": round(points / ms / 1e3, 2), "unit": "Mpts/s", "ms_per_step": round(ms, 4),
+                     "launch_shape": {"chunk_levels": shape[0], "col_lanes": shape[1], "col_block": shape[2]},
+                     "finite": bool(torch.isfinite(V).all().item())}
+        ctx.close()
+        del ens, V
+        torch.cuda.empty_cache()
+    out["mixed"]["speedup_vs_fp64_batch"] = round(out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"], 3)
+    rel = {}
+    for c, n in enumerate(("rho", "u", "v", "w", "rhotheta")):
+        d = (finals["mixed"][c] - finals["fp64"][c]).double()
+        rel[n] = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(finals["fp64"][c].double()))
+    out["mixed"]["rel_l2_vs_fp64_batch"] = rel
+    out["mixed"]["steps_compared"] = W + K
+    # the same members without batching: one FP64 context per grid, the
+    # step time of each member taken from its grid's context
+    per_grid = {}
+    for ne, disc in grids.items():
+        FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda:0")
+        eq = host.CompressibleShallow()
+        host.init_context(FE, Metric, phys, Global)
+        V = torch.as_tensor(pkg.state.jet_state(disc, seed=0), device="cuda:0").clone()
+
+        def step():
+            host.RosenbrockSchur(V, dt, host.FcnGPU, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys, Cache,
+                                 JCache, None, Global, None, eq)
+
+        step()
+        per_grid[ne] = timed(step, K)
+        Global._ctx.close()
+    seq_ms = sum(per_grid[ne] for ne in nes)
+    out["fp64_one_member_at_a_time"] = {"ms_per_step": round(seq_ms, 4),
+                                        "value": round(points / seq_ms / 1e3, 2), "unit": "Mpts/s",
+                                        "ms_per_step_per_grid": {str(k): round(v, 4) for k, v in per_grid.items()}}
+    out["batch_speedup_fp64"] = round(seq_ms / out["fp64"]["ms_per_step"], 3)
+    return out
+
+
 def cpu_info():
     model = ""
     try:
@@ -131,6 +210,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3,
                     help="oracle RW steps for the CPU baseline, about 5 s each at ne=30 (0: skip)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
+    ap.add_argument("--ens-members", type=int, default=32)
+    ap.add_argument("--ens-steps", type=int, default=10)
+    ap.add_argument("--ens-dt", type=float, default=150.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB three times (HBM counter calibration)")
     args = ap.parse_args()
@@ -222,6 +305,11 @@ def main():
         mixed = mixed_precision(host, disc, phys, U0, args, ms_per_step, V_fp64)
         print(f"[bench] mixed precision: {mixed}", file=sys.stderr, flush=True)
 
+    ens = None
+    if not args.no_ensemble:
+        ens = ensemble(pkg, args)
+        print(f"[bench] ensemble: {ens}", file=sys.stderr, flush=True)
+
     cpu = None
     if args.cpu_steps > 0:
         O = ge.load_oracle()
@@ -241,7 +329,7 @@ def main():
            "config": {"workload": workload, "ne": args.ne, "p": args.p,
                       "nz": args.nz, "dt": args.dt, "hyperd": Global.Model.HyperDDiv, "points": Ne,
                       "precision": "fp64", "finite": finite},
-           "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed}
+           "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "ensemble": ens}
     print(json.dumps(out), flush=True)
 
 

@@ -32,6 +32,20 @@ struct cgd_ctx {
   float *d_metp32 = nullptr;                     // compact per-point metric in FP32 (mixed precision)
   double *d_f4x = nullptr;     // [NF][NN][nchunk] chunk-bottom F4 terms
   int *d_nptr = nullptr, *d_ncopy = nullptr;  // global node -> element copies (colour order)
+  double *d_kvis = nullptr;    // [slot][3] hyperviscosity coefficients
+  // batch of grids (cgd_ctx_create_batch): member m owns elements
+  // mem_e0[m]..mem_e0[m+1]-1 and global nodes mem_g0[m]..mem_g0[m+1]-1
+  int nmem = 1;
+  std::vector<int> mem_e0, mem_g0;
+  double *d_M = nullptr, *d_dz = nullptr;  // M and dz in batch numbering (batches only)
+  const double *M = nullptr, *dz = nullptr;  // the M and dz the kernels read
+  // column launch shape of the fused stage kernel: lanes per column (64,
+  // or 32 for two columns per wavefront when Nz <= 32) and workgroup size
+  int colw = 64, colblock = 256;
+  // captured step graph (cgd_rw_graph_create)
+  hipStream_t gstream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
   bool ok = false;
   std::string err;
   // per-pass timing (events on the launch stream)
@@ -238,42 +252,75 @@ extern "C" int cgd_ctx_info(const cgd_ctx *c, int *ncolor, int *chunk_levels, in
   return CGD_OK;
 }
 
-extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
-  if (!out) return CGD_ERR_ARG;
-  *out = nullptr;
-  cgd_ctx *c = new cgd_ctx();
-  *out = c;
-  if (!d) return fail(c, CGD_ERR_ARG, "null descriptor");
-  c->desc = *d;
-  if (d->N < 4 || d->N > 5) return fail(c, CGD_ERR_UNSUPPORTED, "OrdPoly must be 3 or 4 (N = 4 or 5)");
-  if (d->Nz < 3 || d->Nz > 256) return fail(c, CGD_ERR_ARG, "Nz must be in [3, 256]");
-  if (d->NF < 1 || d->NumG < 1) return fail(c, CGD_ERR_ARG, "NF and NumG must be positive");
-  if (d->NumV != 5) return fail(c, CGD_ERR_UNSUPPORTED, "NumV must be 5 (rho, u, v, w, rho*theta)");
-  if (d->NumTr != 0) return fail(c, CGD_ERR_UNSUPPORTED, "tracers (NumTr > 0) are not implemented");
+// Checks one descriptor on its own (dimensions, flags, pointers).
+static int validate_desc(cgd_ctx *c, const cgd_desc *d, const std::string &who) {
+  if (d->N < 4 || d->N > 5) return fail(c, CGD_ERR_UNSUPPORTED, who + "OrdPoly must be 3 or 4 (N = 4 or 5)");
+  if (d->Nz < 3 || d->Nz > 256) return fail(c, CGD_ERR_ARG, who + "Nz must be in [3, 256]");
+  if (d->NF < 1 || d->NumG < 1) return fail(c, CGD_ERR_ARG, who + "NF and NumG must be positive");
+  if (d->NumV != 5) return fail(c, CGD_ERR_UNSUPPORTED, who + "NumV must be 5 (rho, u, v, w, rho*theta)");
+  if (d->NumTr != 0) return fail(c, CGD_ERR_UNSUPPORTED, who + "tracers (NumTr > 0) are not implemented");
   if (d->precision != CGD_FP64 && d->precision != CGD_MIXED)
-    return fail(c, CGD_ERR_UNSUPPORTED, "unknown precision mode");
-  c->mixed = d->precision == CGD_MIXED;
+    return fail(c, CGD_ERR_UNSUPPORTED, who + "unknown precision mode");
   if (d->HorLimit || d->Turbulence || d->VerticalDiffusion || d->SurfaceFlux || d->Forcing || d->Microphysics ||
       d->Damping || d->EDMF)
-    return fail(c, CGD_ERR_UNSUPPORTED, "only the dry CompressibleShallow path without physics hooks is implemented");
-  if (d->HyperDDivW != 0.0) return fail(c, CGD_ERR_UNSUPPORTED, "HyperDDivW > 0 (w hyperviscosity) is not implemented");
+    return fail(c, CGD_ERR_UNSUPPORTED,
+                who + "only the dry CompressibleShallow path without physics hooks is implemented");
+  if (d->HyperDDivW != 0.0)
+    return fail(c, CGD_ERR_UNSUPPORTED, who + "HyperDDivW > 0 (w hyperviscosity) is not implemented");
   if (!d->Glob || !d->DS || !d->DW || !d->dXdxI || !d->J || !d->X || !d->M || !d->dz)
-    return fail(c, CGD_ERR_ARG, "null static input pointer");
-  if ((size_t)d->NF * d->N * d->N * d->Nz > (size_t)1 << 40) return fail(c, CGD_ERR_ARG, "grid too large");
-  // the element kernels index a field with 32-bit offsets
-  if ((size_t)d->NumG * d->Nz >= (size_t)1 << 28 || (size_t)d->N * d->N * d->Nz >= (size_t)1 << 28)
-    return fail(c, CGD_ERR_ARG, "NumG*Nz must be below 2^28");
-  hipStream_t st = S(stream);
+    return fail(c, CGD_ERR_ARG, who + "null static input pointer");
+  return CGD_OK;
+}
+
+// Context over nmem grids ("members") stepped together.  Their meshes are
+// concatenated: member m's elements and global nodes follow those of members
+// 0..m-1, so the state of the batch is one (Nz, sum NumG, 5) array.  Each
+// member is coloured on its own and colour k of the batch is the union of
+// the members' colour k, so one launch per colour covers every member and
+// each node accumulates its element contributions in the order a context of
+// its member alone uses: a member's results are bit-identical either way.
+static int ctx_build(cgd_ctx *c, const cgd_desc *ds, int nmem, hipStream_t st) {
+  int r;
+  for (int m = 0; m < nmem; ++m) {
+    const std::string who = nmem > 1 ? "member " + std::to_string(m) + ": " : "";
+    if ((r = validate_desc(c, &ds[m], who))) return r;
+    const cgd_desc &a = ds[0], &b = ds[m];
+    if (b.N != a.N || b.Nz != a.Nz || b.precision != a.precision)
+      return fail(c, CGD_ERR_ARG, who + "N, Nz and precision must be the same for every member");
+    if (b.Grav != a.Grav || b.Omega != a.Omega || b.Rd != a.Rd || b.Cpd != a.Cpd || b.p0 != a.p0 ||
+        b.kappa != a.kappa)
+      return fail(c, CGD_ERR_ARG, who + "physical constants must be the same for every member");
+  }
+  const cgd_desc *d = &ds[0];
+  c->desc = *d;
+  c->mixed = d->precision == CGD_MIXED;
+  c->nmem = nmem;
   c->N = d->N;
   c->NN = d->N * d->N;
   c->Nz = d->Nz;
-  c->NF = d->NF;
-  c->NumG = d->NumG;
-  const int NN = c->NN, Nz = c->Nz, NF = c->NF;
+  const int NN = c->NN, Nz = c->Nz;
+  size_t nf_tot = 0, ng_tot = 0;
+  c->mem_e0.assign(nmem + 1, 0);
+  c->mem_g0.assign(nmem + 1, 0);
+  for (int m = 0; m < nmem; ++m) {
+    nf_tot += ds[m].NF;
+    ng_tot += ds[m].NumG;
+    if (nf_tot >= (size_t)1 << 31 || ng_tot >= (size_t)1 << 31) return fail(c, CGD_ERR_ARG, "grid too large");
+    c->mem_e0[m + 1] = (int)nf_tot;
+    c->mem_g0[m + 1] = (int)ng_tot;
+  }
+  if (nf_tot * NN * Nz > (size_t)1 << 40) return fail(c, CGD_ERR_ARG, "grid too large");
+  // the element kernels index a field with 32-bit offsets
+  if (ng_tot * Nz >= (size_t)1 << 28 || (size_t)NN * Nz >= (size_t)1 << 28)
+    return fail(c, CGD_ERR_ARG, "NumG*Nz (summed over members) must be below 2^28");
+  c->NF = (int)nf_tot;
+  c->NumG = (int)ng_tot;
+  const int NF = c->NF, NumG = c->NumG;
   // level chunk height NZT (a template constant): a workgroup is one
   // element x NZT levels, N*N*NZT lanes rounded up to whole wavefronts.
   // 8 levels measured fastest at ne=30, nz=64 (5 and 10 fill the
-  // wavefronts better but straddle cache lines); CGD_NZT=16 overrides.
+  // wavefronts better but straddle cache lines); CGD_NZT=16 overrides, and
+  // cgd_set_launch_shape / cgd_autotune change it later.
   {
     int nzt = 8;
     if (const char *ev = getenv("CGD_NZT")) {
@@ -285,39 +332,56 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
     if (const char *ev = getenv("CGD_THOMAS")) c->thomas = atoi(ev) != 0;
   }
   c->block = ((NN * c->nzt + 63) / 64) * 64;
+  c->colw = 64;
+  c->colblock = 256;
 
-  // global numbering -> 0-based int32, range check
-  std::vector<int64_t> hglob((size_t)NN * NF);
-  HIPCHK(c, hipMemcpyAsync(hglob.data(), d->Glob, hglob.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+  // global numbering -> 0-based int32 in batch numbering, range checks, and
+  // each member's colouring
+  std::vector<int> g32((size_t)NN * NF), color(NF, -1);
   std::vector<double> hds(NN), hdw(NN);
-  HIPCHK(c, hipMemcpyAsync(hds.data(), d->DS, NN * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHK(c, hipMemcpyAsync(hdw.data(), d->DW, NN * sizeof(double), hipMemcpyDeviceToHost, st));
-  HIPCHK(c, hipStreamSynchronize(st));
-  std::vector<int> g32((size_t)NN * NF);
-  for (size_t i = 0; i < g32.size(); ++i) {
-    if (hglob[i] < 1 || hglob[i] > d->NumG) return fail(c, CGD_ERR_ARG, "Glob entry outside 1..NumG");
-    g32[i] = (int)(hglob[i] - 1);
+  int ncol = 0;
+  for (int m = 0; m < nmem; ++m) {
+    const cgd_desc &dm = ds[m];
+    std::vector<int64_t> hglob((size_t)NN * dm.NF);
+    std::vector<double> mds(NN), mdw(NN);
+    HIPCHK(c, hipMemcpyAsync(hglob.data(), dm.Glob, hglob.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(mds.data(), dm.DS, NN * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(mdw.data(), dm.DW, NN * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    if (m == 0) {
+      hds = mds;
+      hdw = mdw;
+    } else if (mds != hds || mdw != hdw) {
+      return fail(c, CGD_ERR_ARG, "member " + std::to_string(m) + ": DS and DW must be the same for every member");
+    }
+    std::vector<int> gm(hglob.size());
+    for (size_t i = 0; i < gm.size(); ++i) {
+      if (hglob[i] < 1 || hglob[i] > dm.NumG) return fail(c, CGD_ERR_ARG, "Glob entry outside 1..NumG");
+      gm[i] = (int)(hglob[i] - 1);
+      g32[(size_t)c->mem_e0[m] * NN + i] = gm[i] + c->mem_g0[m];
+    }
+    std::vector<int> cm;
+    const int nc = color_elements(gm, NN, dm.NF, dm.NumG, cm);
+    if (nc < 0) return fail(c, CGD_ERR_ARG, "element colouring needs more than 64 colours");
+    ncol = std::max(ncol, nc);
+    std::copy(cm.begin(), cm.end(), color.begin() + c->mem_e0[m]);
   }
-  // greedy element colouring: no two elements of one colour share a node
-  std::vector<int> color(NF, -1);
-  const int ncol = color_elements(g32, NN, NF, d->NumG, color);
-  if (ncol < 0) return fail(c, CGD_ERR_ARG, "element colouring needs more than 64 colours");
   c->ncolor = ncol;
   c->color_off.assign(ncol + 1, 0);
   for (int e = 0; e < NF; ++e) c->color_off[color[e] + 1]++;
   for (int k = 0; k < ncol; ++k) c->color_off[k + 1] += c->color_off[k];
+  // slot order: colour, then member, then element number within the member
   std::vector<int> elist(NF);
   {
     std::vector<int> fp(c->color_off.begin(), c->color_off.end() - 1);
     for (int e = 0; e < NF; ++e) elist[fp[color[e]]++] = e;
   }
-  // elements are stored in colour order: slot q holds element elist[q]
   std::vector<int> slot(NF);
   for (int q = 0; q < NF; ++q) slot[elist[q]] = q;
   // global node -> element copies (slot*NN + node) in colour order
-  std::vector<int> cptr(d->NumG + 1, 0), ccopy((size_t)NF * NN);
+  std::vector<int> cptr(NumG + 1, 0), ccopy((size_t)NF * NN);
   for (int v : g32) cptr[v + 1]++;
-  for (int i = 0; i < d->NumG; ++i) cptr[i + 1] += cptr[i];
+  for (int i = 0; i < NumG; ++i) cptr[i + 1] += cptr[i];
   {
     std::vector<int> fp(cptr.begin(), cptr.end() - 1);
     for (int q = 0; q < NF; ++q) {
@@ -329,28 +393,61 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   std::vector<int> gslot((size_t)NF * NN);
   for (int q = 0; q < NF; ++q)
     for (int i = 0; i < NN; ++i) gslot[(size_t)q * NN + i] = g32[(size_t)elist[q] * NN + i];
-  for (int gn = 0; gn < d->NumG; ++gn)
+  for (int gn = 0; gn < NumG; ++gn)
     if (cptr[gn + 1] > cptr[gn]) gslot[ccopy[cptr[gn]]] |= (int)0x80000000u;
-  // device buffers (set-up only; nothing is allocated on the step path)
+  // hyperviscosity per slot (the member's coefficients)
+  std::vector<double> kvis((size_t)NF * 3);
+  for (int m = 0; m < nmem; ++m)
+    for (int e = c->mem_e0[m]; e < c->mem_e0[m + 1]; ++e) {
+      kvis[(size_t)slot[e] * 3] = ds[m].HyperDCurl;
+      kvis[(size_t)slot[e] * 3 + 1] = ds[m].HyperDGrad;
+      kvis[(size_t)slot[e] * 3 + 2] = ds[m].HyperDDiv;
+    }
+  // device buffers (set-up only; nothing is allocated on the step path).
+  // The chunk-interface side buffer is sized for the shortest chunk height
+  // (8 levels), so the launch shape can change later.
   const size_t npt = (size_t)NF * NN * Nz;
+  const size_t ng = (size_t)NumG * Nz;
+  const int nchunk_max = (Nz + 7) / 8;
   HIPCHK(c, hipMalloc(&c->d_glob, gslot.size() * sizeof(int)));
   HIPCHK(c, hipMalloc(&c->d_slot, (size_t)NF * sizeof(int)));
   HIPCHK(c, hipMalloc(&c->d_met, npt * CGD_NMET * sizeof(double)));
   HIPCHK(c, hipMalloc(&c->d_met1, npt * CGD_NMET1 * sizeof(double)));
-  HIPCHK(c, hipMalloc(&c->d_ms, (size_t)d->NumG * Nz * sizeof(double)));
-  HIPCHK(c, hipMalloc(&c->d_bwork, (size_t)d->NumG * (Nz - 1) * sizeof(double)));
-  HIPCHK(c, hipMalloc(&c->d_f4x, (size_t)NF * NN * c->nchunk * sizeof(double)));
+  HIPCHK(c, hipMalloc(&c->d_ms, ng * sizeof(double)));
+  HIPCHK(c, hipMalloc(&c->d_bwork, (size_t)NumG * (Nz - 1) * sizeof(double)));
+  HIPCHK(c, hipMalloc(&c->d_f4x, (size_t)NF * NN * nchunk_max * sizeof(double)));
   HIPCHK(c, hipMalloc(&c->d_nptr, cptr.size() * sizeof(int)));
   HIPCHK(c, hipMalloc(&c->d_ncopy, ccopy.size() * sizeof(int)));
+  HIPCHK(c, hipMalloc(&c->d_kvis, kvis.size() * sizeof(double)));
   HIPCHK(c, hipMemcpyAsync(c->d_nptr, cptr.data(), cptr.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_ncopy, ccopy.data(), ccopy.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_glob, gslot.data(), gslot.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_relayout_metric, dim3(grid_for(npt, 256)), dim3(256), 0, st, d->dXdxI, d->J, d->X, c->d_slot,
-                     c->d_met, c->d_met1, NN, Nz, NF, d->Omega);
-  HIPCHK(c, hipGetLastError());
-  const size_t ng = (size_t)d->NumG * Nz;
-  hipLaunchKernelGGL(k_msum, dim3(grid_for(ng, 256)), dim3(256), 0, st, d->M, c->d_ms, ng);
+  HIPCHK(c, hipMemcpyAsync(c->d_kvis, kvis.data(), kvis.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  // M and dz in batch numbering (a single grid's arrays are used in place)
+  if (nmem == 1) {
+    c->M = d->M;
+    c->dz = d->dz;
+  } else {
+    HIPCHK(c, hipMalloc(&c->d_M, 2 * ng * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->d_dz, ng * sizeof(double)));
+    for (int m = 0; m < nmem; ++m) {
+      const size_t gm = (size_t)ds[m].NumG * Nz, o = (size_t)c->mem_g0[m] * Nz;
+      for (int t = 0; t < 2; ++t)
+        HIPCHK(c, hipMemcpyAsync(c->d_M + t * ng + o, ds[m].M + t * gm, gm * sizeof(double),
+                                 hipMemcpyDeviceToDevice, st));
+      HIPCHK(c, hipMemcpyAsync(c->d_dz + o, ds[m].dz, gm * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+    c->M = c->d_M;
+    c->dz = c->d_dz;
+  }
+  for (int m = 0; m < nmem; ++m) {
+    const size_t nptm = (size_t)ds[m].NF * NN * Nz;
+    hipLaunchKernelGGL(k_relayout_metric, dim3(grid_for(nptm, 256)), dim3(256), 0, st, ds[m].dXdxI, ds[m].J,
+                       ds[m].X, c->d_slot + c->mem_e0[m], c->d_met, c->d_met1, NN, Nz, ds[m].NF, d->Omega);
+    HIPCHK(c, hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_msum, dim3(grid_for(ng, 256)), dim3(256), 0, st, c->M, c->d_ms, ng);
   HIPCHK(c, hipGetLastError());
   // compact metric layout when every component but the terrain row has the
   // same bits at all levels (CGD_METRIC=full keeps the full layout)
@@ -393,7 +490,7 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
     HIPCHK(c, hipMalloc(&c->d_ims32, ng * sizeof(float)));
     hipLaunchKernelGGL(k_to_f32, dim3(grid_for(ng, 256)), dim3(256), 0, st, c->d_ms, c->d_ims32, ng);
     HIPCHK(c, hipMalloc(&c->d_m32, 2 * ng * sizeof(float)));
-    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(2 * ng, 256)), dim3(256), 0, st, d->M, c->d_m32, 2 * ng);
+    hipLaunchKernelGGL(k_to_f32, dim3(grid_for(2 * ng, 256)), dim3(256), 0, st, c->M, c->d_m32, 2 * ng);
     if (c->cm) {
       const size_t nmp = npt * CGD_NMETP;
       HIPCHK(c, hipMalloc(&c->d_metp32, nmp * sizeof(float)));
@@ -405,7 +502,7 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
 
   KArgs &a = c->ka;
   a.Nz = Nz;
-  a.NumG = d->NumG;
+  a.NumG = NumG;
   a.nzt = c->nzt;
   a.Grav = d->Grav;
   a.Omega = d->Omega;
@@ -416,9 +513,6 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   a.Rd_p0 = d->Rd / d->p0;
   a.eos_y = 1.0 / (1.0 - d->kappa);
   a.dpd_y = d->kappa / (1.0 - d->kappa);
-  a.Kc = d->HyperDCurl;
-  a.Kg = d->HyperDGrad;
-  a.Kd = d->HyperDDiv;
   // DS, DW arrive column-major (Julia): DS[i,k] at i + N*k
   for (int i = 0; i < c->N; ++i)
     for (int k = 0; k < c->N; ++k) {
@@ -434,20 +528,44 @@ extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
   a.iMs32 = c->d_ims32;
   a.M32 = c->d_m32;
   a.metp32 = c->d_metp32;
-  a.M = d->M;
+  a.M = c->M;
+  a.kvis = c->d_kvis;
   c->ok = true;
   return CGD_OK;
 }
 
+extern "C" int cgd_ctx_create(const cgd_desc *d, cgd_ctx **out, void *stream) {
+  if (!out) return CGD_ERR_ARG;
+  *out = nullptr;
+  cgd_ctx *c = new cgd_ctx();
+  *out = c;
+  if (!d) return fail(c, CGD_ERR_ARG, "null descriptor");
+  return ctx_build(c, d, 1, S(stream));
+}
+
+extern "C" int cgd_ctx_create_batch(const cgd_desc *members, int nmem, cgd_ctx **out, void *stream) {
+  if (!out) return CGD_ERR_ARG;
+  *out = nullptr;
+  cgd_ctx *c = new cgd_ctx();
+  *out = c;
+  if (!members) return fail(c, CGD_ERR_ARG, "null member array");
+  if (nmem < 1 || nmem > 65536) return fail(c, CGD_ERR_ARG, "member count must be in [1, 65536]");
+  return ctx_build(c, members, nmem, S(stream));
+}
+
+static void graph_free(cgd_ctx *c);
 static void prof_free(cgd_ctx *c);
 
 extern "C" int cgd_ctx_destroy(cgd_ctx *c) {
   if (!c) return CGD_ERR_ARG;
   // a context that failed validation holds no device memory: tear it down
   // without touching the HIP runtime
+  graph_free(c);
+  if (c->gstream) (void)hipStreamDestroy(c->gstream);
   for (void *p : {(void *)c->d_glob, (void *)c->d_slot, (void *)c->d_met, (void *)c->d_met1, (void *)c->d_metn,
                   (void *)c->d_met1n, (void *)c->d_ms, (void *)c->d_ims32, (void *)c->d_m32, (void *)c->d_metp32,
-                  (void *)c->d_bwork, (void *)c->d_f4x, (void *)c->d_nptr, (void *)c->d_ncopy})
+                  (void *)c->d_bwork, (void *)c->d_f4x, (void *)c->d_nptr, (void *)c->d_ncopy, (void *)c->d_kvis,
+                  (void *)c->d_M, (void *)c->d_dz})
     if (p) (void)hipFree(p);
   prof_free(c);
   delete c;
@@ -621,7 +739,7 @@ extern "C" int cgd_jac(cgd_ctx *c, double *JRhoW, double *JWRho, double *JWRhoTh
   if (!JRhoW || !JWRho || !JWRhoTh || !JRhoThW || !U || !AuxG) return fail(c, CGD_ERR_ARG, "cgd_jac: null array");
   const size_t n = (size_t)c->NumG * (c->Nz - 1);
   const size_t ng = (size_t)c->NumG * c->Nz;
-  hipLaunchKernelGGL(k_jac, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), c->ka, U, AuxG + 3 * ng, c->desc.dz,
+  hipLaunchKernelGGL(k_jac, dim3(grid_for(n, 256)), dim3(256), 0, S(stream), c->ka, U, AuxG + 3 * ng, c->dz,
                      JRhoW, JWRho, JWRhoTh, JRhoThW);
   HIPCHK(c, hipGetLastError());
   return CGD_OK;
@@ -645,6 +763,17 @@ extern "C" int cgd_schur(cgd_ctx *c, double *k, double *v, const double *JRhoW, 
                        JRhoThW, tri, c->d_bwork, fac, comp_tri);
   HIPCHK(c, hipGetLastError());
   return prof_end(c, st, 2, 1);
+}
+
+// The stage kernel over all NumG columns, colw lanes each, colblock
+// threads per workgroup.
+template <typename R, int W>
+static void launch_stage(cgd_ctx *c, int is, const RWStage<R> &rw, hipStream_t st) {
+  const size_t cols_per_block = (size_t)c->colblock / W;
+  const dim3 g((unsigned)(((size_t)c->NumG + cols_per_block - 1) / cols_per_block)), b(c->colblock);
+  if (is == 0) hipLaunchKernelGGL((k_rw_stage<0, R, W>), g, b, 0, st, c->ka, rw);
+  else if (is == 1) hipLaunchKernelGGL((k_rw_stage<1, R, W>), g, b, 0, st, c->ka, rw);
+  else hipLaunchKernelGGL((k_rw_stage<2, R, W>), g, b, 0, st, c->ka, rw);
 }
 
 // Fused step for Nz <= 64: the first pressure, then per stage the two
@@ -679,7 +808,7 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
     for (int j = 0; j <= is; ++j) rw.av[j] = (is < 2) ? A[is + 1][j] : Mw[j];
     rw.Vn = w->Vn;
     rw.Vnw = w->Vn;
-    rw.dz = c->desc.dz;
+    rw.dz = c->dz;
     rw.V = V;
     rw.k = is < 2 ? kR[is] : nullptr;  // k_3 is consumed in registers
     rw.P = is < 2 ? w->AuxG : nullptr;
@@ -693,10 +822,8 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
     rw.invfac2 = rw.invfac / rw.fac;
     rw.thomas = c->thomas;
     prof_begin(c, st, 2);
-    const dim3 gcol((c->NumG + 3) / 4);
-    if (is == 0) hipLaunchKernelGGL((k_rw_stage<0, R>), gcol, dim3(256), 0, st, c->ka, rw);
-    else if (is == 1) hipLaunchKernelGGL((k_rw_stage<1, R>), gcol, dim3(256), 0, st, c->ka, rw);
-    else hipLaunchKernelGGL((k_rw_stage<2, R>), gcol, dim3(256), 0, st, c->ka, rw);
+    if (c->colw == 32) launch_stage<R, 32>(c, is, rw, st);
+    else launch_stage<R, 64>(c, is, rw, st);
     HIPCHK(c, hipGetLastError());
     if ((r = prof_end(c, st, 2, 1))) return r;
   }
@@ -745,5 +872,158 @@ extern "C" int cgd_rw_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *
       hipLaunchKernelGGL(k_axpy3, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], Mw[0], k[1], Mw[1], k[2], Mw[2], n5);
     HIPCHK(c, hipGetLastError());
   }
+  return CGD_OK;
+}
+
+// ------------------------------------------------------------------------
+// Launch shapes, step graphs and the autotuner
+
+static void graph_free(cgd_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+}
+
+// Admissible shapes: chunk heights 8 (and 16 for N = 5); 64 lanes per
+// column, or 32 when Nz <= 32 and the solve is cyclic reduction; stage
+// workgroups of 64..256 threads in whole wavefronts.
+static bool shape_ok(const cgd_ctx *c, int nzt, int lanes, int block) {
+  if (!(nzt == 8 || (nzt == 16 && c->N == 5))) return false;
+  if (!(lanes == 64 || (lanes == 32 && c->Nz <= 32 && !c->thomas))) return false;
+  return block >= 64 && block <= 256 && block % 64 == 0;
+}
+
+static void apply_shape(cgd_ctx *c, int nzt, int lanes, int block) {
+  c->nzt = nzt;
+  c->nchunk = (c->Nz + nzt - 1) / nzt;
+  c->block = ((c->NN * nzt + 63) / 64) * 64;
+  c->ka.nzt = nzt;
+  c->colw = lanes;
+  c->colblock = block;
+}
+
+extern "C" int cgd_set_launch_shape(cgd_ctx *c, int chunk_levels, int col_lanes, int col_block) {
+  int r = check(c);
+  if (r) return r;
+  const int nzt = chunk_levels ? chunk_levels : c->nzt;
+  const int lanes = col_lanes ? col_lanes : c->colw;
+  const int block = col_block ? col_block : c->colblock;
+  if (!shape_ok(c, nzt, lanes, block))
+    return fail(c, CGD_ERR_ARG,
+                "launch shape: chunk_levels must be 8 (or 16 for N = 5), col_lanes 64 (or 32 when Nz <= 32 "
+                "without the Thomas solve), col_block a multiple of 64 in [64, 256]");
+  graph_free(c);  // a captured graph keeps the old shapes
+  apply_shape(c, nzt, lanes, block);
+  return CGD_OK;
+}
+
+extern "C" int cgd_get_launch_shape(const cgd_ctx *c, int *chunk_levels, int *col_lanes, int *col_block) {
+  if (!c) return CGD_ERR_ARG;
+  if (!c->ok) return CGD_ERR_STATE;
+  if (chunk_levels) *chunk_levels = c->nzt;
+  if (col_lanes) *col_lanes = c->colw;
+  if (col_block) *col_block = c->colblock;
+  return CGD_OK;
+}
+
+extern "C" int cgd_rw_graph_create(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, int nsteps,
+                                   void *stream) {
+  int r = check(c);
+  if (r) return r;
+  if (nsteps < 1 || nsteps > 10000) return fail(c, CGD_ERR_ARG, "cgd_rw_graph_create: nsteps must be in [1, 10000]");
+  if (c->prof) return fail(c, CGD_ERR_STATE, "cgd_rw_graph_create: turn profiling off before capturing");
+  graph_free(c);
+  if (!c->gstream) HIPCHK(c, hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+  // the capture records launches only; the caller's earlier work on stream
+  // is ordered before the replays by hipGraphLaunch on that stream
+  HIPCHK(c, hipStreamSynchronize(S(stream)));
+  HIPCHK(c, hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal));
+  for (int s = 0; s < nsteps && !r; ++s) r = cgd_rw_step(c, V, dt, w, c->gstream);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->gstream, &g);
+  if (r || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    if (r) return r;
+    return fail(c, CGD_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  c->graph = g;
+  HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  return CGD_OK;
+}
+
+extern "C" int cgd_rw_graph_launch(cgd_ctx *c, void *stream) {
+  int r = check(c);
+  if (r) return r;
+  if (!c->gexec) return fail(c, CGD_ERR_STATE, "cgd_rw_graph_launch: no captured step graph");
+  HIPCHK(c, hipGraphLaunch(c->gexec, S(stream)));
+  return CGD_OK;
+}
+
+extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, int reps, void *stream,
+                            int *shape, double *best_ms) {
+  int r = check(c);
+  if (r) return r;
+  if (!V || !w) return fail(c, CGD_ERR_ARG, "cgd_autotune: null array");
+  if (reps < 1 || reps > 100) return fail(c, CGD_ERR_ARG, "cgd_autotune: reps must be in [1, 100]");
+  hipStream_t st = S(stream);
+  const int prof = c->prof;
+  c->prof = 0;
+  graph_free(c);
+  const size_t n5 = (size_t)5 * c->NumG * c->Nz;
+  double *Vsave = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int bn = c->nzt, bl = c->colw, bb = c->colblock;
+  double bms = 1e300;
+  auto run = [&]() -> int {
+    HIPCHK(c, hipMalloc(&Vsave, n5 * sizeof(double)));
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipMemcpyAsync(Vsave, V, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    const int nzts[2] = {8, 16}, lanes[2] = {64, 32}, blocks[3] = {256, 128, 64};
+    int rr;
+    for (int a : nzts)
+      for (int l : lanes)
+        for (int b : blocks) {
+          if (!shape_ok(c, a, l, b)) continue;
+          apply_shape(c, a, l, b);
+          HIPCHK(c, hipMemcpyAsync(V, Vsave, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
+          if ((rr = cgd_rw_step(c, V, dt, w, stream))) return rr;  // warm-up
+          HIPCHK(c, hipEventRecord(e0, st));
+          for (int q = 0; q < reps; ++q)
+            if ((rr = cgd_rw_step(c, V, dt, w, stream))) return rr;
+          HIPCHK(c, hipEventRecord(e1, st));
+          HIPCHK(c, hipEventSynchronize(e1));
+          float ms = 0;
+          HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+          const double per = ms / reps;
+          if (per < bms) {
+            bms = per;
+            bn = a, bl = l, bb = b;
+          }
+        }
+    return CGD_OK;
+  };
+  r = run();
+  // V as it was, whatever happened above
+  if (Vsave) {
+    const hipError_t e = hipMemcpyAsync(V, Vsave, n5 * sizeof(double), hipMemcpyDeviceToDevice, st);
+    const hipError_t e2 = hipStreamSynchronize(st);
+    if (!r && (e != hipSuccess || e2 != hipSuccess))
+      r = fail(c, CGD_ERR_HIP, std::string("cgd_autotune: restoring V: ") +
+                                   hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  if (Vsave) (void)hipFree(Vsave);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  apply_shape(c, bn, bl, bb);
+  c->prof = prof;
+  if (r) return r;
+  if (shape) {
+    shape[0] = bn;
+    shape[1] = bl;
+    shape[2] = bb;
+  }
+  if (best_ms) *best_ms = bms;
   return CGD_OK;
 }

@@ -33,7 +33,6 @@ struct KArgs {
   int Nz, NumG, nzt;
   double Grav, Omega, Rd, Cpd, p0, kappa;
   double Rd_p0, eos_y, dpd_y;      // Rd/p0, 1/(1-kappa), kappa/(1-kappa)
-  double Kc, Kg, Kd;
   double D[CGD_NMAX * CGD_NMAX];   // D[i*N + k] = DS[i,k]
   double DWm[CGD_NMAX * CGD_NMAX]; // DW[i*N + k]
   const int *glob;                 // [slot][NN]: 0-based node | bit 31 set on the node's first copy
@@ -46,6 +45,7 @@ struct KArgs {
   const float *M32;                // M in FP32 (mixed-precision mode)
   const float *metp32;             // compact layout's per-point components in FP32 (mixed precision)
   const double *M;                 // [2][NumG][Nz]
+  const double *kvis;              // [slot][3]: HyperDCurl, HyperDGrad, HyperDDiv of the element's grid
 };
 
 // metric component indices of met (dXdxI[a,b,K] -> dabK, 1-based a,b,K)
@@ -505,7 +505,9 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   const R irho = R(rcp(rho));
   const R c_own = rth * irho;
   const R h = R(0.5), q4 = R(0.25);
-  const R Kc = R(a.Kc), Kg = R(a.Kg), Kd = R(a.Kd);
+  // hyperviscosity of this element's grid (members of an ensemble differ)
+  const double *kv = a.kvis + 3 * e;
+  const R Kc = R(kv[0]), Kg = R(kv[1]), Kd = R(kv[2]);
   // the DSS accumulators of this node (untouched by the other elements of
   // the colour), read with the state rather than after the LDS phases
   R o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0;
@@ -894,10 +896,15 @@ __device__ __forceinline__ double readlane_d(double x, int lane) {
 // A[i,i-1] x[i-1] + A[i,i] x[i] + A[i,i+1] x[i+1] = rhs with A[i,i-1] =
 // tri[3,i-1], A[i,i] = tri[2,i], A[i,i+1] = tri[1,i+1] (the column-oriented
 // storage of SchurSolve.jl); lane i holds t0, t1, t2 = tri[1..3, i].
+// A column occupies W consecutive lanes (W = 64, or 32 when two columns of
+// at most 32 levels share a wavefront); i is the lane within the column.
+// The Thomas branch broadcasts with readlane, so it needs W = 64.
+template <int W>
 __device__ __forceinline__ double col_solve(double t0, double t1, double t2, double rhs, int i, int n, int thomas) {
+  static_assert(W == 32 || W == 64, "a column spans half or all of a wavefront");
   const bool row = i < n;
   double sol = 0.0;
-  if (thomas) {
+  if (W == 64 && thomas) {
     // triSolve!: forward elimination, wave-serial (row j broadcast by readlane)
     double x = t1, b = rhs;
     for (int j = 0; j < n - 1; ++j) {
@@ -916,8 +923,12 @@ __device__ __forceinline__ double col_solve(double t0, double t1, double t2, dou
       if (i == j) sol = (b - t0j1 * sj1) / x;
     }
   } else {
-    // parallel cyclic reduction: log2(64) = 6 elimination sweeps, all lanes busy
-    double ra = __shfl_up(t2, 1), rb = t1, rc = __shfl_down(t0, 1), rd = rhs;
+    // parallel cyclic reduction: log2(W) elimination sweeps, all lanes busy.
+    // Rows n..W-1 are identity rows, so for n < 32 the sixth sweep of a
+    // 64-lane column is an identity on the (then decoupled) rows and W = 32
+    // computes the same solution (up to the compiler's choice of fused
+    // multiply-adds in either instantiation).
+    double ra = __shfl_up(t2, 1, W), rb = t1, rc = __shfl_down(t0, 1, W), rd = rhs;
     if (i == 0 || !row) ra = 0.0;
     if (i >= n - 1) rc = 0.0;
     if (!row) {
@@ -925,16 +936,18 @@ __device__ __forceinline__ double col_solve(double t0, double t1, double t2, dou
       rd = 0.0;
     }
 #pragma unroll
-    for (int st = 1; st < 64; st <<= 1) {
-      double am = __shfl_up(ra, st), bm = __shfl_up(rb, st), cm = __shfl_up(rc, st), dm = __shfl_up(rd, st);
-      double ap = __shfl_down(ra, st), bp = __shfl_down(rb, st), cp = __shfl_down(rc, st), dp = __shfl_down(rd, st);
+    for (int st = 1; st < W; st <<= 1) {
+      double am = __shfl_up(ra, st, W), bm = __shfl_up(rb, st, W), cm = __shfl_up(rc, st, W),
+             dm = __shfl_up(rd, st, W);
+      double ap = __shfl_down(ra, st, W), bp = __shfl_down(rb, st, W), cp = __shfl_down(rc, st, W),
+             dp = __shfl_down(rd, st, W);
       if (i < st) {
         am = 0.0;
         bm = 1.0;
         cm = 0.0;
         dm = 0.0;
       }
-      if (i + st > 63) {
+      if (i + st > W - 1) {
         ap = 0.0;
         bp = 1.0;
         cp = 0.0;
@@ -954,11 +967,12 @@ __device__ __forceinline__ double col_solve(double t0, double t1, double t2, dou
 // TriDiagKernel! (SchurSolve.jl:45-74) for row i from the J blocks of
 // rows i-1, i, i+1 (neighbour rows reached by shuffles)
 // (invfac2 = (1/fac)/fac, formed once on the host)
+template <int W>
 __device__ __forceinline__ void col_tri(double jrw0, double jrw1, double jwr0, double jwr1, double jwt0,
                                         double jwt1, double jtw0, double jtw1, double invfac2, int i, int n,
                                         double &t0, double &t1, double &t2) {
-  const double jwr0m = __shfl_up(jwr0, 1), jwt0m = __shfl_up(jwt0, 1);
-  const double jwr1p = __shfl_down(jwr1, 1), jwt1p = __shfl_down(jwt1, 1);
+  const double jwr0m = __shfl_up(jwr0, 1, W), jwt0m = __shfl_up(jwt0, 1, W);
+  const double jwr1p = __shfl_down(jwr1, 1, W), jwt1p = __shfl_down(jwt1, 1, W);
   t0 = 0;
   t1 = 1;
   t2 = 0;
@@ -1003,7 +1017,7 @@ __global__ void __launch_bounds__(256) k_schur_col(KArgs a, double *__restrict__
   }
   double t0, t1, t2;
   if (comp_tri) {
-    col_tri(jrw0, jrw1, jwr0, jwr1, jwt0, jwt1, jtw0, jtw1, invfac2, i, n, t0, t1, t2);
+    col_tri<64>(jrw0, jrw1, jwr0, jwr1, jwt0, jwt1, jtw0, jtw1, invfac2, i, n, t0, t1, t2);
     if (row) {
       tri[tb] = t0;
       tri[tb + 1] = t1;
@@ -1032,7 +1046,7 @@ __global__ void __launch_bounds__(256) k_schur_col(KArgs a, double *__restrict__
     v4 = v4 / fac + jwr1 * v1 + jwr0 * v1p + jwt1 * v5 + jwt0 * v5p;
     v[c0 + i + 3 * vs] = v4;
   }
-  const double k4 = col_solve(t0, t1, t2, v4, i, n, thomas);
+  const double k4 = col_solve<64>(t0, t1, t2, v4, i, n, thomas);
   // SchurSolveBKernel!
   const double k4m = __shfl_up(k4, 1);
   const double jrw1m = __shfl_up(jrw1, 1), jtw1m = __shfl_up(jtw1, 1);
@@ -1160,12 +1174,14 @@ template <typename R> struct RWStage {
   int thomas;
 };
 
-template <int STAGE, typename R>
+// A column takes W lanes: W = 64, or W = 32 (Nz <= 32) for two columns per
+// wavefront; blockDim.x is any multiple of 64 up to 256.
+template <int STAGE, typename R, int W>
 __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage<R> rw) {
   const int Nz = a.Nz, n = Nz - 1;
-  const int i = threadIdx.x & 63;
-  const size_t ig = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (ig >= (size_t)a.NumG) return;  // uniform per wavefront
+  const int i = threadIdx.x & (W - 1);
+  const size_t ig = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / W;
+  if (ig >= (size_t)a.NumG) return;  // uniform per column (NumG columns, W lanes each)
   const size_t vs = (size_t)a.NumG * Nz;
   const size_t c0 = ig * Nz;
   const bool row = i < n, lev = i < Nz;
@@ -1187,12 +1203,12 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage<R> rw) {
   }
   // frozen Jacobian at interface i (levels i, i+1) and the tri matrix
   const double dpd = lev ? eos_dpdrhoth(a, vn[4]) : 0.0;
-  const double rhoR = __shfl_down(vn[0], 1), rthR = __shfl_down(vn[4], 1);
-  const double dpdR = __shfl_down(dpd, 1), dzR = __shfl_down(dzi, 1);
+  const double rhoR = __shfl_down(vn[0], 1, W), rthR = __shfl_down(vn[4], 1, W);
+  const double dpdR = __shfl_down(dpd, 1, W), dzR = __shfl_down(dzi, 1, W);
   JacBlocks J{0, 0, 0, 0, 0, 0, 0, 0};
   if (row) J = jac_blocks(a, vn[0], rhoR, vn[4], rthR, dpd, dpdR, dzi, dzR);
   double t0, t1, t2;
-  col_tri(J.rw0, J.rw1, J.wr0, J.wr1, J.wt0, J.wt1, J.tw0, J.tw1, rw.invfac2, i, n, t0, t1, t2);
+  col_tri<W>(J.rw0, J.rw1, J.wr0, J.wr1, J.wt0, J.wt1, J.tw0, J.tw1, rw.invfac2, i, n, t0, t1, t2);
   // tendency, chunk-interface w terms, stage terms
   double v[5] = {0, 0, 0, 0, 0};
   double kp[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
@@ -1215,12 +1231,12 @@ __global__ void __launch_bounds__(256) k_rw_stage(KArgs a, RWStage<R> rw) {
     }
   }
   // SchurSolveFKernel!, the tridiagonal solve, SchurSolveBKernel!
-  const double v1p = __shfl_down(v[0], 1), v5p = __shfl_down(v[4], 1);
+  const double v1p = __shfl_down(v[0], 1, W), v5p = __shfl_down(v[4], 1, W);
   double rhs = 0.0;
   if (row) rhs = v[3] * rw.invfac + J.wr1 * v[0] + J.wr0 * v1p + J.wt1 * v[4] + J.wt0 * v5p;
-  const double k4 = col_solve(t0, t1, t2, rhs, i, n, rw.thomas);
-  const double k4m = __shfl_up(k4, 1);
-  const double jrw1m = __shfl_up(J.rw1, 1), jtw1m = __shfl_up(J.tw1, 1);
+  const double k4 = col_solve<W>(t0, t1, t2, rhs, i, n, rw.thomas);
+  const double k4m = __shfl_up(k4, 1, W);
+  const double jrw1m = __shfl_up(J.rw1, 1, W), jtw1m = __shfl_up(J.tw1, 1, W);
   if (!lev) return;
   if (i == 0) {
     v[0] += J.rw0 * k4;

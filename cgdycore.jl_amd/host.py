@@ -86,6 +86,13 @@ def native():
     L.cgd_profile_read.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.cgd_color_elements.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp]
     L.cgd_ctx_info.argtypes = [_vp, _vp, _vp, _vp]
+    L.cgd_ctx_create_batch.argtypes = [C.POINTER(_Desc), C.c_int, C.POINTER(_vp), _vp]
+    L.cgd_set_launch_shape.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    L.cgd_get_launch_shape.argtypes = [_vp, _vp, _vp, _vp]
+    L.cgd_autotune.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), C.c_int, _vp, _vp,
+                               C.POINTER(C.c_double)]
+    L.cgd_rw_graph_create.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), C.c_int, _vp]
+    L.cgd_rw_graph_launch.argtypes = [_vp, _vp]
     _LIB = L
     return L
 
@@ -93,7 +100,8 @@ def native():
 def exported_symbols():
     return ["cgd_ctx_create", "cgd_ctx_destroy", "cgd_last_error", "cgd_prepare", "cgd_fcn", "cgd_jac",
             "cgd_schur", "cgd_rw_step", "cgd_profile_enable", "cgd_profile_read", "cgd_color_elements",
-            "cgd_ctx_info"]
+            "cgd_ctx_info", "cgd_ctx_create_batch", "cgd_set_launch_shape", "cgd_get_launch_shape",
+            "cgd_autotune", "cgd_rw_graph_create", "cgd_rw_graph_launch"]
 
 
 def color_elements(glob: np.ndarray, num_g: int):
@@ -232,37 +240,55 @@ def ssp_knoth():
 
 
 # --------------------------------------------------------------------------
-class NativeContext:
-    """Owns one ``cgd_ctx``: colouring and re-laid-out metric for one grid."""
+def _make_desc(FE: CGQuad, Metric: MetricStruct, Phys: Phys, Global: GlobalStruct) -> _Desc:
+    M = Global.Model
+    d = _Desc()
+    d.N, d.Nz, d.NF, d.NumG = Global.N, Global.Nz, Global.NF, Global.NumG
+    if Global.Precision not in PRECISIONS:
+        raise CGDError(f"unknown precision {Global.Precision!r} (expected one of {sorted(PRECISIONS)})")
+    d.NumV, d.NumTr, d.precision = M.NumV, M.NumTr, PRECISIONS[Global.Precision]
+    d.HorLimit = int(M.HorLimit)
+    d.Turbulence, d.VerticalDiffusion = int(M.Turbulence), int(M.VerticalDiffusion)
+    d.SurfaceFlux, d.Forcing, d.Microphysics = int(M.SurfaceFlux), int(M.Forcing), int(M.Microphysics)
+    d.Damping, d.EDMF = int(M.Damping), int(M.EDMF)
+    d.Grav, d.Omega, d.Rd, d.Cpd, d.p0, d.kappa = (Phys.Grav, Phys.Omega, Phys.Rd, Phys.Cpd, Phys.p0,
+                                                   Phys.kappa)
+    d.HyperDCurl, d.HyperDGrad, d.HyperDDiv, d.HyperDDivW = (M.HyperDCurl, M.HyperDGrad, M.HyperDDiv,
+                                                             M.HyperDDivW)
+    d.Glob, d.DS, d.DW = _ptr(FE.Glob), _ptr(FE.DS), _ptr(FE.DW)
+    d.dXdxI, d.J, d.X = _ptr(Metric.dXdxI), _ptr(Metric.J), _ptr(Metric.X)
+    d.M, d.dz = _ptr(FE.M), _ptr(Metric.dz)
+    return d
 
-    def __init__(self, FE: CGQuad, Metric: MetricStruct, Phys: Phys, Global: GlobalStruct):
+
+def _rw_work(Cache: CacheStruct, JCache: JStruct) -> _RWWork:
+    return _RWWork(_ptr(Cache.Vn), _ptr(Cache.fV), _ptr(Cache.k), _ptr(Cache.AuxG), _ptr(Cache.Temp1),
+                   _ptr(JCache.JRhoW), _ptr(JCache.JWRho), _ptr(JCache.JWRhoTh), _ptr(JCache.JRhoThW),
+                   _ptr(JCache.tri))
+
+
+class NativeContext:
+    """Owns one ``cgd_ctx``: colouring and re-laid-out metric for one grid,
+    or (``members``) for a batch of grids stepped together."""
+
+    def __init__(self, FE: CGQuad, Metric: MetricStruct, Phys: Phys, Global: GlobalStruct, members=None):
         L = native()
-        M = Global.Model
-        d = _Desc()
-        d.N, d.Nz, d.NF, d.NumG = Global.N, Global.Nz, Global.NF, Global.NumG
-        if Global.Precision not in PRECISIONS:
-            raise CGDError(f"unknown precision {Global.Precision!r} (expected one of {sorted(PRECISIONS)})")
-        d.NumV, d.NumTr, d.precision = M.NumV, M.NumTr, PRECISIONS[Global.Precision]
-        d.HorLimit = int(M.HorLimit)
-        d.Turbulence, d.VerticalDiffusion = int(M.Turbulence), int(M.VerticalDiffusion)
-        d.SurfaceFlux, d.Forcing, d.Microphysics = int(M.SurfaceFlux), int(M.Forcing), int(M.Microphysics)
-        d.Damping, d.EDMF = int(M.Damping), int(M.EDMF)
-        d.Grav, d.Omega, d.Rd, d.Cpd, d.p0, d.kappa = (Phys.Grav, Phys.Omega, Phys.Rd, Phys.Cpd, Phys.p0,
-                                                       Phys.kappa)
-        d.HyperDCurl, d.HyperDGrad, d.HyperDDiv, d.HyperDDivW = (M.HyperDCurl, M.HyperDGrad, M.HyperDDiv,
-                                                                 M.HyperDDivW)
-        self._keep = (FE.Glob, FE.DS, FE.DW, Metric.dXdxI, Metric.J, Metric.X, FE.M, Metric.dz)
-        d.Glob, d.DS, d.DW = _ptr(FE.Glob), _ptr(FE.DS), _ptr(FE.DW)
-        d.dXdxI, d.J, d.X = _ptr(Metric.dXdxI), _ptr(Metric.J), _ptr(Metric.X)
-        d.M, d.dz = _ptr(FE.M), _ptr(Metric.dz)
-        h = _vp()
-        rc = L.cgd_ctx_create(C.byref(d), C.byref(h), _stream())
-        self.h = h
         self.L = L
+        self.h = _vp()
+        if members is None:
+            self._keep = (FE.Glob, FE.DS, FE.DW, Metric.dXdxI, Metric.J, Metric.X, FE.M, Metric.dz)
+            d = _make_desc(FE, Metric, Phys, Global)
+            rc = L.cgd_ctx_create(C.byref(d), C.byref(self.h), _stream())
+            what = "cgd_ctx_create"
+        else:
+            self._keep = [(f.Glob, f.DS, f.DW, m.dXdxI, m.J, m.X, f.M, m.dz) for f, m, _ in members]
+            descs = (_Desc * len(members))(*[_make_desc(f, m, Phys, g) for f, m, g in members])
+            rc = L.cgd_ctx_create_batch(descs, len(members), C.byref(self.h), _stream())
+            what = "cgd_ctx_create_batch"
         if rc != 0:
-            msg = L.cgd_last_error(h).decode() if h.value else "cgd_ctx_create failed"
+            msg = L.cgd_last_error(self.h).decode() if self.h.value else f"{what} failed"
             self.close()
-            raise CGDError(f"cgd_ctx_create: {msg} (code {rc})")
+            raise CGDError(f"{what}: {msg} (code {rc})")
 
     def check(self, rc, what):
         if rc != 0:
@@ -284,6 +310,36 @@ class NativeContext:
         nc, nz, cm = C.c_int(), C.c_int(), C.c_int()
         self.check(self.L.cgd_ctx_info(self.h, C.byref(nc), C.byref(nz), C.byref(cm)), "cgd_ctx_info")
         return nc.value, nz.value, bool(cm.value)
+
+    def set_launch_shape(self, chunk_levels=0, col_lanes=0, col_block=0):
+        """Element chunk height, lanes per column and stage workgroup size (0 keeps a value)."""
+        self.check(self.L.cgd_set_launch_shape(self.h, chunk_levels, col_lanes, col_block), "cgd_set_launch_shape")
+
+    def launch_shape(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self.check(self.L.cgd_get_launch_shape(self.h, C.byref(a), C.byref(b), C.byref(c)), "cgd_get_launch_shape")
+        return a.value, b.value, c.value
+
+    def autotune(self, V, dt, Cache, JCache, reps=3):
+        """Time every launch shape on ``V`` (restored afterwards); keep the fastest.
+        Returns ((chunk_levels, col_lanes, col_block), ms per step)."""
+        shape = (C.c_int * 3)()
+        ms = C.c_double()
+        w = _rw_work(Cache, JCache)
+        self.check(self.L.cgd_autotune(self.h, _ptr(V), C.c_double(dt), C.byref(w), int(reps), _stream(), shape,
+                                       C.byref(ms)), "cgd_autotune")
+        return tuple(shape), ms.value
+
+    def graph_create(self, V, dt, Cache, JCache, nsteps=1):
+        """Capture ``nsteps`` Rosenbrock-W steps of ``V`` into a HIP graph."""
+        self._graph_keep = (V, Cache, JCache)
+        w = _rw_work(Cache, JCache)
+        self.check(self.L.cgd_rw_graph_create(self.h, _ptr(V), C.c_double(dt), C.byref(w), int(nsteps),
+                                              _stream()), "cgd_rw_graph_create")
+
+    def graph_launch(self):
+        """Replay the captured steps on torch's current stream."""
+        self.check(self.L.cgd_rw_graph_launch(self.h, _stream()), "cgd_rw_graph_launch")
 
     def profile(self, mask=7):
         """Time the passes in ``mask`` (bit 0 pass 1, bit 1 pass 2, bit 2 column; 0 = off)."""
@@ -368,9 +424,7 @@ def RosenbrockSchur(V, dt, Fcn, FcnPrepare, Jac, CG, Metric, Phys, Cache, JCache
     if Fcn is FcnGPU and FcnPrepare is FcnPrepareGPU and Jac is JacSchurGPU:
         ctx = _ctx(CG, Metric, Phys, Global)
         _check_state(V, Global, 5, "V")
-        w = _RWWork(_ptr(Cache.Vn), _ptr(Cache.fV), _ptr(Cache.k), _ptr(Cache.AuxG), _ptr(Cache.Temp1),
-                    _ptr(JCache.JRhoW), _ptr(JCache.JWRho), _ptr(JCache.JWRhoTh), _ptr(JCache.JRhoThW),
-                    _ptr(JCache.tri))
+        w = _rw_work(Cache, JCache)
         ctx.check(ctx.L.cgd_rw_step(ctx.h, _ptr(V), C.c_double(dt), C.byref(w), _stream()), "RosenbrockSchur")
         JCache.CompTri = False
         return
@@ -411,11 +465,8 @@ def hyperdiffusion_for(ne: int) -> float:
     return 5.0e14 * (30.0 / ne) ** 3.2
 
 
-def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None = None, device="cuda",
-                 nT: int = 5, precision: str = "fp64"):
-    """Upload the static inputs and allocate ``Cache``/``JCache``."""
+def _static_inputs(disc: Discretization, phys: Phys, hyperd, dev, precision):
     hd = hyperdiffusion_for(disc.ne) if hyperd is None else hyperd
-    dev = torch.device(device)
     f64 = dict(dtype=torch.float64, device=dev)
     FE = CGQuad(OrdPoly=disc.p, Glob=torch.as_tensor(disc.glob, dtype=torch.int64, device=dev).contiguous(),
                 DS=torch.as_tensor(np.ascontiguousarray(disc.DS.T), **f64),
@@ -425,12 +476,86 @@ def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None
                           X=torch.as_tensor(disc.X, **f64), dz=torch.as_tensor(disc.dz, **f64))
     Model = ModelStruct(HyperDCurl=hd, HyperDGrad=hd, HyperDDiv=hd)
     Global = GlobalStruct(Model=Model, Nz=disc.nz, NumG=disc.num_g, NF=disc.nf, N=disc.N, Precision=precision)
-    n = (disc.num_g, disc.nz)
+    return FE, Metric, Global
+
+
+def _work_arrays(num_g: int, nz: int, nT: int, dev):
+    f64 = dict(dtype=torch.float64, device=dev)
+    n = (num_g, nz)
     Cache = CacheStruct(AuxG=torch.zeros((4,) + n, **f64), Temp1=torch.zeros((nT,) + n, **f64),
                         k=torch.zeros((3, 5) + n, **f64), fV=torch.zeros((5,) + n, **f64),
                         Vn=torch.zeros((5,) + n, **f64))
-    jn = (disc.num_g, disc.nz - 1)
+    jn = (num_g, nz - 1)
     JCache = JStruct(JRhoW=torch.zeros(jn + (2,), **f64), JWRho=torch.zeros(jn + (2,), **f64),
                      JWRhoTh=torch.zeros(jn + (2,), **f64), JRhoThW=torch.zeros(jn + (2,), **f64),
                      tri=torch.zeros(jn + (3,), **f64))
+    return Cache, JCache
+
+
+def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None = None, device="cuda",
+                 nT: int = 5, precision: str = "fp64"):
+    """Upload the static inputs and allocate ``Cache``/``JCache``."""
+    dev = torch.device(device)
+    FE, Metric, Global = _static_inputs(disc, phys, hyperd, dev, precision)
+    Cache, JCache = _work_arrays(disc.num_g, disc.nz, nT, dev)
     return FE, Metric, Global, Cache, JCache
+
+
+@dataclass
+class EnsembleSetup:
+    """Members on grids of uneven size advanced together by one batch context.
+
+    ``Global`` describes the batch: ``NumG`` and ``NF`` are the members' sums
+    and ``Global._ctx`` is the ``cgd_ctx_create_batch`` context, so the
+    reference entry points (``RosenbrockSchur``, ``FcnGPU`` ...) called with
+    it advance every member at once.  Batch arrays are ``(5, NumG, Nz)``
+    with member ``m`` at ``[:, offsets[m]:offsets[m+1], :]``.
+    """
+
+    FE: list
+    Metric: list
+    Global: GlobalStruct
+    Cache: CacheStruct
+    JCache: JStruct
+    offsets: list
+
+    def member(self, X: torch.Tensor, m: int) -> torch.Tensor:
+        """Member ``m``'s part of a batch array (a view)."""
+        return X[:, self.offsets[m]:self.offsets[m + 1], :]
+
+
+def ensemble_setup(discs, phys: Phys = Phys(), hyperd=None, device="cuda", nT: int = 5,
+                   precision: str = "fp64") -> EnsembleSetup:
+    """Batch context and work arrays for the members ``discs``.
+
+    A Discretization that appears several times is uploaded once and its
+    device arrays are shared by those members.  ``hyperd`` is None (each
+    member's ``hyperdiffusion_for(ne)``) or one coefficient per member.
+    """
+    if not discs:
+        raise CGDError("ensemble_setup: no members")
+    if hyperd is not None and len(hyperd) != len(discs):
+        raise CGDError("ensemble_setup: one hyperviscosity coefficient per member expected")
+    dev = torch.device(device)
+    uploaded = {}
+    members, FEs, Metrics = [], [], []
+    for m, disc in enumerate(discs):
+        hd = None if hyperd is None else hyperd[m]
+        if id(disc) not in uploaded:
+            uploaded[id(disc)] = _static_inputs(disc, phys, None, dev, precision)
+        FE, Metric, G = uploaded[id(disc)]
+        hdm = hyperdiffusion_for(disc.ne) if hd is None else hd
+        Gm = GlobalStruct(Model=ModelStruct(HyperDCurl=hdm, HyperDGrad=hdm, HyperDDiv=hdm), Nz=G.Nz,
+                          NumG=G.NumG, NF=G.NF, N=G.N, Precision=precision)
+        members.append((FE, Metric, Gm))
+        FEs.append(FE)
+        Metrics.append(Metric)
+    offsets = [0]
+    for disc in discs:
+        offsets.append(offsets[-1] + disc.num_g)
+    d0 = discs[0]
+    Global = GlobalStruct(Model=members[0][2].Model, Nz=d0.nz, NumG=offsets[-1], NF=sum(d.nf for d in discs),
+                          N=d0.N, Precision=precision)
+    Global._ctx = NativeContext(None, None, phys, None, members=members)
+    Cache, JCache = _work_arrays(Global.NumG, d0.nz, nT, dev)
+    return EnsembleSetup(FE=FEs, Metric=Metrics, Global=Global, Cache=Cache, JCache=JCache, offsets=offsets)

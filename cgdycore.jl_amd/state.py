@@ -86,3 +86,18 @@ def jet_state(disc: Discretization, seed: int = 0, theta_amp: float = 1.0,
     U[1] = 35.0 * np.sin(2.0 * lat)[:, None] ** 2 * taper(zm, disc.H)[None, :]
     U[4] = rho[None, :] * th
     return U
+
+
+def ensemble_members(n: int = 32, nes=(8, 12, 16)):
+    """The ensemble workload's members: member m uses ne = nes[m mod len(nes)]
+    and theta-noise seed m.  Returns (ne list, seed list)."""
+    return [nes[m % len(nes)] for m in range(n)], list(range(n))
+
+
+def ensemble_state(discs, seeds, theta_amp: float = 1.0, phys: Phys = Phys()) -> np.ndarray:
+    """Members' seeded jet states concatenated along the node axis:
+    numpy (5, sum NumG, Nz), member m's nodes after those of members 0..m-1."""
+    if len(discs) != len(seeds):
+        raise ValueError("one seed per member expected")
+    return np.concatenate([jet_state(d, seed=s, theta_amp=theta_amp, phys=phys) for d, s in zip(discs, seeds)],
+                          axis=1)

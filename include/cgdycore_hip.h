@@ -14,10 +14,13 @@
  * Temp1 may have more than 5 components (the reference allocates 8+NumTr);
  * only the first 5 are read or written.
  *
- * All calls are ordered on the caller's HIP stream (passed as void*), return
- * CGD_OK (0) or a negative error code, and allocate nothing after
- * cgd_ctx_create.  A context is not thread-safe; calls on one context must
- * be issued from one host thread.
+ * All calls are ordered on the caller's HIP stream (passed as void*) and
+ * return CGD_OK (0) or a negative error code.  The step-path calls
+ * (cgd_prepare, cgd_fcn, cgd_jac, cgd_schur, cgd_rw_step,
+ * cgd_rw_graph_launch) allocate nothing; only the set-up calls
+ * (cgd_ctx_create*, cgd_autotune, cgd_rw_graph_create, cgd_profile_enable)
+ * do.  A context is not thread-safe; calls on one context must be issued
+ * from one host thread.
  */
 #ifndef CGDYCORE_HIP_H
 #define CGDYCORE_HIP_H
@@ -76,6 +79,20 @@ typedef struct cgd_ctx cgd_ctx;
  * FcnGPU!'s setup (src/GPU/FcnGPU.jl:552-730) and of CGQuad/MetricStruct
  * consumption; no reference counterpart as a separate call. */
 int cgd_ctx_create(const cgd_desc *desc, cgd_ctx **ctx, void *stream);
+
+/* One context over nmem grids ("members", e.g. an ensemble on grids of
+ * uneven size) that every entry point below then advances together.  The
+ * members must agree on N, Nz, precision and the physical constants; each
+ * keeps its own Glob, DS/DW (equal values), metric, M, dz and
+ * hyperviscosity.  The batch's arrays concatenate the members along NumG:
+ * U, V, F, ... are (Nz, sum_m NumG_m, 5) with member m's nodes at
+ * NumG_0 + ... + NumG_{m-1} onward (and likewise k, AuxG, Temp1, the J
+ * blocks).  Each member is coloured on its own and one launch per colour
+ * covers all members, so a member's results are bit-identical to those of
+ * a context built for it alone with the same launch shape.  No reference
+ * counterpart (the reference steps one grid per process; batch variants of
+ * the calls above, src/GPU/FcnGPU.jl:552 and src/Integration/RosenbrockSchur.jl:1). */
+int cgd_ctx_create_batch(const cgd_desc *members, int nmem, cgd_ctx **ctx, void *stream);
 int cgd_ctx_destroy(cgd_ctx *ctx);
 const char *cgd_last_error(const cgd_ctx *ctx);
 
@@ -121,6 +138,37 @@ int cgd_rw_step(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, voi
  * component except the terrain row dXdxI[3,1:2,:] identical across levels,
  * so stored once per element node).  No reference counterpart. */
 int cgd_ctx_info(const cgd_ctx *ctx, int *ncolor, int *chunk_levels, int *compact_metric);
+
+/* Launch shapes (no reference counterpart; the reference fixes its
+ * workgroup shapes by NumberThreadGPU, src/DyCore/GlobalVariables.jl:240):
+ * chunk_levels = levels per element workgroup (8, or 16 for N = 5);
+ * col_lanes = lanes per column in the Rosenbrock stage kernel (64, or 32 so
+ * that two columns share a wavefront when Nz <= 32); col_block = threads per
+ * stage workgroup (64, 128, 192 or 256).  0 keeps a value.  The workgroup
+ * size changes no bit; the chunk height (chunk-interface w sums) and 32-lane
+ * columns (the compiler's fused multiply-adds) change rounding only, within
+ * the FP64 parity bound.  Setting a shape drops a captured step graph. */
+int cgd_set_launch_shape(cgd_ctx *ctx, int chunk_levels, int col_lanes, int col_block);
+int cgd_get_launch_shape(const cgd_ctx *ctx, int *chunk_levels, int *col_lanes, int *col_block);
+
+/* Autotuner: times reps steps of cgd_rw_step for every admissible launch
+ * shape on the caller's V and work buffers and keeps the fastest.  V is
+ * restored afterwards (the work buffers are scratch).  A set-up call: it
+ * allocates a copy of V and synchronises the stream.  shape (3 ints, may be
+ * null) receives the chosen chunk_levels, col_lanes, col_block, best_ms the
+ * time per step.  No reference counterpart. */
+int cgd_autotune(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, int reps, void *stream, int *shape,
+                 double *best_ms);
+
+/* Step graph: captures nsteps calls of cgd_rw_step(ctx, V, dt, work) into a
+ * HIP graph held by the context (pointers, dt and launch shapes as at the
+ * capture); cgd_rw_graph_launch replays all of them with one launch on
+ * stream, advancing V by nsteps steps.  Capture is a set-up call (it
+ * synchronises stream and creates a capture stream); a replay allocates
+ * nothing.  Profiling must be off while capturing.  No reference
+ * counterpart. */
+int cgd_rw_graph_create(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, int nsteps, void *stream);
+int cgd_rw_graph_launch(cgd_ctx *ctx, void *stream);
 
 /* Host-only helper (no GPU needed): the element colouring that schedules the
  * DSS.  glob is a HOST int64 (N*N, NF) 1-based array; color receives NF

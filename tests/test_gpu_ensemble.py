@@ -1,0 +1,229 @@
+"""Ensembles on grids of uneven size in one batch context, launch shapes,
+the autotuner and step graphs (GPU, through the C ABI).
+
+The batch context concatenates the members' meshes and colours each member
+on its own, so every member's result must be bit-identical to a context
+built for that member alone with the same launch shape; the single-grid
+path is itself checked against the oracle to 1e-12 (and here once more
+directly).  The stage workgroup size changes no bit; 32-lane columns and
+the element chunk height change rounding only.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    den = np.linalg.norm(np.ravel(b))
+    return float(np.linalg.norm(np.ravel(a - b)) / (den if den > 0 else 1.0))
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def _members(pkg, nes, seeds, nz):
+    D = pkg.discretization
+    grids = {ne: D.build(ne, 4, nz, 30000.0) for ne in sorted(set(nes))}
+    return [grids[ne] for ne in nes], seeds
+
+
+def _single_steps(pkg, torch, disc, seed, dt, nsteps, precision, shape=None):
+    host = pkg.host
+    phys = pkg.discretization.Phys()
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda", precision=precision)
+    ctx = host.init_context(FE, Metric, phys, Global)
+    if shape:
+        ctx.set_launch_shape(*shape)
+    V = torch.as_tensor(pkg.state.jet_state(disc, seed=seed), device="cuda").clone()
+    host.TimeStepper(V, dt, nsteps, FE, Metric, phys, Cache, JCache, Global)
+    torch.cuda.synchronize()
+    return V.cpu()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_bitwise_equals_members(pkg, torch_cuda, precision):
+    torch = torch_cuda
+    host = pkg.host
+    nes, seeds = [3, 2, 4, 2], [0, 1, 2, 3]
+    nz, dt, nsteps = 20, 300.0, 2
+    discs, seeds = _members(pkg, nes, seeds, nz)
+    print(f"batch vs members {precision} nes={nes} nz={nz}", flush=True)
+    ens = host.ensemble_setup(discs, precision=precision)
+    assert ens.Global._ctx.info()[0] == 4
+    V = torch.as_tensor(pkg.state.ensemble_state(discs, seeds), device="cuda").clone()
+    host.TimeStepper(V, dt, nsteps, ens.FE[0], ens.Metric[0], pkg.discretization.Phys(), ens.Cache, ens.JCache,
+                     ens.Global)
+    torch.cuda.synchronize()
+    Vb = V.cpu()
+    assert torch.isfinite(Vb).all()
+    for m, (disc, s) in enumerate(zip(discs, seeds)):
+        Vm = _single_steps(pkg, torch, disc, s, dt, nsteps, precision)
+        assert torch.equal(ens.member(Vb, m), Vm), f"member {m} (ne={nes[m]}) differs from its own context"
+
+
+def test_batch_vs_oracle(pkg, oracle_mod, torch_cuda):
+    torch = torch_cuda
+    host = pkg.host
+    phys = pkg.discretization.Phys()
+    nes, seeds = [2, 3], [5, 6]
+    nz, dt, nsteps = 10, 300.0, 2
+    discs, seeds = _members(pkg, nes, seeds, nz)
+    print(f"batch vs oracle nes={nes} nz={nz}", flush=True)
+    ens = host.ensemble_setup(discs)
+    U0 = pkg.state.ensemble_state(discs, seeds)
+    V = torch.as_tensor(U0, device="cuda").clone()
+    host.TimeStepper(V, dt, nsteps, ens.FE[0], ens.Metric[0], phys, ens.Cache, ens.JCache, ens.Global)
+    torch.cuda.synchronize()
+    Vg = V.cpu().numpy()
+    for m, (disc, s) in enumerate(zip(discs, seeds)):
+        hd = host.hyperdiffusion_for(disc.ne)
+        orc = oracle_mod.Oracle(disc, phys, (hd, hd, hd))
+        Vo = pkg.state.jet_state(disc, seed=s)
+        orc.rw_step(Vo, dt, nsteps)
+        Vm = Vg[:, ens.offsets[m]:ens.offsets[m + 1], :]
+        for c in range(5):
+            e = rel_l2(Vm[c], Vo[c])
+            assert e <= 1e-12, f"member {m} V{c + 1} rel L2 {e:.3e}"
+
+
+@pytest.mark.parametrize("nz", [20, 32])
+def test_column_shapes(pkg, torch_cuda, nz):
+    """The workgroup size of the stage kernel changes no bit.  Two columns
+    per wavefront (32 lanes each) run the same operations in exact
+    arithmetic (the sixth cyclic-reduction sweep of a 64-lane column is an
+    identity on decoupled rows), but the compiler may contract a*b + c*d
+    into a different fused multiply-add in that instantiation, so the
+    results agree to rounding: 1e-13 relative L2 per variable after two
+    steps."""
+    torch = torch_cuda
+    host = pkg.host
+    phys = pkg.discretization.Phys()
+    discs, seeds = _members(pkg, [3, 2], [0, 1], nz)
+    print(f"column shapes nz={nz}", flush=True)
+    ens = host.ensemble_setup(discs)
+    ctx = ens.Global._ctx
+    U0 = torch.as_tensor(pkg.state.ensemble_state(discs, seeds), device="cuda")
+    out = {}
+    for shape in [(8, 64, 256), (8, 32, 256), (8, 32, 64), (8, 64, 128), (8, 32, 192)]:
+        ctx.set_launch_shape(*shape)
+        assert ctx.launch_shape() == shape
+        V = U0.clone()
+        host.TimeStepper(V, 300.0, 2, ens.FE[0], ens.Metric[0], phys, ens.Cache, ens.JCache, ens.Global)
+        torch.cuda.synchronize()
+        out[shape] = V.cpu()
+    for w in (64, 32):
+        ref = out[(8, w, 256)]
+        for shape, V in out.items():
+            if shape[1] == w:
+                assert torch.equal(V, ref), f"workgroup size of shape {shape} changed the result"
+    ref = out[(8, 64, 256)].numpy()
+    V = out[(8, 32, 256)].numpy()
+    for c in range(5):
+        assert rel_l2(V[c], ref[c]) <= 1e-13, f"V{c + 1} with 32-lane columns: {rel_l2(V[c], ref[c]):.3e}"
+
+
+def test_chunk_height_within_parity(pkg, oracle_mod, torch_cuda):
+    torch = torch_cuda
+    host = pkg.host
+    phys = pkg.discretization.Phys()
+    disc = pkg.discretization.build(3, 4, 32, 30000.0)
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda")
+    ctx = host.init_context(FE, Metric, phys, Global)
+    ctx.set_launch_shape(16, 32, 128)
+    assert ctx.info()[1] == 16
+    U0 = pkg.state.jet_state(disc, seed=4)
+    V = torch.as_tensor(U0, device="cuda").clone()
+    host.TimeStepper(V, 300.0, 2, FE, Metric, phys, Cache, JCache, Global)
+    torch.cuda.synchronize()
+    hd = host.hyperdiffusion_for(disc.ne)
+    orc = oracle_mod.Oracle(disc, phys, (hd, hd, hd))
+    Vo = U0.copy()
+    orc.rw_step(Vo, 300.0, 2)
+    Vg = V.cpu().numpy()
+    for c in range(5):
+        assert rel_l2(Vg[c], Vo[c]) <= 1e-12, f"V{c + 1}"
+
+
+def test_launch_shape_rejected(pkg, torch_cuda):
+    host = pkg.host
+    phys = pkg.discretization.Phys()
+    disc = pkg.discretization.build(2, 4, 40, 30000.0)
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda")
+    ctx = host.init_context(FE, Metric, phys, Global)
+    for bad in [(12, 0, 0), (0, 32, 0), (0, 16, 0), (0, 0, 96), (0, 0, 320)]:
+        with pytest.raises(host.CGDError):
+            ctx.set_launch_shape(*bad)  # 32 lanes need Nz <= 32; 96 is not whole wavefronts
+    assert ctx.launch_shape() == (8, 64, 256)
+
+
+def test_batch_rejects_mismatched_members(pkg, torch_cuda):
+    host = pkg.host
+    D = pkg.discretization
+    a = D.build(2, 4, 10, 30000.0)
+    b = D.build(2, 4, 12, 30000.0)
+    with pytest.raises(host.CGDError, match="Nz"):
+        host.ensemble_setup([a, b])
+    c = D.build(2, 3, 10, 30000.0)
+    with pytest.raises(host.CGDError):
+        host.ensemble_setup([a, c])
+
+
+def test_autotune_keeps_state_and_parity(pkg, torch_cuda):
+    torch = torch_cuda
+    host = pkg.host
+    phys = pkg.discretization.Phys()
+    discs, seeds = _members(pkg, [2, 3, 2], [0, 1, 2], 32)
+    print("autotune", flush=True)
+    ens = host.ensemble_setup(discs)
+    ctx = ens.Global._ctx
+    U0 = torch.as_tensor(pkg.state.ensemble_state(discs, seeds), device="cuda")
+    V = U0.clone()
+    shape, ms = ctx.autotune(V, 150.0, ens.Cache, ens.JCache, reps=2)
+    torch.cuda.synchronize()
+    print(f"  chosen {shape}, {ms:.3f} ms per step", flush=True)
+    assert torch.equal(V, U0), "autotune must restore V"
+    assert ctx.launch_shape() == shape and ms > 0
+    assert shape[0] in (8, 16) and shape[1] in (32, 64) and shape[2] in (64, 128, 256)
+    # the chosen shape steps like the default one up to the chunk-interface rounding
+    host.TimeStepper(V, 150.0, 2, ens.FE[0], ens.Metric[0], phys, ens.Cache, ens.JCache, ens.Global)
+    ctx.set_launch_shape(8, 64, 256)
+    W = U0.clone()
+    host.TimeStepper(W, 150.0, 2, ens.FE[0], ens.Metric[0], phys, ens.Cache, ens.JCache, ens.Global)
+    torch.cuda.synchronize()
+    for c in range(5):
+        assert rel_l2(V[c].cpu().numpy(), W[c].cpu().numpy()) <= 1e-12, f"V{c + 1}"
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_graph_replays_equal_eager_steps(pkg, torch_cuda, precision):
+    """Two captured steps replayed three times, with eager work and a
+    synchronisation between replays; each replay equals two eager steps."""
+    torch = torch_cuda
+    host = pkg.host
+    phys = pkg.discretization.Phys()
+    discs, seeds = _members(pkg, [2, 3], [0, 1], 20)
+    print(f"graph replay {precision}", flush=True)
+    ens = host.ensemble_setup(discs, precision=precision)
+    ctx = ens.Global._ctx
+    U0 = torch.as_tensor(pkg.state.ensemble_state(discs, seeds), device="cuda")
+    E = U0.clone()
+    G = U0.clone()
+    ctx.graph_create(G, 150.0, ens.Cache, ens.JCache, nsteps=2)
+    torch.cuda.synchronize()
+    assert torch.equal(G, U0), "capturing must not run the steps"
+    for rep in range(3):
+        ctx.graph_launch()
+        torch.cuda.synchronize()
+        Gr = G.clone()
+        # eager steps on the same work buffers in between
+        host.TimeStepper(E, 150.0, 2, ens.FE[0], ens.Metric[0], phys, ens.Cache, ens.JCache, ens.Global)
+        torch.cuda.synchronize()
+        assert torch.equal(Gr, E), f"replay {rep} differs from the eager steps"
+    assert torch.isfinite(G).all()
