@@ -243,10 +243,24 @@ def main():
     D = pkg.discretization
     t0 = time.perf_counter()
     disc = D.build(args.ne, args.p, args.nz, 30000.0)
+    host_build_s = time.perf_counter() - t0
     phys = D.Phys()
     U0 = pkg.state.jet_state(disc, seed=0)
-    print(f"[bench] setup ne={args.ne} p={args.p} nz={args.nz}: {time.perf_counter() - t0:.1f}s", file=sys.stderr,
-          flush=True)
+    print(f"[bench] setup ne={args.ne} p={args.p} nz={args.nz}: {host_build_s:.1f}s", file=sys.stderr, flush=True)
+    # the same static inputs built by the device set-up kernels (numbering on the host)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    dev_build = host.device_build(args.ne, args.p, args.nz, 30000.0, device="cuda:0")
+    torch.cuda.synchronize()
+    device_build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pkg.grid.numbering_cg_quad(pkg.grid.cubed_sphere(args.ne, disc.rad), args.p)
+    numbering_s = time.perf_counter() - t0
+    setup = {"host_build_s": round(host_build_s, 3), "device_build_s": round(device_build_s, 3),
+             "host_numbering_s_of_device_build": round(numbering_s, 3),
+             "what": "grid, numbering, metric (JacobiSphere3), DSS mass, layer sizes; device build = host "
+                     "numbering + set-up kernels"}
+    del dev_build
     FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, hyperd=args.hyperd, device="cuda:0")
     eq = host.CompressibleShallow()
     ctx = host.init_context(FE, Metric, phys, Global)
@@ -329,7 +343,7 @@ def main():
            "config": {"workload": workload, "ne": args.ne, "p": args.p,
                       "nz": args.nz, "dt": args.dt, "hyperd": Global.Model.HyperDDiv, "points": Ne,
                       "precision": "fp64", "finite": finite},
-           "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "ensemble": ens}
+           "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "ensemble": ens, "setup": setup}
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,337 @@
+// Set-up on the device: the shallow-sphere metric with the Sleve
+// terrain-following height (JacobiSphere3Kernel!), the DSS mass
+// (MassCGKernel!) and the layer sizes (GridSizeSphereKernel!), written in
+// the reference's Julia column-major layouts.  Sums over the element copies
+// of a global node run in ascending (element, node) order, so they are
+// deterministic (the reference adds with atomics in no fixed order).  The
+// kernels evaluate every product and sum as written, without fused
+// multiply-adds, so given the same inputs the mass and layer sizes equal a
+// plain IEEE evaluation in the same order bit for bit.
+#include "../../include/cgdycore_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#define CGD_NMAX_SETUP 8  // largest N (OrdPoly + 1) of the set-up kernels
+
+namespace {
+
+thread_local std::string g_setup_err;
+
+int setup_fail(int code, const std::string &msg) {
+  g_setup_err = msg;
+  return code;
+}
+
+#define SCHK(call)                                                                                           \
+  do {                                                                                                       \
+    hipError_t e_ = (call);                                                                                  \
+    if (e_ != hipSuccess) return setup_fail(CGD_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int blocks_for(size_t n, int block) {
+  size_t g = (n + block - 1) / block;
+  return (int)(g < 1 ? 1 : (g > 65536 ? 65536 : g));
+}
+
+// Sleve height (src/Grids/Adapt.jl:19-31) of reference height zref over
+// surface height zs
+__device__ __forceinline__ double sleve_h(double zref, double zs, double H, double etaH, double s) {
+  const double eta = zref / H;
+  if (eta <= etaH) return eta * H + zs * sinh((etaH - eta) / s / etaH) / sinh(1.0 / s);
+  return eta * H;
+}
+
+struct SphereArgs {
+  int N, Nz, NF;
+  double Rad, H, etaH, s;
+  double xw[CGD_NMAX_SETUP];
+  double DS[CGD_NMAX_SETUP * CGD_NMAX_SETUP];  // DS[i + N k] (column-major, as passed)
+};
+
+// JacobiSphere3Kernel! (src/Grids/JacobiSphere3GPU.jl:102-215), one thread
+// per (node ID, layer Iz, element IF), both vertical nodes K.
+// F: corners (4,3,NF); z: (Nz+1) interfaces; zs: (N,N,NF) or null (flat).
+__global__ void k_jacobi_sphere3(SphereArgs a, const double *__restrict__ F, const double *__restrict__ z,
+                                 const double *__restrict__ zs, double *__restrict__ X, double *__restrict__ dXdxI,
+                                 double *__restrict__ Jm) {
+#pragma clang fp contract(off)
+  const int N = a.N, NN = N * N, Nz = a.Nz;
+  const size_t total = (size_t)NN * Nz * a.NF;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+    const int ID = (int)(q % NN);
+    const size_t r1 = q / NN;
+    const int Iz = (int)(r1 % Nz);
+    const size_t IF = r1 / Nz;
+    const int I = ID % N, J = ID / N;
+    const double k1 = a.xw[I], k2 = a.xw[J];
+    const double *Fe = F + 12 * IF;  // F[c,k,IF] at Fe[c + 4 k]
+    double Xc[3];
+    for (int k = 0; k < 3; ++k)
+      Xc[k] = 0.25 * (Fe[0 + 4 * k] * (1.0 - k1) * (1.0 - k2) + Fe[1 + 4 * k] * (1.0 + k1) * (1.0 - k2) +
+                      Fe[2 + 4 * k] * (1.0 + k1) * (1.0 + k2) + Fe[3 + 4 * k] * (1.0 - k1) * (1.0 + k2));
+    const double r = sqrt(Xc[0] * Xc[0] + Xc[1] * Xc[1] + Xc[2] * Xc[2]);
+    const double f = a.Rad / r;
+    // cart2sphere (src/Grids/Geometry.jl:65-89)
+    const double theta = asin(Xc[2] / r);
+    double lam = atan2(Xc[1], Xc[0]);
+    if (Xc[1] < 0.0) lam += 2.0 * M_PI;
+    const double sinlam = sin(lam), coslam = cos(lam), sinth = sin(theta), costh = cos(theta);
+    const double A[3][3] = {
+        {sinlam * sinlam * costh * costh + sinth * sinth, -sinlam * coslam * costh * costh, -coslam * sinth * costh},
+        {-sinlam * coslam * costh * costh, coslam * coslam * costh * costh + sinth * sinth, -sinlam * sinth * costh},
+        {-coslam * sinth, -sinlam * sinth, costh}};
+    // ((((1/4 f) DD) A) B) C, products left to right
+    const double sc = 0.25 * f;
+    const double DD[2][3] = {{sc * -sinlam, sc * coslam, sc * 0.0}, {sc * 0.0, sc * 0.0, sc * 1.0}};
+    double DA[2][3], DAB[2][4];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 3; ++j) DA[i][j] = DD[i][0] * A[0][j] + DD[i][1] * A[1][j] + DD[i][2] * A[2][j];
+    for (int i = 0; i < 2; ++i)
+      for (int c = 0; c < 4; ++c)
+        DAB[i][c] = DA[i][0] * Fe[c + 0] + DA[i][1] * Fe[c + 4] + DA[i][2] * Fe[c + 8];
+    const double C[4][2] = {{-1.0 + k2, -1.0 + k1}, {1.0 - k2, -1.0 - k1}, {1.0 + k2, 1.0 + k1}, {-1.0 - k2, 1.0 - k1}};
+    double L2[2][2];
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j)
+        L2[i][j] = DAB[i][0] * C[0][j] + DAB[i][1] * C[1][j] + DAB[i][2] * C[2][j] + DAB[i][3] * C[3][j];
+    // heights of both vertical nodes and their derivatives along ksi1, ksi2
+    const double z1 = z[Iz], z2 = z[Iz + 1];
+    const double *zse = zs ? zs + (size_t)NN * IF : nullptr;
+    double hR[2], Dx[2], Dy[2];
+    for (int K = 0; K < 2; ++K) {
+      const double zeta = K == 0 ? -1.0 : 1.0;
+      const double zl = 0.5 * ((1.0 - zeta) * z1 + (1.0 + zeta) * z2);
+      hR[K] = sleve_h(zl, zse ? zse[I + N * J] : 0.0, a.H, a.etaH, a.s);
+      double dx = a.DS[I] * sleve_h(zl, zse ? zse[0 + N * J] : 0.0, a.H, a.etaH, a.s);
+      double dy = a.DS[J] * sleve_h(zl, zse ? zse[I + N * 0] : 0.0, a.H, a.etaH, a.s);
+      for (int k = 1; k < N; ++k) {
+        dx = dx + a.DS[I + N * k] * sleve_h(zl, zse ? zse[k + N * J] : 0.0, a.H, a.etaH, a.s);
+        dy = dy + a.DS[J + N * k] * sleve_h(zl, zse ? zse[I + N * k] : 0.0, a.H, a.etaH, a.s);
+      }
+      Dx[K] = dx;
+      Dy[K] = dy;
+    }
+    const double Dz = 0.5 * (hR[1] - hR[0]);
+    for (int K = 0; K < 2; ++K) {
+      for (int c = 0; c < 3; ++c)
+        X[ID + (size_t)NN * (K + 2 * (c + 3 * ((size_t)Iz + (size_t)Nz * IF)))] = Xc[c] / r * (a.Rad + hR[K]);
+      // dXdxLoc = [L11 L12 0; L21 L22 0; Dx Dy Dz]; Det3 and the adjugate
+      const double L11 = L2[0][0], L12 = L2[0][1], L13 = 0.0;
+      const double L21 = L2[1][0], L22 = L2[1][1], L23 = 0.0;
+      const double L31 = Dx[K], L32 = Dy[K], L33 = Dz;
+      const size_t pt = ID + (size_t)NN * ((size_t)Iz + (size_t)Nz * IF);
+      Jm[ID + (size_t)NN * (K + 2 * ((size_t)Iz + (size_t)Nz * IF))] =
+          L11 * (L22 * L33 - L23 * L32) - L12 * (L21 * L33 - L23 * L31) + L13 * (L21 * L32 - L22 * L31);
+      double *d = dXdxI + 9 * (K + 2 * pt);  // dXdxI[i,j,K,...] at d[i + 3 j]
+      d[0 + 3 * 0] = L22 * L33 - L23 * L32;
+      d[1 + 3 * 0] = -(L21 * L33 - L23 * L31);
+      d[2 + 3 * 0] = L21 * L32 - L22 * L31;
+      d[0 + 3 * 1] = -(L12 * L33 - L13 * L32);
+      d[1 + 3 * 1] = L11 * L33 - L13 * L31;
+      d[2 + 3 * 1] = -(L11 * L32 - L12 * L31);
+      d[0 + 3 * 2] = L12 * L23 - L13 * L22;
+      d[1 + 3 * 2] = -(L11 * L23 - L13 * L21);
+      d[2 + 3 * 2] = L11 * L22 - L12 * L21;
+    }
+  }
+}
+
+// Element copies of every global node, ascending (element, node) order:
+// tab[g][0..cnt[g]-1] = IF * NN + ID.
+constexpr int MAXC = 16;
+
+__global__ void k_copies_fill(const int64_t *__restrict__ Glob, int NN, int NF, int NumG, int *__restrict__ cnt,
+                              int *__restrict__ tab, int *__restrict__ bad) {
+  const size_t total = (size_t)NN * NF;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+    const int64_t g = Glob[q] - 1;
+    if (g < 0 || g >= NumG) {
+      atomicOr(bad, 1);
+      continue;
+    }
+    const int slot = atomicAdd(&cnt[g], 1);
+    if (slot < MAXC) tab[g * MAXC + slot] = (int)q;
+    else atomicOr(bad, 2);
+  }
+}
+
+__global__ void k_copies_sort(const int *__restrict__ cnt, int *__restrict__ tab, int NumG) {
+  for (size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x; g < (size_t)NumG; g += (size_t)gridDim.x * blockDim.x) {
+    const int n = cnt[g] < MAXC ? cnt[g] : MAXC;
+    int *t = tab + g * MAXC;
+    for (int i = 1; i < n; ++i) {
+      const int v = t[i];
+      int j = i - 1;
+      while (j >= 0 && t[j] > v) {
+        t[j + 1] = t[j];
+        --j;
+      }
+      t[j + 1] = v;
+    }
+  }
+}
+
+// MassCGKernel! (src/DyCore/MassCG.jl:61-75): M[Iz,g,K] = sum of J[ID,K,Iz,IF]
+// and MMass[Iz,g] = sum of ((0.5 (J1 + J2)) w[I]) w[J] over the copies
+__global__ void k_mass_cg(const int *__restrict__ cnt, const int *__restrict__ tab, const double *__restrict__ Jm,
+                          SphereArgs a, int NumG, const double *__restrict__ w, double *__restrict__ M,
+                          double *__restrict__ MMass) {
+#pragma clang fp contract(off)
+  const int N = a.N, NN = N * N, Nz = a.Nz;
+  const size_t total = (size_t)NumG * Nz;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(q % Nz);
+    const size_t g = q / Nz;
+    double m1 = 0.0, m2 = 0.0, mm = 0.0;
+    for (int c = 0; c < cnt[g]; ++c) {
+      const int e = tab[g * MAXC + c];
+      const size_t IF = e / NN;
+      const int ID = e % NN;
+      const double j1 = Jm[ID + (size_t)NN * (0 + 2 * ((size_t)iz + (size_t)Nz * IF))];
+      const double j2 = Jm[ID + (size_t)NN * (1 + 2 * ((size_t)iz + (size_t)Nz * IF))];
+      m1 += j1;
+      m2 += j2;
+      if (MMass) mm += 0.5 * (j1 + j2) * w[ID % N] * w[ID / N];
+    }
+    M[q] = m1;
+    M[q + total] = m2;
+    if (MMass) MMass[q] = mm;
+  }
+}
+
+// GridSizeSphereKernel! (src/DyCore/DiscretizationCG.jl:119-144) from the
+// last copy in (element, node) order: zP = max(|X1 + X2|/2 - Rad, 0),
+// dz = |X2| - |X1|
+__global__ void k_grid_size_sphere(const int *__restrict__ cnt, const int *__restrict__ tab,
+                                   const double *__restrict__ X, int NN, int Nz, int NumG, double Rad,
+                                   double *__restrict__ zP, double *__restrict__ dz) {
+#pragma clang fp contract(off)
+  const size_t total = (size_t)NumG * Nz;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += (size_t)gridDim.x * blockDim.x) {
+    const int iz = (int)(q % Nz);
+    const size_t g = q / Nz;
+    const int e = tab[g * MAXC + cnt[g] - 1];
+    const size_t IF = e / NN;
+    const int ID = e % NN;
+    double x[2][3];
+    for (int K = 0; K < 2; ++K)
+      for (int c = 0; c < 3; ++c) x[K][c] = X[ID + (size_t)NN * (K + 2 * (c + 3 * ((size_t)iz + (size_t)Nz * IF)))];
+    const double xm = 0.5 * (x[0][0] + x[1][0]), ym = 0.5 * (x[0][1] + x[1][1]), zm = 0.5 * (x[0][2] + x[1][2]);
+    const double rm = sqrt(xm * xm + ym * ym + zm * zm);
+    if (zP) zP[q] = rm - Rad > 0.0 ? rm - Rad : 0.0;
+    const double ra = sqrt(x[0][0] * x[0][0] + x[0][1] * x[0][1] + x[0][2] * x[0][2]);
+    const double rb = sqrt(x[1][0] * x[1][0] + x[1][1] * x[1][1] + x[1][2] * x[1][2]);
+    dz[q] = rb - ra;
+  }
+}
+
+// node -> copies table on the device (set-up only: allocates, synchronises)
+struct Copies {
+  int *cnt = nullptr, *tab = nullptr, *bad = nullptr;
+  ~Copies() {
+    if (cnt) (void)hipFree(cnt);
+    if (tab) (void)hipFree(tab);
+    if (bad) (void)hipFree(bad);
+  }
+};
+
+int build_copies(Copies &cp, const int64_t *Glob, int NN, int NF, int NumG, hipStream_t st) {
+  SCHK(hipMalloc(&cp.cnt, (size_t)NumG * sizeof(int)));
+  SCHK(hipMalloc(&cp.tab, (size_t)NumG * MAXC * sizeof(int)));
+  SCHK(hipMalloc(&cp.bad, sizeof(int)));
+  SCHK(hipMemsetAsync(cp.cnt, 0, (size_t)NumG * sizeof(int), st));
+  SCHK(hipMemsetAsync(cp.bad, 0, sizeof(int), st));
+  hipLaunchKernelGGL(k_copies_fill, dim3(blocks_for((size_t)NN * NF, 256)), dim3(256), 0, st, Glob, NN, NF, NumG,
+                     cp.cnt, cp.tab, cp.bad);
+  SCHK(hipGetLastError());
+  int bad = 0;
+  SCHK(hipMemcpyAsync(&bad, cp.bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  if (bad & 1) return setup_fail(CGD_ERR_ARG, "Glob entry outside 1..NumG");
+  if (bad & 2) return setup_fail(CGD_ERR_UNSUPPORTED, "a global node has more than 16 element copies");
+  hipLaunchKernelGGL(k_copies_sort, dim3(blocks_for(NumG, 256)), dim3(256), 0, st, cp.cnt, cp.tab, NumG);
+  SCHK(hipGetLastError());
+  // every node must have a copy (the grid-size kernel reads the last one)
+  std::vector<int> hc(NumG);
+  SCHK(hipMemcpyAsync(hc.data(), cp.cnt, (size_t)NumG * sizeof(int), hipMemcpyDeviceToHost, st));
+  SCHK(hipStreamSynchronize(st));
+  for (int g = 0; g < NumG; ++g)
+    if (hc[g] < 1) return setup_fail(CGD_ERR_ARG, "global node " + std::to_string(g + 1) + " has no element copy");
+  return CGD_OK;
+}
+
+int dims_ok(int N, int Nz, int NF) {
+  if (N < 2 || N > CGD_NMAX_SETUP) return setup_fail(CGD_ERR_ARG, "N must be in [2, 8]");
+  if (Nz < 1 || NF < 1) return setup_fail(CGD_ERR_ARG, "Nz and NF must be positive");
+  if ((size_t)N * N * NF >= (size_t)1 << 31) return setup_fail(CGD_ERR_ARG, "grid too large");
+  return CGD_OK;
+}
+
+}  // namespace
+
+extern "C" const char *cgd_setup_error(void) { return g_setup_err.c_str(); }
+
+extern "C" int cgd_jacobi_sphere3(int N, int Nz, int NF, const double *F, const double *xw, const double *DS,
+                                  const double *z, const double *zs, double Rad, double etaH, double s, double *X,
+                                  double *dXdxI, double *J, void *stream) {
+  int r = dims_ok(N, Nz, NF);
+  if (r) return r;
+  if (!F || !xw || !DS || !z || !X || !dXdxI || !J) return setup_fail(CGD_ERR_ARG, "null array");
+  if (!(Rad > 0.0) || !(etaH > 0.0) || !(s > 0.0)) return setup_fail(CGD_ERR_ARG, "Rad, etaH and s must be positive");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  SphereArgs a{};
+  a.N = N;
+  a.Nz = Nz;
+  a.NF = NF;
+  a.Rad = Rad;
+  a.etaH = etaH;
+  a.s = s;
+  SCHK(hipMemcpyAsync(a.xw, xw, N * sizeof(double), hipMemcpyDeviceToHost, st));
+  SCHK(hipMemcpyAsync(a.DS, DS, (size_t)N * N * sizeof(double), hipMemcpyDeviceToHost, st));
+  SCHK(hipMemcpyAsync(&a.H, z + Nz, sizeof(double), hipMemcpyDeviceToHost, st));  // H = z[Nz+1]
+  SCHK(hipStreamSynchronize(st));
+  if (!(a.H > 0.0)) return setup_fail(CGD_ERR_ARG, "the top interface z[Nz+1] must be positive");
+  const size_t total = (size_t)N * N * Nz * NF;
+  hipLaunchKernelGGL(k_jacobi_sphere3, dim3(blocks_for(total, 256)), dim3(256), 0, st, a, F, z, zs, X, dXdxI, J);
+  SCHK(hipGetLastError());
+  return CGD_OK;
+}
+
+extern "C" int cgd_mass_cg(int N, int Nz, int NF, int NumG, const int64_t *Glob, const double *J, const double *w,
+                           double *M, double *MMass, void *stream) {
+  int r = dims_ok(N, Nz, NF);
+  if (r) return r;
+  if (NumG < 1 || !Glob || !J || !M || (MMass && !w)) return setup_fail(CGD_ERR_ARG, "bad argument");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  Copies cp;
+  if ((r = build_copies(cp, Glob, N * N, NF, NumG, st))) return r;
+  SphereArgs a{};
+  a.N = N;
+  a.Nz = Nz;
+  a.NF = NF;
+  const size_t total = (size_t)NumG * Nz;
+  hipLaunchKernelGGL(k_mass_cg, dim3(blocks_for(total, 256)), dim3(256), 0, st, cp.cnt, cp.tab, J, a, NumG, w, M,
+                     MMass);
+  SCHK(hipGetLastError());
+  SCHK(hipStreamSynchronize(st));  // the copy table is freed on return
+  return CGD_OK;
+}
+
+extern "C" int cgd_grid_size_sphere(int N, int Nz, int NF, int NumG, const int64_t *Glob, const double *X,
+                                    double Rad, double *zP, double *dz, void *stream) {
+  int r = dims_ok(N, Nz, NF);
+  if (r) return r;
+  if (NumG < 1 || !Glob || !X || !dz) return setup_fail(CGD_ERR_ARG, "bad argument");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  Copies cp;
+  if ((r = build_copies(cp, Glob, N * N, NF, NumG, st))) return r;
+  const size_t total = (size_t)NumG * Nz;
+  hipLaunchKernelGGL(k_grid_size_sphere, dim3(blocks_for(total, 256)), dim3(256), 0, st, cp.cnt, cp.tab, X, N * N, Nz,
+                     NumG, Rad, zP, dz);
+  SCHK(hipGetLastError());
+  SCHK(hipStreamSynchronize(st));
+  return CGD_OK;
+}

@@ -183,13 +183,16 @@ class Discretization:
 
 
 def sphere_metric(face_corners: np.ndarray, xw: np.ndarray, DS: np.ndarray, z: np.ndarray,
-                  rad: float, H: float):
-    """Shallow-sphere metric (JacobiSphere3GPU.jl:102-215) for flat topography.
+                  rad: float, H: float, zs: np.ndarray | None = None):
+    """Shallow-sphere metric (JacobiSphere3GPU.jl:102-215) with the Sleve
+    terrain-following height (Adapt.jl:19-31).
 
-    ``face_corners``: (NF, 4, 3).  Returns ``X, dXdxI, J`` in the numpy
-    shapes of the module docstring.  The horizontal 2x2 block does not depend
-    on height (``f = Rad/r`` uses the bilinear point), the vertical row is the
-    derivative of the Sleve height with ``zs = 0``.
+    ``face_corners``: (NF, 4, 3); ``zs``: surface height at the element
+    nodes, numpy (NF, N, N) = Julia (N, N, NF), or None for flat terrain.
+    Returns ``X, dXdxI, J`` in the numpy shapes of the module docstring.
+    The horizontal 2x2 block does not depend on height (``f = Rad/r`` uses
+    the bilinear point); the vertical row holds the derivatives of the
+    height hR along the element's reference coordinates.
     """
     nf = face_corners.shape[0]
     n = xw.shape[0]
@@ -241,19 +244,28 @@ def sphere_metric(face_corners: np.ndarray, xw: np.ndarray, DS: np.ndarray, z: n
     L = [[DAB[i][0] * C[0][j] + DAB[i][1] * C[1][j] + DAB[i][2] * C[2][j] + DAB[i][3] * C[3][j]
           for j in range(2)] for i in range(2)]  # each (NF, J, I)
 
-    # vertical: hR[K, Iz] (flat topography -> independent of I, J, IF)
+    # vertical: hR at (IF, Iz, J, I, K); flat terrain gives one value per
+    # (Iz, K), held in a (1, Nz, 1, 1, 2) array
     zl = np.empty((2, nz))
     for K in range(2):
         zl[K] = 0.5 * ((one - zeta[K]) * z[:-1] + (one + zeta[K]) * z[1:])
-    hR = sleve(zl, 0.0, H)  # (2, Nz)
-    # DxhR = sum_k D[I,k] hR (same value at every k), DyhR likewise
-    dxh = np.zeros((n, 2, nz))
-    for I in range(n):
-        acc = DS[I, 0] * hR
-        for k in range(1, n):
-            acc = acc + DS[I, k] * hR
-        dxh[I] = acc
-    dzh = 0.5 * (hR[1] - hR[0])  # (Nz,)
+    zlb = zl.T[None, :, None, None, :]
+    if zs is None:
+        hR = sleve(zlb, 0.0, H)
+    else:
+        zs = np.asarray(zs, dtype=np.float64)
+        if zs.shape != (nf, n, n):
+            raise ValueError(f"zs must have shape (NF, N, N) = {(nf, n, n)}")
+        hR = sleve(zlb, zs[:, None, :, :, None], H)
+    # DxhR[I] = sum_k D[I,k] hR[k,J], DyhR[J] = sum_k D[J,k] hR[I,k] (k ascending)
+    nI, nJ = hR.shape[3], hR.shape[2]
+    dxh = DS[:, 0][None, None, None, :, None] * hR[:, :, :, 0:1, :]
+    dyh = DS[:, 0][None, None, :, None, None] * hR[:, :, 0:1, :, :]
+    for k in range(1, n):
+        ki, kj = min(k, nI - 1), min(k, nJ - 1)
+        dxh = dxh + DS[:, k][None, None, None, :, None] * hR[:, :, :, ki:ki + 1, :]
+        dyh = dyh + DS[:, k][None, None, :, None, None] * hR[:, :, kj:kj + 1, :, :]
+    dzh = 0.5 * (hR[..., 1] - hR[..., 0])  # (., Nz, ., .)
 
     # assemble per-point quantities, shape (NF, Nz, J, I, K)
     def hor(a):
@@ -262,9 +274,9 @@ def sphere_metric(face_corners: np.ndarray, xw: np.ndarray, DS: np.ndarray, z: n
     L11, L12, L21, L22 = hor(L[0][0]), hor(L[0][1]), hor(L[1][0]), hor(L[1][1])
     L13 = 0.0
     L23 = 0.0
-    L31 = np.transpose(dxh, (2, 0, 1))[None, :, None, :, :]  # (1, Nz, 1, I, K)
-    L32 = np.transpose(dxh, (2, 0, 1))[None, :, :, None, :]  # (1, Nz, J, 1, K)
-    L33 = np.broadcast_to(dzh[None, :, None, None, None], (1, nz, 1, 1, 2))
+    L31 = dxh
+    L32 = dyh
+    L33 = np.broadcast_to(dzh[..., None], dzh.shape + (2,))
     shape = (nf, nz, n, n, 2)
 
     def full(a):
@@ -285,9 +297,10 @@ def sphere_metric(face_corners: np.ndarray, xw: np.ndarray, DS: np.ndarray, z: n
     J = np.ascontiguousarray(np.transpose(full(jac), (0, 1, 4, 2, 3))).reshape(nf, nz, 2, n * n)
     # X[ID,K,c,Iz,IF] = Xc / r * (Rad + hR)
     X = np.empty((nf, nz, 3, 2, n, n))
+    hRt = np.transpose(hR, (0, 1, 4, 2, 3))  # (., Nz, K, ., .)
     for c, Xc in enumerate((X1, X2, X3)):
         xr = (Xc / r)[:, None, None, :, :]
-        X[:, :, c] = xr * (rad + hR.T[None, :, :, None, None])
+        X[:, :, c] = xr * (rad + hRt)
     X = X.reshape(nf, nz, 3, 2, n * n)
     return X, dXdxI, J
 
@@ -304,6 +317,35 @@ def mass_cg(J: np.ndarray, glob: np.ndarray, num_g: int):
         vals = np.transpose(J[:, :, K, :], (0, 2, 1)).reshape(nf * nn, nz)
         np.add.at(M[K], idx, vals)
     return M
+
+
+def mass_mmass(J: np.ndarray, glob: np.ndarray, num_g: int, w: np.ndarray):
+    """``MMass`` of ``MassCGKernel!`` (MassCG.jl:73): sum over copies of
+    0.5 (J1 + J2) w_I w_J, in ascending (IF, ID) order.  Returns (NumG, Nz)."""
+    nf, nz, _, nn = J.shape
+    n = w.shape[0]
+    # node ID = I + N J (0-based): ((0.5 (J1 + J2)) w[I]) w[J], left to right
+    wI = np.array([w[i % n] for i in range(nn)])
+    wJ = np.array([w[i // n] for i in range(nn)])
+    vals = 0.5 * (J[:, :, 0, :] + J[:, :, 1, :]) * wI[None, None, :] * wJ[None, None, :]
+    vals = np.transpose(vals, (0, 2, 1)).reshape(nf * nn, nz)
+    MM = np.zeros((num_g, nz))
+    np.add.at(MM, glob.reshape(-1) - 1, vals)
+    return MM
+
+
+def layer_center_height(X: np.ndarray, glob: np.ndarray, num_g: int, rad: float):
+    """``zP`` of ``GridSizeSphereKernel!`` (DiscretizationCG.jl:130-134):
+    max(|X1 + X2| / 2 - Rad, 0) from the last copy in (IF, ID) order.
+    Returns (NumG, Nz)."""
+    nf, nz, _, _, nn = X.shape
+    xm = 0.5 * (X[:, :, :, 0, :] + X[:, :, :, 1, :])  # (NF, Nz, 3, N^2)
+    r = np.sqrt(xm[:, :, 0] * xm[:, :, 0] + xm[:, :, 1] * xm[:, :, 1] + xm[:, :, 2] * xm[:, :, 2])
+    vals = np.transpose(np.maximum(r - rad, 0.0), (0, 2, 1)).reshape(nf * nn, nz)
+    idx = glob.reshape(-1) - 1
+    last = np.full(num_g, -1, dtype=np.int64)
+    last[idx] = np.arange(idx.size)
+    return vals[last]
 
 
 def layer_thickness(X: np.ndarray, glob: np.ndarray, num_g: int):

@@ -93,6 +93,10 @@ def native():
                                C.POINTER(C.c_double)]
     L.cgd_rw_graph_create.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), C.c_int, _vp]
     L.cgd_rw_graph_launch.argtypes = [_vp, _vp]
+    L.cgd_setup_error.restype = C.c_char_p
+    L.cgd_jacobi_sphere3.argtypes = [C.c_int, C.c_int, C.c_int] + [_vp] * 5 + [C.c_double] * 3 + [_vp] * 4
+    L.cgd_mass_cg.argtypes = [C.c_int] * 4 + [_vp] * 6
+    L.cgd_grid_size_sphere.argtypes = [C.c_int] * 4 + [_vp, _vp, C.c_double, _vp, _vp, _vp]
     _LIB = L
     return L
 
@@ -101,7 +105,8 @@ def exported_symbols():
     return ["cgd_ctx_create", "cgd_ctx_destroy", "cgd_last_error", "cgd_prepare", "cgd_fcn", "cgd_jac",
             "cgd_schur", "cgd_rw_step", "cgd_profile_enable", "cgd_profile_read", "cgd_color_elements",
             "cgd_ctx_info", "cgd_ctx_create_batch", "cgd_set_launch_shape", "cgd_get_launch_shape",
-            "cgd_autotune", "cgd_rw_graph_create", "cgd_rw_graph_launch"]
+            "cgd_autotune", "cgd_rw_graph_create", "cgd_rw_graph_launch", "cgd_setup_error",
+            "cgd_jacobi_sphere3", "cgd_mass_cg", "cgd_grid_size_sphere"]
 
 
 def color_elements(glob: np.ndarray, num_g: int):
@@ -499,6 +504,89 @@ def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None
     FE, Metric, Global = _static_inputs(disc, phys, hyperd, dev, precision)
     Cache, JCache = _work_arrays(disc.num_g, disc.nz, nT, dev)
     return FE, Metric, Global, Cache, JCache
+
+
+def _setup_check(L, rc, what):
+    if rc != 0:
+        raise CGDError(f"{what}: {L.cgd_setup_error().decode()} (code {rc})")
+
+
+def JacobiSphere3GPU(X, dXdxI, J, xw, DS, F, z, zs, Rad, etaH=0.7, s=0.8):
+    """``JacobiSphere3GPU!`` with the Sleve grid (JacobiSphere3GPU.jl:24-40).
+
+    Device tensors in the numpy shapes of :mod:`.discretization`: ``F``
+    corners (NF, 3, 4) (Julia (4,3,NF)), ``DS`` (N, N) holding DS^T (Julia
+    column-major), ``zs`` (NF, N, N) or None; outputs ``X``, ``dXdxI``, ``J``.
+    """
+    L = native()
+    nf, nz = X.shape[0], X.shape[1]
+    n = xw.shape[0]
+    _setup_check(L, L.cgd_jacobi_sphere3(n, nz, nf, _ptr(F), _ptr(xw), _ptr(DS), _ptr(z),
+                                         _ptr(zs) if zs is not None else None, C.c_double(Rad), C.c_double(etaH),
+                                         C.c_double(s), _ptr(X), _ptr(dXdxI), _ptr(J), _stream()),
+                 "JacobiSphere3GPU")
+
+
+def MassCGGPU(M, MMass, J, Glob, w, num_g):
+    """``MassCGGPU!`` (MassCG.jl:42-75): ``M`` (2, NumG, Nz), ``MMass`` (NumG, Nz) or None."""
+    L = native()
+    nf, nz, _, nn = J.shape
+    n = int(round(nn ** 0.5))
+    _setup_check(L, L.cgd_mass_cg(n, nz, nf, num_g, _ptr(Glob), _ptr(J), _ptr(w), _ptr(M),
+                                  _ptr(MMass) if MMass is not None else None, _stream()), "MassCGGPU")
+
+
+def GridSizeSphereGPU(zP, dz, X, Glob, Rad, num_g):
+    """``GridSizeSphereKernel!`` (DiscretizationCG.jl:119-144): ``dz``, ``zP`` (NumG, Nz)."""
+    L = native()
+    nf, nz, _, _, nn = X.shape
+    n = int(round(nn ** 0.5))
+    _setup_check(L, L.cgd_grid_size_sphere(n, nz, nf, num_g, _ptr(Glob), _ptr(X), C.c_double(Rad),
+                                           _ptr(zP) if zP is not None else None, _ptr(dz), _stream()),
+                 "GridSizeSphereGPU")
+
+
+def device_build(ne: int, p: int = 4, nz: int = 64, H: float = 30000.0, rad: float = 6.37122e6,
+                 phys: Phys = Phys(), hyperd: float | None = None, zs=None, device="cuda", nT: int = 5,
+                 precision: str = "fp64"):
+    """Static inputs built on the device: the cubed-sphere numbering on the
+    host (integer graph work), then the metric, the DSS mass and the layer
+    sizes by the set-up kernels.  ``zs``: surface height, numpy (NF, N, N)
+    or None.  Returns ``(FE, Metric, Global, Cache, JCache, grid_info)``
+    like ``device_setup`` plus a dict with the host numbering."""
+    from . import discretization as _D
+    from . import grid as _G
+
+    dev = torch.device(device)
+    f64 = dict(dtype=torch.float64, device=dev)
+    g = _G.cubed_sphere(ne, rad)
+    glob, num_g = _G.numbering_cg_quad(g, p)
+    DW, DS, xw, w = _D.derivative_matrices(p)
+    z = _D.vertical_grid(nz, H)
+    nf, n = glob.shape[0], p + 1
+    Fc = torch.as_tensor(np.ascontiguousarray(np.transpose(g.face_corners(), (0, 2, 1))), **f64)
+    t_xw, t_w, t_z = (torch.as_tensor(a, **f64) for a in (xw, w, z))
+    t_DS = torch.as_tensor(np.ascontiguousarray(DS.T), **f64)
+    t_zs = None if zs is None else torch.as_tensor(np.ascontiguousarray(zs, dtype=np.float64), **f64)
+    X = torch.empty((nf, nz, 3, 2, n * n), **f64)
+    dXdxI = torch.empty((nf, nz, n * n, 2, 3, 3), **f64)
+    J = torch.empty((nf, nz, 2, n * n), **f64)
+    JacobiSphere3GPU(X, dXdxI, J, t_xw, t_DS, Fc, t_z, t_zs, rad)
+    Glob = torch.as_tensor(glob, dtype=torch.int64, device=dev).contiguous()
+    M = torch.empty((2, num_g, nz), **f64)
+    MMass = torch.empty((num_g, nz), **f64)
+    MassCGGPU(M, MMass, J, Glob, t_w, num_g)
+    dz = torch.empty((num_g, nz), **f64)
+    zP = torch.empty((num_g, nz), **f64)
+    GridSizeSphereGPU(zP, dz, X, Glob, rad, num_g)
+    hd = hyperdiffusion_for(ne) if hyperd is None else hyperd
+    FE = CGQuad(OrdPoly=p, Glob=Glob, DS=t_DS, DW=torch.as_tensor(np.ascontiguousarray(DW.T), **f64), M=M,
+                NumG=num_g)
+    Metric = MetricStruct(dXdxI=dXdxI, J=J, X=X, dz=dz)
+    Model = ModelStruct(HyperDCurl=hd, HyperDGrad=hd, HyperDDiv=hd)
+    Global = GlobalStruct(Model=Model, Nz=nz, NumG=num_g, NF=nf, N=n, Precision=precision)
+    Cache, JCache = _work_arrays(num_g, nz, nT, dev)
+    return FE, Metric, Global, Cache, JCache, {"glob": glob, "num_g": num_g, "MMass": MMass, "zP": zP}
 
 
 @dataclass

