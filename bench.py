@@ -73,8 +73,11 @@ def mixed_precision(host, disc, phys, U0, args, fp64_ms, V_fp64):
     FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, hyperd=args.hyperd, device="cuda:0",
                                                           precision="mixed")
     eq = host.CompressibleShallow()
-    host.init_context(FE, Metric, phys, Global)
+    ctx = host.init_context(FE, Metric, phys, Global)
     V = torch.as_tensor(U0, device="cuda:0").clone()
+    shape = ctx.launch_shape()
+    if not args.no_autotune:
+        shape, _ = ctx.autotune(V, args.dt, Cache, JCache, reps=3)
 
     def step():
         host.RosenbrockSchur(V, args.dt, host.FcnGPU, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys,
@@ -100,6 +103,7 @@ def mixed_precision(host, disc, phys, U0, args, fp64_ms, V_fp64):
     return {"value": round(disc.element_points * args.steps / wall / 1e6, 2), "unit": "Mpts/s",
             "ms_per_step": round(ms, 4), "speedup_vs_fp64": round(fp64_ms / ms, 3),
             "steps_compared": args.warmup + args.steps, "rel_l2_vs_fp64": rel_l2, "rel_linf_vs_fp64": rel_linf,
+            "launch_shape": {"chunk_levels": shape[0], "col_lanes": shape[1], "col_block": shape[2]},
             "precision": "FP32 tendency and DSS; FP64 EOS, w-interface balance, horizontal pressure gradient, "
                          "column solve and stage arithmetic"}
 
@@ -211,6 +215,7 @@ def main():
                     help="oracle RW steps for the CPU baseline, about 5 s each at ne=30 (0: skip)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=10)
     ap.add_argument("--ens-dt", type=float, default=150.0)
@@ -265,6 +270,9 @@ def main():
     eq = host.CompressibleShallow()
     ctx = host.init_context(FE, Metric, phys, Global)
     V = torch.as_tensor(U0, device="cuda:0").clone()
+    shape = ctx.launch_shape()
+    if not args.no_autotune:
+        shape, _ = ctx.autotune(V, args.dt, Cache, JCache, reps=3)  # V is restored
 
     def step():
         host.RosenbrockSchur(V, args.dt, host.FcnGPU, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys,
@@ -342,7 +350,9 @@ def main():
            "dtype": "f64", "data": "synthetic: seeded baroclinic jet (seed 0) on a flat cubed sphere",
            "config": {"workload": workload, "ne": args.ne, "p": args.p,
                       "nz": args.nz, "dt": args.dt, "hyperd": Global.Model.HyperDDiv, "points": Ne,
-                      "precision": "fp64", "finite": finite},
+                      "precision": "fp64", "finite": finite,
+                      "launch_shape": {"chunk_levels": shape[0], "col_lanes": shape[1], "col_block": shape[2],
+                                       "autotuned": not args.no_autotune}},
            "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "ensemble": ens, "setup": setup}
     print(json.dumps(out), flush=True)
 
