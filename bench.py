@@ -72,23 +72,16 @@ def mixed_precision(host, disc, phys, U0, args, fp64_ms, V_fp64):
 
     FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, hyperd=args.hyperd, device="cuda:0",
                                                           precision="mixed")
-    eq = host.CompressibleShallow()
     ctx = host.init_context(FE, Metric, phys, Global)
     V = torch.as_tensor(U0, device="cuda:0").clone()
     shape = ctx.launch_shape()
     if not args.no_autotune:
         shape, _ = ctx.autotune(V, args.dt, Cache, JCache, reps=3)
 
-    def step():
-        host.RosenbrockSchur(V, args.dt, host.FcnGPU, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys,
-                             Cache, JCache, None, Global, None, eq)
-
-    for _ in range(args.warmup):
-        step()
+    host.TimeStepper(V, args.dt, args.warmup, FE, Metric, phys, Cache, JCache, Global)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
+    host.TimeStepper(V, args.dt, args.steps, FE, Metric, phys, Cache, JCache, Global)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = wall / args.steps * 1e3
@@ -103,7 +96,7 @@ def mixed_precision(host, disc, phys, U0, args, fp64_ms, V_fp64):
     return {"value": round(disc.element_points * args.steps / wall / 1e6, 2), "unit": "Mpts/s",
             "ms_per_step": round(ms, 4), "speedup_vs_fp64": round(fp64_ms / ms, 3),
             "steps_compared": args.warmup + args.steps, "rel_l2_vs_fp64": rel_l2, "rel_linf_vs_fp64": rel_linf,
-            "launch_shape": {"chunk_levels": shape[0], "col_lanes": shape[1], "col_block": shape[2]},
+            "launch_shape": shape,
             "precision": "FP32 tendency and DSS; FP64 EOS, w-interface balance, horizontal pressure gradient, "
                          "column solve and stage arithmetic"}
 
@@ -112,11 +105,12 @@ def ensemble(pkg, args):
     """The batched-ensemble workload: args.ens_members members (theta-noise
     seeds 0..n-1) on cubed spheres ne = 8, 12, 16 (member m on
     (8, 12, 16)[m mod 3]), p = 4, nz = 32, dt = args.ens_dt, one batch
-    context, launch shapes from the autotuner, each step replayed from a
-    captured HIP graph.  Reports the mixed-precision batch, the FP64 batch,
-    and the FP64 members stepped one context at a time (the sum of the
-    per-grid step times, each grid timed on its own context), with the
-    mixed batch's error against the FP64 batch after the same steps."""
+    context, launch shapes from the autotuner, the timed steps replayed
+    from one captured HIP graph (one launch).  Reports the mixed-precision
+    batch, the FP64 batch and the FP64 members stepped one context at a
+    time (the sum of the per-grid step times, each grid timed on its own
+    context), with the mixed batch's error against the FP64 batch after the
+    same steps."""
     import torch
 
     host = pkg.host
@@ -145,13 +139,13 @@ def ensemble(pkg, args):
         ctx = ens.Global._ctx
         V = torch.as_tensor(U0, device="cuda:0").clone()
         shape, tune_ms = ctx.autotune(V, dt, ens.Cache, ens.JCache, reps=3)
-        ctx.graph_create(V, dt, ens.Cache, ens.JCache, nsteps=1)
+        ctx.graph_create(V, dt, ens.Cache, ens.JCache, nsteps=K)
         for _ in range(W):
             ctx.graph_launch()
-        ms = timed(ctx.graph_launch, K)
+        ms = timed(ctx.graph_launch, 1) / K
         finals[prec] = V.clone()
         out[prec] = {"value": round(points / ms / 1e3, 2), "unit": "Mpts/s", "ms_per_step": round(ms, 4),
-                     "launch_shape": {"chunk_levels": shape[0], "col_lanes": shape[1], "col_block": shape[2]},
+                     "launch_shape": shape,
                      "finite": bool(torch.isfinite(V).all().item())}
         ctx.close()
         del ens, V
@@ -162,22 +156,17 @@ def ensemble(pkg, args):
         d = (finals["mixed"][c] - finals["fp64"][c]).double()
         rel[n] = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(finals["fp64"][c].double()))
     out["mixed"]["rel_l2_vs_fp64_batch"] = rel
-    out["mixed"]["steps_compared"] = W + K
+    out["mixed"]["steps_compared"] = (W + 1) * K
     # the same members without batching: one FP64 context per grid, the
     # step time of each member taken from its grid's context
     per_grid = {}
     for ne, disc in grids.items():
         FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda:0")
-        eq = host.CompressibleShallow()
         host.init_context(FE, Metric, phys, Global)
         V = torch.as_tensor(pkg.state.jet_state(disc, seed=0), device="cuda:0").clone()
 
-        def step():
-            host.RosenbrockSchur(V, dt, host.FcnGPU, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys, Cache,
-                                 JCache, None, Global, None, eq)
-
-        step()
-        per_grid[ne] = timed(step, K)
+        host.TimeStepper(V, dt, 1, FE, Metric, phys, Cache, JCache, Global)
+        per_grid[ne] = timed(lambda: host.TimeStepper(V, dt, K, FE, Metric, phys, Cache, JCache, Global), 1) / K
         Global._ctx.close()
     seq_ms = sum(per_grid[ne] for ne in nes)
     out["fp64_one_member_at_a_time"] = {"ms_per_step": round(seq_ms, 4),
@@ -267,27 +256,21 @@ def main():
                      "numbering + set-up kernels"}
     del dev_build
     FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, hyperd=args.hyperd, device="cuda:0")
-    eq = host.CompressibleShallow()
     ctx = host.init_context(FE, Metric, phys, Global)
     V = torch.as_tensor(U0, device="cuda:0").clone()
     shape = ctx.launch_shape()
     if not args.no_autotune:
         shape, _ = ctx.autotune(V, args.dt, Cache, JCache, reps=3)  # V is restored
 
-    def step():
-        host.RosenbrockSchur(V, args.dt, host.FcnGPU, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys,
-                             Cache, JCache, None, Global, None, eq)
-
-    for _ in range(args.warmup):
-        step()
+    host.TimeStepper(V, args.dt, args.warmup, FE, Metric, phys, Cache, JCache, Global)
     torch.cuda.synchronize()
-    # timed region: HIP events only around the fused tendency pass (each
-    # event is a stream barrier, so the other passes are not bracketed here)
+    # timed region: one TimeStepper run of the steps (cgd_rw_run), HIP events
+    # only around the fused tendency pass (each event is a stream barrier,
+    # so the other passes are not bracketed here)
     ctx.profile(PASS2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
+    host.TimeStepper(V, args.dt, args.steps, FE, Metric, phys, Cache, JCache, Global)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     p2_ms, p2_launch = ctx.profile_read(1)
@@ -296,8 +279,7 @@ def main():
     # per-pass breakdown from a few extra steps with every pass bracketed
     nb = min(args.steps, 5)
     ctx.profile(ALL_PASSES)
-    for _ in range(nb):
-        step()
+    host.TimeStepper(V, args.dt, nb, FE, Metric, phys, Cache, JCache, Global)
     torch.cuda.synchronize()
     passes = {name: ctx.profile_read(i) for i, name in enumerate(("pass1", "pass2", "column"))}
     ctx.profile(0)
@@ -351,8 +333,7 @@ def main():
            "config": {"workload": workload, "ne": args.ne, "p": args.p,
                       "nz": args.nz, "dt": args.dt, "hyperd": Global.Model.HyperDDiv, "points": Ne,
                       "precision": "fp64", "finite": finite,
-                      "launch_shape": {"chunk_levels": shape[0], "col_lanes": shape[1], "col_block": shape[2],
-                                       "autotuned": not args.no_autotune}},
+                      "launch_shape": dict(shape, autotuned=not args.no_autotune)},
            "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "ensemble": ens, "setup": setup}
     print(json.dumps(out), flush=True)
 

@@ -17,7 +17,8 @@ struct cgd_ctx {
   cgd_desc desc{};
   KArgs ka{};
   int N = 0, NN = 0, Nz = 0, NF = 0, NumG = 0;
-  int nzt = 0, nchunk = 0, block = 0;
+  int nzt = 0, nchunk = 0, block = 0;     // pass 2 (and the chunk-interface side sums)
+  int nzt1 = 0, nchunk1 = 0, block1 = 0;  // pass 1
   int thomas = 0;  // column solve: 0 = cyclic reduction, 1 = serial Thomas (CGD_THOMAS=1)
   int ncolor = 0;
   std::vector<int> color_off;  // first element slot of each colour (elements are stored in colour order)
@@ -42,6 +43,7 @@ struct cgd_ctx {
   // column launch shape of the fused stage kernel: lanes per column (64,
   // or 32 for two columns per wavefront when Nz <= 32) and workgroup size
   int colw = 64, colblock = 256;
+  int items1 = 1;  // pass-1 work items (element, level chunk) per workgroup: 1 or 2
   // captured step graph (cgd_rw_graph_create)
   hipStream_t gstream = nullptr;
   hipGraph_t graph = nullptr;
@@ -61,7 +63,7 @@ struct cgd_ctx {
 // and tile sets of 5 and 4; pass 2 an FP64 D and pressure tile, then D, DW,
 // two sets of 7 tiles and the per-node horizontal metric
 template <typename R> static size_t lds_fcn1(const cgd_ctx *c) {
-  return sizeof(R) * (2 * c->NN + 9 * c->NN * c->nzt);
+  return sizeof(R) * (2 * c->NN + 9 * c->NN * c->nzt1);
 }
 template <typename R> static size_t lds_fcn2(const cgd_ctx *c) {
   // + a 12-entry per-node table of the horizontal metric (compact layout)
@@ -327,13 +329,14 @@ static int ctx_build(cgd_ctx *c, const cgd_desc *ds, int nmem, hipStream_t st) {
       const int o = atoi(ev);
       if (o == 16 && c->N == 5) nzt = o;
     }
-    c->nzt = nzt;
-    c->nchunk = (Nz + nzt - 1) / nzt;
+    c->nzt = c->nzt1 = nzt;
+    c->nchunk = c->nchunk1 = (Nz + nzt - 1) / nzt;
     if (const char *ev = getenv("CGD_THOMAS")) c->thomas = atoi(ev) != 0;
   }
-  c->block = ((NN * c->nzt + 63) / 64) * 64;
+  c->block = c->block1 = ((NN * c->nzt + 63) / 64) * 64;
   c->colw = 64;
   c->colblock = 256;
+  c->items1 = 1;
 
   // global numbering -> 0-based int32 in batch numbering, range checks, and
   // each member's colouring
@@ -641,9 +644,13 @@ extern "C" int cgd_profile_read(cgd_ctx *c, int which, double *total_ms, long lo
 // template dispatch on (N, chunk height, metric layout); R is the tendency
 // type (double, or float in the mixed-precision mode)
 template <int N, int NZT, bool CM, typename R>
-static void fcn1_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const R *UR, R *T1) {
-  hipLaunchKernelGGL((k_fcn1<N, NZT, CM, R>), g, dim3(c->block), lds_fcn1<R>(c), st, c->ka, slot0, c->nchunk, UR,
-                     T1);
+static void fcn1_t(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const R *UR, R *T1) {
+  if (c->items1 == 2)
+    hipLaunchKernelGGL((k_fcn1<N, NZT, CM, R, 2>), dim3((nitems + 1) / 2), dim3(c->block1), lds_fcn1<R>(c), st,
+                       c->ka, slot0, c->nchunk1, nitems, UR, T1);
+  else
+    hipLaunchKernelGGL((k_fcn1<N, NZT, CM, R, 1>), dim3(nitems), dim3(c->block1), lds_fcn1<R>(c), st, c->ka, slot0,
+                       c->nchunk1, nitems, UR, T1);
 }
 template <int N, int NZT, bool CM, typename R>
 static void fcn2_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, const R *UR, const R *T1,
@@ -651,9 +658,9 @@ static void fcn2_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *
   hipLaunchKernelGGL((k_fcn2<N, NZT, CM, R>), g, dim3(c->block), lds_fcn2<R>(c), st, c->ka, slot0, c->nchunk, U,
                      UR, T1, P, F, reinterpret_cast<R *>(c->d_f4x));
 }
-#define CGD_DISPATCH(FN, R, ...)                                   \
+#define CGD_DISPATCH(FN, R, NZTV, ...)                             \
   do {                                                             \
-    if (c->N == 5 && c->nzt == 16) {                               \
+    if (c->N == 5 && (NZTV) == 16) {                               \
       if (c->cm) FN<5, 16, true, R>(c, __VA_ARGS__);               \
       else FN<5, 16, false, R>(c, __VA_ARGS__);                    \
     } else if (c->N == 5) {                                        \
@@ -665,13 +672,13 @@ static void fcn2_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *
     }                                                              \
   } while (0)
 template <typename R>
-static void launch_fcn1(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const R *UR, R *T1) {
-  CGD_DISPATCH(fcn1_t, R, g, st, slot0, UR, T1);
+static void launch_fcn1(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const R *UR, R *T1) {
+  CGD_DISPATCH(fcn1_t, R, c->nzt1, nitems, st, slot0, UR, T1);
 }
 template <typename R>
 static void launch_fcn2(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *U, const R *UR, const R *T1,
                         const double *P, R *F) {
-  CGD_DISPATCH(fcn2_t, R, g, st, slot0, U, UR, T1, P, F);
+  CGD_DISPATCH(fcn2_t, R, c->nzt, g, st, slot0, U, UR, T1, P, F);
 }
 
 extern "C" int cgd_prepare(cgd_ctx *c, const double *U, double *AuxG, void *stream) {
@@ -699,7 +706,7 @@ static int run_fcn(cgd_ctx *c, R *F, const double *U, const R *UR, R *Temp1, con
   for (int k = 0; k < c->ncolor; ++k) {
     const int ne = c->color_off[k + 1] - c->color_off[k];
     if (!ne) continue;
-    launch_fcn1<R>(c, dim3(ne * c->nchunk), st, c->color_off[k], UR, Temp1);
+    launch_fcn1<R>(c, ne * c->nchunk1, st, c->color_off[k], UR, Temp1);
     HIPCHK(c, hipGetLastError());
     ++launches;
   }
@@ -783,7 +790,8 @@ static void launch_stage(cgd_ctx *c, int is, const RWStage<R> &rw, hipStream_t s
 // share the fV work buffer and the FP32 cache sits in Temp1.
 template <typename R>
 static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, hipStream_t st, double *const k[3],
-                         const double A[3][3], const double Cc[3][3], const double Mw[3], double gamma) {
+                         const double A[3][3], const double Cc[3][3], const double Mw[3], double gamma, bool begin,
+                         bool more) {
   int r;
   const size_t ng = (size_t)c->NumG * c->Nz, n5 = 5 * ng;
   const bool mixed = std::is_same<R, float>::value;
@@ -791,13 +799,17 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
   R *T1 = reinterpret_cast<R *>(w->Temp1);
   float *V32 = mixed ? reinterpret_cast<float *>(w->fV) + n5 : nullptr;
   const R *UR = mixed ? reinterpret_cast<const R *>(V32) : reinterpret_cast<const R *>(V);
-  hipLaunchKernelGGL(k_rw_begin, dim3(grid_for((ng + 3) / 4, 256)), dim3(256), 0, st, c->ka, V, w->AuxG, V32, ng);
-  HIPCHK(c, hipGetLastError());
+  // the first pressure (and FP32 shadow) of V, unless the previous step of
+  // a run wrote them with its final state
+  if (begin) {
+    hipLaunchKernelGGL(k_rw_begin, dim3(grid_for((ng + 3) / 4, 256)), dim3(256), 0, st, c->ka, V, w->AuxG, V32, ng);
+    HIPCHK(c, hipGetLastError());
+  }
   for (int is = 0; is < 3; ++is) {
     if ((r = run_fcn<R>(c, F, V, UR, T1, w->AuxG, st, false))) return r;
     RWStage<R> rw{};
     rw.fV = F;
-    rw.V32 = is < 2 ? V32 : nullptr;
+    rw.V32 = (is < 2 || more) ? V32 : nullptr;
     // the stage increments are stored as R (FP32 in the mixed mode: they
     // are ~dt*F, so their rounding is ~1e-7 of an increment)
     R *kR[3] = {reinterpret_cast<R *>(k[0]), reinterpret_cast<R *>(w->k) + n5, reinterpret_cast<R *>(w->k) + 2 * n5};
@@ -811,7 +823,7 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
     rw.dz = c->dz;
     rw.V = V;
     rw.k = is < 2 ? kR[is] : nullptr;  // k_3 is consumed in registers
-    rw.P = is < 2 ? w->AuxG : nullptr;
+    rw.P = (is < 2 || more) ? w->AuxG : nullptr;
     rw.F4x = reinterpret_cast<const R *>(c->d_f4x);
     rw.nptr = c->d_nptr;
     rw.ncopy = c->d_ncopy;
@@ -830,13 +842,16 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
   return CGD_OK;
 }
 
-extern "C" int cgd_rw_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, void *stream) {
+// nsteps SSP-Knoth steps; with the fused path, steps after the first take
+// the pressure (and FP32 shadow) that the previous step's last stage wrote
+static int rw_run(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, int nsteps, void *stream) {
   int r = check(c);
   if (r) return r;
   if (!V || !w || !w->Vn || !w->fV || !w->k || !w->AuxG || !w->Temp1 || !w->JRhoW || !w->JWRho || !w->JWRhoTh ||
       !w->JRhoThW || !w->tri)
     return fail(c, CGD_ERR_ARG, "cgd_rw_step: null array");
   if (!(dt > 0.0)) return fail(c, CGD_ERR_ARG, "cgd_rw_step: dt must be positive");
+  if (nsteps < 1) return fail(c, CGD_ERR_ARG, "cgd_rw_run: nsteps must be positive");
   hipStream_t st = S(stream);
   // SSP-Knoth: a = alpha/Gamma, c = -inv(Gamma), m = Gamma'\b, gamma = 1
   const double A[3][3] = {{0, 0, 0}, {1.0, 0, 0}, {0.25, 0.25, 0}};
@@ -847,32 +862,47 @@ extern "C" int cgd_rw_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *
   const size_t ng = (size_t)c->NumG * c->Nz, n5 = 5 * ng;
   double *k[3] = {w->k, w->k + n5, w->k + 2 * n5};
   if (c->Nz <= 64) {
-    if (c->mixed) return rw_step_fused<float>(c, V, dt, w, st, k, A, Cc, Mw, gamma);
-    return rw_step_fused<double>(c, V, dt, w, st, k, A, Cc, Mw, gamma);
+    for (int s = 0; s < nsteps; ++s) {
+      const bool begin = s == 0, more = s + 1 < nsteps;
+      r = c->mixed ? rw_step_fused<float>(c, V, dt, w, st, k, A, Cc, Mw, gamma, begin, more)
+                   : rw_step_fused<double>(c, V, dt, w, st, k, A, Cc, Mw, gamma, begin, more);
+      if (r) return r;
+    }
+    return CGD_OK;
   }
   // tall columns: the reference's sequence of calls, stage terms by k_axpy
-  HIPCHK(c, hipMemcpyAsync(w->Vn, V, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
-  for (int is = 0; is < 3; ++is) {
-    if ((r = cgd_prepare(c, V, w->AuxG, stream))) return r;
-    if ((r = cgd_fcn(c, w->fV, V, w->Temp1, w->AuxG, stream))) return r;
-    if (is == 0 && (r = cgd_jac(c, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, V, w->AuxG, stream))) return r;
-    const int gb = grid_for(n5, 256);
-    if (is > 0) {
-      hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, w->fV, w->fV, k[0], Cc[is][0] / dt, k[1],
-                         Cc[is][1] / dt, is, n5);
+  for (int s = 0; s < nsteps; ++s) {
+    HIPCHK(c, hipMemcpyAsync(w->Vn, V, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    for (int is = 0; is < 3; ++is) {
+      if ((r = cgd_prepare(c, V, w->AuxG, stream))) return r;
+      if ((r = cgd_fcn(c, w->fV, V, w->Temp1, w->AuxG, stream))) return r;
+      if (is == 0 && (r = cgd_jac(c, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, V, w->AuxG, stream))) return r;
+      const int gb = grid_for(n5, 256);
+      if (is > 0) {
+        hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, w->fV, w->fV, k[0], Cc[is][0] / dt, k[1],
+                           Cc[is][1] / dt, is, n5);
+        HIPCHK(c, hipGetLastError());
+      }
+      if ((r = cgd_schur(c, k[is], w->fV, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma, is == 0,
+                         stream)))
+        return r;
+      if (is < 2)
+        hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], A[is + 1][0], k[1], A[is + 1][1],
+                           is + 1, n5);
+      else
+        hipLaunchKernelGGL(k_axpy3, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], Mw[0], k[1], Mw[1], k[2], Mw[2], n5);
       HIPCHK(c, hipGetLastError());
     }
-    if ((r = cgd_schur(c, k[is], w->fV, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma, is == 0,
-                       stream)))
-      return r;
-    if (is < 2)
-      hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], A[is + 1][0], k[1], A[is + 1][1],
-                         is + 1, n5);
-    else
-      hipLaunchKernelGGL(k_axpy3, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], Mw[0], k[1], Mw[1], k[2], Mw[2], n5);
-    HIPCHK(c, hipGetLastError());
   }
   return CGD_OK;
+}
+
+extern "C" int cgd_rw_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, void *stream) {
+  return rw_run(c, V, dt, w, 1, stream);
+}
+
+extern "C" int cgd_rw_run(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, int nsteps, void *stream) {
+  return rw_run(c, V, dt, w, nsteps, stream);
 }
 
 // ------------------------------------------------------------------------
@@ -888,42 +918,60 @@ static void graph_free(cgd_ctx *c) {
 // Admissible shapes: chunk heights 8 (and 16 for N = 5); 64 lanes per
 // column, or 32 when Nz <= 32 and the solve is cyclic reduction; stage
 // workgroups of 64..256 threads in whole wavefronts.
-static bool shape_ok(const cgd_ctx *c, int nzt, int lanes, int block) {
-  if (!(nzt == 8 || (nzt == 16 && c->N == 5))) return false;
-  if (!(lanes == 64 || (lanes == 32 && c->Nz <= 32 && !c->thomas))) return false;
-  return block >= 64 && block <= 256 && block % 64 == 0;
+static bool shape_ok(const cgd_ctx *c, const cgd_launch_shape &sh) {
+  auto chunk_ok = [&](int z) { return z == 8 || (z == 16 && c->N == 5); };
+  if (!chunk_ok(sh.chunk_levels) || !chunk_ok(sh.pass1_chunk_levels)) return false;
+  if (!(sh.pass1_items == 1 || sh.pass1_items == 2)) return false;
+  if (!(sh.col_lanes == 64 || (sh.col_lanes == 32 && c->Nz <= 32 && !c->thomas))) return false;
+  return sh.col_block >= 64 && sh.col_block <= 256 && sh.col_block % 64 == 0;
 }
 
-static void apply_shape(cgd_ctx *c, int nzt, int lanes, int block) {
-  c->nzt = nzt;
-  c->nchunk = (c->Nz + nzt - 1) / nzt;
-  c->block = ((c->NN * nzt + 63) / 64) * 64;
-  c->ka.nzt = nzt;
-  c->colw = lanes;
-  c->colblock = block;
+static cgd_launch_shape current_shape(const cgd_ctx *c) {
+  cgd_launch_shape sh;
+  sh.chunk_levels = c->nzt;
+  sh.pass1_chunk_levels = c->nzt1;
+  sh.pass1_items = c->items1;
+  sh.col_lanes = c->colw;
+  sh.col_block = c->colblock;
+  return sh;
 }
 
-extern "C" int cgd_set_launch_shape(cgd_ctx *c, int chunk_levels, int col_lanes, int col_block) {
+static void apply_shape(cgd_ctx *c, const cgd_launch_shape &sh) {
+  c->nzt = sh.chunk_levels;
+  c->nchunk = (c->Nz + c->nzt - 1) / c->nzt;
+  c->block = ((c->NN * c->nzt + 63) / 64) * 64;
+  c->ka.nzt = c->nzt;
+  c->nzt1 = sh.pass1_chunk_levels;
+  c->nchunk1 = (c->Nz + c->nzt1 - 1) / c->nzt1;
+  c->block1 = ((c->NN * c->nzt1 + 63) / 64) * 64;
+  c->items1 = sh.pass1_items;
+  c->colw = sh.col_lanes;
+  c->colblock = sh.col_block;
+}
+
+extern "C" int cgd_set_launch_shape(cgd_ctx *c, const cgd_launch_shape *shape) {
   int r = check(c);
   if (r) return r;
-  const int nzt = chunk_levels ? chunk_levels : c->nzt;
-  const int lanes = col_lanes ? col_lanes : c->colw;
-  const int block = col_block ? col_block : c->colblock;
-  if (!shape_ok(c, nzt, lanes, block))
+  if (!shape) return fail(c, CGD_ERR_ARG, "cgd_set_launch_shape: null shape");
+  cgd_launch_shape sh = current_shape(c);
+  if (shape->chunk_levels) sh.chunk_levels = shape->chunk_levels;
+  if (shape->pass1_chunk_levels) sh.pass1_chunk_levels = shape->pass1_chunk_levels;
+  if (shape->pass1_items) sh.pass1_items = shape->pass1_items;
+  if (shape->col_lanes) sh.col_lanes = shape->col_lanes;
+  if (shape->col_block) sh.col_block = shape->col_block;
+  if (!shape_ok(c, sh))
     return fail(c, CGD_ERR_ARG,
-                "launch shape: chunk_levels must be 8 (or 16 for N = 5), col_lanes 64 (or 32 when Nz <= 32 "
-                "without the Thomas solve), col_block a multiple of 64 in [64, 256]");
+                "launch shape: chunk heights must be 8 (or 16 for N = 5), pass1_items 1 or 2, col_lanes 64 (or 32 "
+                "when Nz <= 32 without the Thomas solve), col_block a multiple of 64 in [64, 256]");
   graph_free(c);  // a captured graph keeps the old shapes
-  apply_shape(c, nzt, lanes, block);
+  apply_shape(c, sh);
   return CGD_OK;
 }
 
-extern "C" int cgd_get_launch_shape(const cgd_ctx *c, int *chunk_levels, int *col_lanes, int *col_block) {
-  if (!c) return CGD_ERR_ARG;
+extern "C" int cgd_get_launch_shape(const cgd_ctx *c, cgd_launch_shape *shape) {
+  if (!c || !shape) return CGD_ERR_ARG;
   if (!c->ok) return CGD_ERR_STATE;
-  if (chunk_levels) *chunk_levels = c->nzt;
-  if (col_lanes) *col_lanes = c->colw;
-  if (col_block) *col_block = c->colblock;
+  *shape = current_shape(c);
   return CGD_OK;
 }
 
@@ -939,7 +987,7 @@ extern "C" int cgd_rw_graph_create(cgd_ctx *c, double *V, double dt, const cgd_r
   // is ordered before the replays by hipGraphLaunch on that stream
   HIPCHK(c, hipStreamSynchronize(S(stream)));
   HIPCHK(c, hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal));
-  for (int s = 0; s < nsteps && !r; ++s) r = cgd_rw_step(c, V, dt, w, c->gstream);
+  r = rw_run(c, V, dt, w, nsteps, c->gstream);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(c->gstream, &g);
   if (r || e != hipSuccess) {
@@ -961,7 +1009,7 @@ extern "C" int cgd_rw_graph_launch(cgd_ctx *c, void *stream) {
 }
 
 extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, int reps, void *stream,
-                            int *shape, double *best_ms) {
+                            cgd_launch_shape *chosen, double *best_ms) {
   int r = check(c);
   if (r) return r;
   if (!V || !w) return fail(c, CGD_ERR_ARG, "cgd_autotune: null array");
@@ -973,35 +1021,59 @@ extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work 
   const size_t n5 = (size_t)5 * c->NumG * c->Nz;
   double *Vsave = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  int bn = c->nzt, bl = c->colw, bb = c->colblock;
+  cgd_launch_shape best = current_shape(c);
   double bms = 1e300;
   auto run = [&]() -> int {
     HIPCHK(c, hipMalloc(&Vsave, n5 * sizeof(double)));
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     HIPCHK(c, hipMemcpyAsync(Vsave, V, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
-    const int nzts[2] = {8, 16}, lanes[2] = {64, 32}, blocks[3] = {256, 128, 64};
+    auto try_shape = [&](const cgd_launch_shape &sh) -> int {
+      if (!shape_ok(c, sh)) return CGD_OK;
+      apply_shape(c, sh);
+      int rr;
+      HIPCHK(c, hipMemcpyAsync(V, Vsave, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
+      if ((rr = cgd_rw_step(c, V, dt, w, stream))) return rr;  // warm-up
+      // the fastest of three timings of reps steps (robust to clock ramps
+      // and other work on the device)
+      double per = 1e300;
+      for (int t = 0; t < 3; ++t) {
+        HIPCHK(c, hipEventRecord(e0, st));
+        for (int q = 0; q < reps; ++q)
+          if ((rr = cgd_rw_step(c, V, dt, w, stream))) return rr;
+        HIPCHK(c, hipEventRecord(e1, st));
+        HIPCHK(c, hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+        per = std::min(per, (double)ms / reps);
+      }
+      if (per < bms) {
+        bms = per;
+        best = sh;
+      }
+      return CGD_OK;
+    };
+    // the element kernels' shapes jointly (both passes' chunk heights and
+    // the pass-1 items), then the column shape with the best of them
     int rr;
-    for (int a : nzts)
-      for (int l : lanes)
-        for (int b : blocks) {
-          if (!shape_ok(c, a, l, b)) continue;
-          apply_shape(c, a, l, b);
-          HIPCHK(c, hipMemcpyAsync(V, Vsave, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
-          if ((rr = cgd_rw_step(c, V, dt, w, stream))) return rr;  // warm-up
-          HIPCHK(c, hipEventRecord(e0, st));
-          for (int q = 0; q < reps; ++q)
-            if ((rr = cgd_rw_step(c, V, dt, w, stream))) return rr;
-          HIPCHK(c, hipEventRecord(e1, st));
-          HIPCHK(c, hipEventSynchronize(e1));
-          float ms = 0;
-          HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
-          const double per = ms / reps;
-          if (per < bms) {
-            bms = per;
-            bn = a, bl = l, bb = b;
-          }
+    const cgd_launch_shape b0 = best;
+    for (int z2 : {8, 16})
+      for (int z1 : {8, 16})
+        for (int it : {1, 2}) {
+          cgd_launch_shape sh = b0;
+          sh.chunk_levels = z2;
+          sh.pass1_chunk_levels = z1;
+          sh.pass1_items = it;
+          if ((rr = try_shape(sh))) return rr;
         }
+    const cgd_launch_shape b1 = best;
+    for (int l : {64, 32})
+      for (int b : {256, 128, 64}) {
+        cgd_launch_shape sh = b1;
+        sh.col_lanes = l;
+        sh.col_block = b;
+        if ((rr = try_shape(sh))) return rr;
+      }
     return CGD_OK;
   };
   r = run();
@@ -1016,14 +1088,10 @@ extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work 
   if (Vsave) (void)hipFree(Vsave);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  apply_shape(c, bn, bl, bb);
+  apply_shape(c, best);
   c->prof = prof;
   if (r) return r;
-  if (shape) {
-    shape[0] = bn;
-    shape[1] = bl;
-    shape[2] = bb;
-  }
+  if (chosen) *chosen = best;
   if (best_ms) *best_ms = bms;
   return CGD_OK;
 }

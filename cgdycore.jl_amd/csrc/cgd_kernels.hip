@@ -243,82 +243,101 @@ template <int N, typename R> __device__ __forceinline__ void load_dmats(const KA
 
 // ------------------------------------------------------------------------
 // Pass 1: HyperViscKernel! (src/GPU/ViscKernel.jl:1-84).
-// Grid: one workgroup per (element of the colour, level chunk).
+// Grid: one workgroup per ITEMS consecutive work items of the colour, an
+// item being one (element, level chunk).  With ITEMS = 2 the loads of both
+// items are issued before the first item's derivative phases, so a wave has
+// twice the memory requests in flight (the kernel is latency-bound: its
+// waves mostly wait on memory).  nitems = elements of the colour x nchunk.
 // R is the arithmetic and storage type of the state it reads (U: the state,
 // or its FP32 shadow), of Temp1 and of the LDS tiles: double, or float in
 // the mixed-precision mode.
+template <typename R> struct Pass1In {
+  R S11, S12, S21, S22, Js, iMs;
+  R rho, u, v, rth;
+  R o1, o2, o3, o4, o5;
+  unsigned ui;
+  bool first, valid;
+};
+
 template <int N, int NZT, bool CM, typename R>
-__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, int slot0, int nchunk,
-                                               const R *__restrict__ U, R *__restrict__ T1) {
+__device__ __forceinline__ Pass1In<R> pass1_load(const KArgs &a, int slot0, int nchunk, int nitems, int L,
+                                                 const R *__restrict__ U, const R *__restrict__ T1) {
   constexpr int NN = N * N;
-  extern __shared__ double smem[];
-  R *Ds = reinterpret_cast<R *>(smem), *DWs = Ds + NN;
-  // two tile sets used alternately, so a phase's tiles are never rewritten
-  // before every lane has passed the next phase's barrier
-  Tiles<N, NZT, R> T{Ds + 2 * NN}, TB{Ds + 2 * NN + 5 * NN * NZT};
-  load_dmats<N>(a, Ds, DWs);
   const int Nz = a.Nz;
-  constexpr int nzt = NZT;
-  const int L = xcd_logical(blockIdx.x, gridDim.x);
   const int e = slot0 + L / nchunk;  // element slot (elements are stored in colour order)
   const int chunk = L - (L / nchunk) * nchunk;
   const int t = threadIdx.x;
-  const int node = t / nzt, l = t - node * nzt;
-  const int iz = chunk * nzt + l;
-  const bool in_tile = node < NN;
-  const bool valid = in_tile && iz < Nz;
-  const int I = node % N, J = node / N;
+  const int node = t / NZT, l = t - node * NZT;
+  const int iz = chunk * NZT + l;
   const size_t vs = (size_t)a.NumG * Nz;
   const size_t NNZ = (size_t)NN * Nz;
-
+  Pass1In<R> in{};
+  in.valid = L < nitems && node < NN && iz < Nz;
+  in.Js = 1;
+  in.iMs = 1;
+  if (!in.valid) return in;
   // uniform bases + 32-bit lane offsets (the host checks NumG*Nz < 2^28)
-  const double *metE = a.met1 + (size_t)e * CGD_NMET1 * NNZ;
-  const R *iMsR = std::is_same<R, double>::value ? reinterpret_cast<const R *>(a.iMs)
-                                                  : reinterpret_cast<const R *>(a.iMs32);
-  R S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1, iMs = 1;
-  bool first = false;
-  unsigned ui = 0;
-  R o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0;
-  R uC = 0, vC = 0, uD = 0, vD = 0, Th = 0;
-  if (valid) {
-    const int gv = a.glob[e * NN + node];
-    first = gv < 0;
-    ui = (unsigned)(gv & 0x7fffffff) * (unsigned)Nz + iz;
-    __builtin_assume(ui < (1u << 28));
+  const int gv = a.glob[e * NN + node];
+  in.first = gv < 0;
+  in.ui = (unsigned)(gv & 0x7fffffff) * (unsigned)Nz + iz;
+  __builtin_assume(in.ui < (1u << 28));
+  if constexpr (CM) {
+    const double *mN = a.met1n + (size_t)e * CGD_NMET1 * NN;
+    const unsigned nd = (unsigned)node;
+    in.S11 = R(at(mN, nd));
+    in.S12 = R(at(mN + NN, nd));
+    in.S21 = R(at(mN + 2 * NN, nd));
+    in.S22 = R(at(mN + 3 * NN, nd));
+    in.Js = R(at(mN + 4 * NN, nd));
+  } else {
+    const double *metE = a.met1 + (size_t)e * CGD_NMET1 * NNZ;
     const unsigned mo = (unsigned)(node * Nz + iz);
     __builtin_assume(mo < (1u << 28));
-    if constexpr (CM) {
-      const double *mN = a.met1n + (size_t)e * CGD_NMET1 * NN;
-      const unsigned nd = (unsigned)node;
-      S11 = R(at(mN, nd));
-      S12 = R(at(mN + NN, nd));
-      S21 = R(at(mN + 2 * NN, nd));
-      S22 = R(at(mN + 3 * NN, nd));
-      Js = R(at(mN + 4 * NN, nd));
-    } else {
-      S11 = R(at(metE, mo));
-      S12 = R(at(metE + NNZ, mo));
-      S21 = R(at(metE + 2 * NNZ, mo));
-      S22 = R(at(metE + 3 * NNZ, mo));
-      Js = R(at(metE + 4 * NNZ, mo));
-    }
-    iMs = at(iMsR, ui);
-    // the DSS accumulators of this node: no other element of the colour
-    // touches them, so they are read now, with the state, not after the
-    // derivative phases
-    if (!first) {
-      o1 = at(T1, ui);
-      o2 = at(T1 + vs, ui);
-      o3 = at(T1 + 2 * vs, ui);
-      o4 = at(T1 + 3 * vs, ui);
-      o5 = at(T1 + 4 * vs, ui);
-    }
-    const R rho = at(U, ui), u = at(U + vs, ui), v = at(U + 2 * vs, ui), rth = at(U + 4 * vs, ui);
-    uC = S11 * v - S12 * u;  // Curl12
-    vC = S21 * v - S22 * u;
-    uD = S11 * u + S12 * v;  // Contra12
-    vD = S21 * u + S22 * v;
-    Th = rth * rcp(rho);
+    in.S11 = R(at(metE, mo));
+    in.S12 = R(at(metE + NNZ, mo));
+    in.S21 = R(at(metE + 2 * NNZ, mo));
+    in.S22 = R(at(metE + 3 * NNZ, mo));
+    in.Js = R(at(metE + 4 * NNZ, mo));
+  }
+  const R *iMsR = std::is_same<R, double>::value ? reinterpret_cast<const R *>(a.iMs)
+                                                  : reinterpret_cast<const R *>(a.iMs32);
+  in.iMs = at(iMsR, in.ui);
+  // the DSS accumulators of this node: no other element of the colour
+  // touches them, so they are read now, with the state, not after the
+  // derivative phases
+  if (!in.first) {
+    in.o1 = at(T1, in.ui);
+    in.o2 = at(T1 + vs, in.ui);
+    in.o3 = at(T1 + 2 * vs, in.ui);
+    in.o4 = at(T1 + 3 * vs, in.ui);
+    in.o5 = at(T1 + 4 * vs, in.ui);
+  }
+  in.rho = at(U, in.ui);
+  in.u = at(U + vs, in.ui);
+  in.v = at(U + 2 * vs, in.ui);
+  in.rth = at(U + 4 * vs, in.ui);
+  return in;
+}
+
+// the derivative phases of one item and its DSS write; every lane of the
+// workgroup calls it (two barriers)
+template <int N, int NZT, typename R>
+__device__ __forceinline__ void pass1_compute(const KArgs &a, const Pass1In<R> &in, const R *Ds, const R *DWs,
+                                              const Tiles<N, NZT, R> &T, const Tiles<N, NZT, R> &TB,
+                                              R *__restrict__ T1) {
+  constexpr int NN = N * N;
+  const int t = threadIdx.x;
+  const int node = t / NZT, l = t - node * NZT;
+  const bool in_tile = node < NN;
+  const int I = node % N, J = node / N;
+  const size_t vs = (size_t)a.NumG * a.Nz;
+  R uC = 0, vC = 0, uD = 0, vD = 0, Th = 0;
+  if (in.valid) {
+    uC = in.S11 * in.v - in.S12 * in.u;  // Curl12
+    vC = in.S21 * in.v - in.S22 * in.u;
+    uD = in.S11 * in.u + in.S12 * in.v;  // Contra12
+    vD = in.S21 * in.u + in.S22 * in.v;
+    Th = in.rth * rcp(in.rho);
   }
   if (in_tile) {
     T.at(0, node, l) = uC;
@@ -329,16 +348,16 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
   }
   __syncthreads();
   R Curl = 0, Div = 0, ThCx = 0, ThCy = 0;
-  if (valid) {
+  if (in.valid) {
     const R Dxc = dx<N, NZT, R>(Ds, T, 4, I, J, l);
     const R Dyc = dy<N, NZT, R>(Ds, T, 4, I, J, l);
-    const R iJs = rcp(Js);
+    const R iJs = rcp(in.Js);
     Curl = dxy<N, NZT, R>(Ds, T, 0, 1, I, J, l) * iJs;
     Div = dxy<N, NZT, R>(Ds, T, 2, 3, I, J, l) * iJs;
-    const R GDx = (S11 * Dxc + S21 * Dyc) * iJs;  // Grad12 with J
-    const R GDy = (S12 * Dxc + S22 * Dyc) * iJs;
-    ThCx = S11 * GDx + S12 * GDy;  // Contra12
-    ThCy = S21 * GDx + S22 * GDy;
+    const R GDx = (in.S11 * Dxc + in.S21 * Dyc) * iJs;  // Grad12 with J
+    const R GDy = (in.S12 * Dxc + in.S22 * Dyc) * iJs;
+    ThCx = in.S11 * GDx + in.S12 * GDy;  // Contra12
+    ThCy = in.S21 * GDx + in.S22 * GDy;
   }
   if (in_tile) {
     TB.at(0, node, l) = Curl;
@@ -347,25 +366,51 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
     TB.at(3, node, l) = ThCy;
   }
   __syncthreads();
-  if (valid) {
+  if (in.valid) {
     const R DxCurl = dx<N, NZT, R>(DWs, TB, 0, I, J, l);
     const R DyCurl = dy<N, NZT, R>(DWs, TB, 0, I, J, l);
     const R DxDiv = dx<N, NZT, R>(DWs, TB, 1, I, J, l);
     const R DyDiv = dy<N, NZT, R>(DWs, TB, 1, I, J, l);
     const R DivTh = dxy<N, NZT, R>(DWs, TB, 2, 3, I, J, l);
-    const R FuC = S12 * DxCurl + S22 * DyCurl;  // Rot12
-    const R FvC = -S11 * DxCurl - S21 * DyCurl;
-    const R FuD = S11 * DxDiv + S21 * DyDiv;  // Grad12
-    const R FvD = S12 * DxDiv + S22 * DyDiv;
+    const R FuC = in.S12 * DxCurl + in.S22 * DyCurl;  // Rot12
+    const R FvC = -in.S11 * DxCurl - in.S21 * DyCurl;
+    const R FuD = in.S11 * DxDiv + in.S21 * DyDiv;  // Grad12
+    const R FvD = in.S12 * DxDiv + in.S22 * DyDiv;
     // DSS: the first element (in colour order) holding the node writes,
     // the others accumulate
+    const R iMs = in.iMs;
     const R r1 = -FuC * iMs, r2 = -FvC * iMs, r3 = FuD * iMs, r4 = FvD * iMs, r5 = DivTh * iMs;
     R *T1a = T1, *T1b = T1 + vs, *T1c = T1 + 2 * vs, *T1d = T1 + 3 * vs, *T1e = T1 + 4 * vs;
-    at(T1a, ui) = first ? r1 : o1 + r1;
-    at(T1b, ui) = first ? r2 : o2 + r2;
-    at(T1c, ui) = first ? r3 : o3 + r3;
-    at(T1d, ui) = first ? r4 : o4 + r4;
-    at(T1e, ui) = first ? r5 : o5 + r5;
+    at(T1a, in.ui) = in.first ? r1 : in.o1 + r1;
+    at(T1b, in.ui) = in.first ? r2 : in.o2 + r2;
+    at(T1c, in.ui) = in.first ? r3 : in.o3 + r3;
+    at(T1d, in.ui) = in.first ? r4 : in.o4 + r4;
+    at(T1e, in.ui) = in.first ? r5 : in.o5 + r5;
+  }
+}
+
+template <int N, int NZT, bool CM, typename R, int ITEMS>
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, int slot0, int nchunk, int nitems,
+                                                                       const R *__restrict__ U,
+                                                                       R *__restrict__ T1) {
+  static_assert(ITEMS == 1 || ITEMS == 2, "one or two items per workgroup");
+  constexpr int NN = N * N;
+  extern __shared__ double smem[];
+  R *Ds = reinterpret_cast<R *>(smem), *DWs = Ds + NN;
+  // two tile sets used alternately, so a phase's tiles are never rewritten
+  // before every lane has passed the next phase's barrier (also across the
+  // items of a workgroup)
+  Tiles<N, NZT, R> T{Ds + 2 * NN}, TB{Ds + 2 * NN + 5 * NN * NZT};
+  load_dmats<N>(a, Ds, DWs);
+  const int L0 = xcd_logical(blockIdx.x, gridDim.x) * ITEMS;
+  if constexpr (ITEMS == 1) {
+    const Pass1In<R> in0 = pass1_load<N, NZT, CM, R>(a, slot0, nchunk, nitems, L0, U, T1);
+    pass1_compute<N, NZT, R>(a, in0, Ds, DWs, T, TB, T1);
+  } else {
+    const Pass1In<R> in0 = pass1_load<N, NZT, CM, R>(a, slot0, nchunk, nitems, L0, U, T1);
+    const Pass1In<R> in1 = pass1_load<N, NZT, CM, R>(a, slot0, nchunk, nitems, L0 + 1, U, T1);
+    pass1_compute<N, NZT, R>(a, in0, Ds, DWs, T, TB, T1);
+    pass1_compute<N, NZT, R>(a, in1, Ds, DWs, T, TB, T1);
   }
 }
 

@@ -56,6 +56,17 @@ class _Desc(C.Structure):
                 ("M", _vp), ("dz", _vp)]
 
 
+class LaunchShape(C.Structure):
+    """``cgd_launch_shape``: element chunk heights of both tendency passes,
+    pass-1 items per workgroup, lanes per column and stage workgroup size."""
+
+    _fields_ = [("chunk_levels", C.c_int), ("pass1_chunk_levels", C.c_int), ("pass1_items", C.c_int),
+                ("col_lanes", C.c_int), ("col_block", C.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class _RWWork(C.Structure):
     _fields_ = [(n, _vp) for n in ("Vn", "fV", "k", "AuxG", "Temp1", "JRhoW", "JWRho", "JWRhoTh",
                                    "JRhoThW", "tri")]
@@ -82,14 +93,15 @@ def native():
     L.cgd_jac.argtypes = [_vp] * 8
     L.cgd_schur.argtypes = [_vp] * 8 + [C.c_double, C.c_int, _vp]
     L.cgd_rw_step.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), _vp]
+    L.cgd_rw_run.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), C.c_int, _vp]
     L.cgd_profile_enable.argtypes = [_vp, C.c_int]
     L.cgd_profile_read.argtypes = [_vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]
     L.cgd_color_elements.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp]
     L.cgd_ctx_info.argtypes = [_vp, _vp, _vp, _vp]
     L.cgd_ctx_create_batch.argtypes = [C.POINTER(_Desc), C.c_int, C.POINTER(_vp), _vp]
-    L.cgd_set_launch_shape.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
-    L.cgd_get_launch_shape.argtypes = [_vp, _vp, _vp, _vp]
-    L.cgd_autotune.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), C.c_int, _vp, _vp,
+    L.cgd_set_launch_shape.argtypes = [_vp, C.POINTER(LaunchShape)]
+    L.cgd_get_launch_shape.argtypes = [_vp, C.POINTER(LaunchShape)]
+    L.cgd_autotune.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), C.c_int, _vp, C.POINTER(LaunchShape),
                                C.POINTER(C.c_double)]
     L.cgd_rw_graph_create.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), C.c_int, _vp]
     L.cgd_rw_graph_launch.argtypes = [_vp, _vp]
@@ -106,7 +118,7 @@ def exported_symbols():
             "cgd_schur", "cgd_rw_step", "cgd_profile_enable", "cgd_profile_read", "cgd_color_elements",
             "cgd_ctx_info", "cgd_ctx_create_batch", "cgd_set_launch_shape", "cgd_get_launch_shape",
             "cgd_autotune", "cgd_rw_graph_create", "cgd_rw_graph_launch", "cgd_setup_error",
-            "cgd_jacobi_sphere3", "cgd_mass_cg", "cgd_grid_size_sphere"]
+            "cgd_jacobi_sphere3", "cgd_mass_cg", "cgd_grid_size_sphere", "cgd_rw_run"]
 
 
 def color_elements(glob: np.ndarray, num_g: int):
@@ -316,24 +328,29 @@ class NativeContext:
         self.check(self.L.cgd_ctx_info(self.h, C.byref(nc), C.byref(nz), C.byref(cm)), "cgd_ctx_info")
         return nc.value, nz.value, bool(cm.value)
 
-    def set_launch_shape(self, chunk_levels=0, col_lanes=0, col_block=0):
-        """Element chunk height, lanes per column and stage workgroup size (0 keeps a value)."""
-        self.check(self.L.cgd_set_launch_shape(self.h, chunk_levels, col_lanes, col_block), "cgd_set_launch_shape")
+    def set_launch_shape(self, **fields):
+        """Set launch-shape fields by name (``LaunchShape``); others are kept."""
+        sh = LaunchShape()
+        for k, v in fields.items():
+            if k not in dict(LaunchShape._fields_):
+                raise CGDError(f"unknown launch-shape field {k!r}")
+            setattr(sh, k, int(v))
+        self.check(self.L.cgd_set_launch_shape(self.h, C.byref(sh)), "cgd_set_launch_shape")
 
-    def launch_shape(self):
-        a, b, c = C.c_int(), C.c_int(), C.c_int()
-        self.check(self.L.cgd_get_launch_shape(self.h, C.byref(a), C.byref(b), C.byref(c)), "cgd_get_launch_shape")
-        return a.value, b.value, c.value
+    def launch_shape(self) -> dict:
+        sh = LaunchShape()
+        self.check(self.L.cgd_get_launch_shape(self.h, C.byref(sh)), "cgd_get_launch_shape")
+        return sh.as_dict()
 
     def autotune(self, V, dt, Cache, JCache, reps=3):
-        """Time every launch shape on ``V`` (restored afterwards); keep the fastest.
-        Returns ((chunk_levels, col_lanes, col_block), ms per step)."""
-        shape = (C.c_int * 3)()
+        """Time candidate launch shapes on ``V`` (restored afterwards); keep the
+        fastest.  Returns (shape dict, ms per step)."""
+        sh = LaunchShape()
         ms = C.c_double()
         w = _rw_work(Cache, JCache)
-        self.check(self.L.cgd_autotune(self.h, _ptr(V), C.c_double(dt), C.byref(w), int(reps), _stream(), shape,
-                                       C.byref(ms)), "cgd_autotune")
-        return tuple(shape), ms.value
+        self.check(self.L.cgd_autotune(self.h, _ptr(V), C.c_double(dt), C.byref(w), int(reps), _stream(),
+                                       C.byref(sh), C.byref(ms)), "cgd_autotune")
+        return sh.as_dict(), ms.value
 
     def graph_create(self, V, dt, Cache, JCache, nsteps=1):
         """Capture ``nsteps`` Rosenbrock-W steps of ``V`` into a HIP graph."""
@@ -454,11 +471,17 @@ def RosenbrockSchur(V, dt, Fcn, FcnPrepare, Jac, CG, Metric, Phys, Cache, JCache
 
 
 def TimeStepper(U, dt, nIter, FE, Metric, Phys, Cache, JCache, Global, Exchange=None, Param=None):
-    """Rosenbrock branch of ``TimeStepper!`` (TimeStepper.jl:1-126) without output."""
-    eq = CompressibleShallow()
-    for _ in range(nIter):
-        RosenbrockSchur(U, dt, FcnGPU, FcnPrepareGPU, JacSchurGPU, FE, Metric, Phys, Cache, JCache, Exchange,
-                        Global, Param, eq)
+    """Rosenbrock branch of ``TimeStepper!`` (TimeStepper.jl:1-126) without
+    output: ``nIter`` SSP-Knoth steps in one ``cgd_rw_run`` call (each step
+    hands its final pressure to the next; bit-identical to stepping one call
+    at a time)."""
+    if nIter < 1:
+        return
+    ctx = _ctx(FE, Metric, Phys, Global)
+    _check_state(U, Global, 5, "U")
+    w = _rw_work(Cache, JCache)
+    ctx.check(ctx.L.cgd_rw_run(ctx.h, _ptr(U), C.c_double(dt), C.byref(w), int(nIter), _stream()), "TimeStepper")
+    JCache.CompTri = False
 
 
 # --------------------------------------------------------------------------

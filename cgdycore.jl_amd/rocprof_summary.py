@@ -35,7 +35,7 @@ def kernel_table(db):
 
 
 def short_name(name, width=60):
-    n = name.split("(")[0].replace("void ", "")
+    n = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "")
     return n if len(n) <= width else n[: width - 3] + "..."
 
 

@@ -16,7 +16,7 @@
  *
  * All calls are ordered on the caller's HIP stream (passed as void*) and
  * return CGD_OK (0) or a negative error code.  The step-path calls
- * (cgd_prepare, cgd_fcn, cgd_jac, cgd_schur, cgd_rw_step,
+ * (cgd_prepare, cgd_fcn, cgd_jac, cgd_schur, cgd_rw_step, cgd_rw_run,
  * cgd_rw_graph_launch) allocate nothing; only the set-up calls
  * (cgd_ctx_create*, cgd_autotune, cgd_rw_graph_create, cgd_profile_enable)
  * do.  A context is not thread-safe; calls on one context must be issued
@@ -132,6 +132,13 @@ typedef struct {
 } cgd_rw_work;
 int cgd_rw_step(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, void *stream);
 
+/* The Rosenbrock branch of TimeStepper! (src/Integration/TimeStepper.jl:105-109):
+ * nsteps calls of cgd_rw_step on one stream.  Each step but the last also
+ * writes the pressure (and, in the mixed mode, the FP32 shadow) of its final
+ * state, which the next step then uses instead of recomputing them; the
+ * result is bit-identical to nsteps separate cgd_rw_step calls. */
+int cgd_rw_run(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, int nsteps, void *stream);
+
 /* The schedule and layout cgd_ctx_create chose: the number of element
  * colours (DSS launches per pass), the level chunk height of the element
  * kernels, and whether the metric is stored in the compact layout (every
@@ -140,28 +147,34 @@ int cgd_rw_step(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, voi
 int cgd_ctx_info(const cgd_ctx *ctx, int *ncolor, int *chunk_levels, int *compact_metric);
 
 /* Launch shapes (no reference counterpart; the reference fixes its
- * workgroup shapes by NumberThreadGPU, src/DyCore/GlobalVariables.jl:240):
- * chunk_levels = levels per element workgroup (8, or 16 for N = 5);
- * col_lanes = lanes per column in the Rosenbrock stage kernel (64, or 32 so
- * that two columns share a wavefront when Nz <= 32); col_block = threads per
- * stage workgroup (64, 128, 192 or 256).  0 keeps a value.  The workgroup
- * size changes no bit; the chunk height (chunk-interface w sums) and 32-lane
- * columns (the compiler's fused multiply-adds) change rounding only, within
- * the FP64 parity bound.  Setting a shape drops a captured step graph. */
-int cgd_set_launch_shape(cgd_ctx *ctx, int chunk_levels, int col_lanes, int col_block);
-int cgd_get_launch_shape(const cgd_ctx *ctx, int *chunk_levels, int *col_lanes, int *col_block);
+ * workgroup shapes by NumberThreadGPU, src/DyCore/GlobalVariables.jl:240).
+ * In cgd_set_launch_shape a field of 0 keeps the current value.  The
+ * workgroup sizes and pass1_items change no bit; the pass-2 chunk height
+ * (chunk-interface w sums) and 32-lane columns (the compiler's fused
+ * multiply-adds) change rounding only, within the FP64 parity bound.
+ * Setting a shape drops a captured step graph. */
+typedef struct {
+  int chunk_levels;       /* levels per workgroup of the fused tendency pass: 8, or 16 for N = 5 */
+  int pass1_chunk_levels; /* the same for the first (hyperviscosity) pass */
+  int pass1_items;        /* (element, chunk) items per pass-1 workgroup: 1 or 2 (loads of both in flight) */
+  int col_lanes;          /* lanes per column in the stage kernel: 64, or 32 (two columns per wavefront, Nz <= 32) */
+  int col_block;          /* threads per stage workgroup: 64, 128, 192 or 256 */
+} cgd_launch_shape;
+int cgd_set_launch_shape(cgd_ctx *ctx, const cgd_launch_shape *shape);
+int cgd_get_launch_shape(const cgd_ctx *ctx, cgd_launch_shape *shape);
 
-/* Autotuner: times reps steps of cgd_rw_step for every admissible launch
- * shape on the caller's V and work buffers and keeps the fastest.  V is
- * restored afterwards (the work buffers are scratch).  A set-up call: it
- * allocates a copy of V and synchronises the stream.  shape (3 ints, may be
- * null) receives the chosen chunk_levels, col_lanes, col_block, best_ms the
- * time per step.  No reference counterpart. */
-int cgd_autotune(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, int reps, void *stream, int *shape,
-                 double *best_ms);
+/* Autotuner: times cgd_rw_step per candidate launch shape on the caller's V
+ * and work buffers (the fastest of three timings of reps steps) and keeps
+ * the fastest: the element-kernel shapes jointly (both chunk heights and
+ * pass1_items), then the column shape with the best of them.  V is restored afterwards (the
+ * work buffers are scratch).  A set-up call: it allocates a copy of V and
+ * synchronises the stream.  chosen (may be null) receives the shape, best_ms
+ * the time per step.  No reference counterpart. */
+int cgd_autotune(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, int reps, void *stream,
+                 cgd_launch_shape *chosen, double *best_ms);
 
-/* Step graph: captures nsteps calls of cgd_rw_step(ctx, V, dt, work) into a
- * HIP graph held by the context (pointers, dt and launch shapes as at the
+/* Step graph: captures cgd_rw_run(ctx, V, dt, work, nsteps) into a HIP
+ * graph held by the context (pointers, dt and launch shapes as at the
  * capture); cgd_rw_graph_launch replays all of them with one launch on
  * stream, advancing V by nsteps steps.  Capture is a set-up call (it
  * synchronises stream and creates a capture stream); a replay allocates

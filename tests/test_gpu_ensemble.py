@@ -40,7 +40,7 @@ def _single_steps(pkg, torch, disc, seed, dt, nsteps, precision, shape=None):
     FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda", precision=precision)
     ctx = host.init_context(FE, Metric, phys, Global)
     if shape:
-        ctx.set_launch_shape(*shape)
+        ctx.set_launch_shape(**shape)
     V = torch.as_tensor(pkg.state.jet_state(disc, seed=seed), device="cuda").clone()
     host.TimeStepper(V, dt, nsteps, FE, Metric, phys, Cache, JCache, Global)
     torch.cuda.synchronize()
@@ -111,20 +111,24 @@ def test_column_shapes(pkg, torch_cuda, nz):
     ctx = ens.Global._ctx
     U0 = torch.as_tensor(pkg.state.ensemble_state(discs, seeds), device="cuda")
     out = {}
-    for shape in [(8, 64, 256), (8, 32, 256), (8, 32, 64), (8, 64, 128), (8, 32, 192)]:
-        ctx.set_launch_shape(*shape)
-        assert ctx.launch_shape() == shape
+    # (col_lanes, col_block, pass1_items, pass1_chunk_levels); the pass-2 chunk height stays 8
+    for shape in [(64, 256, 1, 8), (32, 256, 1, 8), (32, 64, 1, 8), (64, 128, 1, 8), (32, 192, 1, 8),
+                  (64, 256, 2, 8), (32, 256, 2, 16), (64, 64, 2, 16)]:
+        ctx.set_launch_shape(chunk_levels=8, col_lanes=shape[0], col_block=shape[1], pass1_items=shape[2],
+                             pass1_chunk_levels=shape[3])
+        assert ctx.launch_shape() == dict(chunk_levels=8, pass1_chunk_levels=shape[3], pass1_items=shape[2],
+                                          col_lanes=shape[0], col_block=shape[1])
         V = U0.clone()
         host.TimeStepper(V, 300.0, 2, ens.FE[0], ens.Metric[0], phys, ens.Cache, ens.JCache, ens.Global)
         torch.cuda.synchronize()
         out[shape] = V.cpu()
     for w in (64, 32):
-        ref = out[(8, w, 256)]
+        ref = out[(w, 256, 1, 8)]
         for shape, V in out.items():
-            if shape[1] == w:
-                assert torch.equal(V, ref), f"workgroup size of shape {shape} changed the result"
-    ref = out[(8, 64, 256)].numpy()
-    V = out[(8, 32, 256)].numpy()
+            if shape[0] == w:
+                assert torch.equal(V, ref), f"shape {shape} changed the result"
+    ref = out[(64, 256, 1, 8)].numpy()
+    V = out[(32, 256, 1, 8)].numpy()
     for c in range(5):
         assert rel_l2(V[c], ref[c]) <= 1e-13, f"V{c + 1} with 32-lane columns: {rel_l2(V[c], ref[c]):.3e}"
 
@@ -136,7 +140,7 @@ def test_chunk_height_within_parity(pkg, oracle_mod, torch_cuda):
     disc = pkg.discretization.build(3, 4, 32, 30000.0)
     FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda")
     ctx = host.init_context(FE, Metric, phys, Global)
-    ctx.set_launch_shape(16, 32, 128)
+    ctx.set_launch_shape(chunk_levels=16, pass1_chunk_levels=16, col_lanes=32, col_block=128)
     assert ctx.info()[1] == 16
     U0 = pkg.state.jet_state(disc, seed=4)
     V = torch.as_tensor(U0, device="cuda").clone()
@@ -157,10 +161,14 @@ def test_launch_shape_rejected(pkg, torch_cuda):
     disc = pkg.discretization.build(2, 4, 40, 30000.0)
     FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda")
     ctx = host.init_context(FE, Metric, phys, Global)
-    for bad in [(12, 0, 0), (0, 32, 0), (0, 16, 0), (0, 0, 96), (0, 0, 320)]:
+    for bad in [dict(chunk_levels=12), dict(pass1_chunk_levels=4), dict(col_lanes=32), dict(col_lanes=16),
+                dict(col_block=96), dict(col_block=320), dict(pass1_items=3)]:
         with pytest.raises(host.CGDError):
-            ctx.set_launch_shape(*bad)  # 32 lanes need Nz <= 32; 96 is not whole wavefronts
-    assert ctx.launch_shape() == (8, 64, 256)
+            ctx.set_launch_shape(**bad)  # 32 lanes need Nz <= 32; 96 is not whole wavefronts
+    with pytest.raises(host.CGDError):
+        ctx.set_launch_shape(colblock=64)
+    assert ctx.launch_shape() == dict(chunk_levels=8, pass1_chunk_levels=8, pass1_items=1, col_lanes=64,
+                                      col_block=256)
 
 
 def test_batch_rejects_mismatched_members(pkg, torch_cuda):
@@ -190,10 +198,11 @@ def test_autotune_keeps_state_and_parity(pkg, torch_cuda):
     print(f"  chosen {shape}, {ms:.3f} ms per step", flush=True)
     assert torch.equal(V, U0), "autotune must restore V"
     assert ctx.launch_shape() == shape and ms > 0
-    assert shape[0] in (8, 16) and shape[1] in (32, 64) and shape[2] in (64, 128, 256)
+    assert shape["chunk_levels"] in (8, 16) and shape["pass1_chunk_levels"] in (8, 16)
+    assert shape["col_lanes"] in (32, 64) and shape["col_block"] in (64, 128, 256) and shape["pass1_items"] in (1, 2)
     # the chosen shape steps like the default one up to the chunk-interface rounding
     host.TimeStepper(V, 150.0, 2, ens.FE[0], ens.Metric[0], phys, ens.Cache, ens.JCache, ens.Global)
-    ctx.set_launch_shape(8, 64, 256)
+    ctx.set_launch_shape(chunk_levels=8, pass1_chunk_levels=8, pass1_items=1, col_lanes=64, col_block=256)
     W = U0.clone()
     host.TimeStepper(W, 150.0, 2, ens.FE[0], ens.Metric[0], phys, ens.Cache, ens.JCache, ens.Global)
     torch.cuda.synchronize()
@@ -227,3 +236,24 @@ def test_graph_replays_equal_eager_steps(pkg, torch_cuda, precision):
         torch.cuda.synchronize()
         assert torch.equal(Gr, E), f"replay {rep} differs from the eager steps"
     assert torch.isfinite(G).all()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_run_equals_single_steps(pkg, torch_cuda, precision):
+    """cgd_rw_run (steps hand their final pressure and FP32 shadow on) gives
+    the bits of the same number of cgd_rw_step calls."""
+    torch = torch_cuda
+    host = pkg.host
+    phys = pkg.discretization.Phys()
+    disc = pkg.discretization.build(3, 4, 20, 30000.0)
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda", precision=precision)
+    eq = host.CompressibleShallow()
+    U0 = torch.as_tensor(pkg.state.jet_state(disc, seed=7), device="cuda")
+    A = U0.clone()
+    for _ in range(3):
+        host.RosenbrockSchur(A, 300.0, host.FcnGPU, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys, Cache,
+                             JCache, None, Global, None, eq)
+    B = U0.clone()
+    host.TimeStepper(B, 300.0, 3, FE, Metric, phys, Cache, JCache, Global)
+    torch.cuda.synchronize()
+    assert torch.equal(A, B)
