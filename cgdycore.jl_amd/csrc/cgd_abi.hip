@@ -20,6 +20,7 @@ struct cgd_ctx {
   int nzt = 0, nchunk = 0, block = 0;     // pass 2 (and the chunk-interface side sums)
   int nzt1 = 0, nchunk1 = 0, block1 = 0;  // pass 1
   int thomas = 0;  // column solve: 0 = cyclic reduction, 1 = serial Thomas (CGD_THOMAS=1)
+  int ntr = 0;     // tracers (components 5..4+ntr of the state arrays, 0-based)
   int ncolor = 0;
   std::vector<int> color_off;  // first element slot of each colour (elements are stored in colour order)
   int *d_glob = nullptr;       // [NF][NN] 0-based
@@ -260,7 +261,9 @@ static int validate_desc(cgd_ctx *c, const cgd_desc *d, const std::string &who) 
   if (d->Nz < 3 || d->Nz > 256) return fail(c, CGD_ERR_ARG, who + "Nz must be in [3, 256]");
   if (d->NF < 1 || d->NumG < 1) return fail(c, CGD_ERR_ARG, who + "NF and NumG must be positive");
   if (d->NumV != 5) return fail(c, CGD_ERR_UNSUPPORTED, who + "NumV must be 5 (rho, u, v, w, rho*theta)");
-  if (d->NumTr != 0) return fail(c, CGD_ERR_UNSUPPORTED, who + "tracers (NumTr > 0) are not implemented");
+  if (d->NumTr < 0 || d->NumTr > 16) return fail(c, CGD_ERR_UNSUPPORTED, who + "NumTr must be in [0, 16]");
+  if (d->NumTr > 0 && d->precision != CGD_FP64)
+    return fail(c, CGD_ERR_UNSUPPORTED, who + "tracers are implemented in the FP64 mode only");
   if (d->precision != CGD_FP64 && d->precision != CGD_MIXED)
     return fail(c, CGD_ERR_UNSUPPORTED, who + "unknown precision mode");
   if (d->HorLimit || d->Turbulence || d->VerticalDiffusion || d->SurfaceFlux || d->Forcing || d->Microphysics ||
@@ -287,8 +290,8 @@ static int ctx_build(cgd_ctx *c, const cgd_desc *ds, int nmem, hipStream_t st) {
     const std::string who = nmem > 1 ? "member " + std::to_string(m) + ": " : "";
     if ((r = validate_desc(c, &ds[m], who))) return r;
     const cgd_desc &a = ds[0], &b = ds[m];
-    if (b.N != a.N || b.Nz != a.Nz || b.precision != a.precision)
-      return fail(c, CGD_ERR_ARG, who + "N, Nz and precision must be the same for every member");
+    if (b.N != a.N || b.Nz != a.Nz || b.precision != a.precision || b.NumTr != a.NumTr)
+      return fail(c, CGD_ERR_ARG, who + "N, Nz, NumTr and precision must be the same for every member");
     if (b.Grav != a.Grav || b.Omega != a.Omega || b.Rd != a.Rd || b.Cpd != a.Cpd || b.p0 != a.p0 ||
         b.kappa != a.kappa)
       return fail(c, CGD_ERR_ARG, who + "physical constants must be the same for every member");
@@ -296,6 +299,7 @@ static int ctx_build(cgd_ctx *c, const cgd_desc *ds, int nmem, hipStream_t st) {
   const cgd_desc *d = &ds[0];
   c->desc = *d;
   c->mixed = d->precision == CGD_MIXED;
+  c->ntr = d->NumTr;
   c->nmem = nmem;
   c->N = d->N;
   c->NN = d->N * d->N;
@@ -671,6 +675,20 @@ static void fcn2_t(cgd_ctx *c, dim3 g, hipStream_t st, int slot0, const double *
       else FN<4, 8, false, R>(c, __VA_ARGS__);                     \
     }                                                              \
   } while (0)
+template <int N, int NZT, bool CM, typename R>
+static void tr1_t(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const double *U, double *T1) {
+  const size_t lds = sizeof(double) * (2 * c->NN + 3 * c->NN * NZT);
+  hipLaunchKernelGGL((k_tr1<N, NZT, CM>), dim3(nitems), dim3(c->block1), lds, st, c->ka, slot0, c->nchunk1, c->ntr,
+                     U, T1);
+}
+template <int N, int NZT, bool CM, typename R>
+static void tr2_t(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const double *U, const double *T1,
+                  double *F) {
+  const size_t lds = sizeof(double) * (2 * c->NN + 6 * c->NN * NZT);
+  hipLaunchKernelGGL((k_tr2<N, NZT, CM>), dim3(nitems), dim3(c->block), lds, st, c->ka, slot0, c->nchunk, c->ntr, U,
+                     T1, F);
+}
+
 template <typename R>
 static void launch_fcn1(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const R *UR, R *T1) {
   CGD_DISPATCH(fcn1_t, R, c->nzt1, nitems, st, slot0, UR, T1);
@@ -710,6 +728,16 @@ static int run_fcn(cgd_ctx *c, R *F, const double *U, const R *UR, R *Temp1, con
     HIPCHK(c, hipGetLastError());
     ++launches;
   }
+  if constexpr (std::is_same<R, double>::value) {
+    if (c->ntr > 0)
+      for (int k = 0; k < c->ncolor; ++k) {
+        const int ne = c->color_off[k + 1] - c->color_off[k];
+        if (!ne) continue;
+        CGD_DISPATCH(tr1_t, double, c->nzt1, ne * c->nchunk1, st, c->color_off[k], U, Temp1);
+        HIPCHK(c, hipGetLastError());
+        ++launches;
+      }
+  }
   if ((r = prof_end(c, st, 0, launches))) return r;
   prof_begin(c, st, 1);
   launches = 0;
@@ -719,6 +747,16 @@ static int run_fcn(cgd_ctx *c, R *F, const double *U, const R *UR, R *Temp1, con
     launch_fcn2<R>(c, dim3(ne * c->nchunk), st, c->color_off[k], U, UR, Temp1, AuxG, F);
     HIPCHK(c, hipGetLastError());
     ++launches;
+  }
+  if constexpr (std::is_same<R, double>::value) {
+    if (c->ntr > 0)
+      for (int k = 0; k < c->ncolor; ++k) {
+        const int ne = c->color_off[k + 1] - c->color_off[k];
+        if (!ne) continue;
+        CGD_DISPATCH(tr2_t, double, c->nzt, ne * c->nchunk, st, c->color_off[k], U, Temp1, F);
+        HIPCHK(c, hipGetLastError());
+        ++launches;
+      }
   }
   if constexpr (std::is_same<R, double>::value) {
     if (f4fix && c->nchunk > 1) {
@@ -769,6 +807,11 @@ extern "C" int cgd_schur(cgd_ctx *c, double *k, double *v, const double *JRhoW, 
     hipLaunchKernelGGL(k_schur, dim3(grid_for(c->NumG, 64)), dim3(64), 0, st, c->ka, k, v, JRhoW, JWRho, JWRhoTh,
                        JRhoThW, tri, c->d_bwork, fac, comp_tri);
   HIPCHK(c, hipGetLastError());
+  if (c->ntr > 0) {
+    const size_t ng = (size_t)c->NumG * c->Nz, nt = (size_t)c->ntr * ng;
+    hipLaunchKernelGGL(k_tr_scale, dim3(grid_for(nt, 256)), dim3(256), 0, st, v + 5 * ng, k + 5 * ng, fac, nt);
+    HIPCHK(c, hipGetLastError());
+  }
   return prof_end(c, st, 2, 1);
 }
 
@@ -794,6 +837,7 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
                          bool more) {
   int r;
   const size_t ng = (size_t)c->NumG * c->Nz, n5 = 5 * ng;
+  const size_t nv = (size_t)(5 + c->ntr) * ng;  // one state array (Nz, NumG, 5 + NumTr)
   const bool mixed = std::is_same<R, float>::value;
   R *F = reinterpret_cast<R *>(w->fV);
   R *T1 = reinterpret_cast<R *>(w->Temp1);
@@ -812,7 +856,7 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
     rw.V32 = (is < 2 || more) ? V32 : nullptr;
     // the stage increments are stored as R (FP32 in the mixed mode: they
     // are ~dt*F, so their rounding is ~1e-7 of an increment)
-    R *kR[3] = {reinterpret_cast<R *>(k[0]), reinterpret_cast<R *>(w->k) + n5, reinterpret_cast<R *>(w->k) + 2 * n5};
+    R *kR[3] = {reinterpret_cast<R *>(k[0]), reinterpret_cast<R *>(w->k) + nv, reinterpret_cast<R *>(w->k) + 2 * nv};
     for (int j = 0; j < is; ++j) {
       rw.kprev[j] = kR[j];
       rw.cf[j] = Cc[is][j] / dt;
@@ -837,6 +881,23 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
     if (c->colw == 32) launch_stage<R, 32>(c, is, rw, st);
     else launch_stage<R, 64>(c, is, rw, st);
     HIPCHK(c, hipGetLastError());
+    if (c->ntr > 0) {  // FP64 only (checked at context creation)
+      const size_t nt = (size_t)c->ntr * ng, o = 5 * ng;
+      const double *fvt = w->fV + o, *k0 = k[0] + o, *k1 = k[1] + o;
+      double *kout = is < 2 ? k[is] + o : nullptr;
+      const double cf0 = is > 0 ? Cc[is][0] / dt : 0.0, cf1 = is > 1 ? Cc[is][1] / dt : 0.0;
+      const dim3 gt(grid_for(nt, 256)), bt(256);
+      if (is == 0)
+        hipLaunchKernelGGL(k_rw_tr<0>, gt, bt, 0, st, fvt, k0, k1, cf0, cf1, rw.av[0], rw.av[1], rw.av[2], rw.fac,
+                           V + o, w->Vn + o, kout, nt);
+      else if (is == 1)
+        hipLaunchKernelGGL(k_rw_tr<1>, gt, bt, 0, st, fvt, k0, k1, cf0, cf1, rw.av[0], rw.av[1], rw.av[2], rw.fac,
+                           V + o, w->Vn + o, kout, nt);
+      else
+        hipLaunchKernelGGL(k_rw_tr<2>, gt, bt, 0, st, fvt, k0, k1, cf0, cf1, rw.av[0], rw.av[1], rw.av[2], rw.fac,
+                           V + o, w->Vn + o, kout, nt);
+      HIPCHK(c, hipGetLastError());
+    }
     if ((r = prof_end(c, st, 2, 1))) return r;
   }
   return CGD_OK;
@@ -859,7 +920,7 @@ static int rw_run(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, int ns
   const double m3 = 2.0 / 3.0, m2 = 1.0 / 6.0 - (-0.75) * m3, m1 = 1.0 / 6.0 - (-0.75) * m3;
   const double Mw[3] = {m1, m2, m3};
   const double gamma = 1.0;
-  const size_t ng = (size_t)c->NumG * c->Nz, n5 = 5 * ng;
+  const size_t ng = (size_t)c->NumG * c->Nz, n5 = (size_t)(5 + c->ntr) * ng;  // (Nz, NumG, 5 + NumTr)
   double *k[3] = {w->k, w->k + n5, w->k + 2 * n5};
   if (c->Nz <= 64) {
     for (int s = 0; s < nsteps; ++s) {

@@ -911,6 +911,274 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
 #undef S22
 }
 
+// ------------------------------------------------------------------------
+// Tracers (NumTr > 0, FP64): components 5..4+NumTr of U/F (0-based), their
+// hyperviscosity caches at the same components of Temp1 (FcnGPU.jl:115-126).
+// Each tracer repeats the rho*theta path: HyperViscTracerKernel!
+// (ViscKernel.jl:169-218) in pass 1, HyperViscTracerKoeffKernel!
+// (:277-327) and the tracer loop of DivRhoTrUpwind3New2Kernel!
+// (OperatorKernel.jl:722-759) in pass 2.  The kernels run after the
+// corresponding main pass, one launch per colour (first-touch DSS as in
+// k_fcn1/k_fcn2), over (element, level chunk) workgroups, all tracers in one
+// launch.  Tracers do not feed back on the dynamics.
+
+// metric component c of element slot e at (node, level), either layout
+template <int NN, bool CM>
+__device__ __forceinline__ double met_at(const KArgs &a, size_t e, int c, int node, int iz) {
+  const size_t Nz = (size_t)a.Nz;
+  if (CM) {
+    if (met_per_point(c)) return a.met[((e * CGD_NMETP + met_pidx(c)) * NN + node) * Nz + iz];
+    return a.metn[(e * CGD_NMETN + met_nidx(c)) * NN + node];
+  }
+  return a.met[((e * CGD_NMET + c) * NN + node) * Nz + iz];
+}
+
+// pass 1: the tracer caches, r = div(grad c) / (M1 + M2) with c = q / rho
+template <int N, int NZT, bool CM>
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_tr1(KArgs a, int slot0, int nchunk, int ntr,
+                                                                      const double *__restrict__ U,
+                                                                      double *__restrict__ T1) {
+  constexpr int NN = N * N;
+  extern __shared__ double smem[];
+  double *Ds = smem, *DWs = Ds + NN;
+  Tiles<N, NZT, double> T{Ds + 2 * NN};  // field 0: c; fields 1, 2: the contravariant gradient
+  load_dmats<N>(a, Ds, DWs);
+  const int Nz = a.Nz;
+  const int L = xcd_logical(blockIdx.x, gridDim.x);
+  const int e = slot0 + L / nchunk;
+  const int chunk = L - (L / nchunk) * nchunk;
+  const int t = threadIdx.x;
+  const int node = t / NZT, l = t - node * NZT;
+  const int iz = chunk * NZT + l;
+  const bool in_tile = node < NN;
+  const bool valid = in_tile && iz < Nz;
+  const int I = node % N, J = node / N;
+  const size_t vs = (size_t)a.NumG * Nz;
+  double S11 = 0, S12 = 0, S21 = 0, S22 = 0, iJs = 1, iMs = 1, irho = 1;
+  bool first = false;
+  size_t ui = 0;
+  if (valid) {
+    const int gv = a.glob[e * NN + node];
+    first = gv < 0;
+    ui = (size_t)(gv & 0x7fffffff) * Nz + iz;
+    if constexpr (CM) {
+      const double *mN = a.met1n + (size_t)e * CGD_NMET1 * NN;
+      S11 = mN[node];
+      S12 = mN[NN + node];
+      S21 = mN[2 * NN + node];
+      S22 = mN[3 * NN + node];
+      iJs = rcp(mN[4 * NN + node]);
+    } else {
+      const size_t NNZ = (size_t)NN * Nz, mo = (size_t)node * Nz + iz;
+      const double *metE = a.met1 + (size_t)e * CGD_NMET1 * NNZ;
+      S11 = metE[mo];
+      S12 = metE[NNZ + mo];
+      S21 = metE[2 * NNZ + mo];
+      S22 = metE[3 * NNZ + mo];
+      iJs = rcp(metE[4 * NNZ + mo]);
+    }
+    iMs = a.iMs[ui];
+    irho = rcp(U[ui]);
+  }
+  for (int tr = 0; tr < ntr; ++tr) {
+    const size_t off = (size_t)(5 + tr) * vs;
+    double o = 0;
+    if (valid) {
+      if (!first) o = T1[off + ui];
+      T.at(0, node, l) = U[off + ui] * irho;
+    } else if (in_tile) {
+      T.at(0, node, l) = 0;
+    }
+    __syncthreads();
+    if (in_tile) {
+      double Cx = 0, Cy = 0;
+      if (valid) {
+        const double Dxc = dx<N, NZT, double>(Ds, T, 0, I, J, l);
+        const double Dyc = dy<N, NZT, double>(Ds, T, 0, I, J, l);
+        const double GDx = (S11 * Dxc + S21 * Dyc) * iJs;  // Grad12 with J
+        const double GDy = (S12 * Dxc + S22 * Dyc) * iJs;
+        Cx = S11 * GDx + S12 * GDy;  // Contra12
+        Cy = S21 * GDx + S22 * GDy;
+      }
+      T.at(1, node, l) = Cx;
+      T.at(2, node, l) = Cy;
+    }
+    __syncthreads();
+    if (valid) {
+      const double r = dxy<N, NZT, double>(DWs, T, 1, 2, I, J, l) * iMs;
+      T1[off + ui] = first ? r : o + r;
+    }
+    __syncthreads();  // the next tracer rewrites the tiles
+  }
+}
+
+// pass 2: -Kd div(rho grad C) / (M1 + M2) from the cache C, the upwind
+// vertical flux of c = q / rho through both interfaces of the layer (the
+// same contravariant w and RecU4 weights as rho*theta) and the horizontal
+// divergence of (-rho v) c
+template <int N, int NZT, bool CM>
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_tr2(KArgs a, int slot0, int nchunk, int ntr,
+                                                                      const double *__restrict__ U,
+                                                                      const double *__restrict__ T1,
+                                                                      double *__restrict__ F) {
+  constexpr int NN = N * N;
+  extern __shared__ double smem[];
+  double *Ds = smem, *DWs = Ds + NN;
+  // fields 0: C, 1: uc c, 2: vc c; 3, 4: the rho-weighted contravariant gradient of C
+  Tiles<N, NZT, double> T{Ds + 2 * NN};
+  load_dmats<N>(a, Ds, DWs);
+  const int Nz = a.Nz;
+  const int L = xcd_logical(blockIdx.x, gridDim.x);
+  const int e = slot0 + L / nchunk;
+  const int chunk = L - (L / nchunk) * nchunk;
+  const int t = threadIdx.x;
+  const int node = t / NZT, l = t - node * NZT;
+  const int iz = chunk * NZT + l;
+  const bool in_tile = node < NN;
+  const bool valid = in_tile && iz < Nz;
+  const int I = node % N, J = node / N;
+  const size_t vs = (size_t)a.NumG * Nz;
+  const bool has_lo = iz > 0, has_hi = iz < Nz - 1;
+  const double Kd = a.kvis[3 * e + 2];
+  double S11 = 0, S12 = 0, S21 = 0, S22 = 0, iJs = 1, iMs = 1, rho = 1, uc = 0, vc = 0;
+  double wlo = 0, wup = 0;                      // contravariant mass fluxes through the interfaces
+  double irh[5] = {1, 1, 1, 1, 1};              // 1/rho at levels iz-2..iz+2 (clamped)
+  double Jc[5] = {1, 1, 1, 1, 1};               // J1 + J2 at levels iz-2..iz+2 (clamped)
+  size_t ui = 0, uk[5] = {0, 0, 0, 0, 0};
+  bool first = false;
+  if (valid) {
+    const int gv = a.glob[e * NN + node];
+    first = gv < 0;
+    const size_t gb = (size_t)(gv & 0x7fffffff) * Nz;
+    ui = gb + iz;
+    for (int k = 0; k < 5; ++k) {
+      const int lz = jmin(jmax(iz + k - 2, 0), Nz - 1);
+      uk[k] = gb + lz;
+      irh[k] = rcp(U[uk[k]]);
+      Jc[k] = met_at<NN, CM>(a, e, MJ1, node, lz) + met_at<NN, CM>(a, e, MJ2, node, lz);
+    }
+    S11 = met_at<NN, CM>(a, e, M111, node, iz) + met_at<NN, CM>(a, e, M112, node, iz);
+    S12 = met_at<NN, CM>(a, e, M121, node, iz) + met_at<NN, CM>(a, e, M122, node, iz);
+    S21 = met_at<NN, CM>(a, e, M211, node, iz) + met_at<NN, CM>(a, e, M212, node, iz);
+    S22 = met_at<NN, CM>(a, e, M221, node, iz) + met_at<NN, CM>(a, e, M222, node, iz);
+    iJs = rcp(Jc[2]);
+    iMs = a.iMs[ui];
+    rho = U[ui];
+    const double u = U[vs + ui], v = U[2 * vs + ui];
+    uc = -rho * (S11 * u + S12 * v);  // Contra12(-rho, u, v)
+    vc = -rho * (S21 * u + S22 * v);
+    if (has_lo) {
+      const size_t um = ui - 1;
+      const double rm = U[um], wm = U[3 * vs + um];
+      wlo = rm * (met_at<NN, CM>(a, e, M312, node, iz - 1) * U[vs + um] +
+                  met_at<NN, CM>(a, e, M322, node, iz - 1) * U[2 * vs + um] +
+                  met_at<NN, CM>(a, e, M332, node, iz - 1) * wm) +
+            rho * (met_at<NN, CM>(a, e, M311, node, iz) * u + met_at<NN, CM>(a, e, M321, node, iz) * v +
+                   met_at<NN, CM>(a, e, M331, node, iz) * wm);
+    }
+    if (has_hi) {
+      const size_t up = ui + 1;
+      const double rp = U[up], w = U[3 * vs + ui];
+      wup = rho * (met_at<NN, CM>(a, e, M312, node, iz) * u + met_at<NN, CM>(a, e, M322, node, iz) * v +
+                   met_at<NN, CM>(a, e, M332, node, iz) * w) +
+            rp * (met_at<NN, CM>(a, e, M311, node, iz + 1) * U[vs + up] +
+                  met_at<NN, CM>(a, e, M321, node, iz + 1) * U[2 * vs + up] +
+                  met_at<NN, CM>(a, e, M331, node, iz + 1) * w);
+    }
+  }
+  for (int tr = 0; tr < ntr; ++tr) {
+    const size_t off = (size_t)(5 + tr) * vs;
+    double o = 0, cz[5] = {0, 0, 0, 0, 0};
+    if (valid) {
+      if (!first) o = F[off + ui];
+      for (int k = 0; k < 5; ++k) cz[k] = U[off + uk[k]] * irh[k];
+    }
+    if (in_tile) {
+      T.at(0, node, l) = valid ? T1[off + ui] : 0.0;
+      T.at(1, node, l) = uc;
+      T.at(2, node, l) = vc;
+      T.at(5, node, l) = cz[2];
+    }
+    __syncthreads();
+    double f = 0;
+    if (in_tile) {
+      double Cx = 0, Cy = 0;
+      if (valid) {
+        const double Dxc = dx<N, NZT, double>(Ds, T, 0, I, J, l);
+        const double Dyc = dy<N, NZT, double>(Ds, T, 0, I, J, l);
+        const double GDx = (S11 * Dxc + S21 * Dyc) * iJs;
+        const double GDy = (S12 * Dxc + S22 * Dyc) * iJs;
+        Cx = rho * (S11 * GDx + S12 * GDy);  // Contra12 with rho
+        Cy = rho * (S21 * GDx + S22 * GDy);
+        // horizontal transport: sum_k D (uc c) over the element row / column
+        f += dxy_prod<N, NZT, double>(Ds, T, 1, 2, 5, I, J, l) * iMs;
+        // upwind vertical fluxes: through the lower interface (this layer
+        // is its right side) and the upper one (its left side)
+        if (has_lo) {
+          double cFL, cFR;
+          recu4<double>(cz[0], cz[1], cz[2], has_hi ? cz[3] : cz[2], Jc[0], Jc[1], Jc[2], has_hi ? Jc[3] : Jc[2],
+                        cFL, cFR);
+          f += 0.25 * ((fabs(wlo) + wlo) * cFL + (-fabs(wlo) + wlo) * cFR) * iMs;
+        }
+        if (has_hi) {
+          double cFL, cFR;
+          recu4<double>(cz[1], cz[2], cz[3], cz[4], Jc[1], Jc[2], Jc[3], Jc[4], cFL, cFR);
+          f -= 0.25 * ((fabs(wup) + wup) * cFL + (-fabs(wup) + wup) * cFR) * iMs;
+        }
+      }
+      T.at(3, node, l) = Cx;
+      T.at(4, node, l) = Cy;
+    }
+    __syncthreads();
+    if (valid) {
+      f += -Kd * dxy<N, NZT, double>(DWs, T, 3, 4, I, J, l) * iMs;
+      F[off + ui] = first ? f : o + f;
+    }
+    __syncthreads();  // the next tracer rewrites the tiles
+  }
+}
+
+// SchurSolveBKernel! for the tracers (SchurSolve.jl:106-109): k = fac v
+__global__ void k_tr_scale(const double *__restrict__ v, double *__restrict__ k, double fac, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    k[i] = fac * v[i];
+}
+
+// Rosenbrock stage arithmetic of the tracers (k = fac v, SchurSolve.jl:106-109):
+// v = fV + sum_j cf[j] k_j; k = fac v; V = Vn + sum_j av[j] k_j (+ av[STAGE] k).
+// Stage 0 reads V and stores it as Vn.
+template <int STAGE>
+__global__ void k_rw_tr(const double *__restrict__ fV, const double *__restrict__ k0, const double *__restrict__ k1,
+                        double cf0, double cf1, double av0, double av1, double av2, double fac, double *__restrict__ V,
+                        double *__restrict__ Vn, double *__restrict__ kout, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double v = fV[i];
+    double kp0 = 0, kp1 = 0;
+    if (STAGE > 0) {
+      kp0 = k0[i];
+      v = v + cf0 * kp0;
+    }
+    if (STAGE > 1) {
+      kp1 = k1[i];
+      v = v + cf1 * kp1;
+    }
+    const double kc = fac * v;
+    double vn;
+    if (STAGE == 0) {
+      vn = V[i];
+      Vn[i] = vn;
+    } else {
+      vn = Vn[i];
+    }
+    double r = vn;
+    if (STAGE > 0) r = r + av0 * kp0;
+    if (STAGE > 1) r = r + av1 * kp1;
+    r = r + (STAGE == 0 ? av0 : (STAGE == 1 ? av1 : av2)) * kc;
+    V[i] = r;
+    if (kout) kout[i] = kc;
+  }
+}
+
 // F4 of the interfaces between level chunks: the lower-interface term of a
 // chunk's first level, summed over the elements sharing the node in the
 // fixed order of nodecopy (colour order), added to F after all colours.

@@ -385,6 +385,11 @@ def init_context(FE, Metric, Phys, Global) -> NativeContext:
     return _ctx(FE, Metric, Phys, Global)
 
 
+def _nv(Global: GlobalStruct) -> int:
+    """Components of a state array: rho, u, v, w, rho*theta and the tracers."""
+    return 5 + (Global.Model.NumTr if Global.Model is not None else 0)
+
+
 def _check_state(t: torch.Tensor, Global: GlobalStruct, nv: int, name: str):
     if t.dtype != torch.float64 or not t.is_cuda:
         raise CGDError(f"{name}: float64 CUDA tensor expected")
@@ -408,9 +413,10 @@ def FcnGPU(F, U, FE, Metric, Phys, Cache, Exchange, Global, Param, Equation):
     if not isinstance(Equation, CompressibleShallow):
         raise CGDError(f"FcnGPU: no method for equation type {type(Equation).__name__}")
     ctx = _ctx(FE, Metric, Phys, Global)
-    _check_state(F, Global, 5, "F")
-    _check_state(U, Global, 5, "U")
-    _check_state(Cache.Temp1, Global, 5, "Cache.Temp1")
+    nv = _nv(Global)
+    _check_state(F, Global, nv, "F")
+    _check_state(U, Global, nv, "U")
+    _check_state(Cache.Temp1, Global, nv, "Cache.Temp1")
     ctx.check(ctx.L.cgd_fcn(ctx.h, _ptr(F), _ptr(U), _ptr(Cache.Temp1), _ptr(Cache.AuxG), _stream()), "FcnGPU")
 
 
@@ -427,8 +433,8 @@ def SchurSolveGPU(k, v, J, fac, Cache, Global):
     ctx = Global._ctx
     if ctx is None:
         raise CGDError("SchurSolveGPU: the native context is created by the first Fcn/Jac call")
-    _check_state(k, Global, 5, "k")
-    _check_state(v, Global, 5, "v")
+    _check_state(k, Global, _nv(Global), "k")
+    _check_state(v, Global, _nv(Global), "v")
     ctx.check(ctx.L.cgd_schur(ctx.h, _ptr(k), _ptr(v), _ptr(J.JRhoW), _ptr(J.JWRho), _ptr(J.JWRhoTh),
                               _ptr(J.JRhoThW), _ptr(J.tri), C.c_double(fac), C.c_int(1 if J.CompTri else 0),
                               _stream()), "SchurSolveGPU")
@@ -445,7 +451,7 @@ def RosenbrockSchur(V, dt, Fcn, FcnPrepare, Jac, CG, Metric, Phys, Cache, JCache
     """
     if Fcn is FcnGPU and FcnPrepare is FcnPrepareGPU and Jac is JacSchurGPU:
         ctx = _ctx(CG, Metric, Phys, Global)
-        _check_state(V, Global, 5, "V")
+        _check_state(V, Global, _nv(Global), "V")
         w = _rw_work(Cache, JCache)
         ctx.check(ctx.L.cgd_rw_step(ctx.h, _ptr(V), C.c_double(dt), C.byref(w), _stream()), "RosenbrockSchur")
         JCache.CompTri = False
@@ -478,7 +484,7 @@ def TimeStepper(U, dt, nIter, FE, Metric, Phys, Cache, JCache, Global, Exchange=
     if nIter < 1:
         return
     ctx = _ctx(FE, Metric, Phys, Global)
-    _check_state(U, Global, 5, "U")
+    _check_state(U, Global, _nv(Global), "U")
     w = _rw_work(Cache, JCache)
     ctx.check(ctx.L.cgd_rw_run(ctx.h, _ptr(U), C.c_double(dt), C.byref(w), int(nIter), _stream()), "TimeStepper")
     JCache.CompTri = False
@@ -493,7 +499,7 @@ def hyperdiffusion_for(ne: int) -> float:
     return 5.0e14 * (30.0 / ne) ** 3.2
 
 
-def _static_inputs(disc: Discretization, phys: Phys, hyperd, dev, precision):
+def _static_inputs(disc: Discretization, phys: Phys, hyperd, dev, precision, NumTr: int = 0):
     hd = hyperdiffusion_for(disc.ne) if hyperd is None else hyperd
     f64 = dict(dtype=torch.float64, device=dev)
     FE = CGQuad(OrdPoly=disc.p, Glob=torch.as_tensor(disc.glob, dtype=torch.int64, device=dev).contiguous(),
@@ -502,17 +508,18 @@ def _static_inputs(disc: Discretization, phys: Phys, hyperd, dev, precision):
                 M=torch.as_tensor(disc.M, **f64), NumG=disc.num_g)
     Metric = MetricStruct(dXdxI=torch.as_tensor(disc.dXdxI, **f64), J=torch.as_tensor(disc.J, **f64),
                           X=torch.as_tensor(disc.X, **f64), dz=torch.as_tensor(disc.dz, **f64))
-    Model = ModelStruct(HyperDCurl=hd, HyperDGrad=hd, HyperDDiv=hd)
+    Model = ModelStruct(HyperDCurl=hd, HyperDGrad=hd, HyperDDiv=hd, NumTr=NumTr)
     Global = GlobalStruct(Model=Model, Nz=disc.nz, NumG=disc.num_g, NF=disc.nf, N=disc.N, Precision=precision)
     return FE, Metric, Global
 
 
-def _work_arrays(num_g: int, nz: int, nT: int, dev):
+def _work_arrays(num_g: int, nz: int, nT: int, dev, NumTr: int = 0):
     f64 = dict(dtype=torch.float64, device=dev)
     n = (num_g, nz)
-    Cache = CacheStruct(AuxG=torch.zeros((4,) + n, **f64), Temp1=torch.zeros((nT,) + n, **f64),
-                        k=torch.zeros((3, 5) + n, **f64), fV=torch.zeros((5,) + n, **f64),
-                        Vn=torch.zeros((5,) + n, **f64))
+    nv = 5 + NumTr
+    Cache = CacheStruct(AuxG=torch.zeros((4,) + n, **f64), Temp1=torch.zeros((max(nT, nv),) + n, **f64),
+                        k=torch.zeros((3, nv) + n, **f64), fV=torch.zeros((nv,) + n, **f64),
+                        Vn=torch.zeros((nv,) + n, **f64))
     jn = (num_g, nz - 1)
     JCache = JStruct(JRhoW=torch.zeros(jn + (2,), **f64), JWRho=torch.zeros(jn + (2,), **f64),
                      JWRhoTh=torch.zeros(jn + (2,), **f64), JRhoThW=torch.zeros(jn + (2,), **f64),
@@ -521,11 +528,12 @@ def _work_arrays(num_g: int, nz: int, nT: int, dev):
 
 
 def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None = None, device="cuda",
-                 nT: int = 5, precision: str = "fp64"):
-    """Upload the static inputs and allocate ``Cache``/``JCache``."""
+                 nT: int = 5, precision: str = "fp64", NumTr: int = 0):
+    """Upload the static inputs and allocate ``Cache``/``JCache`` (state
+    arrays with ``5 + NumTr`` components; ``Temp1`` with at least as many)."""
     dev = torch.device(device)
-    FE, Metric, Global = _static_inputs(disc, phys, hyperd, dev, precision)
-    Cache, JCache = _work_arrays(disc.num_g, disc.nz, nT, dev)
+    FE, Metric, Global = _static_inputs(disc, phys, hyperd, dev, precision, NumTr)
+    Cache, JCache = _work_arrays(disc.num_g, disc.nz, nT, dev, NumTr)
     return FE, Metric, Global, Cache, JCache
 
 

@@ -5,14 +5,17 @@
  *
  * Every array argument is a DEVICE pointer with the reference's Julia
  * column-major layout (first index fastest):
- *   U, F, V, Vn, fV, Temp1 : double (Nz, NumG, 5)       AuxG : double (Nz, NumG, 4)
- *   k                      : double (Nz, NumG, 5, 3)
+ *   U, F, V, Vn, fV        : double (Nz, NumG, 5+NumTr)  AuxG : double (Nz, NumG, 4)
+ *   Temp1                  : double (Nz, NumG, >= 5+NumTr)
+ *   k                      : double (Nz, NumG, 5+NumTr, 3)
  *   JRhoW, JWRho, JWRhoTh, JRhoThW : double (2, Nz-1, NumG)   tri : double (3, Nz-1, NumG)
  *   Glob  : int64 (N*N, NF), 1-based      DS, DW : double (N, N)
  *   dXdxI : double (3,3,2,N*N,Nz,NF)      J : double (N*N,2,Nz,NF)
  *   X     : double (N*N,2,3,Nz,NF)        M : double (Nz,NumG,2)    dz : double (Nz,NumG)
- * Temp1 may have more than 5 components (the reference allocates 8+NumTr);
- * only the first 5 are read or written.
+ * Components 1-5 are rho, u, v, w, rho*theta, then the NumTr tracer
+ * densities.  Temp1 may have more components (the reference allocates
+ * 8+NumTr); its first 5 hold the hyperviscosity cache of the dynamics and
+ * components 6..5+NumTr those of the tracers (FcnGPU.jl:115-126).
  *
  * All calls are ordered on the caller's HIP stream (passed as void*) and
  * return CGD_OK (0) or a negative error code.  The step-path calls
@@ -57,7 +60,7 @@ typedef struct {
   int NF;     /* NumFaces (elements) */
   int NumG;   /* global CG nodes */
   int NumV;   /* must be 5: rho, u, v, w, rho*theta */
-  int NumTr;  /* must be 0 */
+  int NumTr;  /* tracers, 0..16 (FP64 mode only) */
   int precision; /* CGD_FP64 or CGD_MIXED */
   /* model switches that select FcnGPU! branches; only the BASELINE set is
    * implemented: HorLimit=0, KoeffDivW=0, no EDMF/TKE/physics hooks */
@@ -101,8 +104,10 @@ const char *cgd_last_error(const cgd_ctx *ctx);
  * AuxG[:,:,1:3] = (p, T, theta). */
 int cgd_prepare(cgd_ctx *ctx, const double *U, double *AuxG, void *stream);
 
-/* FcnGPU!(F,U,...,Models.CompressibleShallow) -- src/GPU/FcnGPU.jl:552-913.
- * Zeroes and overwrites F and Temp1[:,:,1:5]; reads p = AuxG[:,:,1]. */
+/* FcnGPU!(F,U,...,Models.CompressibleShallow) -- src/GPU/FcnGPU.jl:552-913,
+ * with the tracers of DivRhoTrUpwind3New2Kernel! (src/GPU/OperatorKernel.jl:722-759)
+ * and HyperViscTracer(Koeff)Kernel! (src/GPU/ViscKernel.jl:169-218,277-327).
+ * Zeroes and overwrites F and Temp1[:,:,1:5+NumTr]; reads p = AuxG[:,:,1]. */
 int cgd_fcn(cgd_ctx *ctx, double *F, const double *U, double *Temp1, const double *AuxG, void *stream);
 
 /* JacSchurGPU!(J,U,...) -- src/DyCore/JacSchur.jl:3-75.  Reads AuxG[:,:,4]
@@ -111,7 +116,8 @@ int cgd_jac(cgd_ctx *ctx, double *JRhoW, double *JWRho, double *JWRhoTh, double 
             const double *AuxG, void *stream);
 
 /* SchurSolveGPU!(k,v,J,fac,...) -- src/Integration/SchurSolve.jl:149-177.
- * k: one stage (Nz,NumG,5).  Clobbers v[:,:,1], v[:,:,4], v[:,:,5].
+ * k: one stage (Nz,NumG,5+NumTr), tracers k = fac v.  Clobbers v[:,:,1],
+ * v[:,:,4], v[:,:,5].
  * comp_tri != 0 rebuilds tri from the J blocks first (J.CompTri). */
 int cgd_schur(cgd_ctx *ctx, double *k, double *v, const double *JRhoW, const double *JWRho, const double *JWRhoTh,
               const double *JRhoThW, double *tri, double fac, int comp_tri, void *stream);

@@ -17,7 +17,9 @@
  *   CoriolisShallow                  src/GPU/Coriolis.jl:16-28
  *   GradFullKernel!                  src/GPU/GradKineticKernel.jl:1-129
  *   GravitationShallow               src/GPU/Coriolis.jl:59-67
- *   DivRhoTrUpwind3New2Kernel!       src/GPU/OperatorKernel.jl:637-720
+ *   DivRhoTrUpwind3New2Kernel!       src/GPU/OperatorKernel.jl:637-760 (tracer loop :722-759)
+ *   HyperViscTracerKernel!           src/GPU/ViscKernel.jl:169-218
+ *   HyperViscTracerKoeffKernel!      src/GPU/ViscKernel.jl:277-327
  *   Contra12/Grad12/Curl12/Rot12/Contra3/RecU4
  *                                    src/GPU/OperatorKernel.jl:953-1059
  *   JacSchurKernel!                  src/DyCore/JacSchur.jl:29-75
@@ -464,20 +466,125 @@ static void div_element(const cgdo_grid *g, int IF, int iz, double *F, const dou
     }
 }
 
+/* HyperViscTracerKernel! (koeff = 0: Out = Temp1, c = Tr/rho) and
+ * HyperViscTracerKoeffKernel! (koeff = 1: Out = F, c = the tracer's cache,
+ * rho-weighted) for tracer component v = 5 + t (ViscKernel.jl:169-218,
+ * 277-327); the operations of the theta part of hypervisc_element in the
+ * same order */
+static void tracer_hv_element(const cgdo_grid *g, int IF, int iz, int koeff, int v, double *Out, const double *U,
+                              const double *Cache) {
+  const int N = g->N;
+  double Tr[MAXN][MAXN], Cx[MAXN][MAXN], Cy[MAXN][MAXN];
+  metric_pt m;
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int64_t ind = GLOB(I + J * N, IF);
+      Tr[I][J] = koeff ? Uv(Cache, iz, ind, v) : Uv(U, iz, ind, v) / Uv(U, iz, ind, 0);
+    }
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      load_metric(g, ID, iz, IF, &m);
+      double Dxc = DS(I, 0) * Tr[0][J];
+      double Dyc = DS(J, 0) * Tr[I][0];
+      for (int k = 1; k < N; ++k) {
+        Dxc += DS(I, k) * Tr[k][J];
+        Dyc += DS(J, k) * Tr[I][k];
+      }
+      double gdx, gdy;
+      grad12j(Dxc, Dyc, &m, JJ(ID, 0, iz, IF), JJ(ID, 1, iz, IF), &gdx, &gdy);
+      if (!koeff)
+        contra12(gdx, gdy, &m, &Cx[I][J], &Cy[I][J]);
+      else
+        contra12r(Uv(U, iz, ind, 0), gdx, gdy, &m, &Cx[I][J], &Cy[I][J]);
+    }
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      double DivTr = DW(I, 0) * Cx[0][J] + DW(J, 0) * Cy[I][0];
+      for (int k = 1; k < N; ++k) DivTr += DW(I, k) * Cx[k][J] + DW(J, k) * Cy[I][k];
+      double Ms = MM(iz, ind, 0) + MM(iz, ind, 1);
+      if (!koeff)
+        Uv(Out, iz, ind, v) += DivTr / Ms;
+      else
+        Uv(Out, iz, ind, v) += -g->KoeffDiv * DivTr / Ms;
+    }
+}
+
+/* the tracer loop of DivRhoTrUpwind3New2Kernel! (OperatorKernel.jl:722-759)
+ * for tracer component v: the theta transport of div_element with
+ * c = Tr/rho, in the same order */
+static void tracer_div_element(const cgdo_grid *g, int IF, int iz, int v, double *F, const double *U) {
+  const int N = g->N, Nz = g->Nz;
+  double uc[MAXN][MAXN], vc[MAXN][MAXN], cc[MAXN][MAXN];
+  metric_pt m;
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      load_metric(g, ID, iz, IF, &m);
+      double rho = Uv(U, iz, ind, 0);
+      cc[I][J] = Uv(U, iz, ind, v) / rho;
+      contra12r(-rho, Uv(U, iz, ind, 1), Uv(U, iz, ind, 2), &m, &uc[I][J], &vc[I][J]);
+    }
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      double Ms = MM(iz, ind, 0) + MM(iz, ind, 1);
+      if (iz < Nz - 1) {
+        double rL = Uv(U, iz, ind, 0), rR = Uv(U, iz + 1, ind, 0);
+        double w = Uv(U, iz, ind, 3);
+        double wCon = rL * (DX(2, 0, 1, ID, iz, IF) * Uv(U, iz, ind, 1) + DX(2, 1, 1, ID, iz, IF) * Uv(U, iz, ind, 2) +
+                            DX(2, 2, 1, ID, iz, IF) * w) +
+                      rR * (DX(2, 0, 0, ID, iz + 1, IF) * Uv(U, iz + 1, ind, 1) +
+                            DX(2, 1, 0, ID, iz + 1, IF) * Uv(U, iz + 1, ind, 2) + DX(2, 2, 0, ID, iz + 1, IF) * w);
+        int lm = iz > 0 ? iz - 1 : 0, lp2 = iz + 2 < Nz ? iz + 2 : Nz - 1;
+#define CC(L) (Uv(U, L, ind, v) / Uv(U, L, ind, 0))
+#define JC(L) (JJ(ID, 0, L, IF) + JJ(ID, 1, L, IF))
+        double cFL, cFR;
+        recu4(CC(lm), CC(iz), CC(iz + 1), CC(lp2), JC(lm), JC(iz), JC(iz + 1), JC(lp2), &cFL, &cFR);
+#undef CC
+#undef JC
+        double Msp = MM(iz + 1, ind, 0) + MM(iz + 1, ind, 1);
+        double Flux = 0.25 * ((fabs(wCon) + wCon) * cFL + (-fabs(wCon) + wCon) * cFR);
+        Uv(F, iz, ind, v) += -Flux / Ms;
+        Uv(F, iz + 1, ind, v) += Flux / Msp;
+      }
+      double DivRhoTr = DS(I, 0) * uc[0][J] * cc[0][J] + DS(J, 0) * vc[I][0] * cc[I][0];
+      for (int k = 1; k < N; ++k) DivRhoTr += DS(I, k) * uc[k][J] * cc[k][J] + DS(J, k) * vc[I][k] * cc[I][k];
+      Uv(F, iz, ind, v) += DivRhoTr / Ms;
+    }
+}
+
+/* FcnGPU! with NumTr tracers (no HorLimit): the tracer caches sit at
+ * Temp1[:,:,6:5+NumTr] (FcnGPU.jl:115-126) */
 void cgdo_fcn(const cgdo_grid *g, double *F, const double *U, double *Temp1, const double *AuxG) {
   const size_t ng = (size_t)g->Nz * (size_t)g->NumG;
-  memset(Temp1, 0, 5 * ng * sizeof(double));
+  const int nv = 5 + g->NumTr;
+  memset(Temp1, 0, (size_t)nv * ng * sizeof(double));
   for (int IF = 0; IF < g->NF; ++IF)
     for (int iz = 0; iz < g->Nz; ++iz) hypervisc_element(g, IF, iz, 0, Temp1, U, NULL);
-  memset(F, 0, 5 * ng * sizeof(double));
+  for (int t = 0; t < g->NumTr; ++t)
+    for (int IF = 0; IF < g->NF; ++IF)
+      for (int iz = 0; iz < g->Nz; ++iz) tracer_hv_element(g, IF, iz, 0, 5 + t, Temp1, U, NULL);
+  memset(F, 0, (size_t)nv * ng * sizeof(double));
   for (int IF = 0; IF < g->NF; ++IF)
     for (int iz = 0; iz < g->Nz; ++iz) hypervisc_element(g, IF, iz, 1, F, U, Temp1);
+  for (int t = 0; t < g->NumTr; ++t)
+    for (int IF = 0; IF < g->NF; ++IF)
+      for (int iz = 0; iz < g->Nz; ++iz) tracer_hv_element(g, IF, iz, 1, 5 + t, F, U, Temp1);
   for (int IF = 0; IF < g->NF; ++IF)
     for (int iz = 0; iz < g->Nz; ++iz) momentum_element(g, IF, iz, F, U);
   for (int IF = 0; IF < g->NF; ++IF)
     for (int iz = 0; iz < g->Nz; ++iz) grad_element(g, IF, iz, F, U, AuxG);
   for (int IF = 0; IF < g->NF; ++IF)
     for (int iz = 0; iz < g->Nz; ++iz) div_element(g, IF, iz, F, U);
+  for (int t = 0; t < g->NumTr; ++t)
+    for (int IF = 0; IF < g->NF; ++IF)
+      for (int iz = 0; iz < g->Nz; ++iz) tracer_div_element(g, IF, iz, 5 + t, F, U);
 }
 
 /* JacSchurKernel! (JacSchur.jl:29-75); p = AuxG[:,:,4] for State = Dry */
@@ -505,7 +612,7 @@ void cgdo_jac(const cgdo_grid *g, double *JRhoW, double *JWRho, double *JWRhoTh,
     }
 }
 
-/* SchurSolveGPU! (SchurSolve.jl:149-177) with its kernels, k: (Nz,NumG,5) */
+/* SchurSolveGPU! (SchurSolve.jl:149-177) with its kernels, k: (Nz,NumG,5+NumTr) */
 void cgdo_schur(const cgdo_grid *g, double *k, double *v, const double *JRhoW, const double *JWRho,
                 const double *JWRhoTh, const double *JRhoThW, double *tri, double fac, int comp_tri) {
   const int Nz = g->Nz, n = Nz - 1;
@@ -565,6 +672,7 @@ void cgdo_schur(const cgdo_grid *g, double *k, double *v, const double *JRhoW, c
       Uv(k, iz, ig, 1) = fac * Uv(v, iz, ig, 1);
       Uv(k, iz, ig, 2) = fac * Uv(v, iz, ig, 2);
       Uv(k, iz, ig, 4) = fac * Uv(v, iz, ig, 4);
+      for (int iT = 5; iT < 5 + g->NumTr; ++iT) Uv(k, iz, ig, iT) = fac * Uv(v, iz, ig, iT);
     }
   }
   free(x);
@@ -598,7 +706,7 @@ void cgdo_ssp_knoth(double a[3][3], double c[3][3], double m[3], double *gamma) 
 void cgdo_rw_step(const cgdo_grid *g, double *V, double dt, cgdo_work *w) {
   double a[3][3], c[3][3], m[3], gamma;
   cgdo_ssp_knoth(a, c, m, &gamma);
-  const size_t n5 = 5 * (size_t)g->Nz * (size_t)g->NumG;
+  const size_t n5 = (size_t)(5 + g->NumTr) * (size_t)g->Nz * (size_t)g->NumG;
   memcpy(w->Vn, V, n5 * sizeof(double));
   for (int is = 0; is < 3; ++is) {
     memcpy(V, w->Vn, n5 * sizeof(double));

@@ -2,10 +2,10 @@
  * CPU oracle (test infrastructure only) for the CGDycore.jl FP64 tendency
  * and Rosenbrock-W step.  All arrays are FP64 in the reference's Julia
  * column-major layouts:
- *   U, F, fV, V, Vn, Temp1 : (Nz, NumG, NumV)      AuxG : (Nz, NumG, 4)
+ *   U, F, fV, V, Vn, Temp1 : (Nz, NumG, 5+NumTr)   AuxG : (Nz, NumG, 4)
  *   dXdxI : (3,3,2,N^2,Nz,NF)  J : (N^2,2,Nz,NF)  X : (N^2,2,3,Nz,NF)
  *   M : (Nz,NumG,2)  dz : (Nz,NumG)  Glob : int64 (N^2,NF), 1-based
- *   J blocks : (2,Nz-1,NumG)  tri : (3,Nz-1,NumG)  k : (Nz,NumG,5,3)
+ *   J blocks : (2,Nz-1,NumG)  tri : (3,Nz-1,NumG)  k : (Nz,NumG,5+NumTr,3)
  */
 #ifndef CGD_ORACLE_H
 #define CGD_ORACLE_H
@@ -19,6 +19,7 @@ typedef struct {
   const double *dXdxI, *J, *X, *M, *dz;
   double Grav, Omega, Rd, Cpd, p0, kappa;
   double KoeffCurl, KoeffGrad, KoeffDiv;
+  int NumTr; /* tracers: components 6..5+NumTr of U, F, fV, V, Vn, k and Temp1 */
 } cgdo_grid;
 
 typedef struct {

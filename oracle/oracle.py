@@ -26,7 +26,8 @@ class _Grid(C.Structure):
                 ("dXdxI", _dp), ("J", _dp), ("X", _dp), ("M", _dp), ("dz", _dp),
                 ("Grav", C.c_double), ("Omega", C.c_double), ("Rd", C.c_double),
                 ("Cpd", C.c_double), ("p0", C.c_double), ("kappa", C.c_double),
-                ("KoeffCurl", C.c_double), ("KoeffGrad", C.c_double), ("KoeffDiv", C.c_double)]
+                ("KoeffCurl", C.c_double), ("KoeffGrad", C.c_double), ("KoeffDiv", C.c_double),
+                ("NumTr", C.c_int)]
 
 
 class _Work(C.Structure):
@@ -56,7 +57,7 @@ def _p(a: np.ndarray):
 class Oracle:
     """Binds one discretization (static inputs) to the oracle."""
 
-    def __init__(self, disc, phys=None, koeff=(0.0, 0.0, 0.0)):
+    def __init__(self, disc, phys=None, koeff=(0.0, 0.0, 0.0), NumTr=0):
         from_phys = phys
         if from_phys is None:
             raise ValueError("phys constants required")
@@ -78,6 +79,8 @@ class Oracle:
         g.Grav, g.Omega, g.Rd, g.Cpd = phys.Grav, phys.Omega, phys.Rd, phys.Cpd
         g.p0, g.kappa = phys.p0, phys.kappa
         g.KoeffCurl, g.KoeffGrad, g.KoeffDiv = koeff
+        g.NumTr = int(NumTr)
+        self.nv = 5 + int(NumTr)
         self.g = g
         self.L = lib()
 
@@ -91,8 +94,8 @@ class Oracle:
         return AuxG
 
     def fcn(self, U, AuxG):
-        F = self._state(5)
-        Temp1 = self._state(5)
+        F = self._state(self.nv)
+        Temp1 = self._state(self.nv)
         self.L.cgdo_fcn(C.byref(self.g), _p(F), _p(U), _p(Temp1), _p(AuxG))
         return F, Temp1
 
@@ -103,7 +106,7 @@ class Oracle:
         return blocks  # JRhoW, JWRho, JWRhoTh, JRhoThW
 
     def schur(self, v, blocks, fac, tri=None, comp_tri=True):
-        k = self._state(5)
+        k = self._state(self.nv)
         tri = np.zeros((self.disc.num_g, self.disc.nz - 1, 3)) if tri is None else tri
         self.L.cgdo_schur(C.byref(self.g), _p(k), _p(v), *[_p(b) for b in blocks], _p(tri),
                           C.c_double(fac), C.c_int(1 if comp_tri else 0))
@@ -111,7 +114,7 @@ class Oracle:
 
     def rw_step(self, V, dt, nsteps=1):
         d = self.disc
-        n5 = (5, d.num_g, d.nz)
+        n5 = (self.nv, d.num_g, d.nz)
         w = dict(Vn=np.zeros(n5), fV=np.zeros(n5), k=np.zeros((3,) + n5),
                  AuxG=np.zeros((4, d.num_g, d.nz)), Temp1=np.zeros(n5),
                  JRhoW=np.zeros((d.num_g, d.nz - 1, 2)), JWRho=np.zeros((d.num_g, d.nz - 1, 2)),

@@ -63,7 +63,9 @@ def _desc(pkg, **over):
         (dict(N=3), -2, "OrdPoly"),
         (dict(Nz=2), -1, "Nz"),
         (dict(NumV=6), -2, "NumV"),
-        (dict(NumTr=2), -2, "tracers"),
+        (dict(NumTr=2, precision=1), -2, "tracers are implemented in the FP64 mode only"),
+        (dict(NumTr=17), -2, "NumTr must be in"),
+        (dict(NumTr=-1), -2, "NumTr must be in"),
         (dict(precision=7), -2, "precision"),
         (dict(HorLimit=1), -2, "dry CompressibleShallow"),
         (dict(Microphysics=1), -2, "dry CompressibleShallow"),

@@ -1,0 +1,144 @@
+"""Tracers on the GPU (NumTr > 0, FP64) against the oracle, through the C ABI.
+
+Tracer tendencies, their hyperviscosity caches, the Schur solve's tracer
+part and the Rosenbrock-W steps match the oracle to 1e-12 relative L2 per
+component; the tracers are passive, so the dynamical components are
+bit-identical to a run without tracers.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def rel_l2(a, b):
+    den = np.linalg.norm(np.ravel(b))
+    return float(np.linalg.norm(np.ravel(a - b)) / (den if den > 0 else 1.0))
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def tracer_state(pkg, disc, seed, ntr):
+    """Perturbed jet plus tracers: a noisy mixing ratio, a copy of rho*theta, ..."""
+    U0 = pkg.state.jet_state(disc, seed=seed)
+    rng = np.random.default_rng(seed)
+    U0[1] += 3.0 * rng.standard_normal(U0[1].shape)
+    U0[2] += 3.0 * rng.standard_normal(U0[2].shape)
+    U0[3] += 0.3 * rng.standard_normal(U0[3].shape)
+    U0[3][:, -1] = 0.0
+    tr = [U0[0] * (0.01 + 0.002 * rng.standard_normal(U0[0].shape)), U0[4].copy(), 0.5 * U0[0]]
+    return np.concatenate([U0] + [t[None] for t in tr[:ntr]], axis=0)
+
+
+def setup(pkg, oracle_mod, ne, nz, ntr, seed=1):
+    D, host = pkg.discretization, pkg.host
+    disc = D.build(ne, 4, nz, 30000.0)
+    phys = D.Phys()
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda", NumTr=ntr, nT=8 + ntr)
+    hd = host.hyperdiffusion_for(ne)
+    orc = oracle_mod.Oracle(disc, phys, (hd, hd, hd), NumTr=ntr)
+    return disc, phys, (FE, Metric, Global, Cache, JCache), orc, tracer_state(pkg, disc, seed, ntr)
+
+
+@pytest.mark.parametrize("ne,nz,ntr", [(3, 10, 2), (4, 20, 3), (3, 64, 1)])
+def test_tracer_tendency(pkg, oracle_mod, torch_cuda, ne, nz, ntr):
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, (FE, Metric, Global, Cache, JCache), orc, U0 = setup(pkg, oracle_mod, ne, nz, ntr)
+    print(f"tracer tendency ne={ne} nz={nz} NumTr={ntr}", flush=True)
+    U = torch.as_tensor(U0, device="cuda")
+    F = torch.empty_like(U)
+    eq = host.CompressibleShallow()
+    host.FcnPrepareGPU(U, FE, Metric, phys, Cache, None, Global, None, eq)
+    host.FcnGPU(F, U, FE, Metric, phys, Cache, None, Global, None, eq)
+    torch.cuda.synchronize()
+    Fo, To = orc.fcn(U0.copy(), orc.prepare(U0.copy()))
+    Fg, Tg = F.cpu().numpy(), Cache.Temp1.cpu().numpy()
+    # the constant-mixing-ratio tracer's cache is roundoff (the hyperviscosity
+    # of a constant, ~1e-27 here): measured against the scale of the noisy
+    # tracer's cache
+    floor = np.linalg.norm(To[5])
+    for v in range(5 + ntr):
+        assert rel_l2(Fg[v], Fo[v]) <= (1e-11 if v == 0 else TOL), f"F{v + 1}: {rel_l2(Fg[v], Fo[v]):.3e}"
+        eT = np.linalg.norm(Tg[v] - To[v]) / max(np.linalg.norm(To[v]), floor)
+        assert eT <= TOL, f"Temp1[{v + 1}]: {eT:.3e}"
+    if ntr >= 2:  # the rho*theta tracer repeats F5 through its own kernels
+        assert rel_l2(Fg[6], Fg[4]) <= TOL
+
+
+def test_tracer_schur(pkg, oracle_mod, torch_cuda):
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, (FE, Metric, Global, Cache, JCache), orc, U0 = setup(pkg, oracle_mod, 3, 10, 2)
+    U = torch.as_tensor(U0, device="cuda")
+    eq = host.CompressibleShallow()
+    host.FcnPrepareGPU(U, FE, Metric, phys, Cache, None, Global, None, eq)
+    host.JacSchurGPU(JCache, U, FE, Metric, phys, Cache, Global, None, eq)
+    rng = np.random.default_rng(2)
+    v0 = rng.standard_normal(U0.shape)
+    v = torch.as_tensor(v0, device="cuda")
+    k = torch.empty_like(v)
+    JCache.CompTri = True
+    host.SchurSolveGPU(k, v, JCache, 300.0, Cache, Global)
+    torch.cuda.synchronize()
+    A = orc.prepare(U0.copy())
+    blocks = orc.jac(U0.copy(), A)
+    ko, _ = orc.schur(v0.copy(), blocks, 300.0)
+    kg = k.cpu().numpy()
+    for c in range(7):
+        assert rel_l2(kg[c], ko[c]) <= TOL, f"k{c + 1}"
+    assert np.array_equal(kg[5:], 300.0 * v0[5:])
+
+
+@pytest.mark.parametrize("ne,nz,ntr,nsteps", [(3, 10, 2, 2), (4, 40, 3, 2), (2, 70, 1, 1)])
+def test_tracer_steps(pkg, oracle_mod, torch_cuda, ne, nz, ntr, nsteps):
+    """nz = 70 runs the tall-column path (the reference's call sequence)."""
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, (FE, Metric, Global, Cache, JCache), orc, U0 = setup(pkg, oracle_mod, ne, nz, ntr)
+    print(f"tracer steps ne={ne} nz={nz} NumTr={ntr}", flush=True)
+    V = torch.as_tensor(U0, device="cuda").clone()
+    host.TimeStepper(V, 300.0, nsteps, FE, Metric, phys, Cache, JCache, Global)
+    torch.cuda.synchronize()
+    Vo = U0.copy()
+    orc.rw_step(Vo, 300.0, nsteps)
+    Vg = V.cpu().numpy()
+    assert np.isfinite(Vg).all()
+    for c in range(5 + ntr):
+        e = rel_l2(Vg[c], Vo[c])
+        assert e <= TOL, f"V{c + 1} rel L2 {e:.3e}"
+    # passive: the dynamics equal a run without tracers bit for bit
+    FE0, Metric0, Global0, Cache0, JCache0 = host.device_setup(disc, phys, device="cuda")
+    W = torch.as_tensor(U0[:5], device="cuda").clone()
+    host.TimeStepper(W, 300.0, nsteps, FE0, Metric0, phys, Cache0, JCache0, Global0)
+    torch.cuda.synchronize()
+    assert torch.equal(V[:5], W)
+
+
+def test_tracer_composed_equals_fused(pkg, oracle_mod, torch_cuda):
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, (FE, Metric, Global, Cache, JCache), orc, U0 = setup(pkg, oracle_mod, 3, 20, 2)
+    eq = host.CompressibleShallow()
+    V1 = torch.as_tensor(U0, device="cuda").clone()
+    host.RosenbrockSchur(V1, 300.0, host.FcnGPU, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys, Cache,
+                         JCache, None, Global, None, eq)
+
+    def fcn(*a):
+        return host.FcnGPU(*a)
+
+    V2 = torch.as_tensor(U0, device="cuda").clone()
+    host.RosenbrockSchur(V2, 300.0, fcn, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys, Cache, JCache,
+                         None, Global, None, eq)
+    torch.cuda.synchronize()
+    for c in range(7):
+        assert rel_l2(V2[c].cpu().numpy(), V1[c].cpu().numpy()) <= 1e-14, f"V{c + 1}"

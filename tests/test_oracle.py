@@ -398,3 +398,75 @@ def test_vector_hyperviscosity_damps_rotational_flow(pkg, oracle_mod):
         grad_err.append(abs((Mw * (Fs["grad"][1][:, 1] - Fs["none"][1][:, 1]) * target).sum() / nrm))
     assert curl_err[-1] < 0.06 and _converge(curl_err, 3.0), curl_err
     assert grad_err[-1] < 1e-3 and _converge(grad_err, 10.0), grad_err
+
+
+# ---- tracers (OperatorKernel.jl:722-759, ViscKernel.jl:169-218,277-327) ----
+
+def _tracer_setup(pkg, oracle_mod, ntr, ne=3, nz=8, seed=11):
+    disc = disc_for(pkg, ne, nz)
+    phys = pkg.discretization.Phys()
+    hd = pkg.host.hyperdiffusion_for(ne) if hasattr(pkg, "host") else 5e14
+    U0 = pkg.state.jet_state(disc, seed=seed)
+    rng = np.random.default_rng(seed)
+    U0[1] += 3.0 * rng.standard_normal(U0[1].shape)
+    U0[2] += 3.0 * rng.standard_normal(U0[2].shape)
+    U0[3] += 0.3 * rng.standard_normal(U0[3].shape)
+    U0[3][:, -1] = 0.0
+    orc = oracle_mod.Oracle(disc, phys, (hd, hd, hd), NumTr=ntr)
+    orc0 = oracle_mod.Oracle(disc, phys, (hd, hd, hd))
+    return disc, U0, orc, orc0
+
+
+def test_tracer_equal_to_rhotheta_reproduces_theta(pkg, oracle_mod):
+    """A tracer holding rho*theta has mixing ratio theta; its hyperviscosity
+    cache and tendency are then the rho*theta ones bit for bit, and the
+    dynamics are those of the run without tracers."""
+    disc, U0, orc, orc0 = _tracer_setup(pkg, oracle_mod, 1)
+    U = np.concatenate([U0, U0[4:5]], axis=0)
+    A = orc.prepare(U.copy())
+    F, T1 = orc.fcn(U.copy(), A)
+    assert np.array_equal(F[5], F[4]) and np.array_equal(T1[5], T1[4])
+    F0, T10 = orc0.fcn(U0.copy(), orc0.prepare(U0.copy()))
+    assert np.array_equal(F[:5], F0) and np.array_equal(T1[:5], T10)
+
+
+def test_constant_mixing_ratio(pkg, oracle_mod):
+    """Tracer = c0 * rho: its hyperviscosity is that of a constant (zero up
+    to the rounding of D applied to a constant), and its transport is c0
+    times the mass tendency."""
+    disc, U0, orc, _ = _tracer_setup(pkg, oracle_mod, 1)
+    c0 = 0.37
+    U = np.concatenate([U0, c0 * U0[0:1]], axis=0)
+    F, T1 = orc.fcn(U.copy(), orc.prepare(U.copy()))
+    assert np.abs(T1[5]).max() <= 1e-13 * np.abs(T1[4]).max()
+    err = np.linalg.norm(F[5] - c0 * F[0]) / np.linalg.norm(c0 * F[0])
+    assert err <= 1e-10, err
+
+
+def test_tracer_tendency_scales_linearly(pkg, oracle_mod):
+    """The tracer tendency is linear in the tracer at fixed density and
+    winds (the upwind limiter depends on ratios of mixing-ratio
+    differences only), and each tracer is independent of the others."""
+    disc, U0, orc, _ = _tracer_setup(pkg, oracle_mod, 2)
+    rng = np.random.default_rng(3)
+    q = U0[0] * (1.0 + 0.1 * rng.standard_normal(U0[0].shape))
+    U = np.concatenate([U0, q[None], 4.0 * q[None]], axis=0)
+    F, T1 = orc.fcn(U.copy(), orc.prepare(U.copy()))
+    assert np.allclose(F[6], 4.0 * F[5], rtol=1e-12, atol=1e-12 * np.abs(F[6]).max())
+    assert np.allclose(T1[6], 4.0 * T1[5], rtol=1e-12, atol=1e-12 * np.abs(T1[6]).max())
+
+
+def test_tracers_are_passive_in_the_step(pkg, oracle_mod):
+    """Tracers do not enter the Schur complement (k = fac v for them,
+    SchurSolve.jl:106-109) nor any dynamical tendency: two Rosenbrock-W
+    steps with two tracers leave rho, u, v, w, rho*theta bit-identical to
+    the run without tracers."""
+    disc, U0, orc, orc0 = _tracer_setup(pkg, oracle_mod, 2)
+    rng = np.random.default_rng(5)
+    q = U0[0] * (1.0 + 0.1 * rng.standard_normal(U0[0].shape))
+    V = np.concatenate([U0, q[None], U0[4:5]], axis=0)
+    V0 = U0.copy()
+    orc.rw_step(V, 300.0, 2)
+    orc0.rw_step(V0, 300.0, 2)
+    assert np.array_equal(V[:5], V0)
+    assert np.isfinite(V).all() and not np.array_equal(V[5], q)
