@@ -205,6 +205,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--ntr", type=int, default=0, help="tracers in the FP64 state (reported in config)")
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=10)
     ap.add_argument("--ens-dt", type=float, default=150.0)
@@ -240,6 +241,9 @@ def main():
     host_build_s = time.perf_counter() - t0
     phys = D.Phys()
     U0 = pkg.state.jet_state(disc, seed=0)
+    if args.ntr:
+        # tracers: mixing ratios 0.01 (1 + 0.1 k) of the density
+        U0 = np.concatenate([U0] + [0.01 * (1 + 0.1 * k) * U0[0:1] for k in range(args.ntr)], axis=0)
     print(f"[bench] setup ne={args.ne} p={args.p} nz={args.nz}: {host_build_s:.1f}s", file=sys.stderr, flush=True)
     # the same static inputs built by the device set-up kernels (numbering on the host)
     torch.cuda.synchronize()
@@ -255,7 +259,8 @@ def main():
              "what": "grid, numbering, metric (JacobiSphere3), DSS mass, layer sizes; device build = host "
                      "numbering + set-up kernels"}
     del dev_build
-    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, hyperd=args.hyperd, device="cuda:0")
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, hyperd=args.hyperd, device="cuda:0",
+                                                          NumTr=args.ntr)
     ctx = host.init_context(FE, Metric, phys, Global)
     V = torch.as_tensor(U0, device="cuda:0").clone()
     shape = ctx.launch_shape()
@@ -294,7 +299,7 @@ def main():
     n_eval = 3 * args.steps
     p2_eval_s = p2_ms / 1e3 / n_eval
     achieved = sb["pass2"] / p2_eval_s / 1e9
-    workload = f"rw_step_fp64_ne{args.ne}_p{args.p}_nz{args.nz}"
+    workload = f"rw_step_fp64_ne{args.ne}_p{args.p}_nz{args.nz}" + (f"_tr{args.ntr}" if args.ntr else "")
     traffic = measured_traffic("k_fcn2", workload, sb["pass2"])
     roofline = {"bound": "hbm", "kernel": "k_fcn2 (fused tendency pass)", "achieved": round(achieved, 1),
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
@@ -305,7 +310,7 @@ def main():
                 "pass_ms_per_step": {k: round(v[0] / nb, 4) for k, v in passes.items()}}
 
     mixed = None
-    if not args.no_mixed:
+    if not args.no_mixed and not args.ntr:
         mixed = mixed_precision(host, disc, phys, U0, args, ms_per_step, V_fp64)
         print(f"[bench] mixed precision: {mixed}", file=sys.stderr, flush=True)
 
@@ -315,7 +320,7 @@ def main():
         print(f"[bench] ensemble: {ens}", file=sys.stderr, flush=True)
 
     cpu = None
-    if args.cpu_steps > 0:
+    if args.cpu_steps > 0 and not args.ntr:
         O = ge.load_oracle()
         orc = O.Oracle(disc, phys, (Global.Model.HyperDCurl, Global.Model.HyperDGrad, Global.Model.HyperDDiv))
         Vo = U0.copy()
@@ -332,7 +337,7 @@ def main():
            "dtype": "f64", "data": "synthetic: seeded baroclinic jet (seed 0) on a flat cubed sphere",
            "config": {"workload": workload, "ne": args.ne, "p": args.p,
                       "nz": args.nz, "dt": args.dt, "hyperd": Global.Model.HyperDDiv, "points": Ne,
-                      "precision": "fp64", "finite": finite,
+                      "precision": "fp64", "finite": finite, "tracers": args.ntr,
                       "launch_shape": dict(shape, autotuned=not args.no_autotune)},
            "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "ensemble": ens, "setup": setup}
     print(json.dumps(out), flush=True)
