@@ -13,7 +13,7 @@ The package directory name contains a dot, so it is loaded by path; see
 """
 from . import discretization, grid, state  # noqa: F401
 
-__all__ = ["grid", "discretization", "state", "host", "lib_path"]
+__all__ = ["grid", "discretization", "state", "host", "output", "lib_path"]
 
 
 def lib_path():
@@ -23,8 +23,8 @@ def lib_path():
 
 
 def __getattr__(name):
-    if name == "host":
+    if name in ("host", "output"):
         import importlib
 
-        return importlib.import_module(__name__ + ".host")
+        return importlib.import_module(__name__ + "." + name)
     raise AttributeError(name)

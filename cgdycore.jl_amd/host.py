@@ -118,7 +118,8 @@ def exported_symbols():
             "cgd_schur", "cgd_rw_step", "cgd_profile_enable", "cgd_profile_read", "cgd_color_elements",
             "cgd_ctx_info", "cgd_ctx_create_batch", "cgd_set_launch_shape", "cgd_get_launch_shape",
             "cgd_autotune", "cgd_rw_graph_create", "cgd_rw_graph_launch", "cgd_setup_error",
-            "cgd_jacobi_sphere3", "cgd_mass_cg", "cgd_grid_size_sphere", "cgd_rw_run"]
+            "cgd_jacobi_sphere3", "cgd_mass_cg", "cgd_grid_size_sphere", "cgd_rw_run", "cgd_output_error",
+            "cgd_interpolate"]
 
 
 def color_elements(glob: np.ndarray, num_g: int):

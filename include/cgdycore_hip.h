@@ -219,6 +219,20 @@ int cgd_mass_cg(int N, int Nz, int NF, int NumG, const int64_t *Glob, const doub
 int cgd_grid_size_sphere(int N, int Nz, int NF, int NumG, const int64_t *Glob, const double *X, double Rad,
                          double *zP, double *dz, void *stream);
 
+/* ---- Output (no context).  InterpolateGPU! / InterpolateRhoGPU! /
+ * InterpolateWBGPU! -- src/Outputs/vtkOutputKernel.jl:49-158,215-292: the
+ * values at the cell centres of an OrdPrint x OrdPrint sub-grid of every
+ * element and layer, cCell (OrdPrint, OrdPrint, Nz, NF), from a global-node
+ * field (Nz, NumG) and Inter (OrdPrint, OrdPrint, N, N) = the Lagrange basis
+ * at the sub-cell centres (vtkSphere.jl:193-210).  mode 0: c; mode 1: c/rho
+ * (rho needed); mode 2: c = w at the layer tops, the cell value
+ * 1/2 (w[Iz] + w[Iz-1]) with the free-slip w below the first layer (u, v
+ * and dXdxI needed).  Errors: cgd_output_error(). */
+const char *cgd_output_error(void);
+int cgd_interpolate(int mode, int N, int Nz, int NF, int NumG, int OrdPrint, const int64_t *Glob, const double *Inter,
+                    const double *c, const double *rho, const double *u, const double *v, const double *dXdxI,
+                    double *cCell, void *stream);
+
 /* Host-only helper (no GPU needed): the element colouring that schedules the
  * DSS.  glob is a HOST int64 (N*N, NF) 1-based array; color receives NF
  * colours (0-based), ncolor their count.  Elements of one colour share no
