@@ -40,6 +40,8 @@ struct cgd_ctx {
   int nmem = 1;
   std::vector<int> mem_e0, mem_g0;
   double *d_M = nullptr, *d_dz = nullptr;  // M and dz in batch numbering (batches only)
+  double *d_zP = nullptr, *d_xS = nullptr;  // zP, xS in batch numbering (batches with hooks only)
+  Hooks hooks{};                            // Damping / Forcing (pointwise, after the tendency)
   const double *M = nullptr, *dz = nullptr;  // the M and dz the kernels read
   // column launch shape of the fused stage kernel: lanes per column (64,
   // or 32 for two columns per wavefront when Nz <= 32) and workgroup size
@@ -266,10 +268,13 @@ static int validate_desc(cgd_ctx *c, const cgd_desc *d, const std::string &who) 
     return fail(c, CGD_ERR_UNSUPPORTED, who + "tracers are implemented in the FP64 mode only");
   if (d->precision != CGD_FP64 && d->precision != CGD_MIXED)
     return fail(c, CGD_ERR_UNSUPPORTED, who + "unknown precision mode");
-  if (d->HorLimit || d->Turbulence || d->VerticalDiffusion || d->SurfaceFlux || d->Forcing || d->Microphysics ||
-      d->Damping || d->EDMF)
+  if (d->HorLimit || d->Turbulence || d->VerticalDiffusion || d->SurfaceFlux || d->Microphysics || d->EDMF)
     return fail(c, CGD_ERR_UNSUPPORTED,
-                who + "only the dry CompressibleShallow path without physics hooks is implemented");
+                who + "only the dry CompressibleShallow path is implemented (hooks: Damping, Forcing)");
+  if (d->Damping && (!d->zP || !(d->DampStride > 0.0) || !(d->DampH > 0.0)))
+    return fail(c, CGD_ERR_ARG, who + "Damping needs zP, DampH > 0 and DampStride > 0");
+  if (d->Forcing && (!d->xS || !(d->HS_sigma_b > 0.0 && d->HS_sigma_b < 1.0)))
+    return fail(c, CGD_ERR_ARG, who + "Forcing needs xS and 0 < HS_sigma_b < 1");
   if (d->HyperDDivW != 0.0)
     return fail(c, CGD_ERR_UNSUPPORTED, who + "HyperDDivW > 0 (w hyperviscosity) is not implemented");
   if (!d->Glob || !d->DS || !d->DW || !d->dXdxI || !d->J || !d->X || !d->M || !d->dz)
@@ -295,6 +300,13 @@ static int ctx_build(cgd_ctx *c, const cgd_desc *ds, int nmem, hipStream_t st) {
     if (b.Grav != a.Grav || b.Omega != a.Omega || b.Rd != a.Rd || b.Cpd != a.Cpd || b.p0 != a.p0 ||
         b.kappa != a.kappa)
       return fail(c, CGD_ERR_ARG, who + "physical constants must be the same for every member");
+    if (b.Damping != a.Damping || b.Forcing != a.Forcing ||
+        (a.Damping && (b.DampH != a.DampH || b.DampStride != a.DampStride || b.DampRelax != a.DampRelax)) ||
+        (a.Forcing && (b.HS_k_a != a.HS_k_a || b.HS_k_f != a.HS_k_f || b.HS_k_s != a.HS_k_s ||
+                       b.HS_DeltaT_y != a.HS_DeltaT_y || b.HS_DeltaTh_z != a.HS_DeltaTh_z ||
+                       b.HS_T_equator != a.HS_T_equator || b.HS_T_min != a.HS_T_min ||
+                       b.HS_sigma_b != a.HS_sigma_b)))
+      return fail(c, CGD_ERR_ARG, who + "physics hooks and their constants must be the same for every member");
   }
   const cgd_desc *d = &ds[0];
   c->desc = *d;
@@ -432,10 +444,37 @@ static int ctx_build(cgd_ctx *c, const cgd_desc *ds, int nmem, hipStream_t st) {
   HIPCHK(c, hipMemcpyAsync(c->d_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_kvis, kvis.data(), kvis.size() * sizeof(double), hipMemcpyHostToDevice, st));
   // M and dz in batch numbering (a single grid's arrays are used in place)
+  c->hooks.forcing = d->Forcing != 0;
+  c->hooks.damping = d->Damping != 0;
+  c->hooks.k_a = d->HS_k_a;
+  c->hooks.k_f = d->HS_k_f;
+  c->hooks.k_s = d->HS_k_s;
+  c->hooks.DeltaT_y = d->HS_DeltaT_y;
+  c->hooks.DeltaTh_z = d->HS_DeltaTh_z;
+  c->hooks.T_equator = d->HS_T_equator;
+  c->hooks.T_min = d->HS_T_min;
+  c->hooks.sigma_b = d->HS_sigma_b;
+  c->hooks.H = d->DampH;
+  c->hooks.Stride = d->DampStride;
+  c->hooks.Relax = d->DampRelax;
   if (nmem == 1) {
     c->M = d->M;
     c->dz = d->dz;
+    c->hooks.zP = d->zP;
+    c->hooks.xS = d->xS;
   } else {
+    if (d->Damping) HIPCHK(c, hipMalloc(&c->d_zP, ng * sizeof(double)));
+    if (d->Forcing) HIPCHK(c, hipMalloc(&c->d_xS, 2 * (size_t)NumG * sizeof(double)));
+    for (int m = 0; m < nmem; ++m) {
+      const size_t gm = (size_t)ds[m].NumG * Nz, o = (size_t)c->mem_g0[m] * Nz;
+      if (d->Damping)
+        HIPCHK(c, hipMemcpyAsync(c->d_zP + o, ds[m].zP, gm * sizeof(double), hipMemcpyDeviceToDevice, st));
+      if (d->Forcing)
+        HIPCHK(c, hipMemcpyAsync(c->d_xS + 2 * (size_t)c->mem_g0[m], ds[m].xS, 2 * (size_t)ds[m].NumG * sizeof(double),
+                                 hipMemcpyDeviceToDevice, st));
+    }
+    c->hooks.zP = c->d_zP;
+    c->hooks.xS = c->d_xS;
     HIPCHK(c, hipMalloc(&c->d_M, 2 * ng * sizeof(double)));
     HIPCHK(c, hipMalloc(&c->d_dz, ng * sizeof(double)));
     for (int m = 0; m < nmem; ++m) {
@@ -572,7 +611,7 @@ extern "C" int cgd_ctx_destroy(cgd_ctx *c) {
   for (void *p : {(void *)c->d_glob, (void *)c->d_slot, (void *)c->d_met, (void *)c->d_met1, (void *)c->d_metn,
                   (void *)c->d_met1n, (void *)c->d_ms, (void *)c->d_ims32, (void *)c->d_m32, (void *)c->d_metp32,
                   (void *)c->d_bwork, (void *)c->d_f4x, (void *)c->d_nptr, (void *)c->d_ncopy, (void *)c->d_kvis,
-                  (void *)c->d_M, (void *)c->d_dz})
+                  (void *)c->d_M, (void *)c->d_dz, (void *)c->d_zP, (void *)c->d_xS})
     if (p) (void)hipFree(p);
   prof_free(c);
   delete c;
@@ -757,6 +796,12 @@ static int run_fcn(cgd_ctx *c, R *F, const double *U, const R *UR, R *Temp1, con
         HIPCHK(c, hipGetLastError());
         ++launches;
       }
+  }
+  if (c->hooks.forcing || c->hooks.damping) {
+    const size_t n = (size_t)c->NumG * c->Nz;
+    hipLaunchKernelGGL((k_hooks<R>), dim3(grid_for(n, 256)), dim3(256), 0, st, c->ka, c->hooks, U, AuxG, F, n);
+    HIPCHK(c, hipGetLastError());
+    ++launches;
   }
   if constexpr (std::is_same<R, double>::value) {
     if (f4fix && c->nchunk > 1) {

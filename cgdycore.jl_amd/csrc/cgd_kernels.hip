@@ -912,6 +912,51 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
 }
 
 // ------------------------------------------------------------------------
+// Pointwise physics hooks at the end of FcnGPU! (FcnGPU.jl:953-966), per
+// global point: HeldSuarezForcing (src/Examples/force.jl:5-27) on F2, F3, F5
+// from rho, u, v, the pressure p and the latitude, then DampingW
+// (src/GPU/dampingGPU.jl:5-16) on F4 from w and the layer-centre height.
+struct Hooks {
+  int forcing, damping;
+  double k_a, k_f, k_s, DeltaT_y, DeltaTh_z, T_equator, T_min, sigma_b;
+  double H, Stride, Relax;
+  const double *xS;  // (2, NumG)
+  const double *zP;  // (Nz, NumG)
+};
+template <typename R>
+__global__ void k_hooks(KArgs a, Hooks hk, const double *__restrict__ U, const double *__restrict__ P,
+                        R *__restrict__ F, size_t n) {
+  const size_t vs = n;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    if (hk.forcing) {
+      const size_t ig = i / a.Nz;
+      const double rho = U[i], u = U[vs + i], v = U[2 * vs + i];
+      const double lat = hk.xS[2 * ig + 1];
+      const double Sigma = P[i] / a.p0;
+      const double SigmaPowKappa = exp(a.kappa * log(Sigma));  // fast_powGPU
+      const double hf = jmax(0.0, (Sigma - hk.sigma_b) / (1.0 - hk.sigma_b));
+      const double coslat = cos(lat), sinlat = sin(lat);
+      F[vs + i] = R(double(F[vs + i]) + -(hk.k_f * hf) * u);
+      F[2 * vs + i] = R(double(F[2 * vs + i]) + -(hk.k_f * hf) * v);
+      const double kT = Sigma < 0.7 ? hk.k_a + (hk.k_s - hk.k_a) * hf * coslat * coslat * coslat * coslat : 0.0;
+      double Teq = (hk.T_equator - hk.DeltaT_y * sinlat * sinlat - hk.DeltaTh_z * log(Sigma) * coslat * coslat) *
+                   SigmaPowKappa;
+      Teq = jmax(hk.T_min, Teq);
+      const double DeltaT = kT * (a.p0 * Sigma / (rho * a.Rd) - Teq);
+      F[4 * vs + i] = R(double(F[4 * vs + i]) + -rho * DeltaT / SigmaPowKappa);
+    }
+    if (hk.damping) {
+      const double z = hk.zP[i];
+      if (z >= hk.H - hk.Stride) {
+        const double sn = sin(0.5 * M_PI * (1.0 - (hk.H - z) / hk.Stride));
+        const double Damp = hk.Relax * (sn * sn);
+        F[3 * vs + i] = R(double(F[3 * vs + i]) + -Damp * U[3 * vs + i]);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
 // Tracers (NumTr > 0, FP64): components 5..4+NumTr of U/F (0-based), their
 // hyperviscosity caches at the same components of Temp1 (FcnGPU.jl:115-126).
 // Each tracer repeats the rho*theta path: HyperViscTracerKernel!

@@ -366,6 +366,19 @@ def layer_thickness(X: np.ndarray, glob: np.ndarray, num_g: int):
     return dz
 
 
+def node_lonlat(disc) -> np.ndarray:
+    """``xS`` of the grid: (NumG, 2) = Julia (2, NumG), longitude and latitude
+    of every global node from its first element copy in (IF, ID) order."""
+    X = disc.X
+    idx = disc.glob.reshape(-1) - 1
+    first = np.full(disc.num_g, -1, dtype=np.int64)
+    first[idx[::-1]] = np.arange(idx.size)[::-1]
+    e, i = np.divmod(first, disc.N * disc.N)
+    x, y, z = X[e, 0, 0, 0, i], X[e, 0, 1, 0, i], X[e, 0, 2, 0, i]
+    r = np.sqrt(x * x + y * y + z * z)
+    return np.stack([np.arctan2(y, x), np.arcsin(z / r)], axis=1)
+
+
 def build(ne: int, p: int = 4, nz: int = 64, H: float = 30000.0,
           rad: float = 6.37122e6) -> Discretization:
     """Grid + numbering + metric + mass for one configuration."""

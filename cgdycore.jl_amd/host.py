@@ -53,7 +53,11 @@ class _Desc(C.Structure):
                 ("HyperDCurl", C.c_double), ("HyperDGrad", C.c_double), ("HyperDDiv", C.c_double),
                 ("HyperDDivW", C.c_double),
                 ("Glob", _vp), ("DS", _vp), ("DW", _vp), ("dXdxI", _vp), ("J", _vp), ("X", _vp),
-                ("M", _vp), ("dz", _vp)]
+                ("M", _vp), ("dz", _vp),
+                ("DampH", C.c_double), ("DampStride", C.c_double), ("DampRelax", C.c_double), ("zP", _vp),
+                ("HS_k_a", C.c_double), ("HS_k_f", C.c_double), ("HS_k_s", C.c_double),
+                ("HS_DeltaT_y", C.c_double), ("HS_DeltaTh_z", C.c_double), ("HS_T_equator", C.c_double),
+                ("HS_T_min", C.c_double), ("HS_sigma_b", C.c_double), ("xS", _vp)]
 
 
 class LaunchShape(C.Structure):
@@ -172,6 +176,19 @@ class ModelStruct:
     HyperDGrad: float = 0.0
     HyperDDiv: float = 0.0
     HyperDDivW: float = 0.0
+    # DampingW (src/GPU/dampingGPU.jl:5-16): sponge on w above H - StrideDamp
+    DampH: float = 0.0
+    StrideDamp: float = 0.0
+    Relax: float = 0.0
+    # HeldSuarezForcing constants (ParamHeldSuarezDrySphere, src/Examples/parameters.jl:122-131)
+    HeldSuarez: dict = field(default_factory=lambda: held_suarez_param())
+
+
+def held_suarez_param() -> dict:
+    """``ParamHeldSuarezDrySphere{FT}()`` defaults (parameters.jl:122-131)."""
+    day = 3600.0 * 24.0
+    return dict(k_a=1.0 / (40.0 * day), k_f=1.0 / day, k_s=1.0 / (4.0 * day), DeltaT_y=60.0, DeltaTh_z=10.0,
+                T_equator=315.0, T_min=200.0, sigma_b=7.0 / 10.0)
 
 
 @dataclass
@@ -194,6 +211,8 @@ class MetricStruct:
     J: torch.Tensor
     X: torch.Tensor
     dz: torch.Tensor
+    zP: torch.Tensor | None = None  # (NumG, Nz) layer-centre heights (damping)
+    xS: torch.Tensor | None = None  # (NumG, 2) longitude, latitude (forcing)
 
 
 @dataclass
@@ -276,6 +295,17 @@ def _make_desc(FE: CGQuad, Metric: MetricStruct, Phys: Phys, Global: GlobalStruc
     d.Glob, d.DS, d.DW = _ptr(FE.Glob), _ptr(FE.DS), _ptr(FE.DW)
     d.dXdxI, d.J, d.X = _ptr(Metric.dXdxI), _ptr(Metric.J), _ptr(Metric.X)
     d.M, d.dz = _ptr(FE.M), _ptr(Metric.dz)
+    if M.Damping:
+        if Metric.zP is None:
+            raise CGDError("Damping needs Metric.zP (layer-centre heights)")
+        d.DampH, d.DampStride, d.DampRelax, d.zP = M.DampH, M.StrideDamp, M.Relax, _ptr(Metric.zP)
+    if M.Forcing:
+        if Metric.xS is None:
+            raise CGDError("Forcing needs Metric.xS (node longitude, latitude)")
+        hs = M.HeldSuarez
+        d.HS_k_a, d.HS_k_f, d.HS_k_s = hs["k_a"], hs["k_f"], hs["k_s"]
+        d.HS_DeltaT_y, d.HS_DeltaTh_z, d.HS_T_equator = hs["DeltaT_y"], hs["DeltaTh_z"], hs["T_equator"]
+        d.HS_T_min, d.HS_sigma_b, d.xS = hs["T_min"], hs["sigma_b"], _ptr(Metric.xS)
     return d
 
 
@@ -294,12 +324,13 @@ class NativeContext:
         self.L = L
         self.h = _vp()
         if members is None:
-            self._keep = (FE.Glob, FE.DS, FE.DW, Metric.dXdxI, Metric.J, Metric.X, FE.M, Metric.dz)
+            self._keep = (FE.Glob, FE.DS, FE.DW, Metric.dXdxI, Metric.J, Metric.X, FE.M, Metric.dz, Metric.zP,
+                          Metric.xS)
             d = _make_desc(FE, Metric, Phys, Global)
             rc = L.cgd_ctx_create(C.byref(d), C.byref(self.h), _stream())
             what = "cgd_ctx_create"
         else:
-            self._keep = [(f.Glob, f.DS, f.DW, m.dXdxI, m.J, m.X, f.M, m.dz) for f, m, _ in members]
+            self._keep = [(f.Glob, f.DS, f.DW, m.dXdxI, m.J, m.X, f.M, m.dz, m.zP, m.xS) for f, m, _ in members]
             descs = (_Desc * len(members))(*[_make_desc(f, m, Phys, g) for f, m, g in members])
             rc = L.cgd_ctx_create_batch(descs, len(members), C.byref(self.h), _stream())
             what = "cgd_ctx_create_batch"
@@ -529,11 +560,26 @@ def _work_arrays(num_g: int, nz: int, nT: int, dev, NumTr: int = 0):
 
 
 def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None = None, device="cuda",
-                 nT: int = 5, precision: str = "fp64", NumTr: int = 0):
+                 nT: int = 5, precision: str = "fp64", NumTr: int = 0, damping=None, forcing=None):
     """Upload the static inputs and allocate ``Cache``/``JCache`` (state
-    arrays with ``5 + NumTr`` components; ``Temp1`` with at least as many)."""
+    arrays with ``5 + NumTr`` components; ``Temp1`` with at least as many).
+    ``damping = (H, StrideDamp, Relax)`` switches the DampingW sponge on,
+    ``forcing = True`` (or a dict of Held-Suarez constants) the Held-Suarez
+    forcing."""
+    from . import discretization as _D
+
     dev = torch.device(device)
     FE, Metric, Global = _static_inputs(disc, phys, hyperd, dev, precision, NumTr)
+    f64 = dict(dtype=torch.float64, device=dev)
+    if damping is not None:
+        Global.Model.Damping = True
+        Global.Model.DampH, Global.Model.StrideDamp, Global.Model.Relax = damping
+        Metric.zP = torch.as_tensor(_D.layer_center_height(disc.X, disc.glob, disc.num_g, disc.rad), **f64)
+    if forcing:
+        Global.Model.Forcing = True
+        if isinstance(forcing, dict):
+            Global.Model.HeldSuarez = dict(held_suarez_param(), **forcing)
+        Metric.xS = torch.as_tensor(_D.node_lonlat(disc), **f64).contiguous()
     Cache, JCache = _work_arrays(disc.num_g, disc.nz, nT, dev, NumTr)
     return FE, Metric, Global, Cache, JCache
 

@@ -62,8 +62,9 @@ typedef struct {
   int NumV;   /* must be 5: rho, u, v, w, rho*theta */
   int NumTr;  /* tracers, 0..16 (FP64 mode only) */
   int precision; /* CGD_FP64 or CGD_MIXED */
-  /* model switches that select FcnGPU! branches; only the BASELINE set is
-   * implemented: HorLimit=0, KoeffDivW=0, no EDMF/TKE/physics hooks */
+  /* model switches that select FcnGPU! branches; implemented: Damping
+   * (DampingW) and Forcing (Held-Suarez); HorLimit, Turbulence,
+   * VerticalDiffusion, SurfaceFlux, Microphysics and EDMF must be 0 */
   int HorLimit;
   int Turbulence, VerticalDiffusion, SurfaceFlux, Forcing, Microphysics, Damping, EDMF;
   /* PhysParameters (GlobalVariables.jl:188-230) */
@@ -73,6 +74,17 @@ typedef struct {
   /* static device inputs (reference layouts) */
   const int64_t *Glob;
   const double *DS, *DW, *dXdxI, *J, *X, *M, *dz;
+  /* Damping != 0: DampingW (src/GPU/dampingGPU.jl:5-16), a Rayleigh sponge
+   * on w above DampH - DampStride: F_w -= DampRelax sin^2(pi/2 (1 - (DampH -
+   * zP)/DampStride)) w, with zP (Nz, NumG) the layer-centre heights
+   * (GridSizeSphereKernel!, src/DyCore/DiscretizationCG.jl:130-134) */
+  double DampH, DampStride, DampRelax;
+  const double *zP;
+  /* Forcing != 0: HeldSuarezForcing (src/Examples/force.jl:5-27) with the
+   * ParamHeldSuarezDrySphere constants (src/Examples/parameters.jl:122-131)
+   * and xS (2, NumG) = longitude, latitude of the global nodes */
+  double HS_k_a, HS_k_f, HS_k_s, HS_DeltaT_y, HS_DeltaTh_z, HS_T_equator, HS_T_min, HS_sigma_b;
+  const double *xS;
 } cgd_desc;
 
 typedef struct cgd_ctx cgd_ctx;

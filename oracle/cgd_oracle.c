@@ -20,6 +20,8 @@
  *   DivRhoTrUpwind3New2Kernel!       src/GPU/OperatorKernel.jl:637-760 (tracer loop :722-759)
  *   HyperViscTracerKernel!           src/GPU/ViscKernel.jl:169-218
  *   HyperViscTracerKoeffKernel!      src/GPU/ViscKernel.jl:277-327
+ *   HeldSuarezForcing / ForceKernel! src/Examples/force.jl:5-27, src/GPU/OperatorKernel.jl:1073-1080
+ *   DampingW / DampKernel!           src/GPU/dampingGPU.jl:5-16, src/GPU/OperatorKernel.jl:1061-1071
  *   Contra12/Grad12/Curl12/Rot12/Contra3/RecU4
  *                                    src/GPU/OperatorKernel.jl:953-1059
  *   JacSchurKernel!                  src/DyCore/JacSchur.jl:29-75
@@ -585,6 +587,34 @@ void cgdo_fcn(const cgdo_grid *g, double *F, const double *U, double *Temp1, con
   for (int t = 0; t < g->NumTr; ++t)
     for (int IF = 0; IF < g->NF; ++IF)
       for (int iz = 0; iz < g->Nz; ++iz) tracer_div_element(g, IF, iz, 5 + t, F, U);
+  /* ForceKernel! then DampKernel! (FcnGPU.jl:953-966), pointwise */
+  for (int ig = 0; ig < g->NumG; ++ig)
+    for (int iz = 0; iz < g->Nz; ++iz) {
+      if (g->Forcing) {
+        double p = Uv(AuxG, iz, ig, 0), lat = g->xS[2 * (size_t)ig + 1];
+        double Sigma = p / g->p0;
+        double SigmaPowKappa = pow(Sigma, g->kappa);
+        double height_factor = jmax(0.0, (Sigma - g->HS_sigma_b) / (1.0 - g->HS_sigma_b));
+        double coslat = cos(lat), sinlat = sin(lat);
+        Uv(F, iz, ig, 1) += -(g->HS_k_f * height_factor) * Uv(U, iz, ig, 1);
+        Uv(F, iz, ig, 2) += -(g->HS_k_f * height_factor) * Uv(U, iz, ig, 2);
+        double kT = Sigma < 0.7 ? g->HS_k_a + (g->HS_k_s - g->HS_k_a) * height_factor * coslat * coslat * coslat * coslat
+                                : 0.0;
+        double Teq = (g->HS_T_equator - g->HS_DeltaT_y * sinlat * sinlat -
+                      g->HS_DeltaTh_z * log(Sigma) * coslat * coslat) * SigmaPowKappa;
+        Teq = jmax(g->HS_T_min, Teq);
+        double DeltaT = kT * (g->p0 * Sigma / (Uv(U, iz, ig, 0) * g->Rd) - Teq);
+        Uv(F, iz, ig, 4) += -Uv(U, iz, ig, 0) * DeltaT / SigmaPowKappa;
+      }
+      if (g->Damping) {
+        double z = g->zP[(size_t)iz + (size_t)g->Nz * ig];
+        if (z >= g->DampH - g->DampStride) {
+          const double pi = 3.14159265358979323846; /* Julia's pi as a Float64 */
+          double sn = sin(0.5 * pi * (1.0 - (g->DampH - z) / g->DampStride));
+          Uv(F, iz, ig, 3) += -(g->DampRelax * (sn * sn)) * Uv(U, iz, ig, 3);
+        }
+      }
+    }
 }
 
 /* JacSchurKernel! (JacSchur.jl:29-75); p = AuxG[:,:,4] for State = Dry */

@@ -20,6 +20,12 @@ typedef struct {
   double Grav, Omega, Rd, Cpd, p0, kappa;
   double KoeffCurl, KoeffGrad, KoeffDiv;
   int NumTr; /* tracers: components 6..5+NumTr of U, F, fV, V, Vn, k and Temp1 */
+  /* physics hooks at the end of FcnGPU! (FcnGPU.jl:953-966) */
+  int Damping, Forcing;
+  double DampH, DampStride, DampRelax;
+  const double *zP; /* (Nz, NumG) layer-centre heights */
+  double HS_k_a, HS_k_f, HS_k_s, HS_DeltaT_y, HS_DeltaTh_z, HS_T_equator, HS_T_min, HS_sigma_b;
+  const double *xS; /* (2, NumG) longitude, latitude */
 } cgdo_grid;
 
 typedef struct {

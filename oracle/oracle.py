@@ -27,7 +27,11 @@ class _Grid(C.Structure):
                 ("Grav", C.c_double), ("Omega", C.c_double), ("Rd", C.c_double),
                 ("Cpd", C.c_double), ("p0", C.c_double), ("kappa", C.c_double),
                 ("KoeffCurl", C.c_double), ("KoeffGrad", C.c_double), ("KoeffDiv", C.c_double),
-                ("NumTr", C.c_int)]
+                ("NumTr", C.c_int), ("Damping", C.c_int), ("Forcing", C.c_int),
+                ("DampH", C.c_double), ("DampStride", C.c_double), ("DampRelax", C.c_double), ("zP", _dp),
+                ("HS_k_a", C.c_double), ("HS_k_f", C.c_double), ("HS_k_s", C.c_double),
+                ("HS_DeltaT_y", C.c_double), ("HS_DeltaTh_z", C.c_double), ("HS_T_equator", C.c_double),
+                ("HS_T_min", C.c_double), ("HS_sigma_b", C.c_double), ("xS", _dp)]
 
 
 class _Work(C.Structure):
@@ -49,6 +53,33 @@ def lib():
     return _lib
 
 
+def layer_center_height(disc) -> np.ndarray:
+    """zP (NumG, Nz) of GridSizeSphereKernel! (DiscretizationCG.jl:130-134):
+    max(|X1 + X2| / 2 - Rad, 0) from the last element copy in (IF, ID) order."""
+    X = disc.X
+    nf, nz, _, _, nn = X.shape
+    xm = 0.5 * (X[:, :, :, 0, :] + X[:, :, :, 1, :])
+    r = np.sqrt(xm[:, :, 0] * xm[:, :, 0] + xm[:, :, 1] * xm[:, :, 1] + xm[:, :, 2] * xm[:, :, 2])
+    vals = np.transpose(np.maximum(r - disc.rad, 0.0), (0, 2, 1)).reshape(nf * nn, nz)
+    idx = disc.glob.reshape(-1) - 1
+    last = np.full(disc.num_g, -1, dtype=np.int64)
+    last[idx] = np.arange(idx.size)
+    return vals[last]
+
+
+def node_lonlat(disc) -> np.ndarray:
+    """(NumG, 2) = Julia xS (2, NumG): longitude and latitude of every global
+    node from its first element copy in (IF, ID) order."""
+    X = disc.X
+    idx = disc.glob.reshape(-1) - 1
+    first = np.full(disc.num_g, -1)
+    first[idx[::-1]] = np.arange(idx.size)[::-1]
+    e, i = np.divmod(first, disc.N * disc.N)
+    x, y, z = X[e, 0, 0, 0, i], X[e, 0, 1, 0, i], X[e, 0, 2, 0, i]
+    r = np.sqrt(x * x + y * y + z * z)
+    return np.stack([np.arctan2(y, x), np.arcsin(z / r)], axis=1)
+
+
 def _p(a: np.ndarray):
     assert a.flags["C_CONTIGUOUS"] and a.dtype == np.float64
     return a.ctypes.data_as(_dp)
@@ -57,7 +88,11 @@ def _p(a: np.ndarray):
 class Oracle:
     """Binds one discretization (static inputs) to the oracle."""
 
-    def __init__(self, disc, phys=None, koeff=(0.0, 0.0, 0.0), NumTr=0):
+    def __init__(self, disc, phys=None, koeff=(0.0, 0.0, 0.0), NumTr=0, damping=None, forcing=None):
+        """``damping``: (H, StrideDamp, Relax) of DampingW with zP computed
+        from the metric; ``forcing``: dict of Held-Suarez constants (k_a,
+        k_f, k_s, DeltaT_y, DeltaTh_z, T_equator, T_min, sigma_b) with the
+        node latitudes of the grid."""
         from_phys = phys
         if from_phys is None:
             raise ValueError("phys constants required")
@@ -81,6 +116,19 @@ class Oracle:
         g.KoeffCurl, g.KoeffGrad, g.KoeffDiv = koeff
         g.NumTr = int(NumTr)
         self.nv = 5 + int(NumTr)
+        if damping is not None:
+            zP = np.ascontiguousarray(layer_center_height(disc))
+            self._keep.append(zP)
+            g.Damping = 1
+            g.DampH, g.DampStride, g.DampRelax = damping
+            g.zP = _p(zP)
+        if forcing is not None:
+            xS = np.ascontiguousarray(node_lonlat(disc))
+            self._keep.append(xS)
+            g.Forcing = 1
+            for k in ("k_a", "k_f", "k_s", "DeltaT_y", "DeltaTh_z", "T_equator", "T_min", "sigma_b"):
+                setattr(g, "HS_" + k, forcing[k])
+            g.xS = _p(xS)
         self.g = g
         self.L = lib()
 

@@ -68,6 +68,10 @@ def _desc(pkg, **over):
         (dict(NumTr=-1), -2, "NumTr must be in"),
         (dict(precision=7), -2, "precision"),
         (dict(HorLimit=1), -2, "dry CompressibleShallow"),
+        (dict(Damping=1), -1, "Damping needs zP"),
+        (dict(Damping=1, zP=16, DampH=3e4, DampStride=0.0), -1, "DampStride > 0"),
+        (dict(Forcing=1), -1, "Forcing needs xS"),
+        (dict(Forcing=1, xS=16, HS_sigma_b=1.5), -1, "HS_sigma_b < 1"),
         (dict(Microphysics=1), -2, "dry CompressibleShallow"),
         (dict(HyperDDivW=1e14), -2, "HyperDDivW"),
         (dict(dz=None), -1, "null static input"),

@@ -1,0 +1,97 @@
+"""Physics hooks on the GPU (DampingW sponge, Held-Suarez forcing) against
+the oracle, through the C ABI: tendency and Rosenbrock-W steps to 1e-12
+(FP64), the mixed-precision step to the mixed bound."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+DAMP = (30000.0, 10000.0, 1.0 / 3600.0)
+
+
+def rel_l2(a, b):
+    den = np.linalg.norm(np.ravel(b))
+    return float(np.linalg.norm(np.ravel(a - b)) / (den if den > 0 else 1.0))
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def case(pkg, oracle_mod, ne, nz, precision="fp64", ntr=0):
+    D, host = pkg.discretization, pkg.host
+    disc = D.build(ne, 4, nz, 30000.0)
+    phys = D.Phys()
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda", precision=precision,
+                                                          NumTr=ntr, damping=DAMP, forcing=True)
+    hd = host.hyperdiffusion_for(ne)
+    orc = oracle_mod.Oracle(disc, phys, (hd, hd, hd), NumTr=ntr, damping=DAMP,
+                            forcing=host.held_suarez_param())
+    U0 = pkg.state.jet_state(disc, seed=6)
+    rng = np.random.default_rng(6)
+    U0[3] += 0.3 * rng.standard_normal(U0[3].shape)
+    U0[3][:, -1] = 0.0
+    if ntr:
+        U0 = np.concatenate([U0] + [0.01 * U0[0:1]] * ntr, axis=0)
+    return disc, phys, (FE, Metric, Global, Cache, JCache), orc, U0
+
+
+@pytest.mark.parametrize("ne,nz", [(3, 12), (4, 64)])
+def test_hooks_tendency(pkg, oracle_mod, torch_cuda, ne, nz):
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, (FE, Metric, Global, Cache, JCache), orc, U0 = case(pkg, oracle_mod, ne, nz)
+    print(f"hooks tendency ne={ne} nz={nz}", flush=True)
+    U = torch.as_tensor(U0, device="cuda")
+    F = torch.empty_like(U)
+    eq = host.CompressibleShallow()
+    host.FcnPrepareGPU(U, FE, Metric, phys, Cache, None, Global, None, eq)
+    host.FcnGPU(F, U, FE, Metric, phys, Cache, None, Global, None, eq)
+    torch.cuda.synchronize()
+    Fo, _ = orc.fcn(U0.copy(), orc.prepare(U0.copy()))
+    Fg = F.cpu().numpy()
+    for v in range(5):
+        assert rel_l2(Fg[v], Fo[v]) <= TOL, f"F{v + 1}: {rel_l2(Fg[v], Fo[v]):.3e}"
+
+
+@pytest.mark.parametrize("ne,nz,ntr", [(3, 12, 0), (4, 40, 1)])
+def test_hooks_steps(pkg, oracle_mod, torch_cuda, ne, nz, ntr):
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, (FE, Metric, Global, Cache, JCache), orc, U0 = case(pkg, oracle_mod, ne, nz, ntr=ntr)
+    print(f"hooks steps ne={ne} nz={nz} NumTr={ntr}", flush=True)
+    V = torch.as_tensor(U0, device="cuda").clone()
+    host.TimeStepper(V, 300.0, 2, FE, Metric, phys, Cache, JCache, Global)
+    torch.cuda.synchronize()
+    Vo = U0.copy()
+    orc.rw_step(Vo, 300.0, 2)
+    Vg = V.cpu().numpy()
+    for c in range(5 + ntr):
+        assert rel_l2(Vg[c], Vo[c]) <= TOL, f"V{c + 1}: {rel_l2(Vg[c], Vo[c]):.3e}"
+    # the hooks act: without them the state differs
+    FE0, Metric0, Global0, Cache0, JCache0 = host.device_setup(disc, phys, device="cuda", NumTr=ntr)
+    W = torch.as_tensor(U0, device="cuda").clone()
+    host.TimeStepper(W, 300.0, 2, FE0, Metric0, phys, Cache0, JCache0, Global0)
+    torch.cuda.synchronize()
+    assert rel_l2(W[3].cpu().numpy(), Vg[3]) > 1e-6
+
+
+def test_hooks_mixed_step(pkg, oracle_mod, torch_cuda):
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, (FE, Metric, Global, Cache, JCache), orc, U0 = case(pkg, oracle_mod, 4, 32, precision="mixed")
+    V = torch.as_tensor(U0, device="cuda").clone()
+    host.TimeStepper(V, 300.0, 3, FE, Metric, phys, Cache, JCache, Global)
+    torch.cuda.synchronize()
+    Vo = U0.copy()
+    orc.rw_step(Vo, 300.0, 3)
+    Vg = V.cpu().numpy()
+    errs = [rel_l2(Vg[c], Vo[c]) for c in range(5)]
+    print("  mixed errors", errs, flush=True)
+    assert max(errs[0], errs[1], errs[2], errs[4]) <= 1e-5 and errs[3] <= 1e-4, errs

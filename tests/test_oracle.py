@@ -470,3 +470,70 @@ def test_tracers_are_passive_in_the_step(pkg, oracle_mod):
     orc0.rw_step(V0, 300.0, 2)
     assert np.array_equal(V[:5], V0)
     assert np.isfinite(V).all() and not np.array_equal(V[5], q)
+
+
+# ---- physics hooks (FcnGPU.jl:953-966) ----
+
+def _hooks_case(pkg, oracle_mod, damping=None, forcing=None):
+    disc = disc_for(pkg, 3, 12)
+    phys = pkg.discretization.Phys()
+    U0 = pkg.state.jet_state(disc, seed=9)
+    rng = np.random.default_rng(9)
+    U0[3] += 0.3 * rng.standard_normal(U0[3].shape)
+    U0[3][:, -1] = 0.0
+    hd = 5e14 * (30 / 3) ** 3.2
+    base = oracle_mod.Oracle(disc, phys, (hd, hd, hd))
+    hooked = oracle_mod.Oracle(disc, phys, (hd, hd, hd), damping=damping, forcing=forcing)
+    A = base.prepare(U0.copy())
+    F0, _ = base.fcn(U0.copy(), A)
+    F1, _ = hooked.fcn(U0.copy(), A)
+    return disc, U0, A, F1 - F0, F0
+
+
+def _close(dF, want, F0, rtol=1e-13):
+    """dF = (F0 + term) - F0 carries the rounding of the sum: 2 ulp of |F0|."""
+    return np.all(np.abs(dF - want) <= rtol * np.abs(want) + 4.5e-16 * np.abs(F0) + 1e-300)
+
+
+def test_damping_sponge_profile(pkg, oracle_mod):
+    """DampingW: the w tendency gains -Relax sin^2(pi/2 (1 - (H - z)/S)) w in
+    the top S metres of the layer centres zP and nothing below; the other
+    components are untouched."""
+    H, S, relax = 30000.0, 10000.0, 1.0 / 3600.0
+    disc, U0, A, dF, F0 = _hooks_case(pkg, oracle_mod, damping=(H, S, relax))
+    zP = oracle_mod.layer_center_height(disc)  # (NumG, Nz)
+    assert np.all(dF[[0, 1, 2, 4]] == 0.0)
+    below = zP < H - S
+    assert np.all(dF[3][below] == 0.0)
+    coef = relax * np.sin(0.5 * np.pi * (1.0 - (H - zP) / S)) ** 2
+    want = np.where(below, 0.0, -coef * U0[3])
+    assert _close(dF[3], want, F0[3])
+    top = zP[:, -1]
+    assert np.all(coef[:, -1] <= relax) and np.all(coef[:, -1] > 0.9 * relax)  # near full strength at the lid
+
+
+def test_held_suarez_forcing(pkg, oracle_mod):
+    """HeldSuarezForcing with the reference's branch on sigma < 0.7: winds are
+    damped by k_f max(0, (sigma - sigma_b)/(1 - sigma_b)) (zero above
+    sigma_b), and rho*theta relaxes towards T_eq(lat, sigma) where
+    sigma < 0.7: dF5 = -rho k_T (T - T_eq) / sigma^kappa with T = p/(rho Rd)."""
+    hs = pkg.host.held_suarez_param()
+    disc, U0, A, dF, F0 = _hooks_case(pkg, oracle_mod, forcing=hs)
+    phys = pkg.discretization.Phys()
+    lat = oracle_mod.node_lonlat(disc)[:, 1][:, None]
+    p = A[0]
+    sigma = p / phys.p0
+    hf = np.maximum(0.0, (sigma - hs["sigma_b"]) / (1.0 - hs["sigma_b"]))
+    assert _close(dF[1], -hs["k_f"] * hf * U0[1], F0[1])
+    assert _close(dF[2], -hs["k_f"] * hf * U0[2], F0[2])
+    assert np.all(dF[1][sigma < hs["sigma_b"]] == 0.0) and np.any(dF[1] != 0.0)
+    kT = np.where(sigma < 0.7, hs["k_a"] + (hs["k_s"] - hs["k_a"]) * hf * np.cos(lat) ** 4, 0.0)
+    Teq = np.maximum(hs["T_min"], (hs["T_equator"] - hs["DeltaT_y"] * np.sin(lat) ** 2
+                                   - hs["DeltaTh_z"] * np.log(sigma) * np.cos(lat) ** 2) * sigma ** phys.kappa)
+    T = p / (U0[0] * phys.Rd)
+    want = -U0[0] * kT * (T - Teq) / sigma ** phys.kappa
+    assert _close(dF[4], want, F0[4], rtol=1e-11)
+    # relaxation towards T_eq: the theta tendency opposes T - T_eq
+    m = (kT > 0) & (np.abs(T - Teq) > 1e-6)
+    assert np.all(np.sign(dF[4][m]) == -np.sign((T - Teq)[m]))
+    assert np.all(dF[[0, 3]] == 0.0)
