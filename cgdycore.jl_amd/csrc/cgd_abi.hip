@@ -275,6 +275,14 @@ static int validate_desc(cgd_ctx *c, const cgd_desc *d, const std::string &who) 
     return fail(c, CGD_ERR_ARG, who + "Damping needs zP, DampH > 0 and DampStride > 0");
   if (d->Forcing && (!d->xS || !(d->HS_sigma_b > 0.0 && d->HS_sigma_b < 1.0)))
     return fail(c, CGD_ERR_ARG, who + "Forcing needs xS and 0 < HS_sigma_b < 1");
+  if (d->State != CGD_DRY && d->State != CGD_MOIST) return fail(c, CGD_ERR_UNSUPPORTED, who + "unknown State");
+  if (d->State == CGD_MOIST) {
+    const int hi = 5 + d->NumTr;
+    if (d->RhoVPos < 6 || d->RhoVPos > hi || d->RhoCPos < 6 || d->RhoCPos > hi || d->RhoVPos == d->RhoCPos)
+      return fail(c, CGD_ERR_ARG, who + "Moist needs RhoVPos != RhoCPos among the tracer components 6..5+NumTr");
+    if (!(d->Cpv > 0.0 && d->Cpl > 0.0 && d->Rv > 0.0))
+      return fail(c, CGD_ERR_ARG, who + "Moist needs Cpv, Cpl, Rv > 0");
+  }
   if (d->HyperDDivW != 0.0)
     return fail(c, CGD_ERR_UNSUPPORTED, who + "HyperDDivW > 0 (w hyperviscosity) is not implemented");
   if (!d->Glob || !d->DS || !d->DW || !d->dXdxI || !d->J || !d->X || !d->M || !d->dz)
@@ -300,6 +308,9 @@ static int ctx_build(cgd_ctx *c, const cgd_desc *ds, int nmem, hipStream_t st) {
     if (b.Grav != a.Grav || b.Omega != a.Omega || b.Rd != a.Rd || b.Cpd != a.Cpd || b.p0 != a.p0 ||
         b.kappa != a.kappa)
       return fail(c, CGD_ERR_ARG, who + "physical constants must be the same for every member");
+    if (b.State != a.State || (a.State == CGD_MOIST && (b.RhoVPos != a.RhoVPos || b.RhoCPos != a.RhoCPos ||
+                                                        b.Cpv != a.Cpv || b.Cpl != a.Cpl || b.Rv != a.Rv)))
+      return fail(c, CGD_ERR_ARG, who + "State and the moist constants must be the same for every member");
     if (b.Damping != a.Damping || b.Forcing != a.Forcing ||
         (a.Damping && (b.DampH != a.DampH || b.DampStride != a.DampStride || b.DampRelax != a.DampRelax)) ||
         (a.Forcing && (b.HS_k_a != a.HS_k_a || b.HS_k_f != a.HS_k_f || b.HS_k_s != a.HS_k_s ||
@@ -576,6 +587,11 @@ static int ctx_build(cgd_ctx *c, const cgd_desc *ds, int nmem, hipStream_t st) {
   a.metp32 = c->d_metp32;
   a.M = c->M;
   a.kvis = c->d_kvis;
+  a.iv = d->State == CGD_MOIST ? d->RhoVPos - 1 : -1;
+  a.ic = d->State == CGD_MOIST ? d->RhoCPos - 1 : -1;
+  a.Cpv = d->Cpv;
+  a.Cpl = d->Cpl;
+  a.Rv = d->Rv;
   c->ok = true;
   return CGD_OK;
 }
@@ -893,6 +909,10 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
   if (begin) {
     hipLaunchKernelGGL(k_rw_begin, dim3(grid_for((ng + 3) / 4, 256)), dim3(256), 0, st, c->ka, V, w->AuxG, V32, ng);
     HIPCHK(c, hipGetLastError());
+    if (c->ka.iv >= 0) {
+      hipLaunchKernelGGL(k_moist_p, dim3(grid_for(ng, 256)), dim3(256), 0, st, c->ka, V, w->AuxG, ng);
+      HIPCHK(c, hipGetLastError());
+    }
   }
   for (int is = 0; is < 3; ++is) {
     if ((r = run_fcn<R>(c, F, V, UR, T1, w->AuxG, st, false))) return r;
@@ -941,6 +961,11 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
       else
         hipLaunchKernelGGL(k_rw_tr<2>, gt, bt, 0, st, fvt, k0, k1, cf0, cf1, rw.av[0], rw.av[1], rw.av[2], rw.fac,
                            V + o, w->Vn + o, kout, nt);
+      HIPCHK(c, hipGetLastError());
+    }
+    // Models.Moist: the pressure of the new state needs its tracers too
+    if (c->ka.iv >= 0 && rw.P) {
+      hipLaunchKernelGGL(k_moist_p, dim3(grid_for(ng, 256)), dim3(256), 0, st, c->ka, V, w->AuxG, ng);
       HIPCHK(c, hipGetLastError());
     }
     if ((r = prof_end(c, st, 2, 1))) return r;

@@ -46,6 +46,9 @@ struct KArgs {
   const float *metp32;             // compact layout's per-point components in FP32 (mixed precision)
   const double *M;                 // [2][NumG][Nz]
   const double *kvis;              // [slot][3]: HyperDCurl, HyperDGrad, HyperDDiv of the element's grid
+  // Models.Moist: 0-based components of the vapour and cloud densities (-1: dry)
+  int iv, ic;
+  double Cpv, Cpl, Rv;
 };
 
 // metric component indices of met (dXdxI[a,b,K] -> dabK, 1-based a,b,K)
@@ -1431,14 +1434,46 @@ __global__ void __launch_bounds__(256) k_schur_col(KArgs a, double *__restrict__
 
 // ------------------------------------------------------------------------
 // PressureKernel! with Models.Dry (DiagnosticKernel.jl:96-115)
+// Models.Moist (src/Models/Equation.jl:56-75) at one point: kappa_m = R_m / c_pml
+// of the dry air, vapour and cloud densities; p = (Rd rho*theta /
+// p0^kappa_m)^(1/(1-kappa_m)), T = p / R_m, theta = (p0/p)^kappa_m T
+__device__ __forceinline__ void moist_ptth(const KArgs &a, double rho, double rth, double rv, double rc, double &p,
+                                           double &T, double &th) {
+  const double rd = rho - rv - rc;
+  const double Cpml = a.Cpd * rd + a.Cpv * rv + a.Cpl * rc;
+  const double Rm = a.Rd * rd + a.Rv * rv;
+  const double km = Rm / Cpml;
+  p = pow(a.Rd * rth / pow(a.p0, km), 1.0 / (1.0 - km));
+  T = p / Rm;
+  th = pow(a.p0 / p, km) * T;
+}
+
 __global__ void k_prepare(KArgs a, const double *__restrict__ U, double *__restrict__ AuxG, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     const double rho = U[i], rth = U[i + 4 * n];
+    if (a.iv >= 0) {
+      double p, T, th;
+      moist_ptth(a, rho, rth, U[i + (size_t)a.iv * n], U[i + (size_t)a.ic * n], p, T, th);
+      AuxG[i] = p;
+      AuxG[i + n] = T;
+      AuxG[i + 2 * n] = th;
+      continue;
+    }
     const double p = eos_p(a, rth);
     const double T = p / (a.Rd * rho);
     AuxG[i] = p;
     AuxG[i + n] = T;
     AuxG[i + 2 * n] = pow(a.p0 / p, a.Rd / a.Cpd) * T;
+  }
+}
+
+// the moist pressure of a state, overwriting the dry one that the fused
+// step kernels wrote (only p: FcnGPU! reads AuxG[:,:,1])
+__global__ void k_moist_p(KArgs a, const double *__restrict__ V, double *__restrict__ P, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double p, T, th;
+    moist_ptth(a, V[i], V[i + 4 * n], V[i + (size_t)a.iv * n], V[i + (size_t)a.ic * n], p, T, th);
+    P[i] = p;
   }
 }
 

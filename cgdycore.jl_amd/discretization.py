@@ -41,11 +41,18 @@ class Phys:
     Grav: float = 9.80616
     Cpd: float = 1004.0
     Cvd: float = 717.0
+    Cpv: float = 1885.0
+    Cvv: float = 1424.0
+    Cpl: float = 4186.0
     p0: float = 1.0e5
 
     @property
     def Rd(self):
         return self.Cpd - self.Cvd
+
+    @property
+    def Rv(self):
+        return self.Cpv - self.Cvv
 
     @property
     def kappa(self):

@@ -57,7 +57,9 @@ class _Desc(C.Structure):
                 ("DampH", C.c_double), ("DampStride", C.c_double), ("DampRelax", C.c_double), ("zP", _vp),
                 ("HS_k_a", C.c_double), ("HS_k_f", C.c_double), ("HS_k_s", C.c_double),
                 ("HS_DeltaT_y", C.c_double), ("HS_DeltaTh_z", C.c_double), ("HS_T_equator", C.c_double),
-                ("HS_T_min", C.c_double), ("HS_sigma_b", C.c_double), ("xS", _vp)]
+                ("HS_T_min", C.c_double), ("HS_sigma_b", C.c_double), ("xS", _vp),
+                ("State", C.c_int), ("RhoVPos", C.c_int), ("RhoCPos", C.c_int),
+                ("Cpv", C.c_double), ("Cpl", C.c_double), ("Rv", C.c_double)]
 
 
 class LaunchShape(C.Structure):
@@ -161,7 +163,9 @@ class CompressibleShallow:
 class ModelStruct:
     """The ``Global.Model`` switches the path reads (GlobalVariables.jl:248-329)."""
 
-    State: str = "Dry"
+    State: str = "Dry"  # or "Moist" (vapour and cloud densities at RhoVPos, RhoCPos, 1-based)
+    RhoVPos: int = 0
+    RhoCPos: int = 0
     NumV: int = 5
     NumTr: int = 0
     HorLimit: bool = False
@@ -299,6 +303,11 @@ def _make_desc(FE: CGQuad, Metric: MetricStruct, Phys: Phys, Global: GlobalStruc
         if Metric.zP is None:
             raise CGDError("Damping needs Metric.zP (layer-centre heights)")
         d.DampH, d.DampStride, d.DampRelax, d.zP = M.DampH, M.StrideDamp, M.Relax, _ptr(Metric.zP)
+    if M.State not in ("Dry", "Moist"):
+        raise CGDError(f"unknown State {M.State!r}")
+    if M.State == "Moist":
+        d.State, d.RhoVPos, d.RhoCPos = 1, M.RhoVPos, M.RhoCPos
+        d.Cpv, d.Cpl, d.Rv = Phys.Cpv, Phys.Cpl, Phys.Rv
     if M.Forcing:
         if Metric.xS is None:
             raise CGDError("Forcing needs Metric.xS (node longitude, latitude)")
@@ -560,12 +569,13 @@ def _work_arrays(num_g: int, nz: int, nT: int, dev, NumTr: int = 0):
 
 
 def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None = None, device="cuda",
-                 nT: int = 5, precision: str = "fp64", NumTr: int = 0, damping=None, forcing=None):
+                 nT: int = 5, precision: str = "fp64", NumTr: int = 0, damping=None, forcing=None, moist=None):
     """Upload the static inputs and allocate ``Cache``/``JCache`` (state
     arrays with ``5 + NumTr`` components; ``Temp1`` with at least as many).
     ``damping = (H, StrideDamp, Relax)`` switches the DampingW sponge on,
     ``forcing = True`` (or a dict of Held-Suarez constants) the Held-Suarez
-    forcing."""
+    forcing, ``moist = (RhoVPos, RhoCPos)`` the moist equation of state with
+    vapour and cloud at those (1-based) tracer components."""
     from . import discretization as _D
 
     dev = torch.device(device)
@@ -575,6 +585,9 @@ def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None
         Global.Model.Damping = True
         Global.Model.DampH, Global.Model.StrideDamp, Global.Model.Relax = damping
         Metric.zP = torch.as_tensor(_D.layer_center_height(disc.X, disc.glob, disc.num_g, disc.rad), **f64)
+    if moist is not None:
+        Global.Model.State = "Moist"
+        Global.Model.RhoVPos, Global.Model.RhoCPos = moist
     if forcing:
         Global.Model.Forcing = True
         if isinstance(forcing, dict):

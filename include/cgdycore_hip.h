@@ -85,7 +85,15 @@ typedef struct {
    * and xS (2, NumG) = longitude, latitude of the global nodes */
   double HS_k_a, HS_k_f, HS_k_s, HS_DeltaT_y, HS_DeltaTh_z, HS_T_equator, HS_T_min, HS_sigma_b;
   const double *xS;
+  /* State: CGD_DRY or CGD_MOIST (Models.Moist, src/Models/Equation.jl:56-75):
+   * the pressure from rho, rho*theta and the vapour and cloud densities at
+   * the 1-based components RhoVPos, RhoCPos (tracers, so > 5), with Cpv, Cpl,
+   * Rv (PhysParameters); dPresdRhoTh stays the dry one, as in the reference */
+  int State, RhoVPos, RhoCPos;
+  double Cpv, Cpl, Rv;
 } cgd_desc;
+
+enum { CGD_DRY = 0, CGD_MOIST = 1 };
 
 typedef struct cgd_ctx cgd_ctx;
 
@@ -112,8 +120,8 @@ int cgd_ctx_destroy(cgd_ctx *ctx);
 const char *cgd_last_error(const cgd_ctx *ctx);
 
 /* FcnPrepareGPU!(U,...) -- src/GPU/DiagnosticKernel.jl:180-241 (PressureKernel!
- * :96-115 with Models.Dry, src/Models/Equation.jl:25-32).  Writes
- * AuxG[:,:,1:3] = (p, T, theta). */
+ * :96-115 with Models.Dry or Models.Moist, src/Models/Equation.jl:25-75).
+ * Writes AuxG[:,:,1:3] = (p, T, theta). */
 int cgd_prepare(cgd_ctx *ctx, const double *U, double *AuxG, void *stream);
 
 /* FcnGPU!(F,U,...,Models.CompressibleShallow) -- src/GPU/FcnGPU.jl:552-913,

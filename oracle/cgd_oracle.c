@@ -10,6 +10,7 @@
  * node) order:
  *   PressureKernel!                  src/GPU/DiagnosticKernel.jl:96-115
  *   Dry EOS / dPresdRhoTh            src/Models/Equation.jl:25-38,79
+ *   Moist EOS (Models.Moist)         src/Models/Equation.jl:56-75
  *   FcnGPU! (CompressibleShallow)    src/GPU/FcnGPU.jl:552-913
  *   HyperViscKernel!                 src/GPU/ViscKernel.jl:1-84
  *   HyperViscKoeffKernel!            src/GPU/ViscKernel.jl:86-167
@@ -127,9 +128,22 @@ void cgdo_prepare(const cgdo_grid *g, const double *U, double *AuxG) {
   for (int ig = 0; ig < g->NumG; ++ig)
     for (int iz = 0; iz < Nz; ++iz) {
       double rho = Uv(U, iz, ig, 0), rhoth = Uv(U, iz, ig, 4);
-      double p = eos_pressure(g, rhoth);
-      double T = p / (g->Rd * rho);
-      double th = pow(g->p0 / p, g->Rd / g->Cpd) * T;
+      double p, T, th;
+      if (g->State == 1) {
+        /* Models.Moist (Equation.jl:56-75) */
+        double RhoV = Uv(U, iz, ig, g->RhoVPos - 1), RhoC = Uv(U, iz, ig, g->RhoCPos - 1);
+        double RhoD = rho - RhoV - RhoC;
+        double Cpml = g->Cpd * RhoD + g->Cpv * RhoV + g->Cpl * RhoC;
+        double Rm = g->Rd * RhoD + g->Rv * RhoV;
+        double kappaM = Rm / Cpml;
+        p = pow(g->Rd * rhoth / pow(g->p0, kappaM), 1.0 / (1.0 - kappaM));
+        T = p / Rm;
+        th = pow(g->p0 / p, kappaM) * T;
+      } else {
+        p = eos_pressure(g, rhoth);
+        T = p / (g->Rd * rho);
+        th = pow(g->p0 / p, g->Rd / g->Cpd) * T;
+      }
       Uv(AuxG, iz, ig, 0) = p;
       Uv(AuxG, iz, ig, 1) = T;
       Uv(AuxG, iz, ig, 2) = th;

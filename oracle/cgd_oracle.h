@@ -26,6 +26,8 @@ typedef struct {
   const double *zP; /* (Nz, NumG) layer-centre heights */
   double HS_k_a, HS_k_f, HS_k_s, HS_DeltaT_y, HS_DeltaTh_z, HS_T_equator, HS_T_min, HS_sigma_b;
   const double *xS; /* (2, NumG) longitude, latitude */
+  int State, RhoVPos, RhoCPos; /* State 1: Models.Moist, 1-based vapour / cloud components */
+  double Cpv, Cpl, Rv;
 } cgdo_grid;
 
 typedef struct {

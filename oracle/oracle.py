@@ -31,7 +31,9 @@ class _Grid(C.Structure):
                 ("DampH", C.c_double), ("DampStride", C.c_double), ("DampRelax", C.c_double), ("zP", _dp),
                 ("HS_k_a", C.c_double), ("HS_k_f", C.c_double), ("HS_k_s", C.c_double),
                 ("HS_DeltaT_y", C.c_double), ("HS_DeltaTh_z", C.c_double), ("HS_T_equator", C.c_double),
-                ("HS_T_min", C.c_double), ("HS_sigma_b", C.c_double), ("xS", _dp)]
+                ("HS_T_min", C.c_double), ("HS_sigma_b", C.c_double), ("xS", _dp),
+                ("State", C.c_int), ("RhoVPos", C.c_int), ("RhoCPos", C.c_int),
+                ("Cpv", C.c_double), ("Cpl", C.c_double), ("Rv", C.c_double)]
 
 
 class _Work(C.Structure):
@@ -88,7 +90,7 @@ def _p(a: np.ndarray):
 class Oracle:
     """Binds one discretization (static inputs) to the oracle."""
 
-    def __init__(self, disc, phys=None, koeff=(0.0, 0.0, 0.0), NumTr=0, damping=None, forcing=None):
+    def __init__(self, disc, phys=None, koeff=(0.0, 0.0, 0.0), NumTr=0, damping=None, forcing=None, moist=None):
         """``damping``: (H, StrideDamp, Relax) of DampingW with zP computed
         from the metric; ``forcing``: dict of Held-Suarez constants (k_a,
         k_f, k_s, DeltaT_y, DeltaTh_z, T_equator, T_min, sigma_b) with the
@@ -122,6 +124,10 @@ class Oracle:
             g.Damping = 1
             g.DampH, g.DampStride, g.DampRelax = damping
             g.zP = _p(zP)
+        if moist is not None:
+            # (RhoVPos, RhoCPos, Cpv, Cpl, Rv)
+            g.State = 1
+            g.RhoVPos, g.RhoCPos, g.Cpv, g.Cpl, g.Rv = moist
         if forcing is not None:
             xS = np.ascontiguousarray(node_lonlat(disc))
             self._keep.append(xS)

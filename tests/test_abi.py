@@ -71,6 +71,10 @@ def _desc(pkg, **over):
         (dict(Damping=1), -1, "Damping needs zP"),
         (dict(Damping=1, zP=16, DampH=3e4, DampStride=0.0), -1, "DampStride > 0"),
         (dict(Forcing=1), -1, "Forcing needs xS"),
+        (dict(State=2), -2, "unknown State"),
+        (dict(State=1, NumTr=2, RhoVPos=6, RhoCPos=6, Cpv=1885.0, Cpl=4186.0, Rv=461.0), -1, "Moist needs RhoVPos"),
+        (dict(State=1, NumTr=1, RhoVPos=6, RhoCPos=7, Cpv=1885.0, Cpl=4186.0, Rv=461.0), -1, "Moist needs RhoVPos"),
+        (dict(State=1, NumTr=2, RhoVPos=6, RhoCPos=7), -1, "Cpv, Cpl, Rv"),
         (dict(Forcing=1, xS=16, HS_sigma_b=1.5), -1, "HS_sigma_b < 1"),
         (dict(Microphysics=1), -2, "dry CompressibleShallow"),
         (dict(HyperDDivW=1e14), -2, "HyperDDivW"),

@@ -537,3 +537,40 @@ def test_held_suarez_forcing(pkg, oracle_mod):
     m = (kT > 0) & (np.abs(T - Teq) > 1e-6)
     assert np.all(np.sign(dF[4][m]) == -np.sign((T - Teq)[m]))
     assert np.all(dF[[0, 3]] == 0.0)
+
+
+# ---- moist equation of state (Equation.jl:56-75) ----
+
+def _moist_state(pkg, disc, seed=13):
+    U0 = pkg.state.jet_state(disc, seed=seed)
+    rng = np.random.default_rng(seed)
+    qv = 0.012 * (1.0 + 0.3 * rng.random(U0[0].shape))
+    qc = 2e-4 * rng.random(U0[0].shape)
+    return np.concatenate([U0, (qv * U0[0])[None], (qc * U0[0])[None]], axis=0)
+
+
+def test_moist_eos(pkg, oracle_mod):
+    """p of Models.Moist inverts Rd rho*theta = p^(1-kappa_m) p0^kappa_m with
+    kappa_m = R_m / c_pml of the dry air, vapour and cloud densities;
+    T = p / R_m; theta = (p0/p)^kappa_m T.  Without vapour and cloud it is the
+    dry equation of state."""
+    disc = disc_for(pkg, 3, 8)
+    P = pkg.discretization.Phys()
+    U = _moist_state(pkg, disc)
+    orc = oracle_mod.Oracle(disc, P, NumTr=2, moist=(6, 7, P.Cpv, P.Cpl, P.Rv))
+    A = orc.prepare(U.copy())
+    rho, rth, rv, rc = U[0], U[4], U[5], U[6]
+    rd = rho - rv - rc
+    Rm = P.Rd * rd + P.Rv * rv
+    km = Rm / (P.Cpd * rd + P.Cpv * rv + P.Cpl * rc)
+    p = A[0]
+    assert np.allclose(p ** (1.0 - km) * P.p0 ** km, P.Rd * rth, rtol=1e-13, atol=0)
+    assert np.allclose(A[1], p / Rm, rtol=1e-14, atol=0)
+    assert np.allclose(A[2], (P.p0 / p) ** km * A[1], rtol=1e-13, atol=0)
+    Ud = U.copy()
+    Ud[5:] = 0.0
+    Am = orc.prepare(Ud.copy())
+    Ad = oracle_mod.Oracle(disc, P).prepare(Ud[:5].copy())
+    assert np.allclose(Am, Ad, rtol=1e-13, atol=0)
+    # vapour lowers kappa_m below kappa: the moist pressure differs
+    assert np.abs(A[0] / oracle_mod.Oracle(disc, P).prepare(U[:5].copy())[0] - 1.0).max() > 1e-4
