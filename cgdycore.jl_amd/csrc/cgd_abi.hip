@@ -739,7 +739,7 @@ static void tr1_t(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const doubl
 template <int N, int NZT, bool CM, typename R>
 static void tr2_t(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const double *U, const double *T1,
                   double *F) {
-  const size_t lds = sizeof(double) * (2 * c->NN + 6 * c->NN * NZT);
+  const size_t lds = sizeof(double) * (2 * c->NN + 10 * c->NN * NZT);
   hipLaunchKernelGGL((k_tr2<N, NZT, CM>), dim3(nitems), dim3(c->block), lds, st, c->ka, slot0, c->nchunk, c->ntr, U,
                      T1, F);
 }

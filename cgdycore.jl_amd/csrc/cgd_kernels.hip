@@ -141,6 +141,42 @@ __device__ __forceinline__ void recu4(R cLL, R cL, R cR, R cRR, R JLL, R JL, R J
   }
 }
 
+// RecU4 split in two: the weights of both sides depend on the layer volumes
+// only (computed once, e.g. for several tracers), the limited
+// reconstruction on c; the same operations as recu4, so the same bits
+template <typename R> struct RecW {
+  R kRl, kLl, kLr, kRr;  // left-biased kR, kL; right-biased kL, kR
+};
+template <typename R> __device__ __forceinline__ RecW<R> recu4_weights(R JLL, R JL, R JR, R JRR) {
+  RecW<R> w;
+  const R A = JL + JR;
+  {
+    const R B = JLL + JL, C = B + JR;
+    const R q = rcp(A * B * C);
+    w.kRl = (JL * B) * B * q;
+    w.kLl = (JL * JR) * A * q;
+  }
+  {
+    const R B = JRR + JR, C = A + JRR;
+    const R q = rcp(A * B * C);
+    w.kLr = (JR * B) * B * q;
+    w.kRr = (JR * JL) * A * q;
+  }
+  return w;
+}
+template <typename R>
+__device__ __forceinline__ void recu4_apply(R cLL, R cL, R cR, R cRR, const RecW<R> &w, R &cFL, R &cFR) {
+  const R tiny = R(1.e-20), zero = R(0), one = R(1);
+  {
+    const R r = (cL - cLL) * rcp(cR - cL + tiny);
+    cFL = cL + jmax(zero, jmin(r, jmin(w.kLl * r + w.kRl, one))) * (cR - cL);
+  }
+  {
+    const R r = (cR - cRR) * rcp(cL - cR + tiny);
+    cFR = cR + jmax(zero, jmin(r, jmin(w.kRr * r + w.kLr, one))) * (cL - cR);
+  }
+}
+
 // ------------------------------------------------------------------------
 // LDS tile helpers
 // tile[f][node][level] of values of type R; NZ (levels per chunk) is a
@@ -1063,16 +1099,21 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_tr1(KArgs a, i
 // pass 2: -Kd div(rho grad C) / (M1 + M2) from the cache C, the upwind
 // vertical flux of c = q / rho through both interfaces of the layer (the
 // same contravariant w and RecU4 weights as rho*theta) and the horizontal
-// divergence of (-rho v) c
+// divergence of (-rho v) c.  As in k_fcn2, the values of the neighbour
+// levels come from DPP row shifts (halo loads only at chunk edges), fields
+// are addressed by uniform bases plus 32-bit offsets, and the compact
+// metric layout is read per node.
 template <int N, int NZT, bool CM>
 __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_tr2(KArgs a, int slot0, int nchunk, int ntr,
                                                                       const double *__restrict__ U,
                                                                       const double *__restrict__ T1,
                                                                       double *__restrict__ F) {
   constexpr int NN = N * N;
+  static_assert(NZT >= 3 && NZT <= 16, "chunk heights 3..16 (the DPP neighbour reach is 2)");
   extern __shared__ double smem[];
   double *Ds = smem, *DWs = Ds + NN;
-  // fields 0: C, 1: uc c, 2: vc c; 3, 4: the rho-weighted contravariant gradient of C
+  // fields 0, 1: uc, vc (all tracers); 2..5 and 6..9: C, c, and the
+  // rho-weighted contravariant gradient of C, for alternate tracers
   Tiles<N, NZT, double> T{Ds + 2 * NN};
   load_dmats<N>(a, Ds, DWs);
   const int Nz = a.Nz;
@@ -1085,105 +1126,193 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_tr2(KArgs a, i
   const bool in_tile = node < NN;
   const bool valid = in_tile && iz < Nz;
   const int I = node % N, J = node / N;
-  const size_t vs = (size_t)a.NumG * Nz;
+  const unsigned vs = (unsigned)a.NumG * (unsigned)Nz;
+  const size_t NNZ = (size_t)NN * Nz;
   const bool has_lo = iz > 0, has_hi = iz < Nz - 1;
   const double Kd = a.kvis[3 * e + 2];
-  double S11 = 0, S12 = 0, S21 = 0, S22 = 0, iJs = 1, iMs = 1, rho = 1, uc = 0, vc = 0;
-  double wlo = 0, wup = 0;                      // contravariant mass fluxes through the interfaces
-  double irh[5] = {1, 1, 1, 1, 1};              // 1/rho at levels iz-2..iz+2 (clamped)
-  double Jc[5] = {1, 1, 1, 1, 1};               // J1 + J2 at levels iz-2..iz+2 (clamped)
-  size_t ui = 0, uk[5] = {0, 0, 0, 0, 0};
+  const double *metE = a.met + (size_t)e * (CM ? CGD_NMETP : CGD_NMET) * NNZ;
+  const double *metN = CM ? a.metn + (size_t)e * CGD_NMETN * NN : nullptr;
+  unsigned ui = 0, mo = 0;
   bool first = false;
   if (valid) {
     const int gv = a.glob[e * NN + node];
     first = gv < 0;
-    const size_t gb = (size_t)(gv & 0x7fffffff) * Nz;
-    ui = gb + iz;
-    for (int k = 0; k < 5; ++k) {
-      const int lz = jmin(jmax(iz + k - 2, 0), Nz - 1);
-      uk[k] = gb + lz;
-      irh[k] = rcp(U[uk[k]]);
-      Jc[k] = met_at<NN, CM>(a, e, MJ1, node, lz) + met_at<NN, CM>(a, e, MJ2, node, lz);
+    ui = (unsigned)(gv & 0x7fffffff) * (unsigned)Nz + iz;
+    mo = (unsigned)(node * Nz + iz);
+  }
+  __builtin_assume(ui < (1u << 28));
+  __builtin_assume(mo < (1u << 28));
+#define MET(c, off) at(metE + (size_t)(CM ? met_pidx(c) : (c)) * NNZ, mo + (off))
+#define METN(c) at(metN + (size_t)met_nidx(c) * NN, (unsigned)node)
+#define MET0(c) (CM && !met_per_point(c) ? METN(c) : MET(c, 0))
+  const double *Uu = U + vs, *Uv = U + 2 * (size_t)vs, *Uw = U + 3 * (size_t)vs;
+  double rho = 1, u = 0, v = 0, w = 0, S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1, iMs = 1;
+  double d311 = 0, d312 = 0, d321 = 0, d322 = 0, d331 = 1, d332 = 1;
+  if (valid) {
+    rho = at(U, ui);
+    u = at(Uu, ui);
+    v = at(Uv, ui);
+    w = at(Uw, ui);
+    iMs = at(a.iMs, ui);
+    S11 = MET0(M111) + MET0(M112);
+    S12 = MET0(M121) + MET0(M122);
+    S21 = MET0(M211) + MET0(M212);
+    S22 = MET0(M221) + MET0(M222);
+    Js = MET0(MJ1) + MET0(MJ2);
+    d311 = MET0(M311);
+    d312 = MET0(M312);
+    d321 = MET0(M321);
+    d322 = MET0(M322);
+    d331 = MET0(M331);
+    d332 = MET0(M332);
+  }
+  const double irho = rcp(rho), iJs = rcp(Js);
+  // neighbour levels (a node's chunk is one half of a 16-lane DPP row)
+  const int lr = threadIdx.x & 15;
+  const bool edge_m1 = l < 1 || lr < 1, edge_m2 = l < 2 || lr < 2;
+  const bool edge_p1 = l + 1 >= NZT || lr + 1 > 15, edge_p2 = l + 2 >= NZT || lr + 2 > 15;
+  double rho_m = dpp_from_below<1>(rho), u_m = dpp_from_below<1>(u), v_m = dpp_from_below<1>(v);
+  double w_m = dpp_from_below<1>(w), d312m = dpp_from_below<1>(d312), d322m = dpp_from_below<1>(d322);
+  double d332m = CM ? d332 : dpp_from_below<1>(d332);
+  double rho_p = dpp_from_above<1>(rho), u_p = dpp_from_above<1>(u), v_p = dpp_from_above<1>(v);
+  double d311p = dpp_from_above<1>(d311), d321p = dpp_from_above<1>(d321);
+  double d331p = CM ? d331 : dpp_from_above<1>(d331);
+  double Jm = CM ? Js : dpp_from_below<1>(Js), Jmm = CM ? Js : dpp_from_below<2>(Js);
+  double Jp = CM ? Js : dpp_from_above<1>(Js), Jpp = CM ? Js : dpp_from_above<2>(Js);
+  double irm = dpp_from_below<1>(irho), irmm = dpp_from_below<2>(irho);
+  double irp = dpp_from_above<1>(irho), irpp = dpp_from_above<2>(irho);
+  if (valid) {
+    if (has_lo && edge_m1) {
+      const unsigned um = ui - 1;
+      rho_m = at(U, um);
+      u_m = at(Uu, um);
+      v_m = at(Uv, um);
+      w_m = at(Uw, um);
+      irm = rcp(rho_m);
+      d312m = MET(M312, -1);
+      d322m = MET(M322, -1);
+      if constexpr (!CM) {
+        d332m = MET(M332, -1);
+        Jm = MET(MJ1, -1) + MET(MJ2, -1);
+      }
     }
-    S11 = met_at<NN, CM>(a, e, M111, node, iz) + met_at<NN, CM>(a, e, M112, node, iz);
-    S12 = met_at<NN, CM>(a, e, M121, node, iz) + met_at<NN, CM>(a, e, M122, node, iz);
-    S21 = met_at<NN, CM>(a, e, M211, node, iz) + met_at<NN, CM>(a, e, M212, node, iz);
-    S22 = met_at<NN, CM>(a, e, M221, node, iz) + met_at<NN, CM>(a, e, M222, node, iz);
-    iJs = rcp(Jc[2]);
-    iMs = a.iMs[ui];
-    rho = U[ui];
-    const double u = U[vs + ui], v = U[2 * vs + ui];
-    uc = -rho * (S11 * u + S12 * v);  // Contra12(-rho, u, v)
-    vc = -rho * (S21 * u + S22 * v);
-    if (has_lo) {
-      const size_t um = ui - 1;
-      const double rm = U[um], wm = U[3 * vs + um];
-      wlo = rm * (met_at<NN, CM>(a, e, M312, node, iz - 1) * U[vs + um] +
-                  met_at<NN, CM>(a, e, M322, node, iz - 1) * U[2 * vs + um] +
-                  met_at<NN, CM>(a, e, M332, node, iz - 1) * wm) +
-            rho * (met_at<NN, CM>(a, e, M311, node, iz) * u + met_at<NN, CM>(a, e, M321, node, iz) * v +
-                   met_at<NN, CM>(a, e, M331, node, iz) * wm);
+    if (iz >= 2 && edge_m2) {
+      irmm = rcp(at(U, ui - 2));
+      if constexpr (!CM) Jmm = MET(MJ1, -2) + MET(MJ2, -2);
     }
-    if (has_hi) {
-      const size_t up = ui + 1;
-      const double rp = U[up], w = U[3 * vs + ui];
-      wup = rho * (met_at<NN, CM>(a, e, M312, node, iz) * u + met_at<NN, CM>(a, e, M322, node, iz) * v +
-                   met_at<NN, CM>(a, e, M332, node, iz) * w) +
-            rp * (met_at<NN, CM>(a, e, M311, node, iz + 1) * U[vs + up] +
-                  met_at<NN, CM>(a, e, M321, node, iz + 1) * U[2 * vs + up] +
-                  met_at<NN, CM>(a, e, M331, node, iz + 1) * w);
+    if (has_hi && edge_p1) {
+      const unsigned up = ui + 1;
+      rho_p = at(U, up);
+      u_p = at(Uu, up);
+      v_p = at(Uv, up);
+      irp = rcp(rho_p);
+      d311p = MET(M311, 1);
+      d321p = MET(M321, 1);
+      if constexpr (!CM) {
+        d331p = MET(M331, 1);
+        Jp = MET(MJ1, 1) + MET(MJ2, 1);
+      }
+    }
+    if (iz + 2 <= Nz - 1 && edge_p2) {
+      irpp = rcp(at(U, ui + 2));
+      if constexpr (!CM) Jpp = MET(MJ1, 2) + MET(MJ2, 2);
     }
   }
+  // clamped levels iz-1, iz-2 >= 0 and iz+2 <= Nz-1 (the reference's max/min);
+  // at iz = 0 the shifted values belong to another node's lanes
+  if (iz == 0) Jm = Js;
+  if (iz == 1) {
+    irmm = irm;
+    Jmm = Jm;
+  }
+  if (iz + 2 > Nz - 1) {
+    irpp = irp;
+    Jpp = Jp;
+  }
+  // contravariant mass fluxes through the interfaces (Contra3)
+  double wlo = 0, wup = 0;
+  if (valid && has_lo) wlo = rho_m * (d312m * u_m + d322m * v_m + d332m * w_m) + rho * (d311 * u + d321 * v + d331 * w_m);
+  if (valid && has_hi) wup = rho * (d312 * u + d322 * v + d332 * w) + rho_p * (d311p * u_p + d321p * v_p + d331p * w);
+  // RecU4 weights of the upper interface (and of the lower one, used by the
+  // lanes at a chunk's bottom edge), shared by all tracers
+  const RecW<double> Wup = recu4_weights(Jm, Js, Jp, Jpp);
+  const RecW<double> Wlo = recu4_weights(Jmm, Jm, Js, has_hi ? Jp : Js);
+  // the tracer-independent tiles: (uc, vc) = Contra12(-rho, u, v)
+  if (in_tile) {
+    T.at(0, node, l) = -rho * (S11 * u + S12 * v);
+    T.at(1, node, l) = -rho * (S21 * u + S22 * v);
+  }
   for (int tr = 0; tr < ntr; ++tr) {
-    const size_t off = (size_t)(5 + tr) * vs;
-    double o = 0, cz[5] = {0, 0, 0, 0, 0};
+    // two sets of per-tracer tiles (C, c, Cx, Cy) used alternately, so a
+    // tracer's tiles are never rewritten before every lane has passed the
+    // next tracer's first barrier
+    const int b = 2 + 4 * (tr & 1);
+    const double *Q = U + (size_t)(5 + tr) * vs;
+    const double *C = T1 + (size_t)(5 + tr) * vs;
+    double *Ft = F + (size_t)(5 + tr) * vs;
+    double o = 0, cown = 0, Cown = 0;
     if (valid) {
-      if (!first) o = F[off + ui];
-      for (int k = 0; k < 5; ++k) cz[k] = U[off + uk[k]] * irh[k];
+      if (!first) o = at(Ft, ui);
+      cown = at(Q, ui) * irho;
+      Cown = at(C, ui);
     }
+    double c_m = dpp_from_below<1>(cown), c_mm = dpp_from_below<2>(cown);
+    double c_p = dpp_from_above<1>(cown), c_pp = dpp_from_above<2>(cown);
+    if (valid) {
+      if (has_lo && edge_m1) c_m = at(Q, ui - 1) * irm;
+      if (iz >= 2 && edge_m2) c_mm = at(Q, ui - 2) * irmm;
+      if (has_hi && edge_p1) c_p = at(Q, ui + 1) * irp;
+      if (iz + 2 <= Nz - 1 && edge_p2) c_pp = at(Q, ui + 2) * irpp;
+    }
+    if (iz == 0) c_m = cown;
+    if (iz == 1) c_mm = c_m;
+    if (iz + 2 > Nz - 1) c_pp = c_p;
     if (in_tile) {
-      T.at(0, node, l) = valid ? T1[off + ui] : 0.0;
-      T.at(1, node, l) = uc;
-      T.at(2, node, l) = vc;
-      T.at(5, node, l) = cz[2];
+      T.at(b, node, l) = Cown;
+      T.at(b + 1, node, l) = cown;
+    }
+    // upper-interface flux; the lower one is the upper flux of the level
+    // below, taken from its lane (lanes at a chunk's bottom edge compute it)
+    double Fup = 0;
+    if (valid && has_hi) {
+      double cFL, cFR;
+      recu4_apply(c_m, cown, c_p, c_pp, Wup, cFL, cFR);
+      Fup = 0.25 * ((fabs(wup) + wup) * cFL + (-fabs(wup) + wup) * cFR);
+    }
+    double Flo = dpp_from_below<1>(Fup);
+    if (valid && has_lo && edge_m1) {
+      double cFL, cFR;
+      recu4_apply(c_mm, c_m, cown, has_hi ? c_p : cown, Wlo, cFL, cFR);
+      Flo = 0.25 * ((fabs(wlo) + wlo) * cFL + (-fabs(wlo) + wlo) * cFR);
     }
     __syncthreads();
     double f = 0;
     if (in_tile) {
       double Cx = 0, Cy = 0;
       if (valid) {
-        const double Dxc = dx<N, NZT, double>(Ds, T, 0, I, J, l);
-        const double Dyc = dy<N, NZT, double>(Ds, T, 0, I, J, l);
+        const double Dxc = dx<N, NZT, double>(Ds, T, b, I, J, l);
+        const double Dyc = dy<N, NZT, double>(Ds, T, b, I, J, l);
         const double GDx = (S11 * Dxc + S21 * Dyc) * iJs;
         const double GDy = (S12 * Dxc + S22 * Dyc) * iJs;
         Cx = rho * (S11 * GDx + S12 * GDy);  // Contra12 with rho
         Cy = rho * (S21 * GDx + S22 * GDy);
         // horizontal transport: sum_k D (uc c) over the element row / column
-        f += dxy_prod<N, NZT, double>(Ds, T, 1, 2, 5, I, J, l) * iMs;
-        // upwind vertical fluxes: through the lower interface (this layer
-        // is its right side) and the upper one (its left side)
-        if (has_lo) {
-          double cFL, cFR;
-          recu4<double>(cz[0], cz[1], cz[2], has_hi ? cz[3] : cz[2], Jc[0], Jc[1], Jc[2], has_hi ? Jc[3] : Jc[2],
-                        cFL, cFR);
-          f += 0.25 * ((fabs(wlo) + wlo) * cFL + (-fabs(wlo) + wlo) * cFR) * iMs;
-        }
-        if (has_hi) {
-          double cFL, cFR;
-          recu4<double>(cz[1], cz[2], cz[3], cz[4], Jc[1], Jc[2], Jc[3], Jc[4], cFL, cFR);
-          f -= 0.25 * ((fabs(wup) + wup) * cFL + (-fabs(wup) + wup) * cFR) * iMs;
-        }
+        f += dxy_prod<N, NZT, double>(Ds, T, 0, 1, b + 1, I, J, l) * iMs;
+        if (has_lo) f += Flo * iMs;
+        if (has_hi) f -= Fup * iMs;
       }
-      T.at(3, node, l) = Cx;
-      T.at(4, node, l) = Cy;
+      T.at(b + 2, node, l) = Cx;
+      T.at(b + 3, node, l) = Cy;
     }
     __syncthreads();
     if (valid) {
-      f += -Kd * dxy<N, NZT, double>(DWs, T, 3, 4, I, J, l) * iMs;
-      F[off + ui] = first ? f : o + f;
+      f += -Kd * dxy<N, NZT, double>(DWs, T, b + 2, b + 3, I, J, l) * iMs;
+      at(Ft, ui) = first ? f : o + f;
     }
-    __syncthreads();  // the next tracer rewrites the tiles
   }
+#undef MET
+#undef METN
+#undef MET0
 }
 
 // SchurSolveBKernel! for the tracers (SchurSolve.jl:106-109): k = fac v
