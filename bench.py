@@ -176,6 +176,84 @@ def ensemble(pkg, args):
     return out
 
 
+def adjoint(pkg, args):
+    """The adjoint workload: ne = 16, p = 4, nz = 32, FP64, the seeded jet
+    (seed 0); functional J = sum over the lowest level's global nodes of
+    rho*theta^2 = (rho theta)^2 / rho after 12 steps at dt = 200 s (at
+    dt = 250 or 300 s the step itself -- the C oracle's restatement of the
+    reference as well as this path -- diverges within 12 steps at ne = 16:
+    horizontal acoustics are explicit in this Rosenbrock-W).  The
+    gradient dJ/dV0 comes from one cgd_rw_adjoint call with a checkpoint
+    every 4 steps; it is checked against central differences
+    (J(V0 + eps d) - J(V0 - eps d)) / (2 eps), eps = 1e-6, of the fused
+    forward (cgd_rw_run) along 64 Gaussian directions d ~ N(0, 1) per state
+    entry (numpy generators seeded 1000..1063; w at the model top, which
+    the step keeps fixed, is not perturbed).  J is summed exactly (fsum)."""
+    import math
+
+    import torch
+
+    host = pkg.host
+    D = pkg.discretization
+    ne, nz, dt, nsteps, every, ndir, eps = 16, 32, 200.0, 12, 4, args.adj_dirs, 1e-6
+    phys = D.Phys()
+    disc = D.build(ne, 4, nz, 30000.0)
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda:0")
+    ctx = host.init_context(FE, Metric, phys, Global)
+    U0 = pkg.state.jet_state(disc, seed=0)
+    V0 = torch.as_tensor(U0, device="cuda:0")
+
+    def J_of(Vf):
+        lev = Vf[:, :, 0].cpu().numpy()
+        return math.fsum((lev[4] ** 2 / lev[0]).tolist())
+
+    def run(V):
+        Vt = V.clone()
+        host.TimeStepper(Vt, dt, nsteps, FE, Metric, phys, Cache, JCache, Global)
+        return Vt
+
+    work = ctx.adjoint_work(nsteps, every, device="cuda:0")
+    Vf = torch.empty_like(V0)
+
+    def grad():
+        Vf_fwd = run(V0)  # the forward that a caller's rrule would have run
+        lam = torch.zeros_like(V0)
+        lam[4, :, 0] = 2.0 * Vf_fwd[4, :, 0] / Vf_fwd[0, :, 0]
+        lam[0, :, 0] = -(Vf_fwd[4, :, 0] / Vf_fwd[0, :, 0]) ** 2
+        ctx.rw_adjoint(V0, dt, nsteps, every, lam, Vfinal=Vf, work=work)
+        return lam
+
+    grad()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g = grad()
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    run(V0)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    gn = g.cpu().numpy()
+    errs = []
+    for seed in range(1000, 1000 + ndir):
+        d = np.random.default_rng(seed).standard_normal(U0.shape)
+        d[3][:, -1] = 0.0
+        dt_ = torch.as_tensor(d, device="cuda:0")
+        fd = (J_of(run(V0 + eps * dt_)) - J_of(run(V0 - eps * dt_))) / (2 * eps)
+        ad = float(np.vdot(gn, d))
+        errs.append(abs(fd - ad) / abs(fd))
+    errs = np.array(errs)
+    out = {"ne": ne, "p": 4, "nz": nz, "dt": dt, "steps": nsteps, "checkpoint_every": every, "precision": "fp64",
+           "functional": "sum over lowest-level global nodes of rho*theta^2 after the steps",
+           "J": J_of(Vf), "ms_gradient": round(t_grad * 1e3, 3), "ms_forward": round(t_fwd * 1e3, 3),
+           "gradient_over_forward": round(t_grad / t_fwd, 2),
+           "fd": {"eps": eps, "directions": ndir, "seeds": f"1000..{1000 + ndir - 1}",
+                  "max_rel_err": float(errs.max()), "median_rel_err": float(np.median(errs)),
+                  "pass_1e-6": bool(errs.max() <= 1e-6)}}
+    ctx.close()
+    return out
+
+
 def cpu_info():
     model = ""
     try:
@@ -206,6 +284,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--ntr", type=int, default=0, help="tracers in the FP64 state (reported in config)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient and its FD check")
+    ap.add_argument("--adj-dirs", type=int, default=64, help="FD directions of the adjoint check")
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=10)
     ap.add_argument("--ens-dt", type=float, default=150.0)
@@ -319,6 +399,11 @@ def main():
         ens = ensemble(pkg, args)
         print(f"[bench] ensemble: {ens}", file=sys.stderr, flush=True)
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint(pkg, args)
+        print(f"[bench] adjoint: {adj}", file=sys.stderr, flush=True)
+
     cpu = None
     if args.cpu_steps > 0 and not args.ntr:
         O = ge.load_oracle()
@@ -339,7 +424,8 @@ def main():
                       "nz": args.nz, "dt": args.dt, "hyperd": Global.Model.HyperDDiv, "points": Ne,
                       "precision": "fp64", "finite": finite, "tracers": args.ntr,
                       "launch_shape": dict(shape, autotuned=not args.no_autotune)},
-           "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "ensemble": ens, "setup": setup}
+           "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "ensemble": ens, "adjoint": adj,
+           "setup": setup}
     print(json.dumps(out), flush=True)
 
 

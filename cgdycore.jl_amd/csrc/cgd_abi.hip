@@ -973,6 +973,41 @@ static int rw_step_fused(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w,
   return CGD_OK;
 }
 
+// One SSP-Knoth step as the reference's sequence of calls (FP64): prepare,
+// tendency, the Jacobian at the first stage, the stage terms by k_axpy and
+// the Schur solves.  Used for tall columns and by the adjoint's forward runs.
+static int rw_composed_step(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, hipStream_t st) {
+  int r;
+  const double A[3][3] = {{0, 0, 0}, {1.0, 0, 0}, {0.25, 0.25, 0}};
+  const double Cc[3][3] = {{-1.0, 0, 0}, {-0.0, -1.0, 0}, {-0.75, -0.75, -1.0}};
+  const double m3 = 2.0 / 3.0, m2 = 1.0 / 6.0 - (-0.75) * m3, m1 = 1.0 / 6.0 - (-0.75) * m3;
+  const double Mw[3] = {m1, m2, m3};
+  const double gamma = 1.0;
+  const size_t ng = (size_t)c->NumG * c->Nz, n5 = (size_t)(5 + c->ntr) * ng;  // (Nz, NumG, 5 + NumTr)
+  double *k[3] = {w->k, w->k + n5, w->k + 2 * n5};
+  HIPCHK(c, hipMemcpyAsync(w->Vn, V, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  for (int is = 0; is < 3; ++is) {
+    if ((r = cgd_prepare(c, V, w->AuxG, st))) return r;
+    if ((r = cgd_fcn(c, w->fV, V, w->Temp1, w->AuxG, st))) return r;
+    if (is == 0 && (r = cgd_jac(c, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, V, w->AuxG, st))) return r;
+    const int gb = grid_for(n5, 256);
+    if (is > 0) {
+      hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, w->fV, w->fV, k[0], Cc[is][0] / dt, k[1],
+                         Cc[is][1] / dt, is, n5);
+      HIPCHK(c, hipGetLastError());
+    }
+    if ((r = cgd_schur(c, k[is], w->fV, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma, is == 0, st)))
+      return r;
+    if (is < 2)
+      hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], A[is + 1][0], k[1], A[is + 1][1], is + 1,
+                         n5);
+    else
+      hipLaunchKernelGGL(k_axpy3, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], Mw[0], k[1], Mw[1], k[2], Mw[2], n5);
+    HIPCHK(c, hipGetLastError());
+  }
+  return CGD_OK;
+}
+
 // nsteps SSP-Knoth steps; with the fused path, steps after the first take
 // the pressure (and FP32 shadow) that the previous step's last stage wrote
 static int rw_run(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, int nsteps, void *stream) {
@@ -1002,29 +1037,8 @@ static int rw_run(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, int ns
     return CGD_OK;
   }
   // tall columns: the reference's sequence of calls, stage terms by k_axpy
-  for (int s = 0; s < nsteps; ++s) {
-    HIPCHK(c, hipMemcpyAsync(w->Vn, V, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
-    for (int is = 0; is < 3; ++is) {
-      if ((r = cgd_prepare(c, V, w->AuxG, stream))) return r;
-      if ((r = cgd_fcn(c, w->fV, V, w->Temp1, w->AuxG, stream))) return r;
-      if (is == 0 && (r = cgd_jac(c, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, V, w->AuxG, stream))) return r;
-      const int gb = grid_for(n5, 256);
-      if (is > 0) {
-        hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, w->fV, w->fV, k[0], Cc[is][0] / dt, k[1],
-                           Cc[is][1] / dt, is, n5);
-        HIPCHK(c, hipGetLastError());
-      }
-      if ((r = cgd_schur(c, k[is], w->fV, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma, is == 0,
-                         stream)))
-        return r;
-      if (is < 2)
-        hipLaunchKernelGGL(k_axpy, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], A[is + 1][0], k[1], A[is + 1][1],
-                           is + 1, n5);
-      else
-        hipLaunchKernelGGL(k_axpy3, dim3(gb), dim3(256), 0, st, V, w->Vn, k[0], Mw[0], k[1], Mw[1], k[2], Mw[2], n5);
-      HIPCHK(c, hipGetLastError());
-    }
-  }
+  for (int s = 0; s < nsteps; ++s)
+    if ((r = rw_composed_step(c, V, dt, w, st))) return r;
   return CGD_OK;
 }
 
@@ -1226,3 +1240,7 @@ extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work 
   if (best_ms) *best_ms = bms;
   return CGD_OK;
 }
+
+// ------------------------------------------------------------------------
+// Adjoint of the FP64 path
+#include "cgd_adjoint.hip"

@@ -1,0 +1,1143 @@
+// Reverse mode (adjoint) of the FP64 drop-in path for the dry configuration
+// without tracers or physics hooks:
+//   - the tendency FcnGPU! (src/GPU/FcnGPU.jl:552-968) with its fused DSS:
+//     pass 1 HyperViscKernel! (src/GPU/ViscKernel.jl:1-84), pass 2
+//     HyperViscKoeffKernel! (ViscKernel.jl:86-167),
+//     MomentumVectorInvariantCoriolisKernel! (src/GPU/MomentumKernel.jl:1-178),
+//     GradFullKernel! (src/GPU/GradKineticKernel.jl:1-129) and
+//     DivRhoTrUpwind3New2Kernel! (src/GPU/OperatorKernel.jl:637-720), with
+//     the pressure of PressureKernel! (src/GPU/DiagnosticKernel.jl:96-115);
+//   - the column Schur solve SchurSolveGPU! (src/Integration/SchurSolve.jl:149-177);
+//   - the Jacobian blocks JacSchurKernel! (src/DyCore/JacSchur.jl:29-75);
+//   - the SSP-Knoth step RosenbrockSchur! (src/Integration/RosenbrockSchur.jl:1-33)
+//     and runs of steps with checkpoints.
+// The reference has no adjoint; these are the exact transposes of the
+// derivatives of the forward formulas.  The min/max of the RecU4 limiter
+// and |wCon| of the upwind flux are differentiated on the branch the
+// forward takes, except that |wCon| at wCon = 0 to rounding (w = 0 on a
+// flat metric, e.g. a state at rest vertically) gets the mean of its
+// one-sided derivatives.
+//
+// Element kernels: one thread per (node, level) of one element and level
+// chunk, like the forward passes.  A thread recomputes the forward values of
+// its point that the reverse needs, writes the cotangents of the
+// derivative-matrix outputs of its point to LDS tiles and applies the
+// transposed matrices (a sum over the output index), so the cotangent of
+// every element input is formed by the thread that owns that input.  The
+// cotangents of the global state are summed over the elements that share a
+// node and over neighbouring levels with FP64 atomic adds
+// (global_atomic_add_f64), one launch over all elements: the summation
+// order, and so the last bits, vary from run to run.
+//
+// Included by cgd_abi.hip after cgd_kernels.hip (it uses KArgs, the tile
+// helpers, the context and the forward entry points).
+
+// ------------------------------------------------------------------------
+// Kernels
+
+__device__ __forceinline__ void aadd(double *p, double v) { unsafeAtomicAdd(p, v); }
+
+// sum_i D[i,I] f(i,J): the cotangent of the inputs of dx at node (I, J)
+template <int N, int NZ>
+__device__ __forceinline__ double dxT(const double *Dm, const Tiles<N, NZ, double> &T, int f, int I, int J, int l) {
+  double s = 0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) s += Dm[i * N + I] * T.at(f, i + J * N, l);
+  return s;
+}
+// sum_j D[j,J] f(I,j): the cotangent of the inputs of dy
+template <int N, int NZ>
+__device__ __forceinline__ double dyT(const double *Dm, const Tiles<N, NZ, double> &T, int f, int I, int J, int l) {
+  double s = 0;
+#pragma unroll
+  for (int j = 0; j < N; ++j) s += Dm[j * N + J] * T.at(f, I + j * N, l);
+  return s;
+}
+
+// Limiter factor of RecU4 (OperatorKernel.jl:1044-1059),
+// jmax(0, jmin(r, jmin(k1 r + k0, 1))), and its derivative in r on the
+// branch taken (Julia's min/max order: the second argument wins only when
+// strictly smaller / larger)
+__device__ __forceinline__ double lim_phi(double r, double k1, double k0, double &dphi) {
+  const double lin = k1 * r + k0;
+  double inner = lin, din = k1;
+  if (1.0 < lin) {
+    inner = 1.0;
+    din = 0.0;
+  }
+  double m = r, dm = 1.0;
+  if (inner < r) {
+    m = inner;
+    dm = din;
+  }
+  if (m > 0.0) {
+    dphi = dm;
+    return m;
+  }
+  dphi = 0.0;
+  return 0.0;
+}
+
+// RecU4 forward values and the cotangents of its four c inputs for given
+// cotangents of the two face values (adds into lLL, lL, lR, lRR)
+__device__ __forceinline__ void recu4_rev(double cLL, double cL, double cR, double cRR, double JLL, double JL,
+                                          double JR, double JRR, double &cFL, double &cFR) {
+  const double kR = (JL / (JL + JR)) * ((JLL + JL) / (JLL + JL + JR));
+  const double kL = (JL / (JLL + JL)) * (JR / (JLL + JL + JR));
+  const double den = cR - cL + 1.e-20, r = (cL - cLL) / den;
+  double d;
+  cFL = cL + lim_phi(r, kL, kR, d) * (cR - cL);
+  const double kL2 = (JR / (JR + JL)) * ((JRR + JR) / (JL + JR + JRR));
+  const double kR2 = (JR / (JRR + JR)) * (JL / (JL + JR + JRR));
+  const double den2 = cL - cR + 1.e-20, r2 = (cR - cRR) / den2;
+  cFR = cR + lim_phi(r2, kR2, kL2, d) * (cL - cR);
+}
+__device__ __forceinline__ void recu4_adj(double cLL, double cL, double cR, double cRR, double JLL, double JL,
+                                          double JR, double JRR, double lcFL, double lcFR, double &lLL, double &lL,
+                                          double &lR, double &lRR) {
+  {
+    const double kR = (JL / (JL + JR)) * ((JLL + JL) / (JLL + JL + JR));
+    const double kL = (JL / (JLL + JL)) * (JR / (JLL + JL + JR));
+    const double den = cR - cL + 1.e-20, r = (cL - cLL) / den;
+    double dphi;
+    const double phi = lim_phi(r, kL, kR, dphi);
+    const double lr = lcFL * (cR - cL) * dphi;
+    lL += lcFL * (1.0 - phi) + lr * (1.0 + r) / den;
+    lLL += -lr / den;
+    lR += lcFL * phi - lr * r / den;
+  }
+  {
+    const double kL = (JR / (JR + JL)) * ((JRR + JR) / (JL + JR + JRR));
+    const double kR = (JR / (JRR + JR)) * (JL / (JL + JR + JRR));
+    const double den = cL - cR + 1.e-20, r = (cR - cRR) / den;
+    double dphi;
+    const double phi = lim_phi(r, kR, kL, dphi);
+    const double lr = lcFR * (cL - cR) * dphi;
+    lR += lcFR * (1.0 - phi) + lr * (1.0 + r) / den;
+    lRR += -lr / den;
+    lL += lcFR * phi - lr * r / den;
+  }
+}
+
+// Cotangents at the global nodes before the element reverse passes:
+//   Lam = lamF / (M1 + M2) for rho, u, v, rho*theta; lamF_w / D for w with
+//   the rho-weighted interface mass D = rho(iz+1) M1(iz+1) + rho(iz) M2(iz)
+//   of the w DSS (MomentumKernel.jl:165-176, GradKineticKernel.jl:111-126);
+//   lamU = the rho cotangent through D (F_w = S / D, dF_w/drho = -F_w dD/drho / D),
+//   zero for the other components; lamT1 = 0.
+__global__ void k_adj_pre(KArgs a, const double *__restrict__ U, const double *__restrict__ F,
+                          const double *__restrict__ lamF, double *__restrict__ Lam, double *__restrict__ lamU,
+                          double *__restrict__ lamT1, size_t n) {
+  const int Nz = a.Nz;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const size_t g = q / Nz;
+    const int iz = (int)(q - g * Nz);
+    const double *M1 = a.M, *M2 = a.M + n;
+    const double ims = a.iMs[q];
+    Lam[q] = lamF[q] * ims;
+    Lam[n + q] = lamF[n + q] * ims;
+    Lam[2 * n + q] = lamF[2 * n + q] * ims;
+    Lam[4 * n + q] = lamF[4 * n + q] * ims;
+    double lw = 0.0, lr = 0.0;
+    if (iz < Nz - 1) {
+      const double D = U[q + 1] * M1[q + 1] + U[q] * M2[q];
+      lw = lamF[3 * n + q] / D;
+      lr -= lw * F[3 * n + q] * M2[q];
+    }
+    if (iz > 0) {
+      const double D = U[q] * M1[q] + U[q - 1] * M2[q - 1];
+      lr -= (lamF[3 * n + q - 1] / D) * F[3 * n + q - 1] * M1[q];
+    }
+    Lam[3 * n + q] = lw;
+    lamU[q] = lr;
+    for (int v = 1; v < 5; ++v) lamU[v * n + q] = 0.0;
+    for (int v = 0; v < 5; ++v) lamT1[v * n + q] = 0.0;
+  }
+}
+
+// Reverse of pass 2: momentum, kinetic-energy and pressure gradient,
+// continuity and rho*theta transport, and the hyperviscosity applied to the
+// pass-1 cache, for the cotangents Lam of one element's contributions.
+// Adds into lamU (the state) and lamT1 (the pass-1 cache).
+template <int N, int NZT, bool CM>
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64)
+    k_adj2(KArgs a, int nchunk, const double *__restrict__ U, const double *__restrict__ T1,
+           const double *__restrict__ Lam, double *__restrict__ lamU, double *__restrict__ lamT1) {
+  constexpr int NN = N * N;
+  extern __shared__ double smem[];
+  double *Ds = smem, *DWs = Ds + NN;
+  Tiles<N, NZT, double> T{Ds + 2 * NN};
+  load_dmats<N>(a, Ds, DWs);
+  const int Nz = a.Nz;
+  const int e = blockIdx.x / nchunk, chunk = blockIdx.x - e * nchunk;
+  const int t = threadIdx.x, node = t / NZT, l = t - node * NZT;
+  const int iz = chunk * NZT + l;
+  const bool in_tile = node < NN, valid = in_tile && iz < Nz;
+  const bool lo = iz > 0, hi = iz < Nz - 1, hi2 = iz + 2 <= Nz - 1;
+  const int I = node % N, J = node / N;
+  const size_t vs = (size_t)a.NumG * Nz;
+  auto M = [&](int c, int L) { return met_at<NN, CM>(a, (size_t)e, c, node, L); };
+
+  // ---- loads
+  size_t ui = 0;
+  double rho = 1, u = 0, v = 0, wh = 0, rth = 1;
+  double rho_m = 1, u_m = 0, v_m = 0, w_m = 0, rth_m = 1;
+  double rho_p = 1, u_p = 0, v_p = 0, rth_p = 1, rho_pp = 1, rth_pp = 1;
+  double d111 = 0, d112 = 0, d121 = 0, d122 = 0, d211 = 0, d212 = 0, d221 = 0, d222 = 0;
+  double d311 = 0, d312 = 0, d321 = 0, d322 = 0, d331 = 1, d332 = 1, J1 = 1, J2 = 1, Wc = 0;
+  double d312m = 0, d322m = 0, d332m = 1, Jm = 1, d311p = 0, d321p = 0, d331p = 1, J1p = 1, Jp = 1, Jpp = 1;
+  double c4 = 0;
+  double Lr = 0, Lu = 0, Lv = 0, Lwh = 0, Lrth = 0, Lwl = 0, Lr_p = 0, Lrth_p = 0;
+  double Kc = 0, Kg = 0, Kd = 0;
+  if (valid) {
+    const size_t g = (size_t)(a.glob[(size_t)e * NN + node] & 0x7fffffff);
+    ui = g * Nz + iz;
+    rho = U[ui];
+    u = U[vs + ui];
+    v = U[2 * vs + ui];
+    wh = U[3 * vs + ui];
+    rth = U[4 * vs + ui];
+    d111 = M(M111, iz); d112 = M(M112, iz); d121 = M(M121, iz); d122 = M(M122, iz);
+    d211 = M(M211, iz); d212 = M(M212, iz); d221 = M(M221, iz); d222 = M(M222, iz);
+    d311 = M(M311, iz); d312 = M(M312, iz); d321 = M(M321, iz); d322 = M(M322, iz);
+    d331 = M(M331, iz); d332 = M(M332, iz); J1 = M(MJ1, iz); J2 = M(MJ2, iz); Wc = M(MCOR, iz);
+    c4 = T1[4 * vs + ui];
+    Lr = Lam[ui]; Lu = Lam[vs + ui]; Lv = Lam[2 * vs + ui]; Lwh = hi ? Lam[3 * vs + ui] : 0.0; Lrth = Lam[4 * vs + ui];
+    const double *kv = a.kvis + 3 * (size_t)e;
+    Kc = kv[0]; Kg = kv[1]; Kd = kv[2];
+    if (lo) {
+      rho_m = U[ui - 1]; u_m = U[vs + ui - 1]; v_m = U[2 * vs + ui - 1]; w_m = U[3 * vs + ui - 1];
+      rth_m = U[4 * vs + ui - 1];
+      d312m = M(M312, iz - 1); d322m = M(M322, iz - 1); d332m = M(M332, iz - 1);
+      Jm = M(MJ1, iz - 1) + M(MJ2, iz - 1);
+      Lwl = Lam[3 * vs + ui - 1];
+    }
+    if (hi) {
+      rho_p = U[ui + 1]; u_p = U[vs + ui + 1]; v_p = U[2 * vs + ui + 1]; rth_p = U[4 * vs + ui + 1];
+      d311p = M(M311, iz + 1); d321p = M(M321, iz + 1); d331p = M(M331, iz + 1);
+      J1p = M(MJ1, iz + 1);
+      Jp = J1p + M(MJ2, iz + 1);
+      Lr_p = Lam[ui + 1]; Lrth_p = Lam[4 * vs + ui + 1];
+    }
+    if (hi2) {
+      rho_pp = U[ui + 2]; rth_pp = U[4 * vs + ui + 2];
+      Jpp = M(MJ1, iz + 2) + M(MJ2, iz + 2);
+    }
+  }
+  const double S11 = d111 + d112, S12 = d121 + d122, S21 = d211 + d212, S22 = d221 + d222;
+  const double Js = J1 + J2;
+  const double p = eos_p(a, rth);
+  // w at the lower interface: the free-slip value at the bottom
+  const double wl = lo ? w_m : -(d311 * u + d321 * v) / d331;
+  const double kh = 0.5 * (u * u + v * v);
+  const double K1 = kh + 0.5 * wl * wl, K2 = kh + 0.5 * wh * wh;
+
+  // ---- forward tiles: 0..5 t1[var][K] (var-major), 6..11 t2[var][K],
+  // 12 K1, 13 K2, 14 p, 15 the pass-1 cache of theta
+  if (in_tile) {
+    T.at(0, node, l) = -d121 * wl;
+    T.at(1, node, l) = -d122 * wh;
+    T.at(2, node, l) = -d111 * wl;
+    T.at(3, node, l) = -d112 * wh;
+    T.at(4, node, l) = d121 * u - d111 * v;
+    T.at(5, node, l) = d122 * u - d112 * v;
+    T.at(6, node, l) = -d221 * wl;
+    T.at(7, node, l) = -d222 * wh;
+    T.at(8, node, l) = -d211 * wl;
+    T.at(9, node, l) = -d212 * wh;
+    T.at(10, node, l) = d221 * u - d211 * v;
+    T.at(11, node, l) = d222 * u - d212 * v;
+    T.at(12, node, l) = K1;
+    T.at(13, node, l) = K2;
+    T.at(14, node, l) = p;
+    T.at(15, node, l) = c4;
+  }
+  __syncthreads();
+  const double uz1 = d331 * v - d321 * wl, uz2 = d332 * v - d322 * wh;
+  const double vz1 = d331 * u - d311 * wl, vz2 = d332 * u - d312 * wh;
+  const double wz1 = d321 * u - d311 * v, wz2 = d322 * u - d312 * v;
+  double U1 = 0, U2 = 0, V1 = 0, V2 = 0, W1 = 0, W2 = 0, DXp = 0, DYp = 0, Dxc = 0, Dyc = 0;
+  if (valid) {
+    U1 = 0.5 * (uz2 - uz1) + dxy<N, NZT, double>(Ds, T, 0, 6, I, J, l);
+    U2 = 0.5 * (uz2 - uz1) + dxy<N, NZT, double>(Ds, T, 1, 7, I, J, l);
+    V1 = 0.5 * (vz2 - vz1) + dxy<N, NZT, double>(Ds, T, 2, 8, I, J, l);
+    V2 = 0.5 * (vz2 - vz1) + dxy<N, NZT, double>(Ds, T, 3, 9, I, J, l);
+    W1 = 0.5 * (wz2 - wz1) + dxy<N, NZT, double>(Ds, T, 4, 10, I, J, l);
+    W2 = 0.5 * (wz2 - wz1) + dxy<N, NZT, double>(Ds, T, 5, 11, I, J, l);
+    DXp = dx<N, NZT, double>(Ds, T, 14, I, J, l);
+    DYp = dy<N, NZT, double>(Ds, T, 14, I, J, l);
+    Dxc = dx<N, NZT, double>(Ds, T, 15, I, J, l);
+    Dyc = dy<N, NZT, double>(Ds, T, 15, I, J, l);
+  }
+  __syncthreads();  // the cotangent tiles reuse the forward tiles
+
+  // cotangent accumulators: own level, the level below (m), above (p), two above (pp)
+  double gr = 0, gu = 0, gv = 0, gwh = 0, grth = 0, gwl = 0, gp = 0;
+  double gr_m = 0, gu_m = 0, gv_m = 0, grth_m = 0;
+  double gr_p = 0, gu_p = 0, gv_p = 0, grth_p = 0, gp_p = 0, gr_pp = 0, grth_pp = 0;
+  const double lwl = lo ? Lwl : 0.0, lwh = hi ? Lwh : 0.0;
+
+  // ---- momentum (MomentumKernel.jl:1-178)
+  double lU1 = 0, lU2 = 0, lV1 = 0, lV2 = 0, lW1 = 0, lW2 = 0;
+  {
+    double FUZl = 0, FVZl = 0, FWZl = 0, FUZu = 0, FVZu = 0, FWZu = 0;
+    if (lo) {
+      FUZl = 0.5 * (uz1 - (d332m * v_m - d322m * wl));
+      FVZl = 0.5 * (vz1 - (d332m * u_m - d312m * wl));
+      FWZl = 0.5 * (wz1 - (d322m * u_m - d312m * v_m));
+    }
+    if (hi) {
+      FUZu = 0.5 * ((d331p * v_p - d321p * wh) - uz2);
+      FVZu = 0.5 * ((d331p * u_p - d311p * wh) - vz2);
+      FWZu = 0.5 * ((d321p * u_p - d311p * v_p) - wz2);
+    }
+    const double FW1 = u * FVZl + v * FUZl, FW2 = u * FVZu + v * FUZu;
+    const double Av = u * V1 + v * U1 + FW1, Bv = u * V2 + v * U2 + FW2;
+    const double lA = rho * lwl, lB = rho * lwh;
+    gr += lwl * Av + lwh * Bv;
+    // F_u = (-v W1 - wl V1 - v W2 - wh V2 + FU) / Ms
+    gv += -Lu * (W1 + W2);
+    lW1 += -v * Lu;
+    lW2 += -v * Lu;
+    gwl += -Lu * V1;
+    lV1 += -wl * Lu;
+    gwh += -Lu * V2;
+    lV2 += -wh * Lu;
+    // F_v = (u W1 - wl U1 + u W2 - wh U2 + FV) / Ms
+    gu += Lv * (W1 + W2);
+    lW1 += u * Lv;
+    lW2 += u * Lv;
+    gwl += -Lv * U1;
+    lU1 += -wl * Lv;
+    gwh += -Lv * U2;
+    lU2 += -wh * Lv;
+    // F_w(iz-1) = rho (u V1 + v U1 + FW1) / D, F_w(iz) = rho (u V2 + v U2 + FW2) / D
+    gu += lA * V1 + lB * V2;
+    lV1 += u * lA;
+    lV2 += u * lB;
+    gv += lA * U1 + lB * U2;
+    lU1 += v * lA;
+    lU2 += v * lB;
+    const double lFU = Lu, lFV = Lv, lFW1 = lA, lFW2 = lB;
+    // CoriolisShallow: FU += v Wc (J1 + J2), FV += -u Wc (J1 + J2)
+    gv += lFU * Wc * Js;
+    gu += -lFV * Wc * Js;
+    double luz1 = 0, luz2 = 0, lvz1 = 0, lvz2 = 0, lwz1 = 0, lwz2 = 0;
+    if (lo) {
+      gv += -lFU * FWZl + lFW1 * FUZl;
+      gu += lFV * FWZl + lFW1 * FVZl;
+      gwl += -lFU * FVZl - lFV * FUZl;
+      const double lFWZ = -v * lFU + u * lFV, lFVZ = -wl * lFU + u * lFW1, lFUZ = -wl * lFV + v * lFW1;
+      luz1 += 0.5 * lFUZ;
+      gv_m += -0.5 * d332m * lFUZ;
+      gwl += 0.5 * d322m * lFUZ;
+      lvz1 += 0.5 * lFVZ;
+      gu_m += -0.5 * d332m * lFVZ;
+      gwl += 0.5 * d312m * lFVZ;
+      lwz1 += 0.5 * lFWZ;
+      gu_m += -0.5 * d322m * lFWZ;
+      gv_m += 0.5 * d312m * lFWZ;
+    }
+    if (hi) {
+      gv += -lFU * FWZu + lFW2 * FUZu;
+      gu += lFV * FWZu + lFW2 * FVZu;
+      gwh += -lFU * FVZu - lFV * FUZu;
+      const double lFWZ = -v * lFU + u * lFV, lFVZ = -wh * lFU + u * lFW2, lFUZ = -wh * lFV + v * lFW2;
+      gv_p += 0.5 * d331p * lFUZ;
+      gwh += -0.5 * d321p * lFUZ;
+      luz2 += -0.5 * lFUZ;
+      gu_p += 0.5 * d331p * lFVZ;
+      gwh += -0.5 * d311p * lFVZ;
+      lvz2 += -0.5 * lFVZ;
+      gu_p += 0.5 * d321p * lFWZ;
+      gv_p += -0.5 * d311p * lFWZ;
+      lwz2 += -0.5 * lFWZ;
+    }
+    {
+      const double s = lU1 + lU2;
+      luz2 += 0.5 * s;
+      luz1 -= 0.5 * s;
+    }
+    {
+      const double s = lV1 + lV2;
+      lvz2 += 0.5 * s;
+      lvz1 -= 0.5 * s;
+    }
+    {
+      const double s = lW1 + lW2;
+      lwz2 += 0.5 * s;
+      lwz1 -= 0.5 * s;
+    }
+    gv += d331 * luz1 + d332 * luz2 - d311 * lwz1 - d312 * lwz2;
+    gu += d331 * lvz1 + d332 * lvz2 + d321 * lwz1 + d322 * lwz2;
+    gwl += -d321 * luz1 - d311 * lvz1;
+    gwh += -d322 * luz2 - d312 * lvz2;
+  }
+
+  // ---- kinetic-energy and pressure gradient, gravity (GradKineticKernel.jl:1-129)
+  double lDXp, lDYp, lDXK1, lDYK1, lDXK2, lDYK2, lK1, lK2;
+  {
+    const double GZh = -a.Grav * rho * Js / (d331 + d332);
+    const double Gu = S11 * DXp + S21 * DYp + GZh * (d311 + d312);
+    const double Gv = S12 * DXp + S22 * DYp + GZh * (d321 + d322);
+    gr += (Lu * Gu + Lv * Gv) / (rho * rho);
+    const double lGu = -Lu / rho, lGv = -Lv / rho;
+    const double lGZh = lGu * (d311 + d312) + lGv * (d321 + d322);
+    gr += lGZh * (-a.Grav * Js / (d331 + d332));
+    lDXp = S11 * lGu + S12 * lGv;
+    lDYp = S21 * lGu + S22 * lGv;
+    const double lGwF1 = lwl, lGwF2 = lwh;
+    if (hi) {  // Gradp = -(0.5 (p(iz+1) - p) (d332 + d331(iz+1)) + g (rho J2 + rho(iz+1) J1(iz+1)))
+      const double lGradp = lwh;
+      gr += -a.Grav * J2 * lGradp;
+      gr_p += -a.Grav * J1p * lGradp;
+      const double lGZp = -(d332 + d331p) * lGradp;
+      gp_p += 0.5 * lGZp;
+      gp += -0.5 * lGZp;
+    }
+    const double GZs = 0.5 * (K2 - K1);
+    const double lGZs = -(Lu * (d311 + d312) + Lv * (d321 + d322)) - rho * (lGwF2 * d332 + lGwF1 * d331);
+    gr += -GZs * (lGwF2 * d332 + lGwF1 * d331);
+    lK1 = -0.5 * lGZs;
+    lK2 = 0.5 * lGZs;
+    if (hi) {
+      const double Kb = 0.5 * (u_p * u_p + v_p * v_p) + 0.5 * wh * wh;
+      const double GZu = 0.5 * (Kb - K2);
+      const double lGZu = -(Lu * d312 + Lv * d322 + rho * lGwF2 * d332);
+      gr += -GZu * lGwF2 * d332;
+      const double lKb = 0.5 * lGZu;
+      lK2 += -0.5 * lGZu;
+      gu_p += lKb * u_p;
+      gv_p += lKb * v_p;
+      gwh += lKb * wh;
+    }
+    if (lo) {
+      const double Kt = 0.5 * (u_m * u_m + v_m * v_m) + 0.5 * wl * wl;
+      const double GZl = 0.5 * (K1 - Kt);
+      const double lGZl = -(Lu * d311 + Lv * d321 + rho * lGwF1 * d331);
+      gr += -GZl * lGwF1 * d331;
+      lK1 += 0.5 * lGZl;
+      const double lKt = -0.5 * lGZl;
+      gu_m += lKt * u_m;
+      gv_m += lKt * v_m;
+      gwl += lKt * wl;
+    }
+    lDXK1 = -(d111 * Lu + d121 * Lv);
+    lDYK1 = -(d211 * Lu + d221 * Lv);
+    lDXK2 = -(d112 * Lu + d122 * Lv);
+    lDYK2 = -(d212 * Lu + d222 * Lv);
+  }
+
+  // ---- continuity and rho*theta transport (OperatorKernel.jl:637-720)
+  const double cc = rth / rho;
+  const double uc = -rho * (S11 * u + S12 * v), vc = -rho * (S21 * u + S22 * v);
+  if (valid && hi) {
+    const double lFlux = -Lrth + Lrth_p, lF2 = -Lr + Lr_p;
+    const double aL = d312 * u + d322 * v + d332 * wh, aR = d311p * u_p + d321p * v_p + d331p * wh;
+    const double wCon = rho * aL + rho_p * aR;
+    const double cLL = lo ? rth_m / rho_m : cc, cR = rth_p / rho_p, cRR = hi2 ? rth_pp / rho_pp : cR;
+    const double JLL = lo ? Jm : Js, JRR = hi2 ? Jpp : Jp;
+    double cFL, cFR;
+    recu4_rev(cLL, cc, cR, cRR, JLL, Js, Jp, JRR, cFL, cFR);
+    // d|wCon|/dwCon: the sign, except where wCon is zero to rounding (below
+    // 1e-12 of the vertical-metric-weighted horizontal wind: w = 0 up to
+    // the rounding-level terrain row of a flat metric), where the upwind
+    // switch sits on its kink; there the mean of the one-sided derivatives
+    // (0) is taken, which is what central differences see
+    const double wscale = (fabs(rho * d332) + fabs(rho_p * d331p)) * (fabs(u) + fabs(v) + fabs(u_p) + fabs(v_p));
+    const double aw = fabs(wCon);
+    const double s = aw <= 1e-12 * wscale ? 0.0 : (wCon > 0.0 ? 1.0 : -1.0);
+    const double lcFL = 0.25 * lFlux * (aw + wCon), lcFR = 0.25 * lFlux * (wCon - aw);
+    const double lw = 0.25 * lFlux * ((s + 1.0) * cFL + (1.0 - s) * cFR) + 0.5 * lF2;
+    gr += lw * aL;
+    gr_p += lw * aR;
+    gu += lw * rho * d312;
+    gv += lw * rho * d322;
+    gwh += lw * (rho * d332 + rho_p * d331p);
+    gu_p += lw * rho_p * d311p;
+    gv_p += lw * rho_p * d321p;
+    double lLL = 0, lL = 0, lR = 0, lRR = 0;
+    recu4_adj(cLL, cc, cR, cRR, JLL, Js, Jp, JRR, lcFL, lcFR, lLL, lL, lR, lRR);
+    gr += -lL * cc / rho;
+    grth += lL / rho;
+    gr_p += -lR * cR / rho_p;
+    grth_p += lR / rho_p;
+    if (lo) {
+      gr_m += -lLL * cLL / rho_m;
+      grth_m += lLL / rho_m;
+    } else {
+      gr += -lLL * cc / rho;
+      grth += lLL / rho;
+    }
+    if (hi2) {
+      gr_pp += -lRR * cRR / rho_pp;
+      grth_pp += lRR / rho_pp;
+    } else {
+      gr_p += -lRR * cR / rho_p;
+      grth_p += lRR / rho_p;
+    }
+  }
+
+  // ---- hyperviscosity of pass 2 (ViscKernel.jl:86-167)
+  const double lFuC = Kc * Lu, lFuD = -Kg * Lu, lFvC = Kc * Lv, lFvD = -Kg * Lv, lDivTh = -Kd * Lrth;
+  const double gdx = (S11 * Dxc + S21 * Dyc) / Js, gdy = (S12 * Dxc + S22 * Dyc) / Js;
+
+  // ---- cotangent tiles, round A
+  if (in_tile) {
+    T.at(0, node, l) = lU1;
+    T.at(1, node, l) = lU2;
+    T.at(2, node, l) = lV1;
+    T.at(3, node, l) = lV2;
+    T.at(4, node, l) = lW1;
+    T.at(5, node, l) = lW2;
+    T.at(6, node, l) = lDXp;
+    T.at(7, node, l) = lDYp;
+    T.at(8, node, l) = lDXK1;
+    T.at(9, node, l) = lDYK1;
+    T.at(10, node, l) = lDXK2;
+    T.at(11, node, l) = lDYK2;
+    T.at(12, node, l) = Lrth;  // DivRhoTr
+    T.at(13, node, l) = Lr;    // DivRho
+    T.at(14, node, l) = S12 * lFuC - S11 * lFvC;  // DxCurl
+    T.at(15, node, l) = S22 * lFuC - S21 * lFvC;  // DyCurl
+    T.at(16, node, l) = S11 * lFuD + S12 * lFvD;  // DxDiv
+    T.at(17, node, l) = S21 * lFuD + S22 * lFvD;  // DyDiv
+    T.at(18, node, l) = lDivTh;
+  }
+  __syncthreads();
+  if (valid) {
+    // momentum: t1[var][K] feeds dx, t2[var][K] dy
+    const double M12[2] = {d121, d122}, M22[2] = {d221, d222}, M11[2] = {d111, d112}, M21[2] = {d211, d212};
+#pragma unroll
+    for (int K = 0; K < 2; ++K) {
+      const double a1 = dxT<N, NZT>(Ds, T, K, I, J, l), a2 = dyT<N, NZT>(Ds, T, K, I, J, l);
+      const double b1 = dxT<N, NZT>(Ds, T, 2 + K, I, J, l), b2 = dyT<N, NZT>(Ds, T, 2 + K, I, J, l);
+      const double c1x = dxT<N, NZT>(Ds, T, 4 + K, I, J, l), c2y = dyT<N, NZT>(Ds, T, 4 + K, I, J, l);
+      const double gw = -M12[K] * a1 - M22[K] * a2 - M11[K] * b1 - M21[K] * b2;
+      if (K == 0) gwl += gw;
+      else gwh += gw;
+      gu += M12[K] * c1x + M22[K] * c2y;
+      gv += -M11[K] * c1x - M21[K] * c2y;
+    }
+    // kinetic energy and pressure
+    lK1 += dxT<N, NZT>(Ds, T, 8, I, J, l) + dyT<N, NZT>(Ds, T, 9, I, J, l);
+    lK2 += dxT<N, NZT>(Ds, T, 10, I, J, l) + dyT<N, NZT>(Ds, T, 11, I, J, l);
+    gp += dxT<N, NZT>(Ds, T, 6, I, J, l) + dyT<N, NZT>(Ds, T, 7, I, J, l);
+    gu += (lK1 + lK2) * u;
+    gv += (lK1 + lK2) * v;
+    gwl += lK1 * wl;
+    gwh += lK2 * wh;
+    // horizontal divergence of rho v_contra and rho*theta v_contra
+    const double lPx = dxT<N, NZT>(Ds, T, 12, I, J, l), lQy = dyT<N, NZT>(Ds, T, 12, I, J, l);
+    const double lRx = dxT<N, NZT>(Ds, T, 13, I, J, l), lRy = dyT<N, NZT>(Ds, T, 13, I, J, l);
+    const double luc = lPx * cc + lRx, lvc = lQy * cc + lRy, lcc = lPx * uc + lQy * vc;
+    gu += -rho * (S11 * luc + S21 * lvc);
+    gv += -rho * (S12 * luc + S22 * lvc);
+    gr += -(S11 * u + S12 * v) * luc - (S21 * u + S22 * v) * lvc;
+    grth += lcc / rho;
+    gr += -lcc * cc / rho;
+  }
+  // hyperviscosity: the weak-derivative transposes, then round B
+  double lcu = 0, ldv = 0, lDxc = 0, lDyc = 0;
+  if (valid) {
+    const double lCurl = dxT<N, NZT>(DWs, T, 14, I, J, l) + dyT<N, NZT>(DWs, T, 15, I, J, l);
+    const double lDiv = dxT<N, NZT>(DWs, T, 16, I, J, l) + dyT<N, NZT>(DWs, T, 17, I, J, l);
+    const double lThCx = dxT<N, NZT>(DWs, T, 18, I, J, l), lThCy = dyT<N, NZT>(DWs, T, 18, I, J, l);
+    const double lgdx = rho * (S11 * lThCx + S21 * lThCy), lgdy = rho * (S12 * lThCx + S22 * lThCy);
+    gr += lThCx * (S11 * gdx + S12 * gdy) + lThCy * (S21 * gdx + S22 * gdy);
+    lDxc = (S11 * lgdx + S12 * lgdy) / Js;
+    lDyc = (S21 * lgdx + S22 * lgdy) / Js;
+    lcu = lCurl / Js;
+    ldv = lDiv / Js;
+  }
+  if (in_tile) {
+    T.at(19, node, l) = lcu;
+    T.at(20, node, l) = ldv;
+    T.at(21, node, l) = lDxc;
+    T.at(22, node, l) = lDyc;
+  }
+  __syncthreads();
+  if (!valid) return;
+  {
+    const double lTh = dxT<N, NZT>(Ds, T, 21, I, J, l) + dyT<N, NZT>(Ds, T, 22, I, J, l);
+    const double luC = dxT<N, NZT>(Ds, T, 19, I, J, l), lvC = dyT<N, NZT>(Ds, T, 19, I, J, l);
+    const double luD = dxT<N, NZT>(Ds, T, 20, I, J, l), lvD = dyT<N, NZT>(Ds, T, 20, I, J, l);
+    aadd(&lamT1[ui], -S12 * luC - S22 * lvC);
+    aadd(&lamT1[vs + ui], S11 * luC + S21 * lvC);
+    aadd(&lamT1[2 * vs + ui], S11 * luD + S21 * lvD);
+    aadd(&lamT1[3 * vs + ui], S12 * luD + S22 * lvD);
+    aadd(&lamT1[4 * vs + ui], lTh);
+  }
+  // w at the lower interface: w(iz-1), or the free-slip value of (u, v)
+  if (!lo) {
+    gu += -gwl * d311 / d331;
+    gv += -gwl * d321 / d331;
+  }
+  // pressure -> rho*theta: dp/d(rho theta) = p / ((1 - kappa) rho theta)
+  grth += gp * a.eos_y * p / rth;
+  aadd(&lamU[ui], gr);
+  aadd(&lamU[vs + ui], gu);
+  aadd(&lamU[2 * vs + ui], gv);
+  aadd(&lamU[3 * vs + ui], gwh);
+  aadd(&lamU[4 * vs + ui], grth);
+  if (lo) {
+    aadd(&lamU[ui - 1], gr_m);
+    aadd(&lamU[vs + ui - 1], gu_m);
+    aadd(&lamU[2 * vs + ui - 1], gv_m);
+    aadd(&lamU[3 * vs + ui - 1], gwl);
+    aadd(&lamU[4 * vs + ui - 1], grth_m);
+  }
+  if (hi) {
+    grth_p += gp_p * a.eos_y * eos_p(a, rth_p) / rth_p;
+    aadd(&lamU[ui + 1], gr_p);
+    aadd(&lamU[vs + ui + 1], gu_p);
+    aadd(&lamU[2 * vs + ui + 1], gv_p);
+    aadd(&lamU[4 * vs + ui + 1], grth_p);
+  }
+  if (hi2) {
+    aadd(&lamU[ui + 2], gr_pp);
+    aadd(&lamU[4 * vs + ui + 2], grth_pp);
+  }
+}
+
+// Reverse of pass 1 (HyperViscKernel!) for the cotangent of the DSS'd cache:
+// Lam_f = lamT1_f / (M1 + M2); adds into lamU (rho, u, v, rho*theta).
+template <int N, int NZT, bool CM>
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64)
+    k_adj1(KArgs a, int nchunk, const double *__restrict__ U, const double *__restrict__ lamT1,
+           double *__restrict__ lamU) {
+  constexpr int NN = N * N;
+  extern __shared__ double smem[];
+  double *Ds = smem, *DWs = Ds + NN;
+  Tiles<N, NZT, double> T{Ds + 2 * NN};
+  load_dmats<N>(a, Ds, DWs);
+  const int Nz = a.Nz;
+  const int e = blockIdx.x / nchunk, chunk = blockIdx.x - e * nchunk;
+  const int t = threadIdx.x, node = t / NZT, l = t - node * NZT;
+  const int iz = chunk * NZT + l;
+  const bool in_tile = node < NN, valid = in_tile && iz < Nz;
+  const int I = node % N, J = node / N;
+  const size_t vs = (size_t)a.NumG * Nz;
+  size_t ui = 0;
+  double rho = 1, rth = 1, S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1;
+  double L0 = 0, L1 = 0, L2 = 0, L3 = 0, L4 = 0;
+  if (valid) {
+    const size_t g = (size_t)(a.glob[(size_t)e * NN + node] & 0x7fffffff);
+    ui = g * Nz + iz;
+    rho = U[ui];
+    rth = U[4 * vs + ui];
+    S11 = met_at<NN, CM>(a, e, M111, node, iz) + met_at<NN, CM>(a, e, M112, node, iz);
+    S12 = met_at<NN, CM>(a, e, M121, node, iz) + met_at<NN, CM>(a, e, M122, node, iz);
+    S21 = met_at<NN, CM>(a, e, M211, node, iz) + met_at<NN, CM>(a, e, M212, node, iz);
+    S22 = met_at<NN, CM>(a, e, M221, node, iz) + met_at<NN, CM>(a, e, M222, node, iz);
+    Js = met_at<NN, CM>(a, e, MJ1, node, iz) + met_at<NN, CM>(a, e, MJ2, node, iz);
+    const double ims = a.iMs[ui];
+    L0 = lamT1[ui] * ims;
+    L1 = lamT1[vs + ui] * ims;
+    L2 = lamT1[2 * vs + ui] * ims;
+    L3 = lamT1[3 * vs + ui] * ims;
+    L4 = lamT1[4 * vs + ui] * ims;
+  }
+  // Temp1 += (-FuC, -FvC, FuD, FvD, DivTh) / Ms
+  const double lFuC = -L0, lFvC = -L1, lFuD = L2, lFvD = L3;
+  if (in_tile) {
+    T.at(0, node, l) = S12 * lFuC - S11 * lFvC;
+    T.at(1, node, l) = S22 * lFuC - S21 * lFvC;
+    T.at(2, node, l) = S11 * lFuD + S12 * lFvD;
+    T.at(3, node, l) = S21 * lFuD + S22 * lFvD;
+    T.at(4, node, l) = L4;
+  }
+  __syncthreads();
+  double lcu = 0, ldv = 0, lDxc = 0, lDyc = 0;
+  if (valid) {
+    const double lCurl = dxT<N, NZT>(DWs, T, 0, I, J, l) + dyT<N, NZT>(DWs, T, 1, I, J, l);
+    const double lDiv = dxT<N, NZT>(DWs, T, 2, I, J, l) + dyT<N, NZT>(DWs, T, 3, I, J, l);
+    const double lThCx = dxT<N, NZT>(DWs, T, 4, I, J, l), lThCy = dyT<N, NZT>(DWs, T, 4, I, J, l);
+    const double lgdx = S11 * lThCx + S21 * lThCy, lgdy = S12 * lThCx + S22 * lThCy;
+    lDxc = (S11 * lgdx + S12 * lgdy) / Js;
+    lDyc = (S21 * lgdx + S22 * lgdy) / Js;
+    lcu = lCurl / Js;
+    ldv = lDiv / Js;
+  }
+  if (in_tile) {
+    T.at(5, node, l) = lcu;
+    T.at(6, node, l) = ldv;
+    T.at(7, node, l) = lDxc;
+    T.at(8, node, l) = lDyc;
+  }
+  __syncthreads();
+  if (!valid) return;
+  const double lTh = dxT<N, NZT>(Ds, T, 7, I, J, l) + dyT<N, NZT>(Ds, T, 8, I, J, l);
+  const double luC = dxT<N, NZT>(Ds, T, 5, I, J, l), lvC = dyT<N, NZT>(Ds, T, 5, I, J, l);
+  const double luD = dxT<N, NZT>(Ds, T, 6, I, J, l), lvD = dyT<N, NZT>(Ds, T, 6, I, J, l);
+  // (uC, vC) = Curl12(u, v), (uD, vD) = Contra12(u, v), theta = rho theta / rho
+  const double gu = -S12 * luC - S22 * lvC + S11 * luD + S21 * lvD;
+  const double gv = S11 * luC + S21 * lvC + S12 * luD + S22 * lvD;
+  const double th = rth / rho;
+  aadd(&lamU[ui], -lTh * th / rho);
+  aadd(&lamU[vs + ui], gu);
+  aadd(&lamU[2 * vs + ui], gv);
+  aadd(&lamU[4 * vs + ui], lTh / rho);
+}
+
+// Transpose of the column Schur solve k = S(v) (SchurSolve.jl:45-147) for
+// the frozen Jacobian blocks, one thread per column:
+//   lv = S^T lk, and the cotangents of the four Jacobian blocks (through
+//   the right-hand side, the back substitution and the tridiagonal matrix
+//   T = I/fac^2 - JWRho JRhoW - JWRhoTh JRhoThW) are added into lJ*.
+// kx: the forward k (its w component is the solution x of T x = rhs);
+// v: the forward right-hand side before the solve; bw: Nz doubles per column.
+__global__ void k_schur_adj(KArgs a, const double *__restrict__ lk, const double *__restrict__ kx,
+                            const double *__restrict__ v, const double *__restrict__ JRhoW,
+                            const double *__restrict__ JWRho, const double *__restrict__ JWRhoTh,
+                            const double *__restrict__ JRhoThW, double fac, double *__restrict__ lv,
+                            double *__restrict__ lJRhoW, double *__restrict__ lJWRho, double *__restrict__ lJWRhoTh,
+                            double *__restrict__ lJRhoThW, double *__restrict__ bw) {
+  const int Nz = a.Nz, n = Nz - 1;
+  const size_t vs = (size_t)a.NumG * Nz;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (size_t)a.NumG) return;
+  const size_t b = g * Nz, jb = 2 * (size_t)n * g;
+#define JB(A, c, i) A[jb + 2 * (size_t)(i) + (c)]
+  const double *x = kx + 3 * vs + b;
+  double *l1 = lv + b, *l2 = lv + vs + b, *l3 = lv + 2 * vs + b, *l4 = lv + 3 * vs + b, *l5 = lv + 4 * vs + b;
+  const double *v1 = v + b, *v5 = v + 4 * vs + b;
+  // k1 = fac (v1 + JRhoW x), k5 = fac (v5 + JRhoThW x), k2,3 = fac v2,3
+  for (int iz = 0; iz < Nz; ++iz) {
+    l1[iz] = fac * lk[b + iz];
+    l2[iz] = fac * lk[vs + b + iz];
+    l3[iz] = fac * lk[2 * vs + b + iz];
+    l5[iz] = fac * lk[4 * vs + b + iz];
+  }
+  for (int i = 0; i < n; ++i) {
+    const double xi = x[i];
+    l4[i] = lk[3 * vs + b + i] + JB(JRhoW, 0, i) * l1[i] + JB(JRhoW, 1, i) * l1[i + 1] + JB(JRhoThW, 0, i) * l5[i] +
+            JB(JRhoThW, 1, i) * l5[i + 1];
+    JB(lJRhoW, 0, i) += l1[i] * xi;
+    JB(lJRhoW, 1, i) += l1[i + 1] * xi;
+    JB(lJRhoThW, 0, i) += l5[i] * xi;
+    JB(lJRhoThW, 1, i) += l5[i + 1] * xi;
+  }
+  // T^T y = lx (T[i,i] = t1_i, T[i-1,i] = t0_i, T[i+1,i] = t2_i)
+  const double if2 = 1.0 / (fac * fac);
+  auto t1 = [&](int i) {
+    return if2 - JB(JWRho, 1, i) * JB(JRhoW, 0, i) - JB(JWRho, 0, i) * JB(JRhoW, 1, i) -
+           JB(JWRhoTh, 1, i) * JB(JRhoThW, 0, i) - JB(JWRhoTh, 0, i) * JB(JRhoThW, 1, i);
+  };
+  auto t0 = [&](int i) { return -JB(JWRho, 0, i - 1) * JB(JRhoW, 0, i) - JB(JWRhoTh, 0, i - 1) * JB(JRhoThW, 0, i); };
+  auto t2 = [&](int i) { return -JB(JWRho, 1, i + 1) * JB(JRhoW, 1, i) - JB(JWRhoTh, 1, i + 1) * JB(JRhoThW, 1, i); };
+  double *bp = bw + b;
+  if (n > 0) {
+    bp[0] = t1(0);
+    for (int i = 1; i < n; ++i) {
+      const double m = t0(i) / bp[i - 1];
+      bp[i] = t1(i) - m * t2(i - 1);
+      l4[i] = l4[i] - m * l4[i - 1];
+    }
+    l4[n - 1] = l4[n - 1] / bp[n - 1];
+    for (int i = n - 2; i >= 0; --i) l4[i] = (l4[i] - t2(i) * l4[i + 1]) / bp[i];
+  }
+  for (int i = 0; i < n; ++i) {
+    const double y = l4[i], xi = x[i];
+    // T x = rhs: dT contributes -y_r x_c at T's entry (r, c)
+    const double lt1 = -y * xi;
+    JB(lJWRho, 1, i) += -lt1 * JB(JRhoW, 0, i);
+    JB(lJRhoW, 0, i) += -lt1 * JB(JWRho, 1, i);
+    JB(lJWRho, 0, i) += -lt1 * JB(JRhoW, 1, i);
+    JB(lJRhoW, 1, i) += -lt1 * JB(JWRho, 0, i);
+    JB(lJWRhoTh, 1, i) += -lt1 * JB(JRhoThW, 0, i);
+    JB(lJRhoThW, 0, i) += -lt1 * JB(JWRhoTh, 1, i);
+    JB(lJWRhoTh, 0, i) += -lt1 * JB(JRhoThW, 1, i);
+    JB(lJRhoThW, 1, i) += -lt1 * JB(JWRhoTh, 0, i);
+    if (i >= 1) {
+      const double lt0 = -l4[i - 1] * xi;
+      JB(lJWRho, 0, i - 1) += -lt0 * JB(JRhoW, 0, i);
+      JB(lJRhoW, 0, i) += -lt0 * JB(JWRho, 0, i - 1);
+      JB(lJWRhoTh, 0, i - 1) += -lt0 * JB(JRhoThW, 0, i);
+      JB(lJRhoThW, 0, i) += -lt0 * JB(JWRhoTh, 0, i - 1);
+    }
+    if (i <= n - 2) {
+      const double lt2 = -l4[i + 1] * xi;
+      JB(lJWRho, 1, i + 1) += -lt2 * JB(JRhoW, 1, i);
+      JB(lJRhoW, 1, i) += -lt2 * JB(JWRho, 1, i + 1);
+      JB(lJWRhoTh, 1, i + 1) += -lt2 * JB(JRhoThW, 1, i);
+      JB(lJRhoThW, 1, i) += -lt2 * JB(JWRhoTh, 1, i + 1);
+    }
+    // rhs_i = v4_i / fac + JWRho (v1_i, v1_i+1) + JWRhoTh (v5_i, v5_i+1)
+    JB(lJWRho, 1, i) += y * v1[i];
+    JB(lJWRho, 0, i) += y * v1[i + 1];
+    JB(lJWRhoTh, 1, i) += y * v5[i];
+    JB(lJWRhoTh, 0, i) += y * v5[i + 1];
+    l1[i] += JB(JWRho, 1, i) * y;
+    l1[i + 1] += JB(JWRho, 0, i) * y;
+    l5[i] += JB(JWRhoTh, 1, i) * y;
+    l5[i + 1] += JB(JWRhoTh, 0, i) * y;
+  }
+  for (int i = 0; i < n; ++i) l4[i] = l4[i] / fac;
+  l4[n] = 0.0;
+#undef JB
+}
+
+// Reverse of the Jacobian blocks (JacSchur.jl:29-75) of the state U, one
+// thread per column: adds the rho and rho*theta cotangents into lamU.
+__global__ void k_jac_adj(KArgs a, const double *__restrict__ U, const double *__restrict__ dz,
+                          const double *__restrict__ lJRhoW, const double *__restrict__ lJWRho,
+                          const double *__restrict__ lJWRhoTh, const double *__restrict__ lJRhoThW,
+                          double *__restrict__ lamU) {
+  const int Nz = a.Nz, n = Nz - 1;
+  const size_t vs = (size_t)a.NumG * Nz;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (size_t)a.NumG) return;
+  const size_t b = g * Nz, jb = 2 * (size_t)n * g;
+  for (int i = 0; i < n; ++i) {
+    const size_t q = b + i, o = jb + 2 * (size_t)i;
+    const double RhoL = U[q], RhoR = U[q + 1], RhoThL = U[4 * vs + q], RhoThR = U[4 * vs + q + 1];
+    const double dzL = dz[q], dzR = dz[q + 1], sdz = dzL + dzR, hdz = 0.5 * sdz;
+    const double RhoF = (RhoL * dzL + RhoR * dzR) / sdz;
+    const double dPL = eos_dpdrhoth(a, RhoThL), dPR = eos_dpdrhoth(a, RhoThR);
+    const double wt0 = -dPR / RhoF / hdz, wt1 = dPL / RhoF / hdz;
+    const double wr0 = -a.Grav * dzR / RhoF / sdz, wr1 = -a.Grav * dzL / RhoF / sdz;
+    const double lRhoF = -lJRhoW[o] / dzL + lJRhoW[o + 1] / dzR -
+                         (lJWRhoTh[o] * wt0 + lJWRhoTh[o + 1] * wt1 + lJWRho[o] * wr0 + lJWRho[o + 1] * wr1) / RhoF;
+    lamU[q] += lRhoF * dzL / sdz;
+    lamU[q + 1] += lRhoF * dzR / sdz;
+    // dPdTh = Rd (Rd rho theta / p0)^(kappa / (1 - kappa))
+    const double ldPR = -lJWRhoTh[o] / (RhoF * hdz), ldPL = lJWRhoTh[o + 1] / (RhoF * hdz);
+    const double lRhoThF = -lJRhoThW[o] / dzL + lJRhoThW[o + 1] / dzR;
+    lamU[4 * vs + q] += ldPL * dPL * a.dpd_y / RhoThL + lRhoThF * dzL / sdz;
+    lamU[4 * vs + q + 1] += ldPR * dPR * a.dpd_y / RhoThR + lRhoThF * dzR / sdz;
+  }
+}
+
+// y += a x
+__global__ void k_adj_axpy(double *__restrict__ y, const double *__restrict__ x, double s, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] += s * x[i];
+}
+// y = a x
+__global__ void k_adj_scale(double *__restrict__ y, const double *__restrict__ x, double s, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    y[i] = s * x[i];
+}
+
+// ------------------------------------------------------------------------
+// Host side
+
+// chunk height of the element reverse kernels (LDS: 23 tiles of N*N*8 levels)
+static const int ADJ_NZT = 8;
+
+static size_t adj_lds2(const cgd_ctx *c) { return sizeof(double) * (2 * c->NN + 23 * c->NN * ADJ_NZT); }
+static size_t adj_lds1(const cgd_ctx *c) { return sizeof(double) * (2 * c->NN + 9 * c->NN * ADJ_NZT); }
+
+// zero fill by a kernel (no memset nodes if a caller captures the sequence)
+static int adj_zero(cgd_ctx *c, double *x, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, st, x, n, 0.0);
+  HIPCHK(c, hipGetLastError());
+  return CGD_OK;
+}
+
+static int adj_supported(cgd_ctx *c, const char *who) {
+  int r = check(c);
+  if (r) return r;
+  if (c->ntr > 0 || c->hooks.forcing || c->hooks.damping || c->ka.iv >= 0)
+    return fail(c, CGD_ERR_UNSUPPORTED,
+                std::string(who) + ": the adjoint covers the dry configuration without tracers or physics hooks");
+  return CGD_OK;
+}
+
+// work layout (doubles); ng = NumG*Nz, n5 = 5 ng, nj = 2 (Nz-1) NumG
+struct AdjLayout {
+  size_t ng, n5, nj;
+  size_t F, T1, AuxG, Lam, lT1;                        // tendency adjoint
+  size_t bw, Vi, fV, k, lk, lVi, lfV, Jb, tri, lJ;     // step adjoint
+  size_t ck, seg;                                      // runs: checkpoints, segment states
+  size_t total;
+};
+static AdjLayout adj_layout(const cgd_ctx *c, int nsteps, int every) {
+  AdjLayout L{};
+  L.ng = (size_t)c->NumG * c->Nz;
+  L.n5 = 5 * L.ng;
+  L.nj = 2 * (size_t)(c->Nz - 1) * c->NumG;
+  size_t o = 0;
+  L.F = o; o += L.n5;
+  L.T1 = o; o += L.n5;
+  L.AuxG = o; o += 4 * L.ng;
+  L.Lam = o; o += L.n5;
+  L.lT1 = o; o += L.n5;
+  L.bw = o; o += L.ng;
+  L.Vi = o; o += L.n5;
+  L.fV = o; o += L.n5;
+  L.k = o; o += 3 * L.n5;
+  L.lk = o; o += 3 * L.n5;
+  L.lVi = o; o += L.n5;
+  L.lfV = o; o += L.n5;
+  L.Jb = o; o += 4 * L.nj;
+  L.tri = o; o += 3 * (L.nj / 2);
+  L.lJ = o; o += 4 * L.nj;
+  L.ck = o;
+  if (nsteps > 0) {
+    const int nck = (nsteps + every - 1) / every;
+    o += (size_t)nck * L.n5;
+    L.seg = o;
+    o += (size_t)every * L.n5;
+  } else {
+    L.seg = o;
+  }
+  L.total = o;
+  return L;
+}
+
+extern "C" int cgd_adjoint_work_size(cgd_ctx *c, int nsteps, int every, size_t *ndoubles) {
+  int r = check(c);
+  if (r) return r;
+  if (!ndoubles) return fail(c, CGD_ERR_ARG, "cgd_adjoint_work_size: null output");
+  if (nsteps < 0 || (nsteps > 0 && every < 1)) return fail(c, CGD_ERR_ARG, "cgd_adjoint_work_size: bad step counts");
+  *ndoubles = adj_layout(c, nsteps, every).total;
+  return CGD_OK;
+}
+
+// lamU = (dF/dU)^T lamF at U; the forward tendency is recomputed into work
+static int fcn_adjoint(cgd_ctx *c, const double *U, const double *lamF, double *lamU, double *work,
+                       hipStream_t st) {
+  int r;
+  const AdjLayout L = adj_layout(c, 0, 1);
+  double *F = work + L.F, *T1 = work + L.T1, *AuxG = work + L.AuxG, *Lam = work + L.Lam, *lT1 = work + L.lT1;
+  if ((r = cgd_prepare(c, U, AuxG, st))) return r;
+  if ((r = run_fcn<double>(c, F, U, U, T1, AuxG, st, true))) return r;
+  hipLaunchKernelGGL(k_adj_pre, dim3(grid_for(L.ng, 256)), dim3(256), 0, st, c->ka, U, F, lamF, Lam, lamU, lT1,
+                     L.ng);
+  HIPCHK(c, hipGetLastError());
+  const int nchunk = (c->Nz + ADJ_NZT - 1) / ADJ_NZT;
+  const dim3 grid((unsigned)(c->NF * nchunk)), block((unsigned)((c->NN * ADJ_NZT + 63) / 64 * 64));
+  if (c->N == 5) {
+    if (c->cm) hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, U, T1, Lam, lamU, lT1);
+    else hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, U, T1, Lam, lamU, lT1);
+  } else {
+    if (c->cm) hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, U, T1, Lam, lamU, lT1);
+    else hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, U, T1, Lam, lamU, lT1);
+  }
+  HIPCHK(c, hipGetLastError());
+  if (c->N == 5) {
+    if (c->cm) hipLaunchKernelGGL((k_adj1<5, ADJ_NZT, true>), grid, block, adj_lds1(c), st, c->ka, nchunk, U, lT1, lamU);
+    else hipLaunchKernelGGL((k_adj1<5, ADJ_NZT, false>), grid, block, adj_lds1(c), st, c->ka, nchunk, U, lT1, lamU);
+  } else {
+    if (c->cm) hipLaunchKernelGGL((k_adj1<4, ADJ_NZT, true>), grid, block, adj_lds1(c), st, c->ka, nchunk, U, lT1, lamU);
+    else hipLaunchKernelGGL((k_adj1<4, ADJ_NZT, false>), grid, block, adj_lds1(c), st, c->ka, nchunk, U, lT1, lamU);
+  }
+  HIPCHK(c, hipGetLastError());
+  return CGD_OK;
+}
+
+extern "C" int cgd_fcn_adjoint(cgd_ctx *c, const double *U, const double *lamF, double *lamU, double *work,
+                               void *stream) {
+  int r = adj_supported(c, "cgd_fcn_adjoint");
+  if (r) return r;
+  if (!U || !lamF || !lamU || !work) return fail(c, CGD_ERR_ARG, "cgd_fcn_adjoint: null array");
+  hipStream_t st = S(stream);
+  const AdjLayout L = adj_layout(c, 0, 1);
+  if ((r = adj_zero(c, work + L.AuxG + 3 * L.ng, L.ng, st))) return r;
+  return fcn_adjoint(c, U, lamF, lamU, work, st);
+}
+
+extern "C" int cgd_schur_adjoint(cgd_ctx *c, const double *lamk, const double *k, const double *v,
+                                 const double *JRhoW, const double *JWRho, const double *JWRhoTh,
+                                 const double *JRhoThW, double fac, double *lamv, double *lJRhoW, double *lJWRho,
+                                 double *lJWRhoTh, double *lJRhoThW, double *work, void *stream) {
+  int r = adj_supported(c, "cgd_schur_adjoint");
+  if (r) return r;
+  if (!lamk || !k || !v || !JRhoW || !JWRho || !JWRhoTh || !JRhoThW || !lamv || !lJRhoW || !lJWRho || !lJWRhoTh ||
+      !lJRhoThW || !work)
+    return fail(c, CGD_ERR_ARG, "cgd_schur_adjoint: null array");
+  if (!(fac > 0.0)) return fail(c, CGD_ERR_ARG, "cgd_schur_adjoint: fac must be positive");
+  hipLaunchKernelGGL(k_schur_adj, dim3((unsigned)((c->NumG + 63) / 64)), dim3(64), 0, S(stream), c->ka, lamk, k, v, JRhoW,
+                     JWRho, JWRhoTh, JRhoThW, fac, lamv, lJRhoW, lJWRho, lJWRhoTh, lJRhoThW, work);
+  HIPCHK(c, hipGetLastError());
+  return CGD_OK;
+}
+
+extern "C" int cgd_jac_adjoint(cgd_ctx *c, const double *U, const double *lJRhoW, const double *lJWRho,
+                               const double *lJWRhoTh, const double *lJRhoThW, double *lamU, void *stream) {
+  int r = adj_supported(c, "cgd_jac_adjoint");
+  if (r) return r;
+  if (!U || !lJRhoW || !lJWRho || !lJWRhoTh || !lJRhoThW || !lamU)
+    return fail(c, CGD_ERR_ARG, "cgd_jac_adjoint: null array");
+  hipLaunchKernelGGL(k_jac_adj, dim3((unsigned)((c->NumG + 63) / 64)), dim3(64), 0, S(stream), c->ka, U, c->dz, lJRhoW, lJWRho,
+                     lJWRhoTh, lJRhoThW, lamU);
+  HIPCHK(c, hipGetLastError());
+  return CGD_OK;
+}
+
+// SSP-Knoth tableau (RosenbrockMethod.jl:24-48)
+static void ssp_knoth(double A[3][3], double Cc[3][3], double Mw[3]) {
+  const double a[3][3] = {{0, 0, 0}, {1.0, 0, 0}, {0.25, 0.25, 0}};
+  const double cc[3][3] = {{-1.0, 0, 0}, {-0.0, -1.0, 0}, {-0.75, -0.75, -1.0}};
+  std::memcpy(A, a, sizeof(a));
+  std::memcpy(Cc, cc, sizeof(cc));
+  Mw[2] = 2.0 / 3.0;
+  Mw[1] = 1.0 / 6.0 - (-0.75) * Mw[2];
+  Mw[0] = 1.0 / 6.0 - (-0.75) * Mw[2];
+}
+
+// One step's adjoint: lamV (the cotangent of the step's result) becomes the
+// cotangent of Vn.  The stages are recomputed from Vn (composed FP64 path:
+// prepare, tendency, Jacobian of Vn, Schur solves).
+static int rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, double *lamV, double *work, hipStream_t st) {
+  int r;
+  double A[3][3], Cc[3][3], Mw[3];
+  ssp_knoth(A, Cc, Mw);
+  const double fac = dt;  // dt * gamma, gamma = 1
+  const AdjLayout L = adj_layout(c, 0, 1);
+  const size_t n5 = L.n5, nj = L.nj;
+  double *AuxG = work + L.AuxG, *F = work + L.F, *T1 = work + L.T1, *Vi = work + L.Vi, *fV = work + L.fV;
+  double *k[3] = {work + L.k, work + L.k + n5, work + L.k + 2 * n5};
+  double *lk[3] = {work + L.lk, work + L.lk + n5, work + L.lk + 2 * n5};
+  double *lVi = work + L.lVi, *lfV = work + L.lfV;
+  double *Jb[4] = {work + L.Jb, work + L.Jb + nj, work + L.Jb + 2 * nj, work + L.Jb + 3 * nj};
+  double *lJ[4] = {work + L.lJ, work + L.lJ + nj, work + L.lJ + 2 * nj, work + L.lJ + 3 * nj};
+  double *tri = work + L.tri;
+  const dim3 gb(grid_for(n5, 256)), bb(256);
+  // forward: the Jacobian of Vn and the three stage increments
+  if ((r = cgd_prepare(c, Vn, AuxG, st))) return r;
+  if ((r = cgd_jac(c, Jb[0], Jb[1], Jb[2], Jb[3], Vn, AuxG, st))) return r;
+  for (int is = 0; is < 3; ++is) {
+    hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, Vi, Vn, k[0], A[is][0], k[1], A[is][1], is, n5);
+    HIPCHK(c, hipGetLastError());
+    if ((r = cgd_prepare(c, Vi, AuxG, st))) return r;
+    if ((r = run_fcn<double>(c, fV, Vi, Vi, T1, AuxG, st, true))) return r;
+    if (is > 0) {
+      hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, fV, fV, k[0], Cc[is][0] / dt, k[1], Cc[is][1] / dt, is, n5);
+      HIPCHK(c, hipGetLastError());
+    }
+    if ((r = cgd_schur(c, k[is], fV, Jb[0], Jb[1], Jb[2], Jb[3], tri, fac, is == 0, st))) return r;
+  }
+  // reverse
+  for (int is = 0; is < 3; ++is) {
+    hipLaunchKernelGGL(k_adj_scale, gb, bb, 0, st, lk[is], lamV, Mw[is], n5);
+    HIPCHK(c, hipGetLastError());
+  }
+  if ((r = adj_zero(c, lJ[0], 4 * nj, st))) return r;
+  for (int is = 2; is >= 0; --is) {
+    hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, Vi, Vn, k[0], A[is][0], k[1], A[is][1], is, n5);
+    HIPCHK(c, hipGetLastError());
+    if ((r = cgd_prepare(c, Vi, AuxG, st))) return r;
+    if ((r = run_fcn<double>(c, F, Vi, Vi, T1, AuxG, st, true))) return r;
+    hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, fV, F, k[0], Cc[is][0] / dt, k[1], Cc[is][1] / dt, is, n5);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(k_schur_adj, dim3((unsigned)((c->NumG + 63) / 64)), dim3(64), 0, st, c->ka, lk[is], k[is], fV, Jb[0],
+                       Jb[1], Jb[2], Jb[3], fac, lfV, lJ[0], lJ[1], lJ[2], lJ[3], work + L.bw);
+    HIPCHK(c, hipGetLastError());
+    for (int j = 0; j < is; ++j) {
+      hipLaunchKernelGGL(k_adj_axpy, gb, bb, 0, st, lk[j], lfV, Cc[is][j] / dt, n5);
+      HIPCHK(c, hipGetLastError());
+    }
+    // the tendency's adjoint at Vi; F and T1 hold its forward values
+    hipLaunchKernelGGL(k_adj_pre, dim3(grid_for(L.ng, 256)), dim3(256), 0, st, c->ka, Vi, F, lfV, work + L.Lam, lVi,
+                       work + L.lT1, L.ng);
+    HIPCHK(c, hipGetLastError());
+    {
+      const int nchunk = (c->Nz + ADJ_NZT - 1) / ADJ_NZT;
+      const dim3 grid((unsigned)(c->NF * nchunk)), block((unsigned)((c->NN * ADJ_NZT + 63) / 64 * 64));
+      double *Lam = work + L.Lam, *lT1 = work + L.lT1;
+      if (c->N == 5) {
+        if (c->cm) {
+          hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1, Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj1<5, ADJ_NZT, true>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
+        } else {
+          hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1, Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj1<5, ADJ_NZT, false>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
+        }
+      } else {
+        if (c->cm) {
+          hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1, Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj1<4, ADJ_NZT, true>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
+        } else {
+          hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1, Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj1<4, ADJ_NZT, false>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
+        }
+      }
+      HIPCHK(c, hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_adj_axpy, gb, bb, 0, st, lamV, lVi, 1.0, n5);
+    HIPCHK(c, hipGetLastError());
+    for (int j = 0; j < is; ++j) {
+      hipLaunchKernelGGL(k_adj_axpy, gb, bb, 0, st, lk[j], lVi, A[is][j], n5);
+      HIPCHK(c, hipGetLastError());
+    }
+  }
+  hipLaunchKernelGGL(k_jac_adj, dim3((unsigned)((c->NumG + 63) / 64)), dim3(64), 0, st, c->ka, Vn, c->dz, lJ[0], lJ[1], lJ[2],
+                     lJ[3], lamV);
+  HIPCHK(c, hipGetLastError());
+  return CGD_OK;
+}
+
+static int rw_args(cgd_ctx *c, const char *who, const double *V, double dt, const double *lamV, const double *work) {
+  int r = adj_supported(c, who);
+  if (r) return r;
+  if (!V || !lamV || !work) return fail(c, CGD_ERR_ARG, std::string(who) + ": null array");
+  if (!(dt > 0.0)) return fail(c, CGD_ERR_ARG, std::string(who) + ": dt must be positive");
+  return CGD_OK;
+}
+
+extern "C" int cgd_rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, double *lamV, double *work,
+                                   void *stream) {
+  int r = rw_args(c, "cgd_rw_step_adjoint", Vn, dt, lamV, work);
+  if (r) return r;
+  hipStream_t st = S(stream);
+  const AdjLayout L = adj_layout(c, 0, 1);
+  if ((r = adj_zero(c, work + L.AuxG + 3 * L.ng, L.ng, st))) return r;
+  return rw_step_adjoint(c, Vn, dt, lamV, work, st);
+}
+
+// nsteps steps from V0 with checkpoints every `every` steps: the forward
+// run stores the state at the start of each segment, the reverse sweep
+// recomputes a segment's step states from its checkpoint and runs the step
+// adjoints backwards.  lamV: in, the cotangent of the final state; out, that
+// of V0.  Vfinal (optional) receives the final state.
+extern "C" int cgd_rw_adjoint(cgd_ctx *c, const double *V0, double dt, int nsteps, int every, double *lamV,
+                              double *Vfinal, double *work, void *stream) {
+  int r = rw_args(c, "cgd_rw_adjoint", V0, dt, lamV, work);
+  if (r) return r;
+  if (nsteps < 1 || every < 1) return fail(c, CGD_ERR_ARG, "cgd_rw_adjoint: nsteps and every must be positive");
+  hipStream_t st = S(stream);
+  const AdjLayout L = adj_layout(c, nsteps, every);
+  const size_t n5 = L.n5;
+  if ((r = adj_zero(c, work + L.AuxG + 3 * L.ng, L.ng, st))) return r;
+  // the forward step runs on the composed path with these buffers
+  cgd_rw_work fw{};
+  fw.Vn = work + L.Vi;
+  fw.fV = work + L.fV;
+  fw.k = work + L.k;
+  fw.AuxG = work + L.AuxG;
+  fw.Temp1 = work + L.T1;
+  fw.JRhoW = work + L.Jb;
+  fw.JWRho = work + L.Jb + L.nj;
+  fw.JWRhoTh = work + L.Jb + 2 * L.nj;
+  fw.JRhoThW = work + L.Jb + 3 * L.nj;
+  fw.tri = work + L.tri;
+  const int nseg = (nsteps + every - 1) / every;
+  double *ck = work + L.ck, *seg = work + L.seg;
+  // forward: checkpoints at the segment starts (the last state goes to Vfinal)
+  HIPCHK(c, hipMemcpyAsync(ck, V0, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  for (int s = 1; s <= nseg; ++s) {
+    double *cur = seg;  // run the segment in seg[0]
+    HIPCHK(c, hipMemcpyAsync(cur, ck + (size_t)(s - 1) * n5, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    const int nst = std::min(every, nsteps - (s - 1) * every);
+    for (int q = 0; q < nst; ++q)
+      if ((r = rw_composed_step(c, cur, dt, &fw, st))) return r;
+    if (s < nseg) HIPCHK(c, hipMemcpyAsync(ck + (size_t)s * n5, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    else if (Vfinal) HIPCHK(c, hipMemcpyAsync(Vfinal, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
+  }
+  // reverse, segment by segment
+  for (int s = nseg; s >= 1; --s) {
+    const int nst = std::min(every, nsteps - (s - 1) * every);
+    HIPCHK(c, hipMemcpyAsync(seg, ck + (size_t)(s - 1) * n5, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    for (int q = 1; q < nst; ++q) {
+      HIPCHK(c, hipMemcpyAsync(seg + (size_t)q * n5, seg + (size_t)(q - 1) * n5, n5 * sizeof(double),
+                               hipMemcpyDeviceToDevice, st));
+      if ((r = rw_composed_step(c, seg + (size_t)q * n5, dt, &fw, st))) return r;
+    }
+    for (int q = nst - 1; q >= 0; --q)
+      if ((r = rw_step_adjoint(c, seg + (size_t)q * n5, dt, lamV, work, st))) return r;
+  }
+  return CGD_OK;
+}

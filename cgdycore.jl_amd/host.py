@@ -111,6 +111,12 @@ def native():
                                C.POINTER(C.c_double)]
     L.cgd_rw_graph_create.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), C.c_int, _vp]
     L.cgd_rw_graph_launch.argtypes = [_vp, _vp]
+    L.cgd_adjoint_work_size.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_size_t)]
+    L.cgd_fcn_adjoint.argtypes = [_vp] * 6
+    L.cgd_schur_adjoint.argtypes = [_vp] * 8 + [C.c_double] + [_vp] * 7
+    L.cgd_jac_adjoint.argtypes = [_vp] * 8
+    L.cgd_rw_step_adjoint.argtypes = [_vp, _vp, C.c_double, _vp, _vp, _vp]
+    L.cgd_rw_adjoint.argtypes = [_vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp, _vp, _vp]
     L.cgd_setup_error.restype = C.c_char_p
     L.cgd_jacobi_sphere3.argtypes = [C.c_int, C.c_int, C.c_int] + [_vp] * 5 + [C.c_double] * 3 + [_vp] * 4
     L.cgd_mass_cg.argtypes = [C.c_int] * 4 + [_vp] * 6
@@ -125,7 +131,8 @@ def exported_symbols():
             "cgd_ctx_info", "cgd_ctx_create_batch", "cgd_set_launch_shape", "cgd_get_launch_shape",
             "cgd_autotune", "cgd_rw_graph_create", "cgd_rw_graph_launch", "cgd_setup_error",
             "cgd_jacobi_sphere3", "cgd_mass_cg", "cgd_grid_size_sphere", "cgd_rw_run", "cgd_output_error",
-            "cgd_interpolate"]
+            "cgd_interpolate", "cgd_adjoint_work_size", "cgd_fcn_adjoint", "cgd_schur_adjoint", "cgd_jac_adjoint",
+            "cgd_rw_step_adjoint", "cgd_rw_adjoint"]
 
 
 def color_elements(glob: np.ndarray, num_g: int):
@@ -404,6 +411,51 @@ class NativeContext:
         """Replay the captured steps on torch's current stream."""
         self.check(self.L.cgd_rw_graph_launch(self.h, _stream()), "cgd_rw_graph_launch")
 
+    # ---- adjoint (FP64; dry, no tracers or hooks)
+    def adjoint_work(self, nsteps=0, every=1, device="cuda"):
+        """Work array of the adjoint calls (nsteps = 0: single evaluations)."""
+        n = C.c_size_t()
+        self.check(self.L.cgd_adjoint_work_size(self.h, int(nsteps), int(every), C.byref(n)), "cgd_adjoint_work_size")
+        return torch.empty(n.value, dtype=torch.float64, device=device)
+
+    def fcn_adjoint(self, U, lamF, work=None):
+        """(dF/dU)^T lamF for the tendency F = FcnGPU!(U)."""
+        work = self.adjoint_work(device=U.device) if work is None else work
+        lamU = torch.empty_like(U)
+        self.check(self.L.cgd_fcn_adjoint(self.h, _ptr(U), _ptr(lamF), _ptr(lamU), _ptr(work), _stream()),
+                   "cgd_fcn_adjoint")
+        return lamU
+
+    def schur_adjoint(self, lamk, k, v, JCache, fac, lJ, work=None):
+        """(dk/dv)^T lamk for k = SchurSolveGPU!(v); adds the Jacobian-block
+        cotangents into lJ = (lJRhoW, lJWRho, lJWRhoTh, lJRhoThW)."""
+        work = self.adjoint_work(device=k.device) if work is None else work
+        lamv = torch.empty_like(v)
+        self.check(self.L.cgd_schur_adjoint(self.h, _ptr(lamk), _ptr(k), _ptr(v), _ptr(JCache.JRhoW),
+                                            _ptr(JCache.JWRho), _ptr(JCache.JWRhoTh), _ptr(JCache.JRhoThW),
+                                            C.c_double(fac), _ptr(lamv), *[_ptr(t) for t in lJ], _ptr(work),
+                                            _stream()), "cgd_schur_adjoint")
+        return lamv
+
+    def jac_adjoint(self, U, lJ, lamU):
+        """Adds (dJ/dU)^T lJ into lamU for the blocks of JacSchurGPU!(U)."""
+        self.check(self.L.cgd_jac_adjoint(self.h, _ptr(U), *[_ptr(t) for t in lJ], _ptr(lamU), _stream()),
+                   "cgd_jac_adjoint")
+
+    def rw_step_adjoint(self, Vn, dt, lamV, work=None):
+        """In place: lamV (cotangent of the step's result) -> cotangent of Vn."""
+        work = self.adjoint_work(device=Vn.device) if work is None else work
+        self.check(self.L.cgd_rw_step_adjoint(self.h, _ptr(Vn), C.c_double(dt), _ptr(lamV), _ptr(work),
+                                              _stream()), "cgd_rw_step_adjoint")
+
+    def rw_adjoint(self, V0, dt, nsteps, every, lamV, Vfinal=None, work=None):
+        """In place: lamV (cotangent of the state after nsteps steps from V0)
+        -> cotangent of V0, with a checkpoint every ``every`` steps."""
+        work = self.adjoint_work(nsteps, every, device=V0.device) if work is None else work
+        self.check(self.L.cgd_rw_adjoint(self.h, _ptr(V0), C.c_double(dt), int(nsteps), int(every), _ptr(lamV),
+                                         _ptr(Vfinal) if Vfinal is not None else None, _ptr(work), _stream()),
+                   "cgd_rw_adjoint")
+
     def profile(self, mask=7):
         """Time the passes in ``mask`` (bit 0 pass 1, bit 1 pass 2, bit 2 column; 0 = off)."""
         self.check(self.L.cgd_profile_enable(self.h, int(mask)), "cgd_profile_enable")
@@ -529,6 +581,33 @@ def TimeStepper(U, dt, nIter, FE, Metric, Phys, Cache, JCache, Global, Exchange=
     w = _rw_work(Cache, JCache)
     ctx.check(ctx.L.cgd_rw_run(ctx.h, _ptr(U), C.c_double(dt), C.byref(w), int(nIter), _stream()), "TimeStepper")
     JCache.CompTri = False
+
+
+class RosenbrockSteps(torch.autograd.Function):
+    """``nsteps`` Rosenbrock-W steps of a state as a differentiable function
+    (the counterpart of a ChainRulesCore ``rrule`` around ``TimeStepper!``):
+    the forward is ``cgd_rw_run`` on a copy of ``V0``; the backward is
+    ``cgd_rw_adjoint`` with a checkpoint every ``every`` steps (it recomputes
+    the trajectory with the composed FP64 calls, which agree with the fused
+    step to rounding).  ``Cache``/``JCache`` are the forward's work arrays."""
+
+    @staticmethod
+    def forward(fctx, V0, dt, nsteps, every, nctx, Cache, JCache):
+        V = V0.detach().clone()
+        w = _rw_work(Cache, JCache)
+        nctx.check(nctx.L.cgd_rw_run(nctx.h, _ptr(V), C.c_double(dt), C.byref(w), int(nsteps), _stream()),
+                   "cgd_rw_run")
+        fctx.save_for_backward(V0.detach())
+        fctx.args = (dt, nsteps, every, nctx)
+        return V
+
+    @staticmethod
+    def backward(fctx, gV):
+        (V0,) = fctx.saved_tensors
+        dt, nsteps, every, nctx = fctx.args
+        lam = gV.detach().contiguous().clone()
+        nctx.rw_adjoint(V0, dt, nsteps, every, lam)
+        return lam, None, None, None, None, None, None
 
 
 # --------------------------------------------------------------------------

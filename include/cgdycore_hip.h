@@ -270,6 +270,54 @@ int cgd_color_elements(const int64_t *glob, int NN, int NF, int NumG, int *color
 int cgd_profile_enable(cgd_ctx *ctx, int on);
 int cgd_profile_read(cgd_ctx *ctx, int which, double *total_ms, long long *launches);
 
+/* ---- Adjoint (reverse mode) of the FP64 path.  The reference has no
+ * adjoint: these are the transposes of the derivatives of FcnGPU!
+ * (src/GPU/FcnGPU.jl:552-968), SchurSolveGPU!
+ * (src/Integration/SchurSolve.jl:149-177), JacSchurGPU!
+ * (src/DyCore/JacSchur.jl:3-75) and RosenbrockSchur!
+ * (src/Integration/RosenbrockSchur.jl:1-33), for the dry configuration
+ * without tracers or physics hooks (CGD_ERR_UNSUPPORTED otherwise), in FP64
+ * whatever the context's precision.  The RecU4 limiter and the |wCon|
+ * upwinding are differentiated on the branch the forward takes, except
+ * |wCon| where wCon is zero to rounding (w = 0 on a flat metric): there the
+ * mean of its one-sided derivatives (0) is taken.  State
+ * cotangents are summed with atomic adds, so their last bits vary from run
+ * to run.  work: device doubles, at least cgd_adjoint_work_size(ctx,
+ * nsteps, every) of them (nsteps = 0 for the single-evaluation calls). */
+int cgd_adjoint_work_size(cgd_ctx *ctx, int nsteps, int every, size_t *ndoubles);
+
+/* lamU (Nz,NumG,5) = (dF/dU)^T lamF for F = FcnGPU!(U) (with its
+ * FcnPrepareGPU! pressure); the forward tendency is recomputed in work. */
+int cgd_fcn_adjoint(cgd_ctx *ctx, const double *U, const double *lamF, double *lamU, double *work, void *stream);
+
+/* For k = SchurSolveGPU!(v; J blocks, fac): lamv (Nz,NumG,5) = (dk/dv)^T
+ * lamk, and the Jacobian-block cotangents (each (2,Nz-1,NumG)) are ADDED
+ * into lJRhoW .. lJRhoThW.  k: the forward result (its w component is the
+ * tridiagonal solution); v: the forward right-hand side as it was before
+ * the solve (the solve overwrites it). */
+int cgd_schur_adjoint(cgd_ctx *ctx, const double *lamk, const double *k, const double *v, const double *JRhoW,
+                      const double *JWRho, const double *JWRhoTh, const double *JRhoThW, double fac, double *lamv,
+                      double *lJRhoW, double *lJWRho, double *lJWRhoTh, double *lJRhoThW, double *work,
+                      void *stream);
+
+/* ADDS (dJ/dU)^T lJ into lamU (rho and rho*theta) for the Jacobian blocks
+ * of JacSchurGPU!(U). */
+int cgd_jac_adjoint(cgd_ctx *ctx, const double *U, const double *lJRhoW, const double *lJWRho,
+                    const double *lJWRhoTh, const double *lJRhoThW, double *lamU, void *stream);
+
+/* One RosenbrockSchur! step from Vn: lamV holds the cotangent of the step's
+ * result on entry and that of Vn on return.  The stages are recomputed from
+ * Vn with the composed FP64 calls. */
+int cgd_rw_step_adjoint(cgd_ctx *ctx, const double *Vn, double dt, double *lamV, double *work, void *stream);
+
+/* nsteps steps from V0 with a checkpoint every `every` steps: lamV holds the
+ * cotangent of the final state on entry and that of V0 on return; Vfinal
+ * (may be NULL) receives the final state.  The forward run stores
+ * ceil(nsteps/every) states, the reverse sweep recomputes each segment's
+ * step states from its checkpoint. */
+int cgd_rw_adjoint(cgd_ctx *ctx, const double *V0, double dt, int nsteps, int every, double *lamV, double *Vfinal,
+                   double *work, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

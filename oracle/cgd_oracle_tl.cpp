@@ -1,0 +1,194 @@
+/*
+ * Tangent-linear (forward-mode) build of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the adjoint of the HIP path.
+ * The product never links or calls it.
+ *
+ * cgd_oracle.c is compiled here as C++ with every `double` replaced by a
+ * dual number (value, derivative along one direction).  Arithmetic carries
+ * the derivative by the chain rule, so a run of the oracle on x + eps d
+ * yields f(x) and the exact directional derivative f'(x) d of the
+ * restatement, on the branch of every min / max / abs / comparison that
+ * the unperturbed values take (comparisons look at values only).  This is
+ * the reference derivative for <f'(x)^T lam, d> from the adjoint: unlike
+ * finite differences it has no step, so it neither crosses the kinks of
+ * the RecU4 limiter and of the upwind |wCon| nor loses digits to
+ * cancellation.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+struct cgd_dual {
+  double v, d;
+  cgd_dual() = default;
+  cgd_dual(double x) : v(x), d(0.0) {}
+  cgd_dual(double x, double dx) : v(x), d(dx) {}
+};
+static inline cgd_dual operator+(cgd_dual a, cgd_dual b) { return {a.v + b.v, a.d + b.d}; }
+static inline cgd_dual operator-(cgd_dual a, cgd_dual b) { return {a.v - b.v, a.d - b.d}; }
+static inline cgd_dual operator*(cgd_dual a, cgd_dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+static inline cgd_dual operator/(cgd_dual a, cgd_dual b) {
+  const double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+static inline cgd_dual operator-(cgd_dual a) { return {-a.v, -a.d}; }
+static inline cgd_dual &operator+=(cgd_dual &a, cgd_dual b) { return a = a + b; }
+static inline cgd_dual &operator-=(cgd_dual &a, cgd_dual b) { return a = a - b; }
+static inline cgd_dual &operator*=(cgd_dual &a, cgd_dual b) { return a = a * b; }
+static inline cgd_dual &operator/=(cgd_dual &a, cgd_dual b) { return a = a / b; }
+static inline bool operator<(cgd_dual a, cgd_dual b) { return a.v < b.v; }
+static inline bool operator>(cgd_dual a, cgd_dual b) { return a.v > b.v; }
+static inline bool operator<=(cgd_dual a, cgd_dual b) { return a.v <= b.v; }
+static inline bool operator>=(cgd_dual a, cgd_dual b) { return a.v >= b.v; }
+static inline bool operator==(cgd_dual a, cgd_dual b) { return a.v == b.v; }
+static inline bool operator!=(cgd_dual a, cgd_dual b) { return a.v != b.v; }
+static inline cgd_dual fabs(cgd_dual a) { return a.v < 0.0 ? -a : a; }
+static inline cgd_dual sqrt(cgd_dual a) {
+  const double s = ::sqrt(a.v);
+  return {s, a.d / (2.0 * s)};
+}
+static inline cgd_dual exp(cgd_dual a) {
+  const double e = ::exp(a.v);
+  return {e, a.d * e};
+}
+static inline cgd_dual log(cgd_dual a) { return {::log(a.v), a.d / a.v}; }
+static inline cgd_dual sin(cgd_dual a) { return {::sin(a.v), a.d * ::cos(a.v)}; }
+static inline cgd_dual cos(cgd_dual a) { return {::cos(a.v), -a.d * ::sin(a.v)}; }
+static inline cgd_dual pow(cgd_dual a, cgd_dual b) {
+  const double p = ::pow(a.v, b.v);
+  double dp = 0.0;
+  if (a.d != 0.0) dp += b.v * ::pow(a.v, b.v - 1.0) * a.d;
+  if (b.d != 0.0) dp += p * ::log(a.v) * b.d;
+  return {p, dp};
+}
+
+/* the oracle's structs under other names with FP64 fields (the layouts the
+ * Python wrapper passes), then the oracle itself on dual numbers */
+#define cgdo_grid cgdo_grid_real
+#define cgdo_work cgdo_work_real
+#include "cgd_oracle.h"
+#undef cgdo_grid
+#undef cgdo_work
+#undef CGD_ORACLE_H
+#define double cgd_dual
+#include "cgd_oracle.c"
+#undef double
+#undef NN
+
+namespace {
+
+struct Lifted {
+  std::vector<std::vector<cgd_dual>> keep;
+  const cgd_dual *lift(const double *x, size_t n) {
+    if (!x) return nullptr;
+    keep.emplace_back(n);
+    for (size_t i = 0; i < n; ++i) keep.back()[i] = cgd_dual(x[i]);
+    return keep.back().data();
+  }
+};
+
+// the dual-number grid: the static inputs with zero derivative
+void lift_grid(const cgdo_grid_real *r, cgdo_grid *g, Lifted &L) {
+  const size_t NN = (size_t)r->N * r->N, pts = NN * r->Nz * r->NF, ng = (size_t)r->Nz * r->NumG;
+  g->N = r->N;
+  g->Nz = r->Nz;
+  g->NF = r->NF;
+  g->NumG = r->NumG;
+  g->Glob = r->Glob;
+  g->DS = L.lift(r->DS, NN);
+  g->DW = L.lift(r->DW, NN);
+  g->dXdxI = L.lift(r->dXdxI, 18 * pts);
+  g->J = L.lift(r->J, 2 * pts);
+  g->X = L.lift(r->X, 6 * pts);
+  g->M = L.lift(r->M, 2 * ng);
+  g->dz = L.lift(r->dz, ng);
+  g->Grav = r->Grav;
+  g->Omega = r->Omega;
+  g->Rd = r->Rd;
+  g->Cpd = r->Cpd;
+  g->p0 = r->p0;
+  g->kappa = r->kappa;
+  g->KoeffCurl = r->KoeffCurl;
+  g->KoeffGrad = r->KoeffGrad;
+  g->KoeffDiv = r->KoeffDiv;
+  g->NumTr = r->NumTr;
+  g->Damping = r->Damping;
+  g->Forcing = r->Forcing;
+  g->DampH = r->DampH;
+  g->DampStride = r->DampStride;
+  g->DampRelax = r->DampRelax;
+  g->zP = r->Damping ? L.lift(r->zP, ng) : nullptr;
+  g->HS_k_a = r->HS_k_a;
+  g->HS_k_f = r->HS_k_f;
+  g->HS_k_s = r->HS_k_s;
+  g->HS_DeltaT_y = r->HS_DeltaT_y;
+  g->HS_DeltaTh_z = r->HS_DeltaTh_z;
+  g->HS_T_equator = r->HS_T_equator;
+  g->HS_T_min = r->HS_T_min;
+  g->HS_sigma_b = r->HS_sigma_b;
+  g->xS = r->Forcing ? L.lift(r->xS, 2 * (size_t)r->NumG) : nullptr;
+  g->State = r->State;
+  g->RhoVPos = r->RhoVPos;
+  g->RhoCPos = r->RhoCPos;
+  g->Cpv = r->Cpv;
+  g->Cpl = r->Cpl;
+  g->Rv = r->Rv;
+}
+
+std::vector<cgd_dual> seed(const double *x, const double *dx, size_t n) {
+  std::vector<cgd_dual> r(n);
+  for (size_t i = 0; i < n; ++i) r[i] = cgd_dual(x[i], dx[i]);
+  return r;
+}
+void split(const std::vector<cgd_dual> &x, double *v, double *d) {
+  for (size_t i = 0; i < x.size(); ++i) {
+    v[i] = x[i].v;
+    d[i] = x[i].d;
+  }
+}
+
+}  // namespace
+
+/* F = FcnGPU!(U) after FcnPrepareGPU!(U) and dF = F'(U) dU.  U, dU, F, dF:
+ * (Nz, NumG, 5+NumTr). */
+extern "C" void cgdot_fcn(const cgdo_grid_real *gr, const double *U, const double *dU, double *F, double *dF) {
+  Lifted L;
+  cgdo_grid g;
+  lift_grid(gr, &g, L);
+  const size_t ng = (size_t)gr->Nz * gr->NumG, nv = (size_t)(5 + gr->NumTr) * ng;
+  std::vector<cgd_dual> u = seed(U, dU, nv), f(nv), t1(nv), aux(4 * ng, cgd_dual(0.0));
+  cgdo_prepare(&g, u.data(), aux.data());
+  cgdo_fcn(&g, f.data(), u.data(), t1.data(), aux.data());
+  split(f, F, dF);
+}
+
+/* nsteps RosenbrockSchur! steps of V0 + eps dV0: V (the state) and dV (its
+ * derivative).  V0, dV0, V, dV: (Nz, NumG, 5+NumTr). */
+extern "C" void cgdot_rw_steps(const cgdo_grid_real *gr, const double *V0, const double *dV0, double dt, int nsteps,
+                               double *V, double *dV) {
+  Lifted L;
+  cgdo_grid g;
+  lift_grid(gr, &g, L);
+  const size_t ng = (size_t)gr->Nz * gr->NumG, nv = (size_t)(5 + gr->NumTr) * ng;
+  const size_t nj = 2 * (size_t)(gr->Nz - 1) * gr->NumG;
+  std::vector<cgd_dual> x = seed(V0, dV0, nv);
+  std::vector<cgd_dual> Vn(nv), fV(nv), k(3 * nv), aux(4 * ng, cgd_dual(0.0)), t1(nv);
+  std::vector<cgd_dual> jb(4 * nj), tri(3 * nj / 2);
+  cgdo_work w;
+  w.Vn = Vn.data();
+  w.fV = fV.data();
+  w.k = k.data();
+  w.AuxG = aux.data();
+  w.Temp1 = t1.data();
+  w.JRhoW = jb.data();
+  w.JWRho = jb.data() + nj;
+  w.JWRhoTh = jb.data() + 2 * nj;
+  w.JRhoThW = jb.data() + 3 * nj;
+  w.tri = tri.data();
+  for (int s = 0; s < nsteps; ++s) cgdo_rw_step(&g, x.data(), cgd_dual(dt), &w);
+  split(x, V, dV);
+}

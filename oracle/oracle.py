@@ -55,6 +55,23 @@ def lib():
     return _lib
 
 
+_TL_PATH = os.path.join(_HERE, "libcgd_oracle_tl.so")
+_tl = None
+
+
+def tl_lib():
+    """The tangent-linear build (cgd_oracle_tl.cpp): the restatement on dual
+    numbers, giving f(x) and the exact directional derivative f'(x) d."""
+    global _tl
+    if _tl is None:
+        if not os.path.exists(_TL_PATH):
+            build()
+        _tl = C.CDLL(_TL_PATH)
+        _tl.cgdot_fcn.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp]
+        _tl.cgdot_rw_steps.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_int, _dp, _dp]
+    return _tl
+
+
 def layer_center_height(disc) -> np.ndarray:
     """zP (NumG, Nz) of GridSizeSphereKernel! (DiscretizationCG.jl:130-134):
     max(|X1 + X2| / 2 - Rad, 0) from the last element copy in (IF, ID) order."""
@@ -178,6 +195,23 @@ class Oracle:
         for _ in range(nsteps):
             self.L.cgdo_rw_step(C.byref(self.g), _p(V), C.c_double(dt), C.byref(work))
         return w
+
+    # tangent-linear runs (forward-mode derivative along dU / dV0)
+    def fcn_tl(self, U, dU):
+        """(F, F'(U) dU) for F = FcnGPU!(U) after FcnPrepareGPU!(U)."""
+        F = self._state(self.nv)
+        dF = self._state(self.nv)
+        tl_lib().cgdot_fcn(C.addressof(self.g), _p(np.ascontiguousarray(U)), _p(np.ascontiguousarray(dU)),
+                           _p(F), _p(dF))
+        return F, dF
+
+    def rw_steps_tl(self, V0, dV0, dt, nsteps=1):
+        """(V, dV): nsteps Rosenbrock-W steps of V0 and their derivative along dV0."""
+        V = self._state(self.nv)
+        dV = self._state(self.nv)
+        tl_lib().cgdot_rw_steps(C.addressof(self.g), _p(np.ascontiguousarray(V0)), _p(np.ascontiguousarray(dV0)),
+                                C.c_double(dt), int(nsteps), _p(V), _p(dV))
+        return V, dV
 
     @staticmethod
     def ssp_knoth():

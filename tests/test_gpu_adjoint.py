@@ -1,0 +1,440 @@
+"""Adjoint of the FP64 path (through the C ABI), checked against derivatives
+fixed by the forward maps themselves.
+
+- The Schur solve is linear in its right-hand side, so its transpose must
+  satisfy <lam, S v> = <S^T lam, v> to rounding (1e-12).
+- Every other piece is nonlinear: the directional derivative of
+  <lam, f(x + eps d)> at eps = 0 along seeded Gaussian directions d, by
+  central differences, must match <f'(x)^T lam, d>.  The forward f is the
+  GPU path and, on small grids, the CPU oracle's restatement of the
+  reference, so a dropped term, a wrong sign or a transposed index in the
+  reverse shows up as an O(1) mismatch.  Tolerance 1e-6 relative (the
+  adjoint's bar).  For one tendency evaluation the differences are
+  extrapolated in eps (Richardson, h = 1e-5: the w tendency's
+  near-cancellation makes plain differences at small h noisy); through
+  steps, where the RecU4 limiter's kinks limit h, plain central differences
+  at h = 1e-6 (one step) or 1e-7 (several) are used.
+- Checkpointed runs of steps must give the step-by-step result.
+The direction leaves w at the model top fixed (w[Nz] = 0 is not a degree of
+freedom of the step).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FD_TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+def perturbed(U0, seed):
+    rng = np.random.default_rng(seed)
+    U = U0.copy()
+    U[1] += 5.0 * rng.standard_normal(U[1].shape)
+    U[2] += 5.0 * rng.standard_normal(U[2].shape)
+    U[3] += 0.5 * rng.standard_normal(U[3].shape)
+    U[3][:, -1] = 0.0
+    return U
+
+
+def direction(shape, seed):
+    d = np.random.default_rng(seed).standard_normal(shape)
+    d[3][:, -1] = 0.0
+    return d
+
+
+def _setup(pkg, oracle_mod, ne, nz, seed=0, noisy=True):
+    D = pkg.discretization
+    host = pkg.host
+    disc = D.build(ne, 4, nz, 30000.0)
+    phys = D.Phys()
+    U0 = pkg.state.jet_state(disc, seed=seed)
+    if noisy:
+        U0 = perturbed(U0, 100 + seed)
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda")
+    orc = oracle_mod.Oracle(disc, phys, (Global.Model.HyperDCurl, Global.Model.HyperDGrad, Global.Model.HyperDDiv))
+    ctx = host.init_context(FE, Metric, phys, Global)
+    return disc, phys, U0, (FE, Metric, Global, Cache, JCache), orc, ctx
+
+
+def _rel(a, b):
+    return abs(a - b) / max(abs(a), abs(b), 1e-300)
+
+
+def fd(phi, x, d, h):
+    """Directional derivative of phi at x along d by central differences."""
+    return (phi(x + h * d) - phi(x - h * d)) / (2 * h)
+
+
+def fd_rich(phi, x, d, h=1e-5):
+    """Central differences at h and h/2 combined by Richardson extrapolation."""
+    return (4.0 * fd(phi, x, d, 0.5 * h) - fd(phi, x, d, h)) / 3.0
+
+
+def _fcn_gpu(torch, host, dev, U):
+    FE, Metric, Global, Cache, JCache, phys = dev
+    Ut = torch.as_tensor(U, device="cuda")
+    F = torch.zeros_like(Ut)
+    eq = host.CompressibleShallow()
+    host.FcnPrepareGPU(Ut, FE, Metric, phys, Cache, None, Global, None, eq)
+    host.FcnGPU(F, Ut, FE, Metric, phys, Cache, None, Global, None, eq)
+    return F.cpu().numpy()
+
+
+@pytest.mark.parametrize("ne,nz,noisy", [(3, 8, True), (2, 13, True), (3, 32, False), (4, 10, False)])
+def test_fcn_adjoint_fd(pkg, oracle_mod, torch_cuda, ne, nz, noisy):
+    """Tendency reverse, variable by variable (each cotangent component
+    against each input component), against central differences of the GPU
+    tendency and of the oracle's; on the noisy state and on the jet itself
+    (w = 0: the upwind switch at its kink at every interface)."""
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, U0, (FE, Metric, Global, Cache, JCache), orc, ctx = _setup(pkg, oracle_mod, ne, nz, noisy=noisy)
+    print(f"fcn adjoint ne={ne} nz={nz} noisy={noisy}", flush=True)
+    dev = (FE, Metric, Global, Cache, JCache, phys)
+    rng = np.random.default_rng(7)
+    lamF = rng.standard_normal(U0.shape)
+    lamF[3][:, -1] = 0.0
+    lamU = ctx.fcn_adjoint(torch.as_tensor(U0, device="cuda"), torch.as_tensor(lamF, device="cuda")).cpu().numpy()
+    errs = {}
+    for vin in range(5):
+        d = np.zeros_like(U0)
+        d[vin] = direction(U0.shape, 11 + vin)[vin]
+        Fs = {}
+        for sgn, h in ((1, 1e-5), (-1, 1e-5), (1, 5e-6), (-1, 5e-6)):
+            Fs[(sgn, h)] = _fcn_gpu(torch, host, dev, U0 + sgn * h * d)
+        for vout in list(range(5)) + [None]:
+            lam = lamF if vout is None else np.where(np.arange(5)[:, None, None] == vout, lamF, 0.0)
+            D1 = (np.vdot(lam, Fs[(1, 1e-5)]) - np.vdot(lam, Fs[(-1, 1e-5)])) / 2e-5
+            D2 = (np.vdot(lam, Fs[(1, 5e-6)]) - np.vdot(lam, Fs[(-1, 5e-6)])) / 1e-5
+            fdv = (4.0 * D2 - D1) / 3.0
+            lu = ctx.fcn_adjoint(torch.as_tensor(U0, device="cuda"), torch.as_tensor(lam, device="cuda")).cpu().numpy()
+            ad = np.vdot(lu, d)
+            errs[(vin, vout)] = (_rel(fdv, ad), fdv, ad)
+    bad = {k: v for k, v in errs.items() if v[0] > FD_TOL}
+    for k, v in sorted(errs.items(), key=lambda kv: -kv[1][0])[:6]:
+        print(f"  in {k[0]} out {k[1]}: rel {v[0]:.2e} fd {v[1]:.6e} adj {v[2]:.6e}", flush=True)
+    assert not bad, f"adjoint/FD mismatches (input var, output var): {sorted(bad)}"
+    # the oracle's tendency (CPU restatement of FcnGPU!) in a joint direction
+    d = direction(U0.shape, 99)
+
+    def phi_orc(U):
+        A = orc.prepare(U.copy())
+        F, _ = orc.fcn(U.copy(), A)
+        return np.vdot(lamF, F)
+
+    f = fd_rich(phi_orc, U0, d)
+    ad = np.vdot(lamU, d)
+    print(f"  oracle FD {f:.9e} adjoint {ad:.9e} rel {_rel(f, ad):.2e}", flush=True)
+    assert _rel(f, ad) <= FD_TOL
+
+
+def test_schur_adjoint(pkg, oracle_mod, torch_cuda):
+    """Transpose of the column solve: exact dot-product identity for the
+    right-hand side; Jacobian-block cotangents against central differences
+    of the solve in the blocks, and through JacSchurGPU! in the state."""
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, U0, (FE, Metric, Global, Cache, JCache), orc, ctx = _setup(pkg, oracle_mod, 3, 10)
+    print("schur adjoint ne=3 nz=10", flush=True)
+    eq = host.CompressibleShallow()
+    U = torch.as_tensor(U0, device="cuda")
+    host.FcnPrepareGPU(U, FE, Metric, phys, Cache, None, Global, None, eq)
+    host.JacSchurGPU(JCache, U, FE, Metric, phys, Cache, Global, None, eq)
+    blocks = [JCache.JRhoW, JCache.JWRho, JCache.JWRhoTh, JCache.JRhoThW]
+    fac = 150.0
+    rng = np.random.default_rng(3)
+    v0 = rng.standard_normal(U0.shape) * np.array([1e-3, 1e-2, 1e-2, 1e-2, 1e-1])[:, None, None]
+    lamk = rng.standard_normal(U0.shape)
+
+    def solve(v, JC):
+        k = torch.zeros_like(U)
+        vt = torch.as_tensor(v, device="cuda").clone()
+        JC.CompTri = True
+        host.SchurSolveGPU(k, vt, JC, fac, Cache, Global)
+        return k.cpu().numpy()
+
+    k0 = torch.as_tensor(solve(v0, JCache), device="cuda")
+    lJ = [torch.zeros_like(b) for b in blocks]
+    lamv = ctx.schur_adjoint(torch.as_tensor(lamk, device="cuda"), k0, torch.as_tensor(v0, device="cuda"), JCache,
+                             fac, lJ).cpu().numpy()
+    # linear in v: <lamk, S v> = <S^T lamk, v> (top w of k is 0 by construction)
+    lhs = np.vdot(lamk, k0.cpu().numpy())
+    w = rng.standard_normal(U0.shape)
+    lhs2 = np.vdot(lamk, solve(w, JCache))
+    rhs2 = np.vdot(lamv, w)
+    print(f"  dot identity {lhs2:.12e} {rhs2:.12e} rel {_rel(lhs2, rhs2):.2e}", flush=True)
+    assert _rel(lhs2, rhs2) <= 1e-12
+    assert _rel(lhs, np.vdot(lamv, v0)) <= 1e-12
+    # blocks: central differences of <lamk, S_J(v0)>
+    eps = 1e-7
+    saved = [b.clone() for b in blocks]
+    for i, name in enumerate(["JRhoW", "JWRho", "JWRhoTh", "JRhoThW"]):
+        dJ = torch.as_tensor(np.random.default_rng(20 + i).standard_normal(tuple(blocks[i].shape)), device="cuda")
+        dJ = dJ * saved[i].abs().mean()
+        vals = []
+        for sgn in (1.0, -1.0):
+            for b, s in zip(blocks, saved):
+                b.copy_(s)
+            blocks[i].add_(dJ, alpha=sgn * eps)
+            vals.append(np.vdot(lamk, solve(v0, JCache)))
+        for b, s in zip(blocks, saved):
+            b.copy_(s)
+        fdJ = (vals[0] - vals[1]) / (2 * eps)
+        ad = float((lJ[i] * dJ).sum())
+        print(f"  {name}: fd {fdJ:.9e} adj {ad:.9e} rel {_rel(fdJ, ad):.2e}", flush=True)
+        assert _rel(fdJ, ad) <= FD_TOL, name
+    # JacSchurGPU! in the state: <lJ, J(U)>
+    lamU = torch.zeros_like(U)
+    ctx.jac_adjoint(U, lJ, lamU)
+    d = direction(U0.shape, 5)
+
+    def phi(Un):
+        Ut = torch.as_tensor(Un, device="cuda")
+        host.FcnPrepareGPU(Ut, FE, Metric, phys, Cache, None, Global, None, eq)
+        host.JacSchurGPU(JCache, Ut, FE, Metric, phys, Cache, Global, None, eq)
+        return sum(float((a * b).sum()) for a, b in zip(lJ, [JCache.JRhoW, JCache.JWRho, JCache.JWRhoTh,
+                                                                JCache.JRhoThW]))
+
+    f = fd_rich(phi, U0, d)
+    ad = np.vdot(lamU.cpu().numpy(), d)
+    print(f"  jac: fd {f:.9e} adj {ad:.9e} rel {_rel(f, ad):.2e}", flush=True)
+    assert _rel(f, ad) <= FD_TOL
+
+
+def _rw_gpu(torch, host, ctx, dev, V, dt, nsteps):
+    FE, Metric, Global, Cache, JCache, phys = dev
+    Vt = torch.as_tensor(V, device="cuda").clone()
+    host.TimeStepper(Vt, dt, nsteps, FE, Metric, phys, Cache, JCache, Global)
+    return Vt.cpu().numpy()
+
+
+@pytest.mark.parametrize("ne,nz", [(3, 8), (2, 12)])
+def test_rw_step_adjoint_fd(pkg, oracle_mod, torch_cuda, ne, nz):
+    """One Rosenbrock-W step: central differences of the GPU step and of the
+    oracle's RosenbrockSchur! restatement."""
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, U0, (FE, Metric, Global, Cache, JCache), orc, ctx = _setup(pkg, oracle_mod, ne, nz)
+    print(f"rw step adjoint ne={ne} nz={nz}", flush=True)
+    dev = (FE, Metric, Global, Cache, JCache, phys)
+    dt = 150.0
+    lamV = np.random.default_rng(8).standard_normal(U0.shape)
+    lam = torch.as_tensor(lamV, device="cuda").clone()
+    ctx.rw_step_adjoint(torch.as_tensor(U0, device="cuda"), dt, lam)
+    lam0 = lam.cpu().numpy()
+    for seed in (1, 2):
+        d = direction(U0.shape, seed)
+        f = fd(lambda V: np.vdot(lamV, _rw_gpu(torch, host, ctx, dev, V, dt, 1)), U0, d, 1e-6)
+        ad = np.vdot(lam0, d)
+        print(f"  gpu FD dir {seed}: fd {f:.9e} adj {ad:.9e} rel {_rel(f, ad):.2e}", flush=True)
+        assert _rel(f, ad) <= FD_TOL
+
+    def phi_orc(V):
+        Vc = V.copy()
+        orc.rw_step(Vc, dt)
+        return np.vdot(lamV, Vc)
+
+    d = direction(U0.shape, 3)
+    f = fd(phi_orc, U0, d, 1e-6)
+    ad = np.vdot(lam0, d)
+    print(f"  oracle FD: fd {f:.9e} adj {ad:.9e} rel {_rel(f, ad):.2e}", flush=True)
+    assert _rel(f, ad) <= FD_TOL
+
+
+def test_rw_adjoint_checkpoints(pkg, oracle_mod, torch_cuda):
+    """Runs with checkpoints every 1, 2 and 5 steps agree with each other and
+    with step-by-step adjoints over stored states, and with central
+    differences of cgd_rw_run; Vfinal is the forward's final state."""
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, U0, (FE, Metric, Global, Cache, JCache), orc, ctx = _setup(pkg, oracle_mod, 3, 8)
+    print("rw adjoint checkpoints ne=3 nz=8 nsteps=5", flush=True)
+    dev = (FE, Metric, Global, Cache, JCache, phys)
+    dt, nsteps = 150.0, 5
+    lamV = np.random.default_rng(9).standard_normal(U0.shape)
+    V0 = torch.as_tensor(U0, device="cuda")
+    res = {}
+    for every in (1, 2, 5):
+        lam = torch.as_tensor(lamV, device="cuda").clone()
+        Vf = torch.empty_like(V0)
+        ctx.rw_adjoint(V0, dt, nsteps, every, lam, Vfinal=Vf)
+        res[every] = lam.cpu().numpy()
+    Vf_fwd = _rw_gpu(torch, host, ctx, dev, U0, dt, nsteps)
+    err_vf = np.linalg.norm(Vf.cpu().numpy() - Vf_fwd) / np.linalg.norm(Vf_fwd)
+    print(f"  Vfinal vs cgd_rw_run rel {err_vf:.2e}", flush=True)
+    assert err_vf <= 1e-12
+    for every in (2, 5):
+        e = np.linalg.norm(res[every] - res[1]) / np.linalg.norm(res[1])
+        print(f"  every={every} vs every=1: rel {e:.2e}", flush=True)
+        assert e <= 1e-11
+    # step by step over stored states
+    states = [U0.copy()]
+    for _ in range(nsteps - 1):
+        states.append(_rw_gpu(torch, host, ctx, dev, states[-1], dt, 1))
+    lam = torch.as_tensor(lamV, device="cuda").clone()
+    for s in reversed(states):
+        ctx.rw_step_adjoint(torch.as_tensor(s, device="cuda"), dt, lam)
+    e = np.linalg.norm(lam.cpu().numpy() - res[1]) / np.linalg.norm(res[1])
+    print(f"  step by step vs run: rel {e:.2e}", flush=True)
+    assert e <= 1e-9
+    d = direction(U0.shape, 4)
+    f = fd(lambda V: np.vdot(lamV, _rw_gpu(torch, host, ctx, dev, V, dt, nsteps)), U0, d, 1e-7)
+    ad = np.vdot(res[2], d)
+    print(f"  FD over {nsteps} steps: fd {f:.9e} adj {ad:.9e} rel {_rel(f, ad):.2e}", flush=True)
+    assert _rel(f, ad) <= FD_TOL
+
+
+def test_autograd_function(pkg, oracle_mod, torch_cuda):
+    """RosenbrockSteps as a torch.autograd.Function: the gradient of the sum
+    of rho*theta^2 = (rho theta)^2 / rho over the lowest level against
+    central differences."""
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, U0, (FE, Metric, Global, Cache, JCache), orc, ctx = _setup(pkg, oracle_mod, 3, 8)
+    print("autograd RosenbrockSteps ne=3 nz=8 nsteps=3", flush=True)
+    dt, nsteps, every = 200.0, 3, 2
+
+    def functional(V):
+        return (V[4, :, 0] ** 2 / V[0, :, 0]).sum()
+
+    V0 = torch.as_tensor(U0, device="cuda").clone().requires_grad_(True)
+    V = host.RosenbrockSteps.apply(V0, dt, nsteps, every, ctx, Cache, JCache)
+    functional(V).backward()
+    g = V0.grad.detach().cpu().numpy()
+    d = direction(U0.shape, 6)
+    dev = (FE, Metric, Global, Cache, JCache, phys)
+
+    def Jf(Vn):
+        Vt = torch.as_tensor(_rw_gpu(torch, host, ctx, dev, Vn, dt, nsteps), device="cuda")
+        return float(functional(Vt))
+
+    f = fd(Jf, U0, d, 1e-7)
+    ad = np.vdot(g, d)
+    print(f"  fd {f:.9e} adj {ad:.9e} rel {_rel(f, ad):.2e}", flush=True)
+    assert _rel(f, ad) <= FD_TOL
+
+
+def test_adjoint_rejects_unsupported(pkg, oracle_mod, torch_cuda):
+    torch = torch_cuda
+    host = pkg.host
+    D = pkg.discretization
+    disc = D.build(2, 4, 6, 30000.0)
+    phys = D.Phys()
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda", NumTr=1)
+    ctx = host.init_context(FE, Metric, phys, Global)
+    U = torch.zeros((6, Global.NumG, Global.Nz), dtype=torch.float64, device="cuda")
+    with pytest.raises(host.CGDError, match="dry configuration without tracers"):
+        ctx.fcn_adjoint(U, U)
+
+
+def _tl_check(ctx, orc, torch, U0, d, lamV, dt, nsteps, every):
+    """<adjoint(lamV), d> against <lamV, dV> of the oracle's tangent-linear run."""
+    lam = torch.as_tensor(lamV, device="cuda").clone()
+    ctx.rw_adjoint(torch.as_tensor(U0, device="cuda"), dt, nsteps, every, lam)
+    ad = float(np.vdot(lam.cpu().numpy(), d))
+    _, dV = orc.rw_steps_tl(U0, d, dt, nsteps)
+    return ad, float(np.vdot(lamV, dV))
+
+
+def test_adjoint_vs_tangent_linear_oracle(pkg, oracle_mod, torch_cuda):
+    """Exact check against the oracle's forward-mode derivative (no step, so
+    no kink crossing and no cancellation): the tendency and 2 steps on the
+    noisy state, and 12 steps of the forecast functional at ne = 8, nz = 16,
+    dt = 300 s on the jet (a direction without w: at w = 0 the upwind
+    switch sits on its kink, where the adjoint takes the mean of the
+    one-sided derivatives and the tangent-linear run one side) and on the
+    jet with w noise (full direction).  Bar: 1e-9 relative."""
+    torch = torch_cuda
+    disc, phys, U0, (FE, Metric, Global, Cache, JCache), orc, ctx = _setup(pkg, oracle_mod, 3, 8)
+    print("adjoint vs tangent-linear oracle ne=3 nz=8", flush=True)
+    rng = np.random.default_rng(21)
+    lamF = rng.standard_normal(U0.shape)
+    d = direction(U0.shape, 22)
+    lamU = ctx.fcn_adjoint(torch.as_tensor(U0, device="cuda"), torch.as_tensor(lamF, device="cuda")).cpu().numpy()
+    _, dF = orc.fcn_tl(U0, d)
+    a, t = float(np.vdot(lamU, d)), float(np.vdot(lamF, dF))
+    print(f"  tendency: adj {a:.12e} tl {t:.12e} rel {_rel(a, t):.2e}", flush=True)
+    assert _rel(a, t) <= 1e-9
+    a, t = _tl_check(ctx, orc, torch, U0, d, lamF, 150.0, 2, 1)
+    print(f"  2 steps: adj {a:.12e} tl {t:.12e} rel {_rel(a, t):.2e}", flush=True)
+    assert _rel(a, t) <= 1e-9
+    Global._ctx.close()
+    D = pkg.discretization
+    host = pkg.host
+    disc = D.build(8, 4, 16, 30000.0)
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda")
+    ctx = host.init_context(FE, Metric, phys, Global)
+    orc = oracle_mod.Oracle(disc, phys, (Global.Model.HyperDCurl, Global.Model.HyperDGrad, Global.Model.HyperDDiv))
+    jet = pkg.state.jet_state(disc, seed=0)
+    wn = jet.copy()
+    wn[3] += 0.5 * np.random.default_rng(23).standard_normal(wn[3].shape)
+    wn[3][:, -1] = 0.0
+    for name, S, dd in (("jet, no-w direction", jet, direction(jet.shape, 1000) * (np.arange(5) != 3)[:, None, None]),
+                        ("jet + w noise", wn, direction(jet.shape, 1001))):
+        Vf, _ = orc.rw_steps_tl(S, np.zeros_like(S), 300.0, 12)
+        lamV = np.zeros_like(S)
+        lamV[4, :, 0] = 2.0 * Vf[4, :, 0] / Vf[0, :, 0]
+        lamV[0, :, 0] = -(Vf[4, :, 0] / Vf[0, :, 0]) ** 2
+        a, t = _tl_check(ctx, orc, torch, S, dd, lamV, 300.0, 12, 4)
+        print(f"  forecast 12 steps ne=8 ({name}): adj {a:.12e} tl {t:.12e} rel {_rel(a, t):.2e}", flush=True)
+        assert _rel(a, t) <= 1e-9
+
+
+def test_adjoint_forecast_functional(pkg, oracle_mod, torch_cuda):
+    """The adjoint workload: ne = 16, nz = 32, 12 steps at dt = 200 s (the
+    largest of 150/200/250/300 s at which the step stays finite for 12
+    steps at this resolution) with a checkpoint every 4 steps, J = sum of
+    rho*theta^2 over the lowest level of the seeded jet; the gradient
+    against central differences (eps = 1e-6) of the fused forward along 4
+    of the Gaussian directions.  On this state the differences cross kinks
+    of the upwind switch (w starts at 0 and stays near 0 at many points,
+    and a white-noise rho*theta perturbation moves the stage w by ~1e-5 m/s),
+    which limits their agreement with any derivative to ~1e-5; the bar here
+    is 1e-4 (the exact check is against the tangent-linear oracle above)."""
+    import math
+
+    torch = torch_cuda
+    host = pkg.host
+    D = pkg.discretization
+    disc = D.build(16, 4, 32, 30000.0)
+    phys = D.Phys()
+    FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, device="cuda")
+    ctx = host.init_context(FE, Metric, phys, Global)
+    U0 = pkg.state.jet_state(disc, seed=0)
+    V0 = torch.as_tensor(U0, device="cuda")
+    dt, nsteps, every, eps = 200.0, 12, 4, 1e-6
+    print("adjoint forecast functional ne=16 nz=32 12 steps", flush=True)
+
+    def run(V):
+        Vt = V.clone()
+        host.TimeStepper(Vt, dt, nsteps, FE, Metric, phys, Cache, JCache, Global)
+        return Vt
+
+    def J_of(Vf):
+        lev = Vf[:, :, 0].cpu().numpy()
+        return math.fsum((lev[4] ** 2 / lev[0]).tolist())
+
+    Vf = run(V0)
+    lam = torch.zeros_like(V0)
+    lam[4, :, 0] = 2.0 * Vf[4, :, 0] / Vf[0, :, 0]
+    lam[0, :, 0] = -(Vf[4, :, 0] / Vf[0, :, 0]) ** 2
+    ctx.rw_adjoint(V0, dt, nsteps, every, lam)
+    g = lam.cpu().numpy()
+    assert np.isfinite(g).all()
+    for seed in range(1000, 1004):
+        d = np.random.default_rng(seed).standard_normal(U0.shape)
+        d[3][:, -1] = 0.0
+        dd = torch.as_tensor(d, device="cuda")
+        f = (J_of(run(V0 + eps * dd)) - J_of(run(V0 - eps * dd))) / (2 * eps)
+        ad = float(np.vdot(g, d))
+        print(f"  dir {seed}: fd {f:.9e} adj {ad:.9e} rel {_rel(f, ad):.2e}", flush=True)
+        assert _rel(f, ad) <= 1e-4

@@ -1,0 +1,84 @@
+"""The tangent-linear build of the oracle (oracle/cgd_oracle_tl.cpp), the
+checker of the adjoint, pinned against the plain oracle:
+- its values are the plain oracle's, bit for bit (the same operations);
+- its derivative is linear in the direction and matches central
+  differences of the plain oracle (Richardson-extrapolated, on a state
+  with winds and w away from the kinks of the upwind switch).
+"""
+import numpy as np
+import pytest
+
+
+def _noisy(pkg, ne, nz, seed=0):
+    D = pkg.discretization
+    disc = D.build(ne, 4, nz, 30000.0)
+    phys = D.Phys()
+    U = pkg.state.jet_state(disc, seed=seed)
+    rng = np.random.default_rng(100 + seed)
+    U[1] += 5.0 * rng.standard_normal(U[1].shape)
+    U[2] += 5.0 * rng.standard_normal(U[2].shape)
+    U[3] += 0.5 * rng.standard_normal(U[3].shape)
+    U[3][:, -1] = 0.0
+    return disc, phys, U
+
+
+def _dir(shape, seed):
+    d = np.random.default_rng(seed).standard_normal(shape)
+    d[3][:, -1] = 0.0
+    return d
+
+
+def _fd_rich(phi, x, d, h):
+    D1 = (phi(x + h * d) - phi(x - h * d)) / (2 * h)
+    D2 = (phi(x + 0.5 * h * d) - phi(x - 0.5 * h * d)) / h
+    return (4.0 * D2 - D1) / 3.0
+
+
+@pytest.fixture(scope="module")
+def orc_small(pkg, oracle_mod):
+    disc, phys, U = _noisy(pkg, 2, 6)
+    K = pkg.host.hyperdiffusion_for(2)
+    return oracle_mod.Oracle(disc, phys, (K, K, K)), U
+
+
+def test_tl_values_are_the_oracle(orc_small):
+    orc, U = orc_small
+    d = _dir(U.shape, 1)
+    F, _ = orc.fcn(U.copy(), orc.prepare(U.copy()))
+    Ft, _ = orc.fcn_tl(U, d)
+    assert np.array_equal(F, Ft)
+    V = U.copy()
+    orc.rw_step(V, 150.0, 2)
+    Vt, _ = orc.rw_steps_tl(U, d, 150.0, 2)
+    assert np.array_equal(V, Vt)
+
+
+def test_tl_linear_in_direction(orc_small):
+    orc, U = orc_small
+    d1, d2 = _dir(U.shape, 2), _dir(U.shape, 3)
+    _, a = orc.rw_steps_tl(U, d1, 150.0, 2)
+    _, b = orc.rw_steps_tl(U, d2, 150.0, 2)
+    _, c = orc.rw_steps_tl(U, d1 + 2.0 * d2, 150.0, 2)
+    assert np.linalg.norm(c - (a + 2.0 * b)) <= 1e-13 * np.linalg.norm(c)
+
+
+def test_tl_matches_differences(orc_small):
+    orc, U = orc_small
+    rng = np.random.default_rng(4)
+    lam = rng.standard_normal(U.shape)
+    for seed in (5, 6):
+        d = _dir(U.shape, seed)
+        _, dF = orc.fcn_tl(U, d)
+        f = _fd_rich(lambda x: np.vdot(lam, orc.fcn(x.copy(), orc.prepare(x.copy()))[0]), U, d, 1e-5)
+        t = np.vdot(lam, dF)
+        assert abs(f - t) <= 1e-7 * abs(t), (f, t)
+
+        def step(x):
+            V = x.copy()
+            orc.rw_step(V, 150.0, 1)
+            return np.vdot(lam, V)
+
+        _, dV = orc.rw_steps_tl(U, d, 150.0, 1)
+        f = (step(U + 1e-6 * d) - step(U - 1e-6 * d)) / 2e-6
+        t = np.vdot(lam, dV)
+        assert abs(f - t) <= 1e-7 * abs(t), (f, t)
