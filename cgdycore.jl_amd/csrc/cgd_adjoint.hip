@@ -778,6 +778,130 @@ __global__ void k_schur_adj(KArgs a, const double *__restrict__ lk, const double
 #undef JB
 }
 
+// The same transpose for Nz <= 64: one wavefront per column, lane = level
+// (coalesced loads, neighbour rows by shuffles), the transposed tridiagonal
+// system solved by the forward's parallel cyclic reduction: T^T has
+// sub-diagonal t0_i and super-diagonal t2_i in row i, which is what
+// col_solve reads from the column-storage arguments (t2_{i-1}, t1_i, t0_{i+1}).
+__global__ void __launch_bounds__(256)
+    k_schur_adj_col(KArgs a, const double *__restrict__ lk, const double *__restrict__ kx,
+                    const double *__restrict__ v, const double *__restrict__ JRhoW, const double *__restrict__ JWRho,
+                    const double *__restrict__ JWRhoTh, const double *__restrict__ JRhoThW, double fac, double invfac2,
+                    double *__restrict__ lv, double *__restrict__ lJRhoW, double *__restrict__ lJWRho,
+                    double *__restrict__ lJWRhoTh, double *__restrict__ lJRhoThW) {
+  const int Nz = a.Nz, n = Nz - 1;
+  const int i = threadIdx.x & 63;
+  const size_t ig = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ig >= (size_t)a.NumG) return;  // uniform per wavefront
+  const size_t vs = (size_t)a.NumG * Nz, c0 = ig * Nz;
+  const bool row = i < n, lev = i < Nz;
+  const size_t jb = (ig * (size_t)n + i) * 2;
+  double rw0 = 0, rw1 = 0, wr0 = 0, wr1 = 0, wt0 = 0, wt1 = 0, tw0 = 0, tw1 = 0;
+  if (row) {
+    rw0 = JRhoW[jb];
+    rw1 = JRhoW[jb + 1];
+    wr0 = JWRho[jb];
+    wr1 = JWRho[jb + 1];
+    wt0 = JWRhoTh[jb];
+    wt1 = JWRhoTh[jb + 1];
+    tw0 = JRhoThW[jb];
+    tw1 = JRhoThW[jb + 1];
+  }
+  double l1 = 0, l2 = 0, l3 = 0, l4 = 0, l5 = 0, x = 0, v1 = 0, v5 = 0;
+  if (lev) {
+    l1 = fac * lk[c0 + i];
+    l2 = fac * lk[vs + c0 + i];
+    l3 = fac * lk[2 * vs + c0 + i];
+    l4 = lk[3 * vs + c0 + i];
+    l5 = fac * lk[4 * vs + c0 + i];
+    x = kx[3 * vs + c0 + i];
+    v1 = v[c0 + i];
+    v5 = v[4 * vs + c0 + i];
+  }
+  if (!row) x = 0.0;
+  const double l1p = __shfl_down(l1, 1), l5p = __shfl_down(l5, 1);
+  // k1 = fac (v1 + JRhoW x), k5 = fac (v5 + JRhoThW x), k4 = x
+  double g_rw0 = 0, g_rw1 = 0, g_wr0 = 0, g_wr1 = 0, g_wt0 = 0, g_wt1 = 0, g_tw0 = 0, g_tw1 = 0;
+  double lx = 0;
+  if (row) {
+    lx = l4 + rw0 * l1 + rw1 * l1p + tw0 * l5 + tw1 * l5p;
+    g_rw0 = l1 * x;
+    g_rw1 = l1p * x;
+    g_tw0 = l5 * x;
+    g_tw1 = l5p * x;
+  }
+  double t0, t1, t2;
+  col_tri<64>(rw0, rw1, wr0, wr1, wt0, wt1, tw0, tw1, invfac2, i, n, t0, t1, t2);
+  const double y = col_solve<64>(__shfl_up(t2, 1), t1, __shfl_down(t0, 1), lx, i, n, 0);
+  const double ym = __shfl_up(y, 1), yp = __shfl_down(y, 1);
+  const double wr0m = __shfl_up(wr0, 1), wt0m = __shfl_up(wt0, 1);
+  const double wr1p = __shfl_down(wr1, 1), wt1p = __shfl_down(wt1, 1);
+  const double v1p = __shfl_down(v1, 1), v5p = __shfl_down(v5, 1);
+  // T x = rhs: dT contributes -y_r x_c at entry (r, c)
+  double sendm_wr0 = 0, sendm_wt0 = 0, sendp_wr1 = 0, sendp_wt1 = 0;
+  if (row) {
+    const double lt1 = -y * x;
+    g_wr1 += -lt1 * rw0;
+    g_rw0 += -lt1 * wr1;
+    g_wr0 += -lt1 * rw1;
+    g_rw1 += -lt1 * wr0;
+    g_wt1 += -lt1 * tw0;
+    g_tw0 += -lt1 * wt1;
+    g_wt0 += -lt1 * tw1;
+    g_tw1 += -lt1 * wt0;
+    if (i >= 1) {  // t0_i = -JWRho0_{i-1} JRhoW0_i - JWRhoTh0_{i-1} JRhoThW0_i
+      const double lt0 = -ym * x;
+      sendm_wr0 = -lt0 * rw0;
+      sendm_wt0 = -lt0 * tw0;
+      g_rw0 += -lt0 * wr0m;
+      g_tw0 += -lt0 * wt0m;
+    }
+    if (i <= n - 2) {  // t2_i = -JWRho1_{i+1} JRhoW1_i - JWRhoTh1_{i+1} JRhoThW1_i
+      const double lt2 = -yp * x;
+      sendp_wr1 = -lt2 * rw1;
+      sendp_wt1 = -lt2 * tw1;
+      g_rw1 += -lt2 * wr1p;
+      g_tw1 += -lt2 * wt1p;
+    }
+    // rhs_i = v4_i / fac + JWRho (v1_i, v1_i+1) + JWRhoTh (v5_i, v5_i+1)
+    g_wr1 += y * v1;
+    g_wr0 += y * v1p;
+    g_wt1 += y * v5;
+    g_wt0 += y * v5p;
+  }
+  // the t0 / t2 terms of the neighbouring rows (a shuffle from outside the
+  // wavefront returns the lane's own value: lane 0 has no row below, and
+  // lane 63 is never a row)
+  g_wr0 += __shfl_down(sendm_wr0, 1);
+  g_wt0 += __shfl_down(sendm_wt0, 1);
+  const double rp_wr1 = __shfl_up(sendp_wr1, 1), rp_wt1 = __shfl_up(sendp_wt1, 1);
+  if (i >= 1) {
+    g_wr1 += rp_wr1;
+    g_wt1 += rp_wt1;
+  }
+  // lv1_i += JWRho1_i y_i + JWRho0_{i-1} y_{i-1}, likewise lv5 with JWRhoTh
+  const double yrow = row ? y : 0.0, ymr = (i >= 1) ? ym : 0.0;
+  l1 += wr1 * yrow + wr0m * ymr;
+  l5 += wt1 * yrow + wt0m * ymr;
+  if (lev) {
+    lv[c0 + i] = l1;
+    lv[vs + c0 + i] = l2;
+    lv[2 * vs + c0 + i] = l3;
+    lv[3 * vs + c0 + i] = row ? y / fac : 0.0;
+    lv[4 * vs + c0 + i] = l5;
+  }
+  if (row) {
+    lJRhoW[jb] += g_rw0;
+    lJRhoW[jb + 1] += g_rw1;
+    lJWRho[jb] += g_wr0;
+    lJWRho[jb + 1] += g_wr1;
+    lJWRhoTh[jb] += g_wt0;
+    lJWRhoTh[jb + 1] += g_wt1;
+    lJRhoThW[jb] += g_tw0;
+    lJRhoThW[jb + 1] += g_tw1;
+  }
+}
+
 // Reverse of the Jacobian blocks (JacSchur.jl:29-75) of the state U, one
 // thread per column: adds the rho and rho*theta cotangents into lamU.
 __global__ void k_jac_adj(KArgs a, const double *__restrict__ U, const double *__restrict__ dz,
@@ -806,6 +930,49 @@ __global__ void k_jac_adj(KArgs a, const double *__restrict__ U, const double *_
     const double lRhoThF = -lJRhoThW[o] / dzL + lJRhoThW[o + 1] / dzR;
     lamU[4 * vs + q] += ldPL * dPL * a.dpd_y / RhoThL + lRhoThF * dzL / sdz;
     lamU[4 * vs + q + 1] += ldPR * dPR * a.dpd_y / RhoThR + lRhoThF * dzR / sdz;
+  }
+}
+
+// The same for Nz <= 64: one wavefront per column, lane i = interface i;
+// the contributions to level i + 1 reach its lane by a shuffle.
+__global__ void __launch_bounds__(256)
+    k_jac_adj_col(KArgs a, const double *__restrict__ U, const double *__restrict__ dz,
+                  const double *__restrict__ lJRhoW, const double *__restrict__ lJWRho,
+                  const double *__restrict__ lJWRhoTh, const double *__restrict__ lJRhoThW,
+                  double *__restrict__ lamU) {
+  const int Nz = a.Nz, n = Nz - 1;
+  const int i = threadIdx.x & 63;
+  const size_t ig = (size_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (ig >= (size_t)a.NumG) return;  // uniform per wavefront
+  const size_t vs = (size_t)a.NumG * Nz, q = ig * Nz + i, o = (ig * (size_t)n + i) * 2;
+  const bool row = i < n, lev = i < Nz;
+  double rho = 1, rth = 1, dzi = 1;
+  if (lev) {
+    rho = U[q];
+    rth = U[4 * vs + q];
+    dzi = dz[q];
+  }
+  const double RhoR = __shfl_down(rho, 1), RhoThR = __shfl_down(rth, 1), dzR = __shfl_down(dzi, 1);
+  double gL_r = 0, gR_r = 0, gL_t = 0, gR_t = 0;
+  if (row) {
+    const double RhoL = rho, RhoThL = rth, dzL = dzi, sdz = dzL + dzR, hdz = 0.5 * sdz;
+    const double RhoF = (RhoL * dzL + RhoR * dzR) / sdz;
+    const double dPL = eos_dpdrhoth(a, RhoThL), dPR = eos_dpdrhoth(a, RhoThR);
+    const double wt0 = -dPR / RhoF / hdz, wt1 = dPL / RhoF / hdz;
+    const double wr0 = -a.Grav * dzR / RhoF / sdz, wr1 = -a.Grav * dzL / RhoF / sdz;
+    const double lRhoF = -lJRhoW[o] / dzL + lJRhoW[o + 1] / dzR -
+                         (lJWRhoTh[o] * wt0 + lJWRhoTh[o + 1] * wt1 + lJWRho[o] * wr0 + lJWRho[o + 1] * wr1) / RhoF;
+    gL_r = lRhoF * dzL / sdz;
+    gR_r = lRhoF * dzR / sdz;
+    const double ldPR = -lJWRhoTh[o] / (RhoF * hdz), ldPL = lJWRhoTh[o + 1] / (RhoF * hdz);
+    const double lRhoThF = -lJRhoThW[o] / dzL + lJRhoThW[o + 1] / dzR;
+    gL_t = ldPL * dPL * a.dpd_y / RhoThL + lRhoThF * dzL / sdz;
+    gR_t = ldPR * dPR * a.dpd_y / RhoThR + lRhoThF * dzR / sdz;
+  }
+  const double fr = __shfl_up(gR_r, 1), ft = __shfl_up(gR_t, 1);
+  if (lev) {
+    lamU[q] += gL_r + (i >= 1 ? fr : 0.0);
+    lamU[4 * vs + q] += gL_t + (i >= 1 ? ft : 0.0);
   }
 }
 
@@ -850,6 +1017,7 @@ struct AdjLayout {
   size_t ng, n5, nj;
   size_t F, T1, AuxG, Lam, lT1;                        // tendency adjoint
   size_t bw, Vi, fV, k, lk, lVi, lfV, Jb, tri, lJ;     // step adjoint
+  size_t Fs, T1s;                                      // the three stages' tendencies and caches
   size_t ck, seg;                                      // runs: checkpoints, segment states
   size_t total;
 };
@@ -874,6 +1042,8 @@ static AdjLayout adj_layout(const cgd_ctx *c, int nsteps, int every) {
   L.Jb = o; o += 4 * L.nj;
   L.tri = o; o += 3 * (L.nj / 2);
   L.lJ = o; o += 4 * L.nj;
+  L.Fs = o; o += 3 * L.n5;
+  L.T1s = o; o += 3 * L.n5;
   L.ck = o;
   if (nsteps > 0) {
     const int nck = (nsteps + every - 1) / every;
@@ -939,6 +1109,25 @@ extern "C" int cgd_fcn_adjoint(cgd_ctx *c, const double *U, const double *lamF, 
   return fcn_adjoint(c, U, lamF, lamU, work, st);
 }
 
+static void launch_schur_adj(cgd_ctx *c, hipStream_t st, const double *lamk, const double *k, const double *v,
+                             const double *const Jb[4], double fac, double *lamv, double *const lJ[4],
+                             double *bw) {
+  if (c->Nz <= 64)
+    hipLaunchKernelGGL(k_schur_adj_col, dim3((unsigned)((c->NumG + 3) / 4)), dim3(256), 0, st, c->ka, lamk, k, v,
+                       Jb[0], Jb[1], Jb[2], Jb[3], fac, (1.0 / fac) / fac, lamv, lJ[0], lJ[1], lJ[2], lJ[3]);
+  else
+    hipLaunchKernelGGL(k_schur_adj, dim3((unsigned)((c->NumG + 63) / 64)), dim3(64), 0, st, c->ka, lamk, k, v,
+                       Jb[0], Jb[1], Jb[2], Jb[3], fac, lamv, lJ[0], lJ[1], lJ[2], lJ[3], bw);
+}
+static void launch_jac_adj(cgd_ctx *c, hipStream_t st, const double *U, const double *const lJ[4], double *lamU) {
+  if (c->Nz <= 64)
+    hipLaunchKernelGGL(k_jac_adj_col, dim3((unsigned)((c->NumG + 3) / 4)), dim3(256), 0, st, c->ka, U, c->dz, lJ[0],
+                       lJ[1], lJ[2], lJ[3], lamU);
+  else
+    hipLaunchKernelGGL(k_jac_adj, dim3((unsigned)((c->NumG + 63) / 64)), dim3(64), 0, st, c->ka, U, c->dz, lJ[0],
+                       lJ[1], lJ[2], lJ[3], lamU);
+}
+
 extern "C" int cgd_schur_adjoint(cgd_ctx *c, const double *lamk, const double *k, const double *v,
                                  const double *JRhoW, const double *JWRho, const double *JWRhoTh,
                                  const double *JRhoThW, double fac, double *lamv, double *lJRhoW, double *lJWRho,
@@ -949,8 +1138,9 @@ extern "C" int cgd_schur_adjoint(cgd_ctx *c, const double *lamk, const double *k
       !lJRhoThW || !work)
     return fail(c, CGD_ERR_ARG, "cgd_schur_adjoint: null array");
   if (!(fac > 0.0)) return fail(c, CGD_ERR_ARG, "cgd_schur_adjoint: fac must be positive");
-  hipLaunchKernelGGL(k_schur_adj, dim3((unsigned)((c->NumG + 63) / 64)), dim3(64), 0, S(stream), c->ka, lamk, k, v, JRhoW,
-                     JWRho, JWRhoTh, JRhoThW, fac, lamv, lJRhoW, lJWRho, lJWRhoTh, lJRhoThW, work);
+  const double *Jb[4] = {JRhoW, JWRho, JWRhoTh, JRhoThW};
+  double *lJ[4] = {lJRhoW, lJWRho, lJWRhoTh, lJRhoThW};
+  launch_schur_adj(c, S(stream), lamk, k, v, Jb, fac, lamv, lJ, work);
   HIPCHK(c, hipGetLastError());
   return CGD_OK;
 }
@@ -961,8 +1151,8 @@ extern "C" int cgd_jac_adjoint(cgd_ctx *c, const double *U, const double *lJRhoW
   if (r) return r;
   if (!U || !lJRhoW || !lJWRho || !lJWRhoTh || !lJRhoThW || !lamU)
     return fail(c, CGD_ERR_ARG, "cgd_jac_adjoint: null array");
-  hipLaunchKernelGGL(k_jac_adj, dim3((unsigned)((c->NumG + 63) / 64)), dim3(64), 0, S(stream), c->ka, U, c->dz, lJRhoW, lJWRho,
-                     lJWRhoTh, lJRhoThW, lamU);
+  const double *lJ[4] = {lJRhoW, lJWRho, lJWRhoTh, lJRhoThW};
+  launch_jac_adj(c, S(stream), U, lJ, lamU);
   HIPCHK(c, hipGetLastError());
   return CGD_OK;
 }
@@ -996,18 +1186,21 @@ static int rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, double *lamV
   double *lJ[4] = {work + L.lJ, work + L.lJ + nj, work + L.lJ + 2 * nj, work + L.lJ + 3 * nj};
   double *tri = work + L.tri;
   const dim3 gb(grid_for(n5, 256)), bb(256);
-  // forward: the Jacobian of Vn and the three stage increments
+  double *Fs[3] = {work + L.Fs, work + L.Fs + n5, work + L.Fs + 2 * n5};
+  double *T1s[3] = {work + L.T1s, work + L.T1s + n5, work + L.T1s + 2 * n5};
+  (void)F;
+  (void)T1;
+  // forward: the Jacobian of Vn and the three stage increments; each
+  // stage's tendency and pass-1 cache are kept for its reverse
   if ((r = cgd_prepare(c, Vn, AuxG, st))) return r;
   if ((r = cgd_jac(c, Jb[0], Jb[1], Jb[2], Jb[3], Vn, AuxG, st))) return r;
   for (int is = 0; is < 3; ++is) {
     hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, Vi, Vn, k[0], A[is][0], k[1], A[is][1], is, n5);
     HIPCHK(c, hipGetLastError());
     if ((r = cgd_prepare(c, Vi, AuxG, st))) return r;
-    if ((r = run_fcn<double>(c, fV, Vi, Vi, T1, AuxG, st, true))) return r;
-    if (is > 0) {
-      hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, fV, fV, k[0], Cc[is][0] / dt, k[1], Cc[is][1] / dt, is, n5);
-      HIPCHK(c, hipGetLastError());
-    }
+    if ((r = run_fcn<double>(c, Fs[is], Vi, Vi, T1s[is], AuxG, st, true))) return r;
+    hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, fV, Fs[is], k[0], Cc[is][0] / dt, k[1], Cc[is][1] / dt, is, n5);
+    HIPCHK(c, hipGetLastError());
     if ((r = cgd_schur(c, k[is], fV, Jb[0], Jb[1], Jb[2], Jb[3], tri, fac, is == 0, st))) return r;
   }
   // reverse
@@ -1019,20 +1212,17 @@ static int rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, double *lamV
   for (int is = 2; is >= 0; --is) {
     hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, Vi, Vn, k[0], A[is][0], k[1], A[is][1], is, n5);
     HIPCHK(c, hipGetLastError());
-    if ((r = cgd_prepare(c, Vi, AuxG, st))) return r;
-    if ((r = run_fcn<double>(c, F, Vi, Vi, T1, AuxG, st, true))) return r;
-    hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, fV, F, k[0], Cc[is][0] / dt, k[1], Cc[is][1] / dt, is, n5);
+    hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, fV, Fs[is], k[0], Cc[is][0] / dt, k[1], Cc[is][1] / dt, is, n5);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(k_schur_adj, dim3((unsigned)((c->NumG + 63) / 64)), dim3(64), 0, st, c->ka, lk[is], k[is], fV, Jb[0],
-                       Jb[1], Jb[2], Jb[3], fac, lfV, lJ[0], lJ[1], lJ[2], lJ[3], work + L.bw);
+    launch_schur_adj(c, st, lk[is], k[is], fV, Jb, fac, lfV, lJ, work + L.bw);
     HIPCHK(c, hipGetLastError());
     for (int j = 0; j < is; ++j) {
       hipLaunchKernelGGL(k_adj_axpy, gb, bb, 0, st, lk[j], lfV, Cc[is][j] / dt, n5);
       HIPCHK(c, hipGetLastError());
     }
-    // the tendency's adjoint at Vi; F and T1 hold its forward values
-    hipLaunchKernelGGL(k_adj_pre, dim3(grid_for(L.ng, 256)), dim3(256), 0, st, c->ka, Vi, F, lfV, work + L.Lam, lVi,
-                       work + L.lT1, L.ng);
+    // the tendency's adjoint at Vi from its kept forward values
+    hipLaunchKernelGGL(k_adj_pre, dim3(grid_for(L.ng, 256)), dim3(256), 0, st, c->ka, Vi, Fs[is], lfV, work + L.Lam,
+                       lVi, work + L.lT1, L.ng);
     HIPCHK(c, hipGetLastError());
     {
       const int nchunk = (c->Nz + ADJ_NZT - 1) / ADJ_NZT;
@@ -1040,18 +1230,18 @@ static int rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, double *lamV
       double *Lam = work + L.Lam, *lT1 = work + L.lT1;
       if (c->N == 5) {
         if (c->cm) {
-          hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1, Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1s[is], Lam, lVi, lT1);
           hipLaunchKernelGGL((k_adj1<5, ADJ_NZT, true>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
         } else {
-          hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1, Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1s[is], Lam, lVi, lT1);
           hipLaunchKernelGGL((k_adj1<5, ADJ_NZT, false>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
         }
       } else {
         if (c->cm) {
-          hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1, Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1s[is], Lam, lVi, lT1);
           hipLaunchKernelGGL((k_adj1<4, ADJ_NZT, true>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
         } else {
-          hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1, Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1s[is], Lam, lVi, lT1);
           hipLaunchKernelGGL((k_adj1<4, ADJ_NZT, false>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
         }
       }
@@ -1064,8 +1254,7 @@ static int rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, double *lamV
       HIPCHK(c, hipGetLastError());
     }
   }
-  hipLaunchKernelGGL(k_jac_adj, dim3((unsigned)((c->NumG + 63) / 64)), dim3(64), 0, st, c->ka, Vn, c->dz, lJ[0], lJ[1], lJ[2],
-                     lJ[3], lamV);
+  launch_jac_adj(c, st, Vn, lJ, lamV);
   HIPCHK(c, hipGetLastError());
   return CGD_OK;
 }
@@ -1091,7 +1280,9 @@ extern "C" int cgd_rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, doub
 // nsteps steps from V0 with checkpoints every `every` steps: the forward
 // run stores the state at the start of each segment, the reverse sweep
 // recomputes a segment's step states from its checkpoint and runs the step
-// adjoints backwards.  lamV: in, the cotangent of the final state; out, that
+// adjoints backwards.  The checkpoint runs use the fused step (equal to the
+// composed calls to rounding); each step's reverse recomputes its stages
+// with the composed calls.  lamV: in, the cotangent of the final state; out, that
 // of V0.  Vfinal (optional) receives the final state.
 extern "C" int cgd_rw_adjoint(cgd_ctx *c, const double *V0, double dt, int nsteps, int every, double *lamV,
                               double *Vfinal, double *work, void *stream) {
@@ -1116,14 +1307,21 @@ extern "C" int cgd_rw_adjoint(cgd_ctx *c, const double *V0, double dt, int nstep
   fw.tri = work + L.tri;
   const int nseg = (nsteps + every - 1) / every;
   double *ck = work + L.ck, *seg = work + L.seg;
+  // the checkpoint runs: the fused FP64 step (the composed calls when the
+  // context steps in mixed precision, or for tall columns inside rw_run)
+  auto forward = [&](double *V, int nst) -> int {
+    if (!c->mixed) return rw_run(c, V, dt, &fw, nst, st);
+    for (int q = 0; q < nst; ++q)
+      if (int e = rw_composed_step(c, V, dt, &fw, st)) return e;
+    return CGD_OK;
+  };
   // forward: checkpoints at the segment starts (the last state goes to Vfinal)
   HIPCHK(c, hipMemcpyAsync(ck, V0, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
   for (int s = 1; s <= nseg; ++s) {
     double *cur = seg;  // run the segment in seg[0]
     HIPCHK(c, hipMemcpyAsync(cur, ck + (size_t)(s - 1) * n5, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
     const int nst = std::min(every, nsteps - (s - 1) * every);
-    for (int q = 0; q < nst; ++q)
-      if ((r = rw_composed_step(c, cur, dt, &fw, st))) return r;
+    if ((r = forward(cur, nst))) return r;
     if (s < nseg) HIPCHK(c, hipMemcpyAsync(ck + (size_t)s * n5, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
     else if (Vfinal) HIPCHK(c, hipMemcpyAsync(Vfinal, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
@@ -1134,7 +1332,7 @@ extern "C" int cgd_rw_adjoint(cgd_ctx *c, const double *V0, double dt, int nstep
     for (int q = 1; q < nst; ++q) {
       HIPCHK(c, hipMemcpyAsync(seg + (size_t)q * n5, seg + (size_t)(q - 1) * n5, n5 * sizeof(double),
                                hipMemcpyDeviceToDevice, st));
-      if ((r = rw_composed_step(c, seg + (size_t)q * n5, dt, &fw, st))) return r;
+      if ((r = forward(seg + (size_t)q * n5, 1))) return r;
     }
     for (int q = nst - 1; q >= 0; --q)
       if ((r = rw_step_adjoint(c, seg + (size_t)q * n5, dt, lamV, work, st))) return r;
