@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--ntr", type=int, default=0, help="tracers in the FP64 state (reported in config)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient and its FD check")
     ap.add_argument("--adj-dirs", type=int, default=64, help="FD directions of the adjoint check")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="run only the adjoint workload and print its JSON (not the benchmark line)")
     ap.add_argument("--ens-members", type=int, default=32)
     ap.add_argument("--ens-steps", type=int, default=10)
     ap.add_argument("--ens-dt", type=float, default=150.0)
@@ -314,6 +316,9 @@ def main():
         return
 
     pkg = ge.load_package()
+    if args.only_adjoint:  # profiling aid: the adjoint workload alone, its own JSON line
+        print(json.dumps({"adjoint": adjoint(pkg, args)}), flush=True)
+        return
     host = pkg.host
     D = pkg.discretization
     t0 = time.perf_counter()
