@@ -162,7 +162,10 @@ def test_jac_schur_parity(pkg, oracle_mod, torch_cuda, ne, nz):
     assert np.all(kg[3][:, -1] == 0.0)
 
 
-@pytest.mark.parametrize("ne,nz,dt,nsteps", [(3, 8, 300.0, 2), (6, 10, 300.0, 3), (4, 64, 300.0, 2)])
+# (6, 10, 300 s, 10 steps): the reference's CPU configuration exactly; (30, 64,
+# 100 s, 2 steps): the FP64 drop-in workload of the benchmark
+@pytest.mark.parametrize("ne,nz,dt,nsteps", [(3, 8, 300.0, 2), (6, 10, 300.0, 3), (4, 64, 300.0, 2),
+                                             (6, 10, 300.0, 10), (30, 64, 100.0, 2)])
 def test_rw_step_parity(pkg, oracle_mod, torch_cuda, ne, nz, dt, nsteps):
     torch = torch_cuda
     host = pkg.host
