@@ -284,6 +284,8 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--ntr", type=int, default=0, help="tracers in the FP64 state (reported in config)")
+    ap.add_argument("--horlimit", action="store_true",
+                    help="limited horizontal tracer transport (HorLimit; needs --ntr)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint gradient and its FD check")
     ap.add_argument("--adj-dirs", type=int, default=64, help="FD directions of the adjoint check")
     ap.add_argument("--only-adjoint", action="store_true",
@@ -296,6 +298,8 @@ def main():
     args = ap.parse_args()
     if args.dt is None:
         args.dt = 100.0 if args.ne <= 30 else 3000.0 / args.ne
+    if args.horlimit and not args.ntr:
+        ap.error("--horlimit needs tracers (--ntr)")
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU halo exchange is not implemented", file=sys.stderr)
@@ -328,7 +332,11 @@ def main():
     U0 = pkg.state.jet_state(disc, seed=0)
     if args.ntr:
         # tracers: mixing ratios 0.01 (1 + 0.1 k) of the density
-        U0 = np.concatenate([U0] + [0.01 * (1 + 0.1 * k) * U0[0:1] for k in range(args.ntr)], axis=0)
+        # (with --horlimit times 1.5 where |longitude| < 0.8, a step for the limiter to act on)
+        shape_q = np.ones((1,) + U0.shape[1:])
+        if args.horlimit:
+            shape_q[0] += 0.5 * (np.abs(D.node_lonlat(disc)[:, 0]) < 0.8)[:, None]
+        U0 = np.concatenate([U0] + [0.01 * (1 + 0.1 * k) * shape_q * U0[0:1] for k in range(args.ntr)], axis=0)
     print(f"[bench] setup ne={args.ne} p={args.p} nz={args.nz}: {host_build_s:.1f}s", file=sys.stderr, flush=True)
     # the same static inputs built by the device set-up kernels (numbering on the host)
     torch.cuda.synchronize()
@@ -345,7 +353,7 @@ def main():
                      "numbering + set-up kernels"}
     del dev_build
     FE, Metric, Global, Cache, JCache = host.device_setup(disc, phys, hyperd=args.hyperd, device="cuda:0",
-                                                          NumTr=args.ntr)
+                                                          NumTr=args.ntr, horlimit=args.horlimit)
     ctx = host.init_context(FE, Metric, phys, Global)
     V = torch.as_tensor(U0, device="cuda:0").clone()
     shape = ctx.launch_shape()
@@ -384,7 +392,8 @@ def main():
     n_eval = 3 * args.steps
     p2_eval_s = p2_ms / 1e3 / n_eval
     achieved = sb["pass2"] / p2_eval_s / 1e9
-    workload = f"rw_step_fp64_ne{args.ne}_p{args.p}_nz{args.nz}" + (f"_tr{args.ntr}" if args.ntr else "")
+    workload = f"rw_step_fp64_ne{args.ne}_p{args.p}_nz{args.nz}" + (f"_tr{args.ntr}" if args.ntr else "") + (
+        "_lim" if args.horlimit else "")
     traffic = measured_traffic("k_fcn2", workload, sb["pass2"])
     roofline = {"bound": "hbm", "kernel": "k_fcn2 (fused tendency pass)", "achieved": round(achieved, 1),
                 "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
@@ -428,7 +437,7 @@ def main():
            "config": {"workload": workload, "ne": args.ne, "p": args.p,
                       "nz": args.nz, "dt": args.dt, "hyperd": Global.Model.HyperDDiv, "points": Ne,
                       "precision": "fp64", "finite": finite, "tracers": args.ntr,
-                      "launch_shape": dict(shape, autotuned=not args.no_autotune)},
+                      "horlimit": args.horlimit, "launch_shape": dict(shape, autotuned=not args.no_autotune)},
            "roofline": roofline, "cpu_baseline": cpu, "mixed": mixed, "ensemble": ens, "adjoint": adj,
            "setup": setup}
     print(json.dumps(out), flush=True)

@@ -35,6 +35,13 @@ struct cgd_ctx {
   double *d_f4x = nullptr;     // [NF][NN][nchunk] chunk-bottom F4 terms
   int *d_nptr = nullptr, *d_ncopy = nullptr;  // global node -> element copies (colour order)
   double *d_kvis = nullptr;    // [slot][3] hyperviscosity coefficients
+  // HorLimit: each slot's stencil (itself and the elements sharing a node,
+  // 13 slot numbers), the element bounds of the mixing ratios
+  // [tracer][slot][Nz], and the stage time step (dtauStage)
+  int horlimit = 0;
+  double dtau = 0.0;
+  int *d_stencil = nullptr;
+  double *d_qmin = nullptr, *d_qmax = nullptr;
   // batch of grids (cgd_ctx_create_batch): member m owns elements
   // mem_e0[m]..mem_e0[m+1]-1 and global nodes mem_g0[m]..mem_g0[m+1]-1
   int nmem = 1;
@@ -268,9 +275,11 @@ static int validate_desc(cgd_ctx *c, const cgd_desc *d, const std::string &who) 
     return fail(c, CGD_ERR_UNSUPPORTED, who + "tracers are implemented in the FP64 mode only");
   if (d->precision != CGD_FP64 && d->precision != CGD_MIXED)
     return fail(c, CGD_ERR_UNSUPPORTED, who + "unknown precision mode");
-  if (d->HorLimit || d->Turbulence || d->VerticalDiffusion || d->SurfaceFlux || d->Microphysics || d->EDMF)
+  if (d->Turbulence || d->VerticalDiffusion || d->SurfaceFlux || d->Microphysics || d->EDMF)
     return fail(c, CGD_ERR_UNSUPPORTED,
-                who + "only the dry CompressibleShallow path is implemented (hooks: Damping, Forcing)");
+                who + "only the CompressibleShallow path is implemented (hooks: Damping, Forcing, HorLimit)");
+  if (d->HorLimit && (d->NumTr < 1 || !d->w))
+    return fail(c, CGD_ERR_ARG, who + "HorLimit needs tracers (NumTr > 0) and the GLL weights w");
   if (d->Damping && (!d->zP || !(d->DampStride > 0.0) || !(d->DampH > 0.0)))
     return fail(c, CGD_ERR_ARG, who + "Damping needs zP, DampH > 0 and DampStride > 0");
   if (d->Forcing && (!d->xS || !(d->HS_sigma_b > 0.0 && d->HS_sigma_b < 1.0)))
@@ -318,6 +327,8 @@ static int ctx_build(cgd_ctx *c, const cgd_desc *ds, int nmem, hipStream_t st) {
                        b.HS_T_equator != a.HS_T_equator || b.HS_T_min != a.HS_T_min ||
                        b.HS_sigma_b != a.HS_sigma_b)))
       return fail(c, CGD_ERR_ARG, who + "physics hooks and their constants must be the same for every member");
+    if ((b.HorLimit != 0) != (a.HorLimit != 0))
+      return fail(c, CGD_ERR_ARG, who + "HorLimit must be the same for every member");
   }
   const cgd_desc *d = &ds[0];
   c->desc = *d;
@@ -433,6 +444,25 @@ static int ctx_build(cgd_ctx *c, const cgd_desc *ds, int nmem, hipStream_t st) {
       kvis[(size_t)slot[e] * 3 + 1] = ds[m].HyperDGrad;
       kvis[(size_t)slot[e] * 3 + 2] = ds[m].HyperDDiv;
     }
+  // HorLimit: the stencil of every slot (FE.Stencil, NumberingFem.jl:133-157:
+  // the element and those sharing a grid node with it, padded with itself)
+  c->horlimit = d->HorLimit != 0;
+  std::vector<int> stencil;
+  if (c->horlimit) {
+    std::vector<int> aptr, adj;
+    element_adjacency(g32, NN, NF, NumG, aptr, adj);
+    stencil.assign((size_t)NF * 13, 0);
+    for (int e = 0; e < NF; ++e) {
+      const int q = slot[e];
+      if (aptr[e + 1] - aptr[e] > 12) return fail(c, CGD_ERR_ARG, "HorLimit: an element has more than 12 neighbours");
+      for (int k = 0; k < 13; ++k) stencil[(size_t)q * 13 + k] = q;
+      for (int k = aptr[e]; k < aptr[e + 1]; ++k) stencil[(size_t)q * 13 + 1 + (k - aptr[e])] = slot[adj[k]];
+    }
+    std::vector<double> wg(c->N);
+    HIPCHK(c, hipMemcpyAsync(wg.data(), d->w, c->N * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipStreamSynchronize(st));
+    for (int i = 0; i < c->N; ++i) c->ka.wg[i] = wg[i];
+  }
   // device buffers (set-up only; nothing is allocated on the step path).
   // The chunk-interface side buffer is sized for the shortest chunk height
   // (8 levels), so the launch shape can change later.
@@ -454,6 +484,13 @@ static int ctx_build(cgd_ctx *c, const cgd_desc *ds, int nmem, hipStream_t st) {
   HIPCHK(c, hipMemcpyAsync(c->d_glob, gslot.data(), gslot.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_slot, slot.data(), slot.size() * sizeof(int), hipMemcpyHostToDevice, st));
   HIPCHK(c, hipMemcpyAsync(c->d_kvis, kvis.data(), kvis.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  if (c->horlimit) {
+    const size_t nb = (size_t)c->ntr * NF * Nz;
+    HIPCHK(c, hipMalloc(&c->d_stencil, stencil.size() * sizeof(int)));
+    HIPCHK(c, hipMalloc(&c->d_qmin, nb * sizeof(double)));
+    HIPCHK(c, hipMalloc(&c->d_qmax, nb * sizeof(double)));
+    HIPCHK(c, hipMemcpyAsync(c->d_stencil, stencil.data(), stencil.size() * sizeof(int), hipMemcpyHostToDevice, st));
+  }
   // M and dz in batch numbering (a single grid's arrays are used in place)
   c->hooks.forcing = d->Forcing != 0;
   c->hooks.damping = d->Damping != 0;
@@ -624,6 +661,8 @@ extern "C" int cgd_ctx_destroy(cgd_ctx *c) {
   // without touching the HIP runtime
   graph_free(c);
   if (c->gstream) (void)hipStreamDestroy(c->gstream);
+  for (void *p : {(void *)c->d_stencil, (void *)c->d_qmin, (void *)c->d_qmax})
+    if (p) (void)hipFree(p);
   for (void *p : {(void *)c->d_glob, (void *)c->d_slot, (void *)c->d_met, (void *)c->d_met1, (void *)c->d_metn,
                   (void *)c->d_met1n, (void *)c->d_ms, (void *)c->d_ims32, (void *)c->d_m32, (void *)c->d_metp32,
                   (void *)c->d_bwork, (void *)c->d_f4x, (void *)c->d_nptr, (void *)c->d_ncopy, (void *)c->d_kvis,
@@ -744,6 +783,13 @@ static void tr2_t(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const doubl
                      T1, F);
 }
 
+template <int N, int NZT, bool CM, typename R>
+static void tr2lim_t(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const double *U, double *F) {
+  const size_t lds = sizeof(double) * (c->NN + 4 * c->NN * NZT + 9 * NZT);
+  hipLaunchKernelGGL((k_tr2_lim<N, NZT, CM>), dim3(nitems), dim3(c->block), lds, st, c->ka, slot0, c->nchunk, c->ntr,
+                     U, c->d_stencil, c->NF, c->d_qmin, c->d_qmax, c->dtau, F);
+}
+
 template <typename R>
 static void launch_fcn1(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const R *UR, R *T1) {
   CGD_DISPATCH(fcn1_t, R, c->nzt1, nitems, st, slot0, UR, T1);
@@ -784,7 +830,7 @@ static int run_fcn(cgd_ctx *c, R *F, const double *U, const R *UR, R *Temp1, con
     ++launches;
   }
   if constexpr (std::is_same<R, double>::value) {
-    if (c->ntr > 0)
+    if (c->ntr > 0 && !c->horlimit)  // HorLimit: no tracer hyperviscosity (FcnGPU.jl:747-751)
       for (int k = 0; k < c->ncolor; ++k) {
         const int ne = c->color_off[k + 1] - c->color_off[k];
         if (!ne) continue;
@@ -804,7 +850,22 @@ static int run_fcn(cgd_ctx *c, R *F, const double *U, const R *UR, R *Temp1, con
     ++launches;
   }
   if constexpr (std::is_same<R, double>::value) {
-    if (c->ntr > 0)
+    if (c->ntr > 0 && c->horlimit) {
+      // LimitKernel! on the state, then DivRhoTrUpwind3LimKernel! per colour
+      if (!(c->dtau > 0.0)) return fail(c, CGD_ERR_ARG, "HorLimit: the stage time step is not set (cgd_set_dtau)");
+      const size_t nb = (size_t)c->ntr * c->NF * c->Nz;
+      hipLaunchKernelGGL(k_lim_bounds, dim3(grid_for(nb, 256)), dim3(256), 0, st, c->ka, c->NF, c->NN, c->ntr, U,
+                         c->d_qmin, c->d_qmax);
+      HIPCHK(c, hipGetLastError());
+      ++launches;
+      for (int k = 0; k < c->ncolor; ++k) {
+        const int ne = c->color_off[k + 1] - c->color_off[k];
+        if (!ne) continue;
+        CGD_DISPATCH(tr2lim_t, double, c->nzt, ne * c->nchunk, st, c->color_off[k], U, F);
+        HIPCHK(c, hipGetLastError());
+        ++launches;
+      }
+    } else if (c->ntr > 0) {
       for (int k = 0; k < c->ncolor; ++k) {
         const int ne = c->color_off[k + 1] - c->color_off[k];
         if (!ne) continue;
@@ -812,6 +873,7 @@ static int run_fcn(cgd_ctx *c, R *F, const double *U, const R *UR, R *Temp1, con
         HIPCHK(c, hipGetLastError());
         ++launches;
       }
+    }
   }
   if (c->hooks.forcing || c->hooks.damping) {
     const size_t n = (size_t)c->NumG * c->Nz;
@@ -836,6 +898,14 @@ extern "C" int cgd_fcn(cgd_ctx *c, double *F, const double *U, double *Temp1, co
   if (r) return r;
   if (!F || !U || !Temp1 || !AuxG) return fail(c, CGD_ERR_ARG, "cgd_fcn: null array");
   return run_fcn<double>(c, F, U, U, Temp1, AuxG, S(stream), true);
+}
+
+extern "C" int cgd_set_dtau(cgd_ctx *c, double dtau) {
+  int r = check(c);
+  if (r) return r;
+  if (!(dtau > 0.0)) return fail(c, CGD_ERR_ARG, "cgd_set_dtau: dtau must be positive");
+  c->dtau = dtau;
+  return CGD_OK;
 }
 
 extern "C" int cgd_jac(cgd_ctx *c, double *JRhoW, double *JWRho, double *JWRhoTh, double *JRhoThW, const double *U,
@@ -1019,6 +1089,7 @@ static int rw_run(cgd_ctx *c, double *V, double dt, const cgd_rw_work *w, int ns
   if (!(dt > 0.0)) return fail(c, CGD_ERR_ARG, "cgd_rw_step: dt must be positive");
   if (nsteps < 1) return fail(c, CGD_ERR_ARG, "cgd_rw_run: nsteps must be positive");
   hipStream_t st = S(stream);
+  c->dtau = dt;  // Global.TimeStepper.dtauStage = dt (RosenbrockSchur.jl:9)
   // SSP-Knoth: a = alpha/Gamma, c = -inv(Gamma), m = Gamma'\b, gamma = 1
   const double A[3][3] = {{0, 0, 0}, {1.0, 0, 0}, {0.25, 0.25, 0}};
   const double Cc[3][3] = {{-1.0, 0, 0}, {-0.0, -1.0, 0}, {-0.75, -0.75, -1.0}};

@@ -49,6 +49,7 @@ struct KArgs {
   // Models.Moist: 0-based components of the vapour and cloud densities (-1: dry)
   int iv, ic;
   double Cpv, Cpl, Rv;
+  double wg[CGD_NMAX];  // GLL weights (HorLimit)
 };
 
 // metric component indices of met (dXdxI[a,b,K] -> dabK, 1-based a,b,K)
@@ -1313,6 +1314,256 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_tr2(KArgs a, i
 #undef MET
 #undef METN
 #undef MET0
+}
+
+// ------------------------------------------------------------------------
+// HorLimit (src/GPU/HorLimiterKernel.jl)
+
+// LimitKernel! (HorLimiterKernel.jl:2-22): per tracer, element slot and
+// level, the min and max over the element's nodes of the mixing ratio;
+// qmin/qmax[(tr*NF + slot)*Nz + iz]
+__global__ void k_lim_bounds(KArgs a, int NF, int NN, int ntr, const double *__restrict__ U,
+                             double *__restrict__ qmin, double *__restrict__ qmax) {
+  const int Nz = a.Nz;
+  const size_t vs = (size_t)a.NumG * Nz, n = (size_t)ntr * NF * Nz;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+    const size_t tq = q / ((size_t)NF * Nz), r = q - tq * NF * Nz;
+    const int e = (int)(r / Nz), iz = (int)(r - (size_t)e * Nz);
+    const double *Tr = U + (5 + tq) * vs;
+    double mn = 1.0 / 0.0, mx = -1.0 / 0.0;
+    for (int ID = 0; ID < NN; ++ID) {
+      const size_t ui = (size_t)(a.glob[(size_t)e * NN + ID] & 0x7fffffff) * Nz + iz;
+      const double v = Tr[ui] / U[ui];
+      mn = jmin(mn, v);
+      mx = jmax(mx, v);
+    }
+    qmin[q] = mn;
+    qmax[q] = mx;
+  }
+}
+
+// medianGPU (HorLimiterKernel.jl:389-406)
+__device__ __forceinline__ double median3(double a1, double a2, double a3) {
+  if (a1 <= a2) {
+    if (a2 <= a3) return a2;
+    if (a1 <= a3) return a3;
+    return a1;
+  }
+  if (a1 <= a3) return a1;
+  if (a2 <= a3) return a3;
+  return a2;
+}
+
+// DivRhoTrUpwind3LimKernel! (HorLimiterKernel.jl:24-196) for all tracers,
+// one launch per colour, one thread per (node, level) of an element and
+// level chunk: the upwind vertical flux through both interfaces of the
+// layer (as k_tr2), then the limited horizontal transport: the predicted
+// rho*q and rho after dtau, the mixing ratio clipped to the bounds of the
+// element's stencil after a level-wide shift lambda, lambda from the
+// reference's secant iteration on the element-level tracer mass.  The
+// level sums run over the nodes in node order (LDS), one thread per level
+// driving the iteration.  Divisions as in the reference (no reciprocals):
+// the iteration's thresholds are absolute.  The DSS mass is M1 + M2 (the
+// reference indexes its 3-D M with two indices here).
+template <int N, int NZT, bool CM>
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64)
+    k_tr2_lim(KArgs a, int slot0, int nchunk, int ntr, const double *__restrict__ U, const int *__restrict__ stencil,
+              int NF, const double *__restrict__ qmin, const double *__restrict__ qmax, double dtau,
+              double *__restrict__ F) {
+  constexpr int NN = N * N;
+  extern __shared__ double smem[];
+  double *Ds = smem;
+  // fields 0, 1: uc, vc; 2: c; 3: the node terms of a level sum
+  Tiles<N, NZT, double> T{Ds + NN};
+  double *lv = Ds + NN + 4 * NN * NZT;  // per level: qMinS, qMaxS, sumJ, res, resp, resc, lp, lc, conv
+  if (threadIdx.x < NN) Ds[threadIdx.x] = a.D[threadIdx.x];
+  const int Nz = a.Nz;
+  const int L = xcd_logical(blockIdx.x, gridDim.x);
+  const int e = slot0 + L / nchunk;
+  const int chunk = L - (L / nchunk) * nchunk;
+  const int t = threadIdx.x;
+  const int node = t / NZT, l = t - node * NZT;
+  const int iz = chunk * NZT + l;
+  const bool in_tile = node < NN;
+  const bool valid = in_tile && iz < Nz;
+  const bool master = node == 0 && iz < Nz;  // drives the level's iteration
+  const int I = node % N, J = node / N;
+  const size_t vs = (size_t)a.NumG * Nz;
+  const bool has_lo = iz > 0, has_hi = iz < Nz - 1;
+  auto M = [&](int c, int Lv) { return met_at<NN, CM>(a, (size_t)e, c, node, Lv); };
+  size_t ui = 0;
+  bool first = false;
+  double rho = 1, u = 0, v = 0, S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1, Ms = 1, wgt = 0;
+  double wlo = 0, wup = 0, rho_m = 1, rho_mm = 1, rho_p = 1, rho_pp = 1;
+  double Jm = 1, Jmm = 1, Jp = 1, Jpp = 1;
+  if (valid) {
+    const int gv = a.glob[(size_t)e * NN + node];
+    first = gv < 0;
+    ui = (size_t)(gv & 0x7fffffff) * Nz + iz;
+    rho = U[ui];
+    u = U[vs + ui];
+    v = U[2 * vs + ui];
+    S11 = M(M111, iz) + M(M112, iz);
+    S12 = M(M121, iz) + M(M122, iz);
+    S21 = M(M211, iz) + M(M212, iz);
+    S22 = M(M221, iz) + M(M222, iz);
+    Js = M(MJ1, iz) + M(MJ2, iz);
+    Ms = a.M[ui] + a.M[vs + ui];
+    wgt = Js * a.wg[I] * a.wg[J];
+    const int lm = has_lo ? iz - 1 : iz, lmm = iz >= 2 ? iz - 2 : 0;
+    const int lp = has_hi ? iz + 1 : iz, lpp = iz + 2 <= Nz - 1 ? iz + 2 : Nz - 1;
+    rho_m = U[ui - iz + lm];
+    rho_mm = U[ui - iz + lmm];
+    rho_p = U[ui - iz + lp];
+    rho_pp = U[ui - iz + lpp];
+    Jm = M(MJ1, lm) + M(MJ2, lm);
+    Jmm = M(MJ1, lmm) + M(MJ2, lmm);
+    Jp = M(MJ1, lp) + M(MJ2, lp);
+    Jpp = M(MJ1, lpp) + M(MJ2, lpp);
+    // Contra3 at the two interfaces (the reference's operation order)
+    if (has_lo) {
+      const double w = U[3 * vs + ui - 1];
+      wlo = rho_m * (M(M312, iz - 1) * U[vs + ui - 1] + M(M322, iz - 1) * U[2 * vs + ui - 1] + M(M332, iz - 1) * w) +
+            rho * (M(M311, iz) * u + M(M321, iz) * v + M(M331, iz) * w);
+    }
+    if (has_hi) {
+      const double w = U[3 * vs + ui];
+      wup = rho * (M(M312, iz) * u + M(M322, iz) * v + M(M332, iz) * w) +
+            rho_p * (M(M311, iz + 1) * U[vs + ui + 1] + M(M321, iz + 1) * U[2 * vs + ui + 1] + M(M331, iz + 1) * w);
+    }
+  }
+  if (in_tile) {
+    T.at(0, node, l) = -rho * (S11 * u + S12 * v);  // Contra12(-rho, u, v)
+    T.at(1, node, l) = -rho * (S21 * u + S22 * v);
+  }
+  __syncthreads();  // D and the tracer-independent tiles
+  // the level's sum of J
+  if (in_tile) T.at(3, node, l) = valid ? Js : 0.0;
+  __syncthreads();
+  if (master) {
+    double s = 0.0;
+    for (int k = 0; k < NN; ++k) s += T.at(3, k, l);
+    lv[2 * NZT + l] = s;
+  }
+  __syncthreads();
+  const double sumJ = iz < Nz ? lv[2 * NZT + l] : 1.0;
+  const double eta = 1.e-12, dlFD = 1.e-8, l0 = 0.0;
+  for (int tr = 0; tr < ntr; ++tr) {
+    const double *Tr = U + (size_t)(5 + tr) * vs;
+    double *Ft = F + (size_t)(5 + tr) * vs;
+    double o = 0, Trs = 0, c = 0, f = 0;
+    if (valid) {
+      if (!first) o = Ft[ui];
+      Trs = Tr[ui];
+      c = Trs / rho;
+      // upwind vertical fluxes through both interfaces (the unlimited kernel's)
+      const double cm = has_lo ? Tr[ui - 1] / rho_m : c;
+      const double cmm = iz >= 2 ? Tr[ui - 2] / rho_mm : cm;
+      const double cp = has_hi ? Tr[ui + 1] / rho_p : c;
+      const double cpp = iz + 2 <= Nz - 1 ? Tr[ui + 2] / rho_pp : cp;
+      if (has_lo) {
+        double cFL, cFR;
+        recu4(cmm, cm, c, has_hi ? cp : c, Jmm, Jm, Js, has_hi ? Jp : Js, cFL, cFR);
+        f += 0.25 * ((fabs(wlo) + wlo) * cFL + (-fabs(wlo) + wlo) * cFR) / Ms;
+      }
+      if (has_hi) {
+        double cFL, cFR;
+        recu4(cm, c, cp, cpp, Jm, Js, Jp, Jpp, cFL, cFR);
+        f += -(0.25 * ((fabs(wup) + wup) * cFL + (-fabs(wup) + wup) * cFR)) / Ms;
+      }
+    }
+    if (in_tile) T.at(2, node, l) = c;
+    if (master) {  // the stencil's bounds of this level
+      const int *st = stencil + 13 * (size_t)e;
+      const double *mn = qmin + (size_t)tr * NF * Nz, *mx = qmax + (size_t)tr * NF * Nz;
+      double qn = mn[(size_t)st[0] * Nz + iz], qx = mx[(size_t)st[0] * Nz + iz];
+      for (int iS = 1; iS < 13; ++iS) {
+        qn = jmin(mn[(size_t)st[iS] * Nz + iz], qn);
+        qx = jmax(mx[(size_t)st[iS] * Nz + iz], qx);
+      }
+      lv[l] = qn;
+      lv[NZT + l] = qx;
+    }
+    __syncthreads();
+    double RhoTrS = 1, RhoS = 1, qMinS = 0, qMaxS = 0;
+    if (valid) {
+      double DivRhoTr = Ds[I * N] * T.at(0, J * N, l) * T.at(2, J * N, l);
+      DivRhoTr += Ds[J * N] * T.at(1, I, l) * T.at(2, I, l);
+      double DivRho = Ds[I * N] * T.at(0, J * N, l);
+      DivRho += Ds[J * N] * T.at(1, I, l);
+#pragma unroll
+      for (int k = 1; k < N; ++k) {
+        DivRhoTr += Ds[I * N + k] * T.at(0, k + J * N, l) * T.at(2, k + J * N, l);
+        DivRhoTr += Ds[J * N + k] * T.at(1, I + k * N, l) * T.at(2, I + k * N, l);
+        DivRho += Ds[I * N + k] * T.at(0, k + J * N, l);
+        DivRho += Ds[J * N + k] * T.at(1, I + k * N, l);
+      }
+      RhoTrS = Trs + dtau * DivRhoTr / Js;
+      RhoS = rho + dtau * DivRho / Js;
+      qMinS = lv[l];
+      qMaxS = lv[NZT + l];
+    }
+    const double pred = RhoTrS / RhoS;
+    // one level sum of wgt/sumJ (q RhoS - RhoTrS) for q = median(qMinS, pred + lam, qMaxS)
+    auto level_sum = [&](double lam, bool active, double &q) {
+      double term = 0.0;
+      if (valid && active) {
+        q = median3(qMinS, pred + lam, qMaxS);
+        term = wgt / sumJ * (q * RhoS - RhoTrS);
+      }
+      if (in_tile) T.at(3, node, l) = term;
+      __syncthreads();
+      if (master && active) {
+        double s = 0.0;
+        for (int k = 0; k < NN; ++k) s += T.at(3, k, l);
+        lv[3 * NZT + l] = s;
+      }
+      __syncthreads();
+    };
+    double q = 0.0, qfd = 0.0;
+    level_sum(l0, true, q);
+    if (master) {
+      lv[4 * NZT + l] = lv[3 * NZT + l];  // resp
+      lv[8 * NZT + l] = fabs(lv[3 * NZT + l]) <= eta ? 0.0 : 1.0;
+    }
+    __syncthreads();
+    const bool fd = iz < Nz && lv[8 * NZT + l] != 0.0;
+    level_sum(l0 + dlFD, fd, qfd);
+    if (master && fd) {
+      const double resp = lv[4 * NZT + l], resc = lv[3 * NZT + l];
+      if (fabs(resc - resp) <= 1.e-13) {
+        lv[8 * NZT + l] = 0.0;
+      } else {
+        const double alpha = dlFD / (resc - resp);
+        lv[6 * NZT + l] = l0;                   // lp
+        lv[7 * NZT + l] = l0 - alpha * resp;    // lc
+        lv[4 * NZT + l] = 0.0;                  // resp (reset as the reference does)
+      }
+    }
+    __syncthreads();
+    for (int it = 0; it < 8; ++it) {
+      const bool conv = iz < Nz && lv[8 * NZT + l] != 0.0;
+      const double lc = conv ? lv[7 * NZT + l] : 0.0;
+      level_sum(lc, conv, q);
+      if (master && conv) {
+        const double resp = lv[4 * NZT + l], resc = lv[3 * NZT + l], lp = lv[6 * NZT + l];
+        if (fabs(resc - resp) <= 1.e-13) {
+          lv[8 * NZT + l] = 0.0;
+        } else {
+          const double alpha = (lp - lc) / (resp - resc);
+          lv[4 * NZT + l] = resc;
+          lv[6 * NZT + l] = lc;
+          lv[7 * NZT + l] = lc - alpha * resc;
+        }
+      }
+      __syncthreads();
+    }
+    if (valid) {
+      f += (q * RhoS - Trs) * Js / dtau / Ms;
+      Ft[ui] = first ? f : o + f;
+    }
+    __syncthreads();  // the next tracer rewrites the tiles and level values
+  }
 }
 
 // SchurSolveBKernel! for the tracers (SchurSolve.jl:106-109): k = fac v

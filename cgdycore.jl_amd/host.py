@@ -59,7 +59,7 @@ class _Desc(C.Structure):
                 ("HS_DeltaT_y", C.c_double), ("HS_DeltaTh_z", C.c_double), ("HS_T_equator", C.c_double),
                 ("HS_T_min", C.c_double), ("HS_sigma_b", C.c_double), ("xS", _vp),
                 ("State", C.c_int), ("RhoVPos", C.c_int), ("RhoCPos", C.c_int),
-                ("Cpv", C.c_double), ("Cpl", C.c_double), ("Rv", C.c_double)]
+                ("Cpv", C.c_double), ("Cpl", C.c_double), ("Rv", C.c_double), ("w", _vp)]
 
 
 class LaunchShape(C.Structure):
@@ -111,6 +111,7 @@ def native():
                                C.POINTER(C.c_double)]
     L.cgd_rw_graph_create.argtypes = [_vp, _vp, C.c_double, C.POINTER(_RWWork), C.c_int, _vp]
     L.cgd_rw_graph_launch.argtypes = [_vp, _vp]
+    L.cgd_set_dtau.argtypes = [_vp, C.c_double]
     L.cgd_adjoint_work_size.argtypes = [_vp, C.c_int, C.c_int, C.POINTER(C.c_size_t)]
     L.cgd_fcn_adjoint.argtypes = [_vp] * 6
     L.cgd_schur_adjoint.argtypes = [_vp] * 8 + [C.c_double] + [_vp] * 7
@@ -132,7 +133,7 @@ def exported_symbols():
             "cgd_autotune", "cgd_rw_graph_create", "cgd_rw_graph_launch", "cgd_setup_error",
             "cgd_jacobi_sphere3", "cgd_mass_cg", "cgd_grid_size_sphere", "cgd_rw_run", "cgd_output_error",
             "cgd_interpolate", "cgd_adjoint_work_size", "cgd_fcn_adjoint", "cgd_schur_adjoint", "cgd_jac_adjoint",
-            "cgd_rw_step_adjoint", "cgd_rw_adjoint"]
+            "cgd_rw_step_adjoint", "cgd_rw_adjoint", "cgd_set_dtau"]
 
 
 def color_elements(glob: np.ndarray, num_g: int):
@@ -212,6 +213,7 @@ class CGQuad:
     DW: torch.Tensor
     M: torch.Tensor  # (2, NumG, Nz)
     NumG: int
+    w: torch.Tensor | None = None  # (N,) GLL weights (HorLimit)
 
 
 @dataclass
@@ -262,6 +264,9 @@ class GlobalStruct:
     # (FP64 for the EOS, the w-interface balance, the horizontal pressure
     # gradient and the column solve); the separate Fcn/Jac/Schur calls stay FP64
     Precision: str = "fp64"
+    # Global.TimeStepper.dtauStage: the limited tracer transport's time step
+    # (RosenbrockSchur! and TimeStepper set it to dt)
+    dtauStage: float = 0.0
     _ctx: object = field(default=None, repr=False)
 
 
@@ -322,6 +327,10 @@ def _make_desc(FE: CGQuad, Metric: MetricStruct, Phys: Phys, Global: GlobalStruc
         d.HS_k_a, d.HS_k_f, d.HS_k_s = hs["k_a"], hs["k_f"], hs["k_s"]
         d.HS_DeltaT_y, d.HS_DeltaTh_z, d.HS_T_equator = hs["DeltaT_y"], hs["DeltaTh_z"], hs["T_equator"]
         d.HS_T_min, d.HS_sigma_b, d.xS = hs["T_min"], hs["sigma_b"], _ptr(Metric.xS)
+    if M.HorLimit:
+        if FE.w is None:
+            raise CGDError("HorLimit needs the GLL weights FE.w")
+        d.w = _ptr(FE.w)
     return d
 
 
@@ -510,6 +519,8 @@ def FcnGPU(F, U, FE, Metric, Phys, Cache, Exchange, Global, Param, Equation):
     _check_state(F, Global, nv, "F")
     _check_state(U, Global, nv, "U")
     _check_state(Cache.Temp1, Global, nv, "Cache.Temp1")
+    if Global.Model.HorLimit:
+        ctx.check(ctx.L.cgd_set_dtau(ctx.h, C.c_double(Global.dtauStage)), "FcnGPU (dtauStage)")
     ctx.check(ctx.L.cgd_fcn(ctx.h, _ptr(F), _ptr(U), _ptr(Cache.Temp1), _ptr(Cache.AuxG), _stream()), "FcnGPU")
 
 
@@ -542,6 +553,7 @@ def RosenbrockSchur(V, dt, Fcn, FcnPrepare, Jac, CG, Metric, Phys, Cache, JCache
     whole step runs as one ``cgd_rw_step`` call; other callables are
     composed stage by stage exactly as the reference does.
     """
+    Global.dtauStage = dt  # RosenbrockSchur.jl:9
     if Fcn is FcnGPU and FcnPrepare is FcnPrepareGPU and Jac is JacSchurGPU:
         ctx = _ctx(CG, Metric, Phys, Global)
         _check_state(V, Global, _nv(Global), "V")
@@ -578,6 +590,7 @@ def TimeStepper(U, dt, nIter, FE, Metric, Phys, Cache, JCache, Global, Exchange=
         return
     ctx = _ctx(FE, Metric, Phys, Global)
     _check_state(U, Global, _nv(Global), "U")
+    Global.dtauStage = dt
     w = _rw_work(Cache, JCache)
     ctx.check(ctx.L.cgd_rw_run(ctx.h, _ptr(U), C.c_double(dt), C.byref(w), int(nIter), _stream()), "TimeStepper")
     JCache.CompTri = False
@@ -648,13 +661,15 @@ def _work_arrays(num_g: int, nz: int, nT: int, dev, NumTr: int = 0):
 
 
 def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None = None, device="cuda",
-                 nT: int = 5, precision: str = "fp64", NumTr: int = 0, damping=None, forcing=None, moist=None):
+                 nT: int = 5, precision: str = "fp64", NumTr: int = 0, damping=None, forcing=None, moist=None,
+                 horlimit: bool = False):
     """Upload the static inputs and allocate ``Cache``/``JCache`` (state
     arrays with ``5 + NumTr`` components; ``Temp1`` with at least as many).
     ``damping = (H, StrideDamp, Relax)`` switches the DampingW sponge on,
     ``forcing = True`` (or a dict of Held-Suarez constants) the Held-Suarez
     forcing, ``moist = (RhoVPos, RhoCPos)`` the moist equation of state with
-    vapour and cloud at those (1-based) tracer components."""
+    vapour and cloud at those (1-based) tracer components, ``horlimit`` the
+    limited horizontal tracer transport (HorLimit)."""
     from . import discretization as _D
 
     dev = torch.device(device)
@@ -667,6 +682,9 @@ def device_setup(disc: Discretization, phys: Phys = Phys(), hyperd: float | None
     if moist is not None:
         Global.Model.State = "Moist"
         Global.Model.RhoVPos, Global.Model.RhoCPos = moist
+    if horlimit:
+        Global.Model.HorLimit = True
+        FE.w = torch.as_tensor(disc.w, **f64).contiguous()
     if forcing:
         Global.Model.Forcing = True
         if isinstance(forcing, dict):

@@ -63,8 +63,9 @@ typedef struct {
   int NumTr;  /* tracers, 0..16 (FP64 mode only) */
   int precision; /* CGD_FP64 or CGD_MIXED */
   /* model switches that select FcnGPU! branches; implemented: Damping
-   * (DampingW) and Forcing (Held-Suarez); HorLimit, Turbulence,
-   * VerticalDiffusion, SurfaceFlux, Microphysics and EDMF must be 0 */
+   * (DampingW), Forcing (Held-Suarez) and HorLimit (the limited horizontal
+   * tracer transport, with tracers, FP64); Turbulence, VerticalDiffusion,
+   * SurfaceFlux, Microphysics and EDMF must be 0 */
   int HorLimit;
   int Turbulence, VerticalDiffusion, SurfaceFlux, Forcing, Microphysics, Damping, EDMF;
   /* PhysParameters (GlobalVariables.jl:188-230) */
@@ -91,6 +92,8 @@ typedef struct {
    * Rv (PhysParameters); dPresdRhoTh stays the dry one, as in the reference */
   int State, RhoVPos, RhoCPos;
   double Cpv, Cpl, Rv;
+  /* HorLimit != 0: the N GLL weights (FE.w), device pointer */
+  const double *w;
 } cgd_desc;
 
 enum { CGD_DRY = 0, CGD_MOIST = 1 };
@@ -126,9 +129,17 @@ int cgd_prepare(cgd_ctx *ctx, const double *U, double *AuxG, void *stream);
 
 /* FcnGPU!(F,U,...,Models.CompressibleShallow) -- src/GPU/FcnGPU.jl:552-913,
  * with the tracers of DivRhoTrUpwind3New2Kernel! (src/GPU/OperatorKernel.jl:722-759)
- * and HyperViscTracer(Koeff)Kernel! (src/GPU/ViscKernel.jl:169-218,277-327).
+ * and HyperViscTracer(Koeff)Kernel! (src/GPU/ViscKernel.jl:169-218,277-327),
+ * or with HorLimit LimitKernel! and DivRhoTrUpwind3LimKernel!
+ * (src/GPU/HorLimiterKernel.jl:2-196) and no tracer hyperviscosity.
  * Zeroes and overwrites F and Temp1[:,:,1:5+NumTr]; reads p = AuxG[:,:,1]. */
 int cgd_fcn(cgd_ctx *ctx, double *F, const double *U, double *Temp1, const double *AuxG, void *stream);
+
+/* Global.TimeStepper.dtauStage, the time step of the limited tracer
+ * transport (HorLimit): RosenbrockSchur! sets it to dt
+ * (src/Integration/RosenbrockSchur.jl:9), and so do cgd_rw_step and
+ * cgd_rw_run; cgd_fcn with HorLimit needs it set. */
+int cgd_set_dtau(cgd_ctx *ctx, double dtau);
 
 /* JacSchurGPU!(J,U,...) -- src/DyCore/JacSchur.jl:3-75.  Reads AuxG[:,:,4]
  * (identically zero for State=Dry). */

@@ -23,6 +23,8 @@
  *   HyperViscTracerKoeffKernel!      src/GPU/ViscKernel.jl:277-327
  *   HeldSuarezForcing / ForceKernel! src/Examples/force.jl:5-27, src/GPU/OperatorKernel.jl:1073-1080
  *   DampingW / DampKernel!           src/GPU/dampingGPU.jl:5-16, src/GPU/OperatorKernel.jl:1061-1071
+ *   LimitKernel!, DivRhoTrUpwind3LimKernel!, medianGPU, FE.Stencil
+ *                                    src/GPU/HorLimiterKernel.jl:2-196,389-406, src/DyCore/NumberingFem.jl:133-157
  *   Contra12/Grad12/Curl12/Rot12/Contra3/RecU4
  *                                    src/GPU/OperatorKernel.jl:953-1059
  *   JacSchurKernel!                  src/DyCore/JacSchur.jl:29-75
@@ -575,6 +577,200 @@ static void tracer_div_element(const cgdo_grid *g, int IF, int iz, int v, double
     }
 }
 
+/* ---- HorLimit: LimitKernel! and DivRhoTrUpwind3LimKernel!
+ * (src/GPU/HorLimiterKernel.jl:2-196) ---- */
+
+/* medianGPU (HorLimiterKernel.jl:389-406) */
+static double median3(double a1, double a2, double a3) {
+  if (a1 <= a2) {
+    if (a2 <= a3) return a2;
+    if (a1 <= a3) return a3;
+    return a1;
+  }
+  if (a1 <= a3) return a1;
+  if (a2 <= a3) return a3;
+  return a2;
+}
+
+/* FE.Stencil (NumberingFem.jl:133-157): the element and the elements that
+ * share a grid node with it, padded with the element itself to 13 entries;
+ * on a conforming quadrilateral mesh these are the elements sharing a GLL
+ * node, which is how they are found here.  Returns NF*13 entries. */
+static int *lim_stencil(const cgdo_grid *g) {
+  const int NF = g->NF;
+  int *st = (int *)malloc(sizeof(int) * 13 * (size_t)NF);
+  int *cnt = (int *)calloc((size_t)g->NumG + 1, sizeof(int));
+  for (int IF = 0; IF < NF; ++IF)
+    for (int ID = 0; ID < NN; ++ID) cnt[GLOB(ID, IF) + 1]++;
+  for (int i = 0; i < g->NumG; ++i) cnt[i + 1] += cnt[i];
+  int *els = (int *)malloc(sizeof(int) * (size_t)cnt[g->NumG]);
+  int *fill = (int *)malloc(sizeof(int) * (size_t)g->NumG);
+  memcpy(fill, cnt, sizeof(int) * (size_t)g->NumG);
+  for (int IF = 0; IF < NF; ++IF)
+    for (int ID = 0; ID < NN; ++ID) els[fill[GLOB(ID, IF)]++] = IF;
+  for (int IF = 0; IF < NF; ++IF) {
+    int *s = st + 13 * (size_t)IF, n = 1;
+    for (int k = 0; k < 13; ++k) s[k] = IF;
+    for (int ID = 0; ID < NN; ++ID) {
+      const int64_t gn = GLOB(ID, IF);
+      for (int q = cnt[gn]; q < cnt[gn + 1]; ++q) {
+        int jF = els[q], in = 0;
+        for (int k = 0; k < n; ++k) in |= s[k] == jF;
+        if (!in && n < 13) s[n++] = jF;
+      }
+    }
+  }
+  free(cnt);
+  free(els);
+  free(fill);
+  return st;
+}
+
+/* LimitKernel!: qmin/qmax[iz + Nz*IF] = min/max over the element's nodes of
+ * the mixing ratio of component v */
+static void lim_bounds(const cgdo_grid *g, int v, const double *U, double *qmin, double *qmax) {
+  for (int IF = 0; IF < g->NF; ++IF)
+    for (int iz = 0; iz < g->Nz; ++iz) {
+      double mn = 1.0 / 0.0, mx = -1.0 / 0.0;
+      for (int ID = 0; ID < NN; ++ID) {
+        int64_t ind = GLOB(ID, IF);
+        double q = Uv(U, iz, ind, v) / Uv(U, iz, ind, 0);
+        mn = jmin(mn, q);
+        mx = jmax(mx, q);
+      }
+      qmin[(size_t)iz + (size_t)g->Nz * IF] = mn;
+      qmax[(size_t)iz + (size_t)g->Nz * IF] = mx;
+    }
+}
+
+/* DivRhoTrUpwind3LimKernel! at level iz of element IF for tracer component
+ * v: the upwind vertical flux as in the unlimited kernel, then the
+ * horizontal transport as a limited update: the predicted rho*q and rho
+ * after dtau, the mixing ratio clipped to the bounds of the element's
+ * stencil after a shift lambda, and lambda from the secant iteration on the
+ * element-level tracer mass (operations in the reference's order, including
+ * its reset of the previous residual after the first step).  The DSS mass
+ * is M1 + M2, as in every other kernel (the reference indexes its 3-D M with
+ * two indices here, HorLimiterKernel.jl:111,113,196). */
+static void tracer_lim_element(const cgdo_grid *g, int IF, int iz, int v, double *F, const double *U,
+                               const double *qmin, const double *qmax, const int *stencil) {
+  const int N = g->N, Nz = g->Nz;
+  const double dt = g->dtau, eta = 1.e-12, dlFD = 1.e-8, l0 = 0.0;
+  double uc[MAXN][MAXN], vc[MAXN][MAXN], cc[MAXN][MAXN];
+  double RhoTrS[MAXN * MAXN], RhoS[MAXN * MAXN], q[MAXN * MAXN], Jc[MAXN * MAXN];
+  metric_pt m;
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      load_metric(g, ID, iz, IF, &m);
+      double rho = Uv(U, iz, ind, 0);
+      cc[I][J] = Uv(U, iz, ind, v) / rho;
+      contra12r(-rho, Uv(U, iz, ind, 1), Uv(U, iz, ind, 2), &m, &uc[I][J], &vc[I][J]);
+    }
+  const int *st = stencil + 13 * (size_t)IF;
+  double qMinS = qmin[(size_t)iz + (size_t)Nz * st[0]], qMaxS = qmax[(size_t)iz + (size_t)Nz * st[0]];
+  for (int iS = 1; iS < 13; ++iS) {
+    qMinS = jmin(qmin[(size_t)iz + (size_t)Nz * st[iS]], qMinS);
+    qMaxS = jmax(qmax[(size_t)iz + (size_t)Nz * st[iS]], qMaxS);
+  }
+  double sumJ = 0.0;
+  for (int ID = 0; ID < N * N; ++ID) sumJ += JJ(ID, 0, iz, IF) + JJ(ID, 1, iz, IF);
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      double Ms = MM(iz, ind, 0) + MM(iz, ind, 1);
+      if (iz < Nz - 1) {
+        double rL = Uv(U, iz, ind, 0), rR = Uv(U, iz + 1, ind, 0);
+        double w = Uv(U, iz, ind, 3);
+        double wCon = rL * (DX(2, 0, 1, ID, iz, IF) * Uv(U, iz, ind, 1) + DX(2, 1, 1, ID, iz, IF) * Uv(U, iz, ind, 2) +
+                            DX(2, 2, 1, ID, iz, IF) * w) +
+                      rR * (DX(2, 0, 0, ID, iz + 1, IF) * Uv(U, iz + 1, ind, 1) +
+                            DX(2, 1, 0, ID, iz + 1, IF) * Uv(U, iz + 1, ind, 2) + DX(2, 2, 0, ID, iz + 1, IF) * w);
+        int lm = iz > 0 ? iz - 1 : 0, lp2 = iz + 2 < Nz ? iz + 2 : Nz - 1;
+#define CC(L) (Uv(U, L, ind, v) / Uv(U, L, ind, 0))
+#define JC(L) (JJ(ID, 0, L, IF) + JJ(ID, 1, L, IF))
+        double cFL, cFR;
+        recu4(CC(lm), CC(iz), CC(iz + 1), CC(lp2), JC(lm), JC(iz), JC(iz + 1), JC(lp2), &cFL, &cFR);
+#undef CC
+#undef JC
+        double Msp = MM(iz + 1, ind, 0) + MM(iz + 1, ind, 1);
+        double Flux = 0.25 * ((fabs(wCon) + wCon) * cFL + (-fabs(wCon) + wCon) * cFR);
+        Uv(F, iz, ind, v) += -Flux / Ms;
+        Uv(F, iz + 1, ind, v) += Flux / Msp;
+      }
+      double DivRhoTr = DS(I, 0) * uc[0][J] * cc[0][J];
+      DivRhoTr += DS(J, 0) * vc[I][0] * cc[I][0];
+      double DivRho = DS(I, 0) * uc[0][J];
+      DivRho += DS(J, 0) * vc[I][0];
+      for (int k = 1; k < N; ++k) {
+        DivRhoTr += DS(I, k) * uc[k][J] * cc[k][J];
+        DivRhoTr += DS(J, k) * vc[I][k] * cc[I][k];
+        DivRho += DS(I, k) * uc[k][J];
+        DivRho += DS(J, k) * vc[I][k];
+      }
+      Jc[ID] = JJ(ID, 0, iz, IF) + JJ(ID, 1, iz, IF);
+      RhoTrS[ID] = Uv(U, iz, ind, v) + dt * DivRhoTr / Jc[ID];
+      RhoS[ID] = Uv(U, iz, ind, 0) + dt * DivRho / Jc[ID];
+    }
+  /* element-level tracer-mass residual of a shift lambda; q gets the clipped mixing ratios */
+#define RESID(lam, store)                                                                   \
+  do {                                                                                      \
+    res = 0.0;                                                                              \
+    for (int J = 0; J < N; ++J)                                                             \
+      for (int I = 0; I < N; ++I) {                                                         \
+        int ID = I + J * N;                                                                 \
+        double qq = median3(qMinS, RhoTrS[ID] / RhoS[ID] + (lam), qMaxS);                   \
+        if (store) q[ID] = qq;                                                              \
+        res += Jc[ID] * g->w[I] * g->w[J] / sumJ * (qq * RhoS[ID] - RhoTrS[ID]);            \
+      }                                                                                     \
+  } while (0)
+  double res, resp, resc = 0.0, alpha, lp = 0.0, lc = 0.0;
+  int conv = 1;
+  RESID(l0, 1);
+  resp = res;
+  if (fabs(resp) <= eta) {
+    conv = 0;
+  } else {
+    RESID(l0 + dlFD, 0);
+    resc = res;
+  }
+  if (conv) {
+    if (fabs(resc - resp) <= 1.e-13) {
+      conv = 0;
+    } else {
+      alpha = dlFD / (resc - resp);
+      lp = l0;
+      lc = lp - alpha * resp;
+      resp = 0.0;
+      resc = 0.0;
+    }
+  }
+  for (int iTer = 0; iTer < 8; ++iTer) {
+    if (!conv) break;
+    RESID(lc, 1);
+    resc = res;
+    if (fabs(resc - resp) <= 1.e-13) {
+      conv = 0;
+    } else {
+      alpha = (lp - lc) / (resp - resc);
+      resp = resc;
+      lp = lc;
+      lc = lc - alpha * resc;
+      resc = 0.0;
+    }
+  }
+#undef RESID
+  for (int J = 0; J < N; ++J)
+    for (int I = 0; I < N; ++I) {
+      int ID = I + J * N;
+      int64_t ind = GLOB(ID, IF);
+      double Ms = MM(iz, ind, 0) + MM(iz, ind, 1);
+      Uv(F, iz, ind, v) += (q[ID] * RhoS[ID] - Uv(U, iz, ind, v)) * Jc[ID] / dt / Ms;
+    }
+}
+
 /* FcnGPU! with NumTr tracers (no HorLimit): the tracer caches sit at
  * Temp1[:,:,6:5+NumTr] (FcnGPU.jl:115-126) */
 void cgdo_fcn(const cgdo_grid *g, double *F, const double *U, double *Temp1, const double *AuxG) {
@@ -583,13 +779,14 @@ void cgdo_fcn(const cgdo_grid *g, double *F, const double *U, double *Temp1, con
   memset(Temp1, 0, (size_t)nv * ng * sizeof(double));
   for (int IF = 0; IF < g->NF; ++IF)
     for (int iz = 0; iz < g->Nz; ++iz) hypervisc_element(g, IF, iz, 0, Temp1, U, NULL);
-  for (int t = 0; t < g->NumTr; ++t)
+  const int hv_tr = !g->HorLimit; /* the limited path has no tracer hyperviscosity (FcnGPU.jl:747-751,861-866) */
+  for (int t = 0; t < g->NumTr * hv_tr; ++t)
     for (int IF = 0; IF < g->NF; ++IF)
       for (int iz = 0; iz < g->Nz; ++iz) tracer_hv_element(g, IF, iz, 0, 5 + t, Temp1, U, NULL);
   memset(F, 0, (size_t)nv * ng * sizeof(double));
   for (int IF = 0; IF < g->NF; ++IF)
     for (int iz = 0; iz < g->Nz; ++iz) hypervisc_element(g, IF, iz, 1, F, U, Temp1);
-  for (int t = 0; t < g->NumTr; ++t)
+  for (int t = 0; t < g->NumTr * hv_tr; ++t)
     for (int IF = 0; IF < g->NF; ++IF)
       for (int iz = 0; iz < g->Nz; ++iz) tracer_hv_element(g, IF, iz, 1, 5 + t, F, U, Temp1);
   for (int IF = 0; IF < g->NF; ++IF)
@@ -598,9 +795,24 @@ void cgdo_fcn(const cgdo_grid *g, double *F, const double *U, double *Temp1, con
     for (int iz = 0; iz < g->Nz; ++iz) grad_element(g, IF, iz, F, U, AuxG);
   for (int IF = 0; IF < g->NF; ++IF)
     for (int iz = 0; iz < g->Nz; ++iz) div_element(g, IF, iz, F, U);
-  for (int t = 0; t < g->NumTr; ++t)
-    for (int IF = 0; IF < g->NF; ++IF)
-      for (int iz = 0; iz < g->Nz; ++iz) tracer_div_element(g, IF, iz, 5 + t, F, U);
+  if (g->HorLimit && g->NumTr > 0) {
+    /* LimitKernel! on the state (FcnGPU.jl:735-737), then the limited transport */
+    const size_t nb = (size_t)g->Nz * g->NF;
+    double *qmin = (double *)malloc(sizeof(double) * nb), *qmax = (double *)malloc(sizeof(double) * nb);
+    int *stencil = lim_stencil(g);
+    for (int t = 0; t < g->NumTr; ++t) {
+      lim_bounds(g, 5 + t, U, qmin, qmax);
+      for (int IF = 0; IF < g->NF; ++IF)
+        for (int iz = 0; iz < g->Nz; ++iz) tracer_lim_element(g, IF, iz, 5 + t, F, U, qmin, qmax, stencil);
+    }
+    free(qmin);
+    free(qmax);
+    free(stencil);
+  } else {
+    for (int t = 0; t < g->NumTr; ++t)
+      for (int IF = 0; IF < g->NF; ++IF)
+        for (int iz = 0; iz < g->Nz; ++iz) tracer_div_element(g, IF, iz, 5 + t, F, U);
+  }
   /* ForceKernel! then DampKernel! (FcnGPU.jl:953-966), pointwise */
   for (int ig = 0; ig < g->NumG; ++ig)
     for (int iz = 0; iz < g->Nz; ++iz) {
@@ -748,6 +960,8 @@ void cgdo_ssp_knoth(double a[3][3], double c[3][3], double m[3], double *gamma) 
 
 /* RosenbrockSchur! (RosenbrockSchur.jl:1-33).  Work arrays in cgdo_work. */
 void cgdo_rw_step(const cgdo_grid *g, double *V, double dt, cgdo_work *w) {
+  /* Global.TimeStepper.dtauStage = dt (RosenbrockSchur.jl:9) */
+  ((cgdo_grid *)g)->dtau = dt;
   double a[3][3], c[3][3], m[3], gamma;
   cgdo_ssp_knoth(a, c, m, &gamma);
   const size_t n5 = (size_t)(5 + g->NumTr) * (size_t)g->Nz * (size_t)g->NumG;

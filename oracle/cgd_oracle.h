@@ -28,6 +28,11 @@ typedef struct {
   const double *xS; /* (2, NumG) longitude, latitude */
   int State, RhoVPos, RhoCPos; /* State 1: Models.Moist, 1-based vapour / cloud components */
   double Cpv, Cpl, Rv;
+  /* HorLimit: the limited horizontal tracer transport (HorLimiterKernel.jl)
+   * with the N GLL weights w and the stage time step dtau (dtauStage) */
+  int HorLimit;
+  const double *w;
+  double dtau;
 } cgdo_grid;
 
 typedef struct {

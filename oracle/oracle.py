@@ -33,7 +33,8 @@ class _Grid(C.Structure):
                 ("HS_DeltaT_y", C.c_double), ("HS_DeltaTh_z", C.c_double), ("HS_T_equator", C.c_double),
                 ("HS_T_min", C.c_double), ("HS_sigma_b", C.c_double), ("xS", _dp),
                 ("State", C.c_int), ("RhoVPos", C.c_int), ("RhoCPos", C.c_int),
-                ("Cpv", C.c_double), ("Cpl", C.c_double), ("Rv", C.c_double)]
+                ("Cpv", C.c_double), ("Cpl", C.c_double), ("Rv", C.c_double),
+                ("HorLimit", C.c_int), ("w", _dp), ("dtau", C.c_double)]
 
 
 class _Work(C.Structure):
@@ -107,11 +108,14 @@ def _p(a: np.ndarray):
 class Oracle:
     """Binds one discretization (static inputs) to the oracle."""
 
-    def __init__(self, disc, phys=None, koeff=(0.0, 0.0, 0.0), NumTr=0, damping=None, forcing=None, moist=None):
+    def __init__(self, disc, phys=None, koeff=(0.0, 0.0, 0.0), NumTr=0, damping=None, forcing=None, moist=None,
+                 horlimit=False):
         """``damping``: (H, StrideDamp, Relax) of DampingW with zP computed
         from the metric; ``forcing``: dict of Held-Suarez constants (k_a,
         k_f, k_s, DeltaT_y, DeltaTh_z, T_equator, T_min, sigma_b) with the
-        node latitudes of the grid."""
+        node latitudes of the grid; ``horlimit``: the limited horizontal
+        tracer transport (its stage time step is ``dtau``, set by
+        ``rw_step`` to dt as RosenbrockSchur! does)."""
         from_phys = phys
         if from_phys is None:
             raise ValueError("phys constants required")
@@ -152,8 +156,22 @@ class Oracle:
             for k in ("k_a", "k_f", "k_s", "DeltaT_y", "DeltaTh_z", "T_equator", "T_min", "sigma_b"):
                 setattr(g, "HS_" + k, forcing[k])
             g.xS = _p(xS)
+        if horlimit:
+            wg = np.ascontiguousarray(disc.w, dtype=np.float64)
+            self._keep.append(wg)
+            g.HorLimit = 1
+            g.w = _p(wg)
         self.g = g
         self.L = lib()
+
+    @property
+    def dtau(self):
+        """Global.TimeStepper.dtauStage: the limiter's time step."""
+        return self.g.dtau
+
+    @dtau.setter
+    def dtau(self, v):
+        self.g.dtau = float(v)
 
     # shapes
     def _state(self, nv=5):

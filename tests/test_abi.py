@@ -67,7 +67,8 @@ def _desc(pkg, **over):
         (dict(NumTr=17), -2, "NumTr must be in"),
         (dict(NumTr=-1), -2, "NumTr must be in"),
         (dict(precision=7), -2, "precision"),
-        (dict(HorLimit=1), -2, "dry CompressibleShallow"),
+        (dict(HorLimit=1), -1, "HorLimit needs tracers"),
+        (dict(HorLimit=1, NumTr=1), -1, "GLL weights w"),
         (dict(Damping=1), -1, "Damping needs zP"),
         (dict(Damping=1, zP=16, DampH=3e4, DampStride=0.0), -1, "DampStride > 0"),
         (dict(Forcing=1), -1, "Forcing needs xS"),
@@ -76,7 +77,7 @@ def _desc(pkg, **over):
         (dict(State=1, NumTr=1, RhoVPos=6, RhoCPos=7, Cpv=1885.0, Cpl=4186.0, Rv=461.0), -1, "Moist needs RhoVPos"),
         (dict(State=1, NumTr=2, RhoVPos=6, RhoCPos=7), -1, "Cpv, Cpl, Rv"),
         (dict(Forcing=1, xS=16, HS_sigma_b=1.5), -1, "HS_sigma_b < 1"),
-        (dict(Microphysics=1), -2, "dry CompressibleShallow"),
+        (dict(Microphysics=1), -2, "only the CompressibleShallow path"),
         (dict(HyperDDivW=1e14), -2, "HyperDDivW"),
         (dict(dz=None), -1, "null static input"),
         (dict(NF=0), -1, "positive"),

@@ -574,3 +574,59 @@ def test_moist_eos(pkg, oracle_mod):
     assert np.allclose(Am, Ad, rtol=1e-13, atol=0)
     # vapour lowers kappa_m below kappa: the moist pressure differs
     assert np.abs(A[0] / oracle_mod.Oracle(disc, P).prepare(U[:5].copy())[0] - 1.0).max() > 1e-4
+
+
+# ---- HorLimit: LimitKernel! + DivRhoTrUpwind3LimKernel! (HorLimiterKernel.jl:2-196)
+def _lim_state(pkg, ne=4, nz=8):
+    D = pkg.discretization
+    disc = D.build(ne, 4, nz, 30000.0)
+    U = pkg.state.jet_state(disc, seed=0)
+    rng = np.random.default_rng(1)
+    U[1] += 5 * rng.standard_normal(U[1].shape)
+    U[3] += 0.2 * rng.standard_normal(U[3].shape)
+    U[3][:, -1] = 0.0
+    return disc, D.Phys(), U
+
+
+def test_horlimit_inactive_and_constant(pkg, oracle_mod):
+    """A horizontally uniform mixing ratio q(z) leaves nothing to limit: the
+    limited tendency equals the unlimited transport (whose tracer
+    hyperviscosity vanishes for it); a constant mixing ratio c0 gives
+    c0 * F_rho; the dynamics are untouched."""
+    disc, phys, U = _lim_state(pkg)
+    K = pkg.host.hyperdiffusion_for(4)
+    ol = oracle_mod.Oracle(disc, phys, (K, K, K), NumTr=1, horlimit=True)
+    ol.dtau = 150.0
+    ou = oracle_mod.Oracle(disc, phys, (K, K, K), NumTr=1)
+    qz = 1e-3 * (1 + np.linspace(0, 1, disc.nz))[None, :]
+    U2 = np.concatenate([U, (qz * U[0])[None]], 0)
+    Fl, _ = ol.fcn(U2.copy(), ol.prepare(U2.copy()))
+    Fu, _ = ou.fcn(U2.copy(), ou.prepare(U2.copy()))
+    assert np.linalg.norm(Fl[5] - Fu[5]) <= 1e-12 * np.linalg.norm(Fu[5])
+    assert np.array_equal(Fl[:5], Fu[:5])
+    U3 = np.concatenate([U, (1e-3 * U[0])[None]], 0)
+    Fl, _ = ol.fcn(U3.copy(), ol.prepare(U3.copy()))
+    assert np.linalg.norm(Fl[5] - 1e-3 * Fl[0]) <= 1e-9 * np.linalg.norm(1e-3 * Fl[0])
+
+
+def test_horlimit_bounds_a_step(pkg, oracle_mod):
+    """A step in the mixing ratio: an explicit update with the limited
+    tendency over dtau keeps q inside the initial range (to rounding) at
+    time steps where the unlimited transport overshoots."""
+    disc, phys, U = _lim_state(pkg)
+    K = pkg.host.hyperdiffusion_for(4)
+    ol = oracle_mod.Oracle(disc, phys, (K, K, K), NumTr=1, horlimit=True)
+    ou = oracle_mod.Oracle(disc, phys, (K, K, K), NumTr=1)
+    ll = oracle_mod.node_lonlat(disc)
+    q = np.where(np.abs(ll[:, 0]) < 0.8, 2e-3, 1e-3)[:, None] * np.ones((1, disc.nz))
+    U3 = np.concatenate([U, (q * U[0])[None]], 0)
+    over = []
+    for dt in (150.0, 600.0, 1200.0):
+        ol.dtau = dt
+        Fl, _ = ol.fcn(U3.copy(), ol.prepare(U3.copy()))
+        Fu, _ = ou.fcn(U3.copy(), ou.prepare(U3.copy()))
+        ql = (U3[5] + dt * Fl[5]) / (U3[0] + dt * Fl[0])
+        qu = (U3[5] + dt * Fu[5]) / (U3[0] + dt * Fu[0])
+        assert ql.min() >= 1e-3 * (1 - 1e-12) and ql.max() <= 2e-3 * (1 + 1e-12), dt
+        over.append(qu.min() < 1e-3 * (1 - 1e-6) or qu.max() > 2e-3 * (1 + 1e-6))
+    assert all(over)
