@@ -145,3 +145,49 @@ def test_horlimit_bounds_at_size(pkg, oracle_mod, torch_cuda, ne, nz, dtaus):
         qc = (U[7] + dtau * F[7]) / rho
         assert float((qc - 0.5).abs().max()) <= 1e-10, dtau
     torch.cuda.synchronize()
+
+
+def test_horlimit_composed_equals_fused(pkg, oracle_mod, torch_cuda):
+    """host.RosenbrockSchur through a Python tendency callback (the composed
+    call sequence, dtauStage = dt set by the stepper) equals the fused step."""
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, (FE, Metric, Global, Cache, JCache), _, U0 = setup(pkg, oracle_mod, 3, 20, 2, oracle=False)
+    eq = host.CompressibleShallow()
+    V1 = torch.as_tensor(U0, device="cuda").clone()
+    host.RosenbrockSchur(V1, 300.0, host.FcnGPU, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys, Cache,
+                         JCache, None, Global, None, eq)
+
+    def fcn(*a):
+        return host.FcnGPU(*a)
+
+    V2 = torch.as_tensor(U0, device="cuda").clone()
+    host.RosenbrockSchur(V2, 300.0, fcn, host.FcnPrepareGPU, host.JacSchurGPU, FE, Metric, phys, Cache, JCache,
+                         None, Global, None, eq)
+    torch.cuda.synchronize()
+    assert Global.dtauStage == 300.0
+    for c in range(7):
+        assert rel_l2(V2[c].cpu().numpy(), V1[c].cpu().numpy()) <= 1e-14, f"V{c + 1}"
+
+
+def test_horlimit_graph_replays_equal_eager_steps(pkg, oracle_mod, torch_cuda):
+    """Two captured steps with tracers and HorLimit replayed three times, with
+    eager steps and a synchronisation in between; each replay equals two
+    eager steps bit for bit."""
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, (FE, Metric, Global, Cache, JCache), _, U0 = setup(pkg, oracle_mod, 3, 20, 2, oracle=False)
+    ctx = host.init_context(FE, Metric, phys, Global)
+    G = torch.as_tensor(U0, device="cuda").clone()
+    E = G.clone()
+    ctx.graph_create(G, 150.0, Cache, JCache, nsteps=2)
+    torch.cuda.synchronize()
+    assert torch.equal(G, E), "capturing must not run the steps"
+    for rep in range(3):
+        ctx.graph_launch()
+        torch.cuda.synchronize()
+        Gr = G.clone()
+        host.TimeStepper(E, 150.0, 2, FE, Metric, phys, Cache, JCache, Global)
+        torch.cuda.synchronize()
+        assert torch.equal(Gr, E), f"replay {rep} differs from the eager steps"
+    assert torch.isfinite(G).all()
