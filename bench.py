@@ -267,6 +267,42 @@ def cpu_info():
     return model
 
 
+def cpu_baseline(pkg, O, disc, phys, U0, Global, args, Ne):
+    """The C oracle (the restated reference path; its results do not depend
+    on the thread count) timed on the host: args.cpu_steps RW steps of the
+    benchmark workload on every thread OpenMP is given (OMP_NUM_THREADS, else
+    every core) and on one thread, and the reference's CPU configuration
+    (ne=6, p=4, nz=10, 10 steps of 300 s) on both."""
+    nall = O.set_threads(0)
+
+    def timed(d, K, U, dt, steps, nt):
+        O.set_threads(nt)
+        orc = O.Oracle(d, phys, (K, K, K) if np.isscalar(K) else K)
+        V = U.copy()
+        t0 = time.perf_counter()
+        orc.rw_step(V, dt, steps)
+        return time.perf_counter() - t0
+
+    hd = (Global.Model.HyperDCurl, Global.Model.HyperDGrad, Global.Model.HyperDDiv)
+    tn = timed(disc, hd, U0, args.dt, args.cpu_steps, nall)
+    t1 = timed(disc, hd, U0, args.dt, args.cpu_steps, 1)
+    d6 = pkg.discretization.build(6, args.p, 10, 30000.0)
+    U6 = pkg.state.jet_state(d6, seed=0)
+    K6 = pkg.host.hyperdiffusion_for(6)
+    c1 = timed(d6, K6, U6, 300.0, 10, 1)
+    cn = timed(d6, K6, U6, 300.0, 10, nall)
+    O.set_threads(nall)
+    ne6 = d6.nf * d6.N * d6.N * d6.nz
+    rate = lambda pts, steps, t: round(pts * steps / t / 1e6, 4)  # noqa: E731
+    return {"value": rate(Ne, args.cpu_steps, tn), "unit": "Mpts/s", "cores": nall, "kind": "port",
+            "sample": f"{args.cpu_steps} RW step(s) of the C oracle (OpenMP, {nall} threads) on the same "
+                      f"workload ({tn:.1f}s; {cpu_info()}, {os.cpu_count()} logical CPUs visible)",
+            "single_core": {"value": rate(Ne, args.cpu_steps, t1), "cores": 1, "seconds": round(t1, 2)},
+            "cpu_config": {"workload": f"rw_step_fp64_ne6_p{args.p}_nz10, 10 steps, dt=300 s",
+                           "mpts_s_1_core": rate(ne6, 10, c1), f"mpts_s_{nall}_threads": rate(ne6, 10, cn),
+                           "seconds_1_core": round(c1, 3), f"seconds_{nall}_threads": round(cn, 3)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -420,15 +456,8 @@ def main():
 
     cpu = None
     if args.cpu_steps > 0 and not args.ntr:
-        O = ge.load_oracle()
-        orc = O.Oracle(disc, phys, (Global.Model.HyperDCurl, Global.Model.HyperDGrad, Global.Model.HyperDDiv))
-        Vo = U0.copy()
-        t0 = time.perf_counter()
-        orc.rw_step(Vo, args.dt, args.cpu_steps)
-        ct = time.perf_counter() - t0
-        cpu = {"value": round(Ne * args.cpu_steps / ct / 1e6, 4), "unit": "Mpts/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} RW step(s) of the C oracle on the same workload "
-                         f"({ct:.1f}s, 1 thread, {cpu_info()})"}
+        cpu = cpu_baseline(pkg, ge.load_oracle(), disc, phys, U0, Global, args, Ne)
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
     out = {"metric": "RW step Mpts/s + % HBM roofline, FP64 vs FP32-mixed at stated err",
            "value": round(mpts, 2), "unit": "Mpts/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -6,8 +6,14 @@
  * path and the timed CPU baseline; the product never links or calls it.
  *
  * It restates the reference kernels one by one, with the atomic DSS
- * scatter-adds replaced by serial adds in ascending (element, level,
- * node) order:
+ * scatter-adds replaced by adds in a fixed order: the elements are coloured
+ * greedily (ascending element number, smallest colour not taken by an
+ * element sharing a node), the colours run one after the other and the
+ * elements of a colour in parallel (OpenMP), each element over its levels
+ * and nodes in ascending order.  No two elements of a colour share a node,
+ * so every global node receives its contributions in (colour, element,
+ * level) order and the results do not depend on the thread count.  Columns
+ * and pointwise loops run in parallel too.  The restated kernels:
  *   PressureKernel!                  src/GPU/DiagnosticKernel.jl:96-115
  *   Dry EOS / dPresdRhoTh            src/Models/Equation.jl:25-38,79
  *   Moist EOS (Models.Moist)         src/Models/Equation.jl:56-75
@@ -45,8 +51,12 @@
 #include "cgd_oracle.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define NN (g->N * g->N)
 #define Uv(U, iz, ig, v) (U)[(size_t)(iz) + (size_t)g->Nz * ((size_t)(ig) + (size_t)g->NumG * (size_t)(v))]
@@ -127,6 +137,7 @@ static inline double dpres_drhoth(const cgdo_grid *g, double rhoth) {
 
 void cgdo_prepare(const cgdo_grid *g, const double *U, double *AuxG) {
   const int Nz = g->Nz;
+#pragma omp parallel for schedule(static)
   for (int ig = 0; ig < g->NumG; ++ig)
     for (int iz = 0; iz < Nz; ++iz) {
       double rho = Uv(U, iz, ig, 0), rhoth = Uv(U, iz, ig, 4);
@@ -592,6 +603,87 @@ static double median3(double a1, double a2, double a3) {
   return a2;
 }
 
+/* node -> elements (CSR over the global nodes, elements ascending) */
+static void node_elements(const cgdo_grid *g, int **ptr, int **els) {
+  int *cnt = (int *)calloc((size_t)g->NumG + 1, sizeof(int));
+  for (int IF = 0; IF < g->NF; ++IF)
+    for (int ID = 0; ID < NN; ++ID) cnt[GLOB(ID, IF) + 1]++;
+  for (int i = 0; i < g->NumG; ++i) cnt[i + 1] += cnt[i];
+  int *e = (int *)malloc(sizeof(int) * (size_t)(cnt[g->NumG] > 0 ? cnt[g->NumG] : 1));
+  int *fill = (int *)malloc(sizeof(int) * (size_t)(g->NumG > 0 ? g->NumG : 1));
+  memcpy(fill, cnt, sizeof(int) * (size_t)g->NumG);
+  for (int IF = 0; IF < g->NF; ++IF)
+    for (int ID = 0; ID < NN; ++ID) e[fill[GLOB(ID, IF)]++] = IF;
+  free(fill);
+  *ptr = cnt;
+  *els = e;
+}
+
+/* The DSS order: a greedy colouring of the elements (ascending element
+ * number, the smallest colour that no element sharing a node with it has);
+ * order[] lists the elements by (colour, element), colour c is
+ * order[off[c] .. off[c+1]). */
+typedef struct {
+  int ncol;
+  int *order, *off;
+} cgdo_colouring;
+
+static cgdo_colouring colour_elements(const cgdo_grid *g) {
+  const int NF = g->NF;
+  int *ptr, *els;
+  node_elements(g, &ptr, &els);
+  int *col = (int *)malloc(sizeof(int) * (size_t)(NF > 0 ? NF : 1));
+  int ncol = 0;
+  for (int IF = 0; IF < NF; ++IF) {
+    uint64_t used = 0;
+    for (int ID = 0; ID < NN; ++ID) {
+      const int64_t gn = GLOB(ID, IF);
+      for (int q = ptr[gn]; q < ptr[gn + 1]; ++q)
+        if (els[q] < IF) used |= (uint64_t)1 << col[els[q]];
+    }
+    int c = 0;
+    while (c < 63 && ((used >> c) & 1)) ++c;
+    col[IF] = c;
+    if (c + 1 > ncol) ncol = c + 1;
+  }
+  cgdo_colouring C;
+  C.ncol = ncol;
+  C.off = (int *)calloc((size_t)ncol + 1, sizeof(int));
+  C.order = (int *)malloc(sizeof(int) * (size_t)(NF > 0 ? NF : 1));
+  for (int IF = 0; IF < NF; ++IF) C.off[col[IF] + 1]++;
+  for (int c = 0; c < ncol; ++c) C.off[c + 1] += C.off[c];
+  int *fill = (int *)malloc(sizeof(int) * (size_t)(ncol > 0 ? ncol : 1));
+  memcpy(fill, C.off, sizeof(int) * (size_t)ncol);
+  for (int IF = 0; IF < NF; ++IF) C.order[fill[col[IF]]++] = IF;
+  free(fill);
+  free(col);
+  free(ptr);
+  free(els);
+  return C;
+}
+
+/* every element of the colouring, colour by colour, the elements of a
+ * colour in parallel; BODY sees the element as IF */
+#define FOR_ELEMENTS(C, BODY)                                            \
+  for (int c_ = 0; c_ < (C).ncol; ++c_) {                                \
+    _Pragma("omp parallel for schedule(static)")                         \
+    for (int k_ = (C).off[c_]; k_ < (C).off[c_ + 1]; ++k_) {             \
+      const int IF = (C).order[k_];                                      \
+      BODY                                                               \
+    }                                                                    \
+  }
+
+/* threads of the parallel loops (1 = serial); returns the number in use */
+int cgdo_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 /* FE.Stencil (NumberingFem.jl:133-157): the element and the elements that
  * share a grid node with it, padded with the element itself to 13 entries;
  * on a conforming quadrilateral mesh these are the elements sharing a GLL
@@ -599,15 +691,8 @@ static double median3(double a1, double a2, double a3) {
 static int *lim_stencil(const cgdo_grid *g) {
   const int NF = g->NF;
   int *st = (int *)malloc(sizeof(int) * 13 * (size_t)NF);
-  int *cnt = (int *)calloc((size_t)g->NumG + 1, sizeof(int));
-  for (int IF = 0; IF < NF; ++IF)
-    for (int ID = 0; ID < NN; ++ID) cnt[GLOB(ID, IF) + 1]++;
-  for (int i = 0; i < g->NumG; ++i) cnt[i + 1] += cnt[i];
-  int *els = (int *)malloc(sizeof(int) * (size_t)cnt[g->NumG]);
-  int *fill = (int *)malloc(sizeof(int) * (size_t)g->NumG);
-  memcpy(fill, cnt, sizeof(int) * (size_t)g->NumG);
-  for (int IF = 0; IF < NF; ++IF)
-    for (int ID = 0; ID < NN; ++ID) els[fill[GLOB(ID, IF)]++] = IF;
+  int *cnt, *els;
+  node_elements(g, &cnt, &els);
   for (int IF = 0; IF < NF; ++IF) {
     int *s = st + 13 * (size_t)IF, n = 1;
     for (int k = 0; k < 13; ++k) s[k] = IF;
@@ -622,13 +707,13 @@ static int *lim_stencil(const cgdo_grid *g) {
   }
   free(cnt);
   free(els);
-  free(fill);
   return st;
 }
 
 /* LimitKernel!: qmin/qmax[iz + Nz*IF] = min/max over the element's nodes of
  * the mixing ratio of component v */
 static void lim_bounds(const cgdo_grid *g, int v, const double *U, double *qmin, double *qmax) {
+#pragma omp parallel for schedule(static)
   for (int IF = 0; IF < g->NF; ++IF)
     for (int iz = 0; iz < g->Nz; ++iz) {
       double mn = 1.0 / 0.0, mx = -1.0 / 0.0;
@@ -776,25 +861,19 @@ static void tracer_lim_element(const cgdo_grid *g, int IF, int iz, int v, double
 void cgdo_fcn(const cgdo_grid *g, double *F, const double *U, double *Temp1, const double *AuxG) {
   const size_t ng = (size_t)g->Nz * (size_t)g->NumG;
   const int nv = 5 + g->NumTr;
+  const cgdo_colouring C = colour_elements(g);
   memset(Temp1, 0, (size_t)nv * ng * sizeof(double));
-  for (int IF = 0; IF < g->NF; ++IF)
-    for (int iz = 0; iz < g->Nz; ++iz) hypervisc_element(g, IF, iz, 0, Temp1, U, NULL);
+  FOR_ELEMENTS(C, for (int iz = 0; iz < g->Nz; ++iz) hypervisc_element(g, IF, iz, 0, Temp1, U, NULL);)
   const int hv_tr = !g->HorLimit; /* the limited path has no tracer hyperviscosity (FcnGPU.jl:747-751,861-866) */
   for (int t = 0; t < g->NumTr * hv_tr; ++t)
-    for (int IF = 0; IF < g->NF; ++IF)
-      for (int iz = 0; iz < g->Nz; ++iz) tracer_hv_element(g, IF, iz, 0, 5 + t, Temp1, U, NULL);
+    FOR_ELEMENTS(C, for (int iz = 0; iz < g->Nz; ++iz) tracer_hv_element(g, IF, iz, 0, 5 + t, Temp1, U, NULL);)
   memset(F, 0, (size_t)nv * ng * sizeof(double));
-  for (int IF = 0; IF < g->NF; ++IF)
-    for (int iz = 0; iz < g->Nz; ++iz) hypervisc_element(g, IF, iz, 1, F, U, Temp1);
+  FOR_ELEMENTS(C, for (int iz = 0; iz < g->Nz; ++iz) hypervisc_element(g, IF, iz, 1, F, U, Temp1);)
   for (int t = 0; t < g->NumTr * hv_tr; ++t)
-    for (int IF = 0; IF < g->NF; ++IF)
-      for (int iz = 0; iz < g->Nz; ++iz) tracer_hv_element(g, IF, iz, 1, 5 + t, F, U, Temp1);
-  for (int IF = 0; IF < g->NF; ++IF)
-    for (int iz = 0; iz < g->Nz; ++iz) momentum_element(g, IF, iz, F, U);
-  for (int IF = 0; IF < g->NF; ++IF)
-    for (int iz = 0; iz < g->Nz; ++iz) grad_element(g, IF, iz, F, U, AuxG);
-  for (int IF = 0; IF < g->NF; ++IF)
-    for (int iz = 0; iz < g->Nz; ++iz) div_element(g, IF, iz, F, U);
+    FOR_ELEMENTS(C, for (int iz = 0; iz < g->Nz; ++iz) tracer_hv_element(g, IF, iz, 1, 5 + t, F, U, Temp1);)
+  FOR_ELEMENTS(C, for (int iz = 0; iz < g->Nz; ++iz) momentum_element(g, IF, iz, F, U);)
+  FOR_ELEMENTS(C, for (int iz = 0; iz < g->Nz; ++iz) grad_element(g, IF, iz, F, U, AuxG);)
+  FOR_ELEMENTS(C, for (int iz = 0; iz < g->Nz; ++iz) div_element(g, IF, iz, F, U);)
   if (g->HorLimit && g->NumTr > 0) {
     /* LimitKernel! on the state (FcnGPU.jl:735-737), then the limited transport */
     const size_t nb = (size_t)g->Nz * g->NF;
@@ -802,18 +881,20 @@ void cgdo_fcn(const cgdo_grid *g, double *F, const double *U, double *Temp1, con
     int *stencil = lim_stencil(g);
     for (int t = 0; t < g->NumTr; ++t) {
       lim_bounds(g, 5 + t, U, qmin, qmax);
-      for (int IF = 0; IF < g->NF; ++IF)
-        for (int iz = 0; iz < g->Nz; ++iz) tracer_lim_element(g, IF, iz, 5 + t, F, U, qmin, qmax, stencil);
+      FOR_ELEMENTS(C, for (int iz = 0; iz < g->Nz; ++iz)
+                          tracer_lim_element(g, IF, iz, 5 + t, F, U, qmin, qmax, stencil);)
     }
     free(qmin);
     free(qmax);
     free(stencil);
   } else {
     for (int t = 0; t < g->NumTr; ++t)
-      for (int IF = 0; IF < g->NF; ++IF)
-        for (int iz = 0; iz < g->Nz; ++iz) tracer_div_element(g, IF, iz, 5 + t, F, U);
+      FOR_ELEMENTS(C, for (int iz = 0; iz < g->Nz; ++iz) tracer_div_element(g, IF, iz, 5 + t, F, U);)
   }
+  free(C.order);
+  free(C.off);
   /* ForceKernel! then DampKernel! (FcnGPU.jl:953-966), pointwise */
+#pragma omp parallel for schedule(static)
   for (int ig = 0; ig < g->NumG; ++ig)
     for (int iz = 0; iz < g->Nz; ++iz) {
       if (g->Forcing) {
@@ -847,6 +928,7 @@ void cgdo_fcn(const cgdo_grid *g, double *F, const double *U, double *Temp1, con
 void cgdo_jac(const cgdo_grid *g, double *JRhoW, double *JWRho, double *JWRhoTh, double *JRhoThW, const double *U,
               const double *AuxG) {
   const int Nz = g->Nz;
+#pragma omp parallel for schedule(static)
   for (int ig = 0; ig < g->NumG; ++ig)
     for (int iz = 0; iz < Nz - 1; ++iz) {
       size_t o = 2 * ((size_t)iz + (size_t)(Nz - 1) * (size_t)ig);
@@ -876,6 +958,7 @@ void cgdo_schur(const cgdo_grid *g, double *k, double *v, const double *JRhoW, c
 #define TR(c, i, ig) tri[(size_t)(c) + 3 * ((size_t)(i) + (size_t)n * (size_t)(ig))]
   if (comp_tri) {
     double invfac = 1.0 / fac, invfac2 = invfac / fac;
+#pragma omp parallel for schedule(static)
     for (int ig = 0; ig < g->NumG; ++ig)
       for (int i = 0; i < n; ++i) {
         double d = invfac2 - JB(JWRho, 1, i, ig) * JB(JRhoW, 0, i, ig) - JB(JWRho, 0, i, ig) * JB(JRhoW, 1, i, ig) -
@@ -889,8 +972,11 @@ void cgdo_schur(const cgdo_grid *g, double *k, double *v, const double *JRhoW, c
                                           JB(JWRhoTh, 1, i + 1, ig) * JB(JRhoThW, 1, i, ig);
       }
   }
+#pragma omp parallel
+  {
   double *x = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
   double *b = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+#pragma omp for schedule(static)
   for (int ig = 0; ig < g->NumG; ++ig) {
     /* SchurSolveFKernel! */
     for (int iz = 0; iz < Nz; ++iz) {
@@ -933,6 +1019,7 @@ void cgdo_schur(const cgdo_grid *g, double *k, double *v, const double *JRhoW, c
   }
   free(x);
   free(b);
+  }
 #undef JB
 #undef TR
 }
@@ -970,7 +1057,9 @@ void cgdo_rw_step(const cgdo_grid *g, double *V, double dt, cgdo_work *w) {
     memcpy(V, w->Vn, n5 * sizeof(double));
     for (int js = 0; js < is; ++js) {
       const double *kj = w->k + (size_t)js * n5;
-      for (size_t i = 0; i < n5; ++i) V[i] = V[i] + a[is][js] * kj[i];
+      const double aij = a[is][js];
+#pragma omp parallel for schedule(static)
+      for (size_t i = 0; i < n5; ++i) V[i] = V[i] + aij * kj[i];
     }
     cgdo_prepare(g, V, w->AuxG);
     cgdo_fcn(g, w->fV, V, w->Temp1, w->AuxG);
@@ -978,7 +1067,9 @@ void cgdo_rw_step(const cgdo_grid *g, double *V, double dt, cgdo_work *w) {
     for (int js = 0; js < is; ++js) {
       double fac = c[is][js] / dt;
       const double *kj = w->k + (size_t)js * n5;
-      for (size_t i = 0; i < n5; ++i) w->fV[i] = w->fV[i] + fac * kj[i];
+      double *fV = w->fV;
+#pragma omp parallel for schedule(static)
+      for (size_t i = 0; i < n5; ++i) fV[i] = fV[i] + fac * kj[i];
     }
     cgdo_schur(g, w->k + (size_t)is * n5, w->fV, w->JRhoW, w->JWRho, w->JWRhoTh, w->JRhoThW, w->tri, dt * gamma,
                is == 0);
@@ -986,6 +1077,8 @@ void cgdo_rw_step(const cgdo_grid *g, double *V, double dt, cgdo_work *w) {
   memcpy(V, w->Vn, n5 * sizeof(double));
   for (int is = 0; is < 3; ++is) {
     const double *ki = w->k + (size_t)is * n5;
-    for (size_t i = 0; i < n5; ++i) V[i] = V[i] + m[is] * ki[i];
+    const double mi = m[is];
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n5; ++i) V[i] = V[i] + mi * ki[i];
   }
 }

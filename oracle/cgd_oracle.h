@@ -48,5 +48,9 @@ void cgdo_schur(const cgdo_grid *g, double *k, double *v, const double *JRhoW, c
                 const double *JWRhoTh, const double *JRhoThW, double *tri, double fac, int comp_tri);
 void cgdo_ssp_knoth(double a[3][3], double c[3][3], double m[3], double *gamma);
 void cgdo_rw_step(const cgdo_grid *g, double *V, double dt, cgdo_work *w);
+/* threads of the parallel loops (n <= 0 keeps the current number); returns
+ * the number in use (1 in a build without OpenMP).  Results do not depend
+ * on it. */
+int cgdo_set_threads(int n);
 
 #endif

@@ -53,7 +53,17 @@ def lib():
             build()
         _lib = C.CDLL(_LIB_PATH)
         _lib.cgdo_ssp_knoth.argtypes = [_dp, _dp, _dp, _dp]
+        _lib.cgdo_set_threads.argtypes = [C.c_int]
+        _lib.cgdo_set_threads.restype = C.c_int
     return _lib
+
+
+def set_threads(n: int = 0) -> int:
+    """Threads of the oracle's parallel loops (n <= 0 keeps the current
+    number; OpenMP's default is OMP_NUM_THREADS or every core).  Returns the
+    number in use.  The results do not depend on it: the DSS adds run in a
+    fixed (colour, element) order."""
+    return int(lib().cgdo_set_threads(int(n)))
 
 
 _TL_PATH = os.path.join(_HERE, "libcgd_oracle_tl.so")

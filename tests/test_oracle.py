@@ -630,3 +630,27 @@ def test_horlimit_bounds_a_step(pkg, oracle_mod):
         assert ql.min() >= 1e-3 * (1 - 1e-12) and ql.max() <= 2e-3 * (1 + 1e-12), dt
         over.append(qu.min() < 1e-3 * (1 - 1e-6) or qu.max() > 2e-3 * (1 + 1e-6))
     assert all(over)
+
+
+def test_oracle_threads_do_not_change_results(pkg, oracle_mod):
+    """The DSS adds run in a fixed (colour, element) order, so the tendency
+    (with tracers and HorLimit) and the steps are bit-identical at 1 and at
+    4 threads."""
+    disc, phys, U = _lim_state(pkg, ne=3, nz=6)
+    U2 = np.concatenate([U, (1e-3 * (1 + 0.3 * np.sign(U[1])) * U[0])[None]], 0)
+    K = pkg.host.hyperdiffusion_for(3)
+    prev = oracle_mod.set_threads(0)
+    try:
+        out = []
+        for nt in (1, 4):
+            assert oracle_mod.set_threads(nt) in (1, nt)
+            orc = oracle_mod.Oracle(disc, phys, (K, K, K), NumTr=1, horlimit=True)
+            orc.dtau = 300.0
+            F, T1 = orc.fcn(U2.copy(), orc.prepare(U2.copy()))
+            V = U2.copy()
+            orc.rw_step(V, 300.0, 2)
+            out.append((F, T1, V))
+        for a, b in zip(*out):
+            assert np.array_equal(a, b)
+    finally:
+        oracle_mod.set_threads(prev)
