@@ -785,7 +785,7 @@ static void tr2_t(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const doubl
 
 template <int N, int NZT, bool CM, typename R>
 static void tr2lim_t(cgd_ctx *c, int nitems, hipStream_t st, int slot0, const double *U, double *F) {
-  const size_t lds = sizeof(double) * (c->NN + 4 * c->NN * NZT + 9 * NZT);
+  const size_t lds = sizeof(double) * (c->NN + 7 * c->NN * NZT);
   hipLaunchKernelGGL((k_tr2_lim<N, NZT, CM>), dim3(nitems), dim3(c->block), lds, st, c->ka, slot0, c->nchunk, c->ntr,
                      U, c->d_stencil, c->NF, c->d_qmin, c->d_qmax, c->dtau, F);
 }

@@ -115,6 +115,14 @@ template <typename T> __device__ __forceinline__ const T &at(const T *base, unsi
   return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + i * (unsigned)sizeof(T));
 }
 
+// The value of x in lane `lane` (uniform), on every lane.
+__device__ __forceinline__ double readlane_d(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // RecU4 (src/GPU/OperatorKernel.jl:1044-1059): the J-weighted limited
 // reconstruction of c at the face between cL and cR.  The weights
 //   kR = JL/(JL+JR) * (JLL+JL)/(JLL+JL+JR),  kL = JL/(JLL+JL) * JR/(JLL+JL+JR)
@@ -1354,28 +1362,48 @@ __device__ __forceinline__ double median3(double a1, double a2, double a3) {
   return a2;
 }
 
+// The sum of x over the 32 lanes of each half of the wavefront, on every
+// lane of the half: each 16-lane row pairwise at strides 8, 4, 2, 1 (lane i
+// adds lane i + s, DPP row shifts), then row 0 + row 1 of the half.  This is
+// the limiter's level-sum order, which the oracle restates.
+__device__ __forceinline__ double half_tree_sum(double x) {
+  x = x + dpp_from_above<8>(x);
+  x = x + dpp_from_above<4>(x);
+  x = x + dpp_from_above<2>(x);
+  x = x + dpp_from_above<1>(x);
+  const double h0 = readlane_d(x, 0) + readlane_d(x, 16), h1 = readlane_d(x, 32) + readlane_d(x, 48);
+  return (threadIdx.x & 32) ? h1 : h0;
+}
+
 // DivRhoTrUpwind3LimKernel! (HorLimiterKernel.jl:24-196) for all tracers,
 // one launch per colour, one thread per (node, level) of an element and
 // level chunk: the upwind vertical flux through both interfaces of the
 // layer (as k_tr2), then the limited horizontal transport: the predicted
 // rho*q and rho after dtau, the mixing ratio clipped to the bounds of the
 // element's stencil after a level-wide shift lambda, lambda from the
-// reference's secant iteration on the element-level tracer mass.  The
-// level sums run over the nodes in node order (LDS), one thread per level
-// driving the iteration.  Divisions as in the reference (no reciprocals):
-// the iteration's thresholds are absolute.  The DSS mass is M1 + M2 (the
-// reference indexes its 3-D M with two indices here).
+// reference's secant iteration on the element-level tracer mass.
+// Per tracer, three phases and three barriers: (A) the node threads form
+// rho*q and rho after dtau and put them in LDS; (B) one wavefront per level
+// (lane = node) takes the stencil's bounds and runs the whole iteration,
+// every residual a sum over the lanes in node order (readlane), so the sums
+// and the iteration's decisions are those of the serial restatement, with
+// no barrier inside the iteration; (C) the node threads read the clipped q
+// back and write the tendency.  Divisions as in the reference (no
+// reciprocals): the iteration's thresholds are absolute.  The DSS mass is
+// M1 + M2 (the reference indexes its 3-D M with two indices here).
 template <int N, int NZT, bool CM>
 __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64)
     k_tr2_lim(KArgs a, int slot0, int nchunk, int ntr, const double *__restrict__ U, const int *__restrict__ stencil,
               int NF, const double *__restrict__ qmin, const double *__restrict__ qmax, double dtau,
               double *__restrict__ F) {
   constexpr int NN = N * N;
+  static_assert(NN <= 32, "a 32-lane half of a wavefront holds the nodes of an element level");
   extern __shared__ double smem[];
   double *Ds = smem;
-  // fields 0, 1: uc, vc; 2: c; 3: the node terms of a level sum
+  // tiles 0, 1: uc, vc; 2: c.  Then per (level, node), node fastest: J,
+  // and per tracer rho after dtau, rho*q after dtau and the clipped q
   Tiles<N, NZT, double> T{Ds + NN};
-  double *lv = Ds + NN + 4 * NN * NZT;  // per level: qMinS, qMaxS, sumJ, res, resp, resc, lp, lc, conv
+  double *wJ = Ds + NN + 3 * NN * NZT, *wS = wJ + NN * NZT, *wT = wS + NN * NZT, *wQ = wT + NN * NZT;
   if (threadIdx.x < NN) Ds[threadIdx.x] = a.D[threadIdx.x];
   const int Nz = a.Nz;
   const int L = xcd_logical(blockIdx.x, gridDim.x);
@@ -1386,14 +1414,13 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64)
   const int iz = chunk * NZT + l;
   const bool in_tile = node < NN;
   const bool valid = in_tile && iz < Nz;
-  const bool master = node == 0 && iz < Nz;  // drives the level's iteration
   const int I = node % N, J = node / N;
   const size_t vs = (size_t)a.NumG * Nz;
   const bool has_lo = iz > 0, has_hi = iz < Nz - 1;
   auto M = [&](int c, int Lv) { return met_at<NN, CM>(a, (size_t)e, c, node, Lv); };
   size_t ui = 0;
   bool first = false;
-  double rho = 1, u = 0, v = 0, S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1, Ms = 1, wgt = 0;
+  double rho = 1, u = 0, v = 0, S11 = 0, S12 = 0, S21 = 0, S22 = 0, Js = 1, Ms = 1;
   double wlo = 0, wup = 0, rho_m = 1, rho_mm = 1, rho_p = 1, rho_pp = 1;
   double Jm = 1, Jmm = 1, Jp = 1, Jpp = 1;
   if (valid) {
@@ -1409,7 +1436,6 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64)
     S22 = M(M221, iz) + M(M222, iz);
     Js = M(MJ1, iz) + M(MJ2, iz);
     Ms = a.M[ui] + a.M[vs + ui];
-    wgt = Js * a.wg[I] * a.wg[J];
     const int lm = has_lo ? iz - 1 : iz, lmm = iz >= 2 ? iz - 2 : 0;
     const int lp = has_hi ? iz + 1 : iz, lpp = iz + 2 <= Nz - 1 ? iz + 2 : Nz - 1;
     rho_m = U[ui - iz + lm];
@@ -1435,57 +1461,88 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64)
   if (in_tile) {
     T.at(0, node, l) = -rho * (S11 * u + S12 * v);  // Contra12(-rho, u, v)
     T.at(1, node, l) = -rho * (S21 * u + S22 * v);
+    wJ[l * NN + node] = Js;
   }
   __syncthreads();  // D and the tracer-independent tiles
-  // the level's sum of J
-  if (in_tile) T.at(3, node, l) = valid ? Js : 0.0;
-  __syncthreads();
-  if (master) {
-    double s = 0.0;
-    for (int k = 0; k < NN; ++k) s += T.at(3, k, l);
-    lv[2 * NZT + l] = s;
-  }
-  __syncthreads();
-  const double sumJ = iz < Nz ? lv[2 * NZT + l] : 1.0;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int ln = lane & 31, In = ln % N, Jn = ln / N;
   const double eta = 1.e-12, dlFD = 1.e-8, l0 = 0.0;
+  // the stencil's bounds of level lw: min / max over the 13 stencil slots
+  // (the same addresses on every lane of a half)
+  const int *st = stencil + 13 * (size_t)e;
+  auto bounds = [&](int tr, int izw, double &qMinS, double &qMaxS) {
+    const double *mn = qmin + (size_t)tr * NF * Nz, *mx = qmax + (size_t)tr * NF * Nz;
+    qMinS = mn[(size_t)st[0] * Nz + izw];
+    qMaxS = mx[(size_t)st[0] * Nz + izw];
+    for (int iS = 1; iS < 13; ++iS) {
+      qMinS = jmin(mn[(size_t)st[iS] * Nz + izw], qMinS);
+      qMaxS = jmax(mx[(size_t)st[iS] * Nz + izw], qMaxS);
+    }
+  };
+  // RecU4 weights of both interfaces, shared by all tracers
+  const RecW<double> Wup = recu4_weights(Jm, Js, Jp, Jpp);
+  const RecW<double> Wlo = recu4_weights(Jmm, Jm, Js, has_hi ? Jp : Js);
+  // a node's levels sit in one DPP row (or half of one), so the neighbour
+  // levels' mixing ratios and the lower interface's flux come from the
+  // neighbouring lanes; lanes at a chunk edge load or compute their own
+  const int lr = threadIdx.x & 15;
+  const bool edge_m1 = l < 1 || lr < 1, edge_m2 = l < 2 || lr < 2;
+  const bool edge_p1 = l + 1 >= NZT || lr + 1 > 15, edge_p2 = l + 2 >= NZT || lr + 2 > 15;
+  const int lw0 = 2 * wave + (lane >> 5), iz0 = chunk * NZT + lw0;  // this half's first level in phase B
+  const bool live0 = 2 * wave < NZT && lw0 < NZT && iz0 < Nz;
+  // this level's tracer density and DSS accumulator, one tracer ahead
+  double Trs_n = 0, o_n = 0;
+  if (valid && ntr > 0) {
+    Trs_n = U[5 * vs + ui];
+    if (!first) o_n = F[5 * vs + ui];
+  }
   for (int tr = 0; tr < ntr; ++tr) {
     const double *Tr = U + (size_t)(5 + tr) * vs;
     double *Ft = F + (size_t)(5 + tr) * vs;
-    double o = 0, Trs = 0, c = 0, f = 0;
+    // phase B's bounds of the first level pair, loaded ahead of phase A
+    double qMin0 = 0, qMax0 = 0;
+    if (live0) bounds(tr, iz0, qMin0, qMax0);
+    const double Trs = Trs_n, o = o_n;
+    if (valid && tr + 1 < ntr) {
+      Trs_n = Tr[vs + ui];
+      if (!first) o_n = Ft[vs + ui];
+    }
+    double c = 0, f = 0;
+    if (valid) c = Trs / rho;
+    double cm = dpp_from_below<1>(c), cmm = dpp_from_below<2>(c);
+    double cp = dpp_from_above<1>(c), cpp = dpp_from_above<2>(c);
     if (valid) {
-      if (!first) o = Ft[ui];
-      Trs = Tr[ui];
-      c = Trs / rho;
-      // upwind vertical fluxes through both interfaces (the unlimited kernel's)
-      const double cm = has_lo ? Tr[ui - 1] / rho_m : c;
-      const double cmm = iz >= 2 ? Tr[ui - 2] / rho_mm : cm;
-      const double cp = has_hi ? Tr[ui + 1] / rho_p : c;
-      const double cpp = iz + 2 <= Nz - 1 ? Tr[ui + 2] / rho_pp : cp;
-      if (has_lo) {
-        double cFL, cFR;
-        recu4(cmm, cm, c, has_hi ? cp : c, Jmm, Jm, Js, has_hi ? Jp : Js, cFL, cFR);
-        f += 0.25 * ((fabs(wlo) + wlo) * cFL + (-fabs(wlo) + wlo) * cFR) / Ms;
-      }
-      if (has_hi) {
-        double cFL, cFR;
-        recu4(cm, c, cp, cpp, Jm, Js, Jp, Jpp, cFL, cFR);
-        f += -(0.25 * ((fabs(wup) + wup) * cFL + (-fabs(wup) + wup) * cFR)) / Ms;
-      }
+      if (has_lo && edge_m1) cm = Tr[ui - 1] / rho_m;
+      if (iz >= 2 && edge_m2) cmm = Tr[ui - 2] / rho_mm;
+      if (has_hi && edge_p1) cp = Tr[ui + 1] / rho_p;
+      if (iz + 2 <= Nz - 1 && edge_p2) cpp = Tr[ui + 2] / rho_pp;
+    }
+    // clamped levels (the reference's max/min)
+    if (!has_lo) cm = c;
+    if (iz < 2) cmm = cm;
+    if (!has_hi) cp = c;
+    if (iz + 2 > Nz - 1) cpp = cp;
+    // upwind vertical flux through the upper interface; the lower one is the
+    // upper flux of the level below (the unlimited kernel's fluxes)
+    double Fup = 0;
+    if (valid && has_hi) {
+      double cFL, cFR;
+      recu4_apply(cm, c, cp, cpp, Wup, cFL, cFR);
+      Fup = 0.25 * ((fabs(wup) + wup) * cFL + (-fabs(wup) + wup) * cFR);
+    }
+    double Flo = dpp_from_below<1>(Fup);
+    if (valid && has_lo && edge_m1) {
+      double cFL, cFR;
+      recu4_apply(cmm, cm, c, has_hi ? cp : c, Wlo, cFL, cFR);
+      Flo = 0.25 * ((fabs(wlo) + wlo) * cFL + (-fabs(wlo) + wlo) * cFR);
+    }
+    if (valid) {
+      if (has_lo) f += Flo / Ms;
+      if (has_hi) f += -Fup / Ms;
     }
     if (in_tile) T.at(2, node, l) = c;
-    if (master) {  // the stencil's bounds of this level
-      const int *st = stencil + 13 * (size_t)e;
-      const double *mn = qmin + (size_t)tr * NF * Nz, *mx = qmax + (size_t)tr * NF * Nz;
-      double qn = mn[(size_t)st[0] * Nz + iz], qx = mx[(size_t)st[0] * Nz + iz];
-      for (int iS = 1; iS < 13; ++iS) {
-        qn = jmin(mn[(size_t)st[iS] * Nz + iz], qn);
-        qx = jmax(mx[(size_t)st[iS] * Nz + iz], qx);
-      }
-      lv[l] = qn;
-      lv[NZT + l] = qx;
-    }
-    __syncthreads();
-    double RhoTrS = 1, RhoS = 1, qMinS = 0, qMaxS = 0;
+    __syncthreads();  // (A) c of every node
+    double RhoS = 1;
     if (valid) {
       double DivRhoTr = Ds[I * N] * T.at(0, J * N, l) * T.at(2, J * N, l);
       DivRhoTr += Ds[J * N] * T.at(1, I, l) * T.at(2, I, l);
@@ -1498,71 +1555,71 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64)
         DivRho += Ds[I * N + k] * T.at(0, k + J * N, l);
         DivRho += Ds[J * N + k] * T.at(1, I + k * N, l);
       }
-      RhoTrS = Trs + dtau * DivRhoTr / Js;
       RhoS = rho + dtau * DivRho / Js;
-      qMinS = lv[l];
-      qMaxS = lv[NZT + l];
+      wS[l * NN + node] = RhoS;
+      wT[l * NN + node] = Trs + dtau * DivRhoTr / Js;
     }
-    const double pred = RhoTrS / RhoS;
-    // one level sum of wgt/sumJ (q RhoS - RhoTrS) for q = median(qMinS, pred + lam, qMaxS)
-    auto level_sum = [&](double lam, bool active, double &q) {
-      double term = 0.0;
-      if (valid && active) {
-        q = median3(qMinS, pred + lam, qMaxS);
-        term = wgt / sumJ * (q * RhoS - RhoTrS);
+    __syncthreads();  // (B) rho and rho*q after dtau of every node
+    // two levels per wavefront, one per 32-lane half (lane = node)
+    for (int pr = wave; pr < (NZT + 1) / 2; pr += nwave) {
+      const int lw = 2 * pr + (lane >> 5);
+      const int izw = chunk * NZT + lw;
+      const bool live = lw < NZT && izw < Nz;  // uniform in each half
+      const bool node_lane = live && ln < NN;
+      double qMinS = qMin0, qMaxS = qMax0, Jl = 0, RhoSl = 1, RhoTrSl = 0;
+      if (live && pr != wave) bounds(tr, izw, qMinS, qMaxS);
+      if (node_lane) {
+        Jl = wJ[lw * NN + ln];
+        RhoSl = wS[lw * NN + ln];
+        RhoTrSl = wT[lw * NN + ln];
       }
-      if (in_tile) T.at(3, node, l) = term;
-      __syncthreads();
-      if (master && active) {
-        double s = 0.0;
-        for (int k = 0; k < NN; ++k) s += T.at(3, k, l);
-        lv[3 * NZT + l] = s;
-      }
-      __syncthreads();
-    };
-    double q = 0.0, qfd = 0.0;
-    level_sum(l0, true, q);
-    if (master) {
-      lv[4 * NZT + l] = lv[3 * NZT + l];  // resp
-      lv[8 * NZT + l] = fabs(lv[3 * NZT + l]) <= eta ? 0.0 : 1.0;
-    }
-    __syncthreads();
-    const bool fd = iz < Nz && lv[8 * NZT + l] != 0.0;
-    level_sum(l0 + dlFD, fd, qfd);
-    if (master && fd) {
-      const double resp = lv[4 * NZT + l], resc = lv[3 * NZT + l];
-      if (fabs(resc - resp) <= 1.e-13) {
-        lv[8 * NZT + l] = 0.0;
-      } else {
-        const double alpha = dlFD / (resc - resp);
-        lv[6 * NZT + l] = l0;                   // lp
-        lv[7 * NZT + l] = l0 - alpha * resp;    // lc
-        lv[4 * NZT + l] = 0.0;                  // resp (reset as the reference does)
-      }
-    }
-    __syncthreads();
-    for (int it = 0; it < 8; ++it) {
-      const bool conv = iz < Nz && lv[8 * NZT + l] != 0.0;
-      const double lc = conv ? lv[7 * NZT + l] : 0.0;
-      level_sum(lc, conv, q);
-      if (master && conv) {
-        const double resp = lv[4 * NZT + l], resc = lv[3 * NZT + l], lp = lv[6 * NZT + l];
-        if (fabs(resc - resp) <= 1.e-13) {
-          lv[8 * NZT + l] = 0.0;
+      const double sumJ = half_tree_sum(Jl);
+      // the weight J w_I w_J / sum J of the reference's residual terms
+      const double wgs = Jl * a.wg[In] * a.wg[Jn] / sumJ, pred = RhoTrSl / RhoSl;
+      double q = 0.0;
+      // the level's tracer-mass residual of a shift lam (every lane takes
+      // part in the reduction); q takes the clipped mixing ratios on `store`
+      auto resid = [&](double lam, bool store) {
+        const double qq = median3(qMinS, pred + lam, qMaxS);
+        if (store) q = qq;
+        return half_tree_sum(node_lane ? wgs * (qq * RhoSl - RhoTrSl) : 0.0);
+      };
+      double resp = resid(l0, true), lp = 0.0, lc = 0.0;
+      bool conv = live && !(fabs(resp) <= eta);
+      const double resc0 = resid(l0 + dlFD, false);
+      if (conv) {
+        if (fabs(resc0 - resp) <= 1.e-13) {
+          conv = false;
         } else {
-          const double alpha = (lp - lc) / (resp - resc);
-          lv[4 * NZT + l] = resc;
-          lv[6 * NZT + l] = lc;
-          lv[7 * NZT + l] = lc - alpha * resc;
+          const double alpha = dlFD / (resc0 - resp);
+          lp = l0;
+          lc = lp - alpha * resp;
+          resp = 0.0;  // the reference resets it here
         }
       }
-      __syncthreads();
+      for (int it = 0; it < 8; ++it) {
+        if (__ballot(conv) == 0) break;  // uniform
+        const double resc = resid(lc, conv);
+        if (conv) {
+          if (fabs(resc - resp) <= 1.e-13) {
+            conv = false;
+          } else {
+            const double alpha = (lp - lc) / (resp - resc);
+            resp = resc;
+            lp = lc;
+            lc = lc - alpha * resc;
+          }
+        }
+      }
+      if (node_lane) wQ[lw * NN + ln] = q;
     }
+    __syncthreads();  // (C) the clipped q of every node
     if (valid) {
-      f += (q * RhoS - Trs) * Js / dtau / Ms;
+      f += (wQ[l * NN + node] * RhoS - Trs) * Js / dtau / Ms;
       Ft[ui] = first ? f : o + f;
     }
-    __syncthreads();  // the next tracer rewrites the tiles and level values
+    // the next tracer writes tile 2, wS and wT after barrier (A), and wQ
+    // after (B), which every lane reaches only after reading them here
   }
 }
 
@@ -1623,13 +1680,6 @@ __global__ void k_f4fix(const int *__restrict__ nptr, const int *__restrict__ nc
     for (int c = nptr[g]; c < nptr[g + 1]; ++c) s += F4x[(size_t)ncopy[c] * nchunk + ch];
     F[3 * vs + (size_t)g * Nz + iz - 1] += s;
   }
-}
-
-__device__ __forceinline__ double readlane_d(double x, int lane) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // Solution of the Schur tridiagonal system of one column, one row per lane

@@ -728,6 +728,22 @@ static void lim_bounds(const cgdo_grid *g, int v, const double *U, double *qmin,
     }
 }
 
+/* A level sum of the limiter (sum J and the tracer-mass residuals): the n
+ * node values padded with zeros to 32; nodes 0-15 and 16-31 each added
+ * pairwise at strides 8, 4, 2, 1 (t[i] = t[i] + t[i+s]), then the two sums
+ * added.  The reference adds them with atomics in no fixed order; this order
+ * is the one the GPU's wavefront reduction takes. */
+static double tree32(const double *v, int n) {
+  double t[32];
+  for (int i = 0; i < 32; ++i) t[i] = i < n ? v[i] : 0.0;
+  for (int st = 8; st >= 1; st /= 2)
+    for (int i = 0; i < st; ++i) {
+      t[i] = t[i] + t[i + st];
+      t[16 + i] = t[16 + i] + t[16 + i + st];
+    }
+  return t[0] + t[16];
+}
+
 /* DivRhoTrUpwind3LimKernel! at level iz of element IF for tracer component
  * v: the upwind vertical flux as in the unlimited kernel, then the
  * horizontal transport as a limited update: the predicted rho*q and rho
@@ -759,8 +775,10 @@ static void tracer_lim_element(const cgdo_grid *g, int IF, int iz, int v, double
     qMinS = jmin(qmin[(size_t)iz + (size_t)Nz * st[iS]], qMinS);
     qMaxS = jmax(qmax[(size_t)iz + (size_t)Nz * st[iS]], qMaxS);
   }
-  double sumJ = 0.0;
-  for (int ID = 0; ID < N * N; ++ID) sumJ += JJ(ID, 0, iz, IF) + JJ(ID, 1, iz, IF);
+  double Jn[MAXN * MAXN], terms[MAXN * MAXN];
+  if (N * N > 32) abort(); /* the level sums cover at most 32 nodes (OrdPoly <= 4) */
+  for (int ID = 0; ID < N * N; ++ID) Jn[ID] = JJ(ID, 0, iz, IF) + JJ(ID, 1, iz, IF);
+  const double sumJ = tree32(Jn, N * N);
   for (int J = 0; J < N; ++J)
     for (int I = 0; I < N; ++I) {
       int ID = I + J * N;
@@ -802,14 +820,14 @@ static void tracer_lim_element(const cgdo_grid *g, int IF, int iz, int v, double
   /* element-level tracer-mass residual of a shift lambda; q gets the clipped mixing ratios */
 #define RESID(lam, store)                                                                   \
   do {                                                                                      \
-    res = 0.0;                                                                              \
     for (int J = 0; J < N; ++J)                                                             \
       for (int I = 0; I < N; ++I) {                                                         \
         int ID = I + J * N;                                                                 \
         double qq = median3(qMinS, RhoTrS[ID] / RhoS[ID] + (lam), qMaxS);                   \
         if (store) q[ID] = qq;                                                              \
-        res += Jc[ID] * g->w[I] * g->w[J] / sumJ * (qq * RhoS[ID] - RhoTrS[ID]);            \
+        terms[ID] = Jc[ID] * g->w[I] * g->w[J] / sumJ * (qq * RhoS[ID] - RhoTrS[ID]);       \
       }                                                                                     \
+    res = tree32(terms, N * N);                                                             \
   } while (0)
   double res, resp, resc = 0.0, alpha, lp = 0.0, lc = 0.0;
   int conv = 1;
