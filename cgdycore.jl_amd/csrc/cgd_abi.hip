@@ -1234,7 +1234,7 @@ extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work 
   const int prof = c->prof;
   c->prof = 0;
   graph_free(c);
-  const size_t n5 = (size_t)5 * c->NumG * c->Nz;
+  const size_t n5 = (size_t)(5 + c->ntr) * c->NumG * c->Nz;  // every component, tracers included
   double *Vsave = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   cgd_launch_shape best = current_shape(c);
@@ -1244,15 +1244,14 @@ extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work 
     HIPCHK(c, hipEventCreate(&e0));
     HIPCHK(c, hipEventCreate(&e1));
     HIPCHK(c, hipMemcpyAsync(Vsave, V, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
-    auto try_shape = [&](const cgd_launch_shape &sh) -> int {
-      if (!shape_ok(c, sh)) return CGD_OK;
+    // per-step time of one shape: the fastest of three timings of reps steps
+    // (robust to clock ramps and other work on the device)
+    auto time_shape = [&](const cgd_launch_shape &sh, double &per) -> int {
       apply_shape(c, sh);
       int rr;
       HIPCHK(c, hipMemcpyAsync(V, Vsave, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
       if ((rr = cgd_rw_step(c, V, dt, w, stream))) return rr;  // warm-up
-      // the fastest of three timings of reps steps (robust to clock ramps
-      // and other work on the device)
-      double per = 1e300;
+      per = 1e300;
       for (int t = 0; t < 3; ++t) {
         HIPCHK(c, hipEventRecord(e0, st));
         for (int q = 0; q < reps; ++q)
@@ -1263,33 +1262,51 @@ extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work 
         HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
         per = std::min(per, (double)ms / reps);
       }
-      if (per < bms) {
-        bms = per;
-        best = sh;
-      }
+      return CGD_OK;
+    };
+    // the fastest of a set of shapes: two sweeps over all of them, each
+    // shape keeping its better time, so a drift of the device's speed during
+    // a sweep does not favour the shapes timed in its fast part
+    auto pick = [&](const std::vector<cgd_launch_shape> &cand) -> int {
+      std::vector<double> tmin(cand.size(), 1e300);
+      for (int sweep = 0; sweep < 2; ++sweep)
+        for (size_t i = 0; i < cand.size(); ++i) {
+          double per;
+          int rr;
+          if ((rr = time_shape(cand[i], per))) return rr;
+          tmin[i] = std::min(tmin[i], per);
+        }
+      for (size_t i = 0; i < cand.size(); ++i)
+        if (tmin[i] < bms) {
+          bms = tmin[i];
+          best = cand[i];
+        }
       return CGD_OK;
     };
     // the element kernels' shapes jointly (both passes' chunk heights and
     // the pass-1 items), then the column shape with the best of them
     int rr;
-    const cgd_launch_shape b0 = best;
+    std::vector<cgd_launch_shape> cand;
     for (int z2 : {8, 16})
       for (int z1 : {8, 16})
         for (int it : {1, 2}) {
-          cgd_launch_shape sh = b0;
+          cgd_launch_shape sh = best;
           sh.chunk_levels = z2;
           sh.pass1_chunk_levels = z1;
           sh.pass1_items = it;
-          if ((rr = try_shape(sh))) return rr;
+          if (shape_ok(c, sh)) cand.push_back(sh);
         }
+    if ((rr = pick(cand))) return rr;
     const cgd_launch_shape b1 = best;
+    cand.clear();
     for (int l : {64, 32})
       for (int b : {256, 128, 64}) {
         cgd_launch_shape sh = b1;
         sh.col_lanes = l;
         sh.col_block = b;
-        if ((rr = try_shape(sh))) return rr;
+        if (shape_ok(c, sh)) cand.push_back(sh);
       }
+    if ((rr = pick(cand))) return rr;
     return CGD_OK;
   };
   r = run();

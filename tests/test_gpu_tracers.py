@@ -142,3 +142,17 @@ def test_tracer_composed_equals_fused(pkg, oracle_mod, torch_cuda):
     torch.cuda.synchronize()
     for c in range(7):
         assert rel_l2(V2[c].cpu().numpy(), V1[c].cpu().numpy()) <= 1e-14, f"V{c + 1}"
+
+
+def test_autotune_restores_tracers(pkg, oracle_mod, torch_cuda):
+    """cgd_autotune times its shapes on V and puts every component back,
+    the tracers included."""
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, (FE, Metric, Global, Cache, JCache), orc, U0 = setup(pkg, oracle_mod, 3, 20, 2)
+    ctx = host.init_context(FE, Metric, phys, Global)
+    V = torch.as_tensor(U0, device="cuda").clone()
+    shape, ms = ctx.autotune(V, 300.0, Cache, JCache, reps=1)
+    torch.cuda.synchronize()
+    assert ms > 0
+    assert torch.equal(V, torch.as_tensor(U0, device="cuda"))
