@@ -183,8 +183,10 @@ def adjoint(pkg, args):
     dt = 250 or 300 s the step itself -- the C oracle's restatement of the
     reference as well as this path -- diverges within 12 steps at ne = 16:
     horizontal acoustics are explicit in this Rosenbrock-W).  The
-    gradient dJ/dV0 comes from one cgd_rw_adjoint call with a checkpoint
-    every 4 steps; it is checked against central differences
+    gradient dJ/dV0 comes from the forward with a checkpoint every 4 steps
+    (cgd_rw_forward_ckpt, which also gives the final state J needs) and the
+    reverse sweep from those checkpoints (cgd_rw_adjoint_ckpt); "ms_gradient"
+    times both.  It is checked against central differences
     (J(V0 + eps d) - J(V0 - eps d)) / (2 eps), eps = 1e-6, of the fused
     forward (cgd_rw_run) along 64 Gaussian directions d ~ N(0, 1) per state
     entry (numpy generators seeded 1000..1063; w at the model top, which
@@ -216,11 +218,12 @@ def adjoint(pkg, args):
     Vf = torch.empty_like(V0)
 
     def grad():
-        Vf_fwd = run(V0)  # the forward that a caller's rrule would have run
+        # the rrule's forward keeps its checkpoints; the pullback uses them
+        ctx.rw_forward_ckpt(V0, dt, nsteps, every, Vf, work)
         lam = torch.zeros_like(V0)
-        lam[4, :, 0] = 2.0 * Vf_fwd[4, :, 0] / Vf_fwd[0, :, 0]
-        lam[0, :, 0] = -(Vf_fwd[4, :, 0] / Vf_fwd[0, :, 0]) ** 2
-        ctx.rw_adjoint(V0, dt, nsteps, every, lam, Vfinal=Vf, work=work)
+        lam[4, :, 0] = 2.0 * Vf[4, :, 0] / Vf[0, :, 0]
+        lam[0, :, 0] = -(Vf[4, :, 0] / Vf[0, :, 0]) ** 2
+        ctx.rw_adjoint_ckpt(dt, nsteps, every, lam, work)
         return lam
 
     grad()

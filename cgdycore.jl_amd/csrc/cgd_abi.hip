@@ -40,6 +40,11 @@ struct cgd_ctx {
   // [tracer][slot][Nz], and the stage time step (dtauStage)
   int horlimit = 0;
   double dtau = 0.0;
+  // the checkpoints that cgd_rw_forward_ckpt left in a work array, for the
+  // reverse sweep of cgd_rw_adjoint_ckpt (nullptr: none)
+  const double *ck_work = nullptr;
+  double ck_dt = 0.0;
+  int ck_nsteps = 0, ck_every = 0;
   int *d_stencil = nullptr;
   double *d_qmin = nullptr, *d_qmax = nullptr;
   // batch of grids (cgd_ctx_create_batch): member m owns elements

@@ -1284,16 +1284,8 @@ extern "C" int cgd_rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, doub
 // composed calls to rounding); each step's reverse recomputes its stages
 // with the composed calls.  lamV: in, the cotangent of the final state; out, that
 // of V0.  Vfinal (optional) receives the final state.
-extern "C" int cgd_rw_adjoint(cgd_ctx *c, const double *V0, double dt, int nsteps, int every, double *lamV,
-                              double *Vfinal, double *work, void *stream) {
-  int r = rw_args(c, "cgd_rw_adjoint", V0, dt, lamV, work);
-  if (r) return r;
-  if (nsteps < 1 || every < 1) return fail(c, CGD_ERR_ARG, "cgd_rw_adjoint: nsteps and every must be positive");
-  hipStream_t st = S(stream);
-  const AdjLayout L = adj_layout(c, nsteps, every);
-  const size_t n5 = L.n5;
-  if ((r = adj_zero(c, work + L.AuxG + 3 * L.ng, L.ng, st))) return r;
-  // the forward step runs on the composed path with these buffers
+// The forward buffers of the checkpoint runs inside the adjoint work array.
+static cgd_rw_work ckpt_fw(const AdjLayout &L, double *work) {
   cgd_rw_work fw{};
   fw.Vn = work + L.Vi;
   fw.fV = work + L.fV;
@@ -1305,37 +1297,102 @@ extern "C" int cgd_rw_adjoint(cgd_ctx *c, const double *V0, double dt, int nstep
   fw.JWRhoTh = work + L.Jb + 2 * L.nj;
   fw.JRhoThW = work + L.Jb + 3 * L.nj;
   fw.tri = work + L.tri;
+  return fw;
+}
+
+// nst steps of V in place: the fused FP64 step (the composed calls when the
+// context steps in mixed precision, or for tall columns inside rw_run)
+static int ckpt_steps(cgd_ctx *c, double *V, double dt, const cgd_rw_work &fw, int nst, hipStream_t st) {
+  if (!c->mixed) return rw_run(c, V, dt, &fw, nst, st);
+  for (int q = 0; q < nst; ++q)
+    if (int e = rw_composed_step(c, V, dt, &fw, st)) return e;
+  return CGD_OK;
+}
+
+// The forward half: nsteps steps from V0, a checkpoint at the start of every
+// segment of `every` steps kept in work, the final state in Vfinal.
+static int ckpt_forward(cgd_ctx *c, const double *V0, double dt, int nsteps, int every, double *Vfinal, double *work,
+                        hipStream_t st) {
+  const AdjLayout L = adj_layout(c, nsteps, every);
+  const size_t n5 = L.n5;
+  int r;
+  if ((r = adj_zero(c, work + L.AuxG + 3 * L.ng, L.ng, st))) return r;
+  const cgd_rw_work fw = ckpt_fw(L, work);
   const int nseg = (nsteps + every - 1) / every;
   double *ck = work + L.ck, *seg = work + L.seg;
-  // the checkpoint runs: the fused FP64 step (the composed calls when the
-  // context steps in mixed precision, or for tall columns inside rw_run)
-  auto forward = [&](double *V, int nst) -> int {
-    if (!c->mixed) return rw_run(c, V, dt, &fw, nst, st);
-    for (int q = 0; q < nst; ++q)
-      if (int e = rw_composed_step(c, V, dt, &fw, st)) return e;
-    return CGD_OK;
-  };
-  // forward: checkpoints at the segment starts (the last state goes to Vfinal)
   HIPCHK(c, hipMemcpyAsync(ck, V0, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
   for (int s = 1; s <= nseg; ++s) {
     double *cur = seg;  // run the segment in seg[0]
     HIPCHK(c, hipMemcpyAsync(cur, ck + (size_t)(s - 1) * n5, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
     const int nst = std::min(every, nsteps - (s - 1) * every);
-    if ((r = forward(cur, nst))) return r;
+    if ((r = ckpt_steps(c, cur, dt, fw, nst, st))) return r;
     if (s < nseg) HIPCHK(c, hipMemcpyAsync(ck + (size_t)s * n5, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
     else if (Vfinal) HIPCHK(c, hipMemcpyAsync(Vfinal, cur, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
-  // reverse, segment by segment
+  c->ck_work = work;
+  c->ck_dt = dt;
+  c->ck_nsteps = nsteps;
+  c->ck_every = every;
+  return CGD_OK;
+}
+
+// The reverse half: segment by segment, the segment's step states
+// recomputed from its checkpoint, then the steps' adjoints in reverse.
+static int ckpt_reverse(cgd_ctx *c, double dt, int nsteps, int every, double *lamV, double *work, hipStream_t st) {
+  const AdjLayout L = adj_layout(c, nsteps, every);
+  const size_t n5 = L.n5;
+  int r;
+  if ((r = adj_zero(c, work + L.AuxG + 3 * L.ng, L.ng, st))) return r;
+  const cgd_rw_work fw = ckpt_fw(L, work);
+  const int nseg = (nsteps + every - 1) / every;
+  const double *ck = work + L.ck;
+  double *seg = work + L.seg;
   for (int s = nseg; s >= 1; --s) {
     const int nst = std::min(every, nsteps - (s - 1) * every);
     HIPCHK(c, hipMemcpyAsync(seg, ck + (size_t)(s - 1) * n5, n5 * sizeof(double), hipMemcpyDeviceToDevice, st));
     for (int q = 1; q < nst; ++q) {
       HIPCHK(c, hipMemcpyAsync(seg + (size_t)q * n5, seg + (size_t)(q - 1) * n5, n5 * sizeof(double),
                                hipMemcpyDeviceToDevice, st));
-      if ((r = forward(seg + (size_t)q * n5, 1))) return r;
+      if ((r = ckpt_steps(c, seg + (size_t)q * n5, dt, fw, 1, st))) return r;
     }
     for (int q = nst - 1; q >= 0; --q)
       if ((r = rw_step_adjoint(c, seg + (size_t)q * n5, dt, lamV, work, st))) return r;
   }
   return CGD_OK;
+}
+
+static int ckpt_args(cgd_ctx *c, const char *who, int nsteps, int every) {
+  if (nsteps < 1 || every < 1) return fail(c, CGD_ERR_ARG, std::string(who) + ": nsteps and every must be positive");
+  return CGD_OK;
+}
+
+extern "C" int cgd_rw_adjoint(cgd_ctx *c, const double *V0, double dt, int nsteps, int every, double *lamV,
+                              double *Vfinal, double *work, void *stream) {
+  int r = rw_args(c, "cgd_rw_adjoint", V0, dt, lamV, work);
+  if (r || (r = ckpt_args(c, "cgd_rw_adjoint", nsteps, every))) return r;
+  hipStream_t st = S(stream);
+  if ((r = ckpt_forward(c, V0, dt, nsteps, every, Vfinal, work, st))) return r;
+  return ckpt_reverse(c, dt, nsteps, every, lamV, work, st);
+}
+
+extern "C" int cgd_rw_forward_ckpt(cgd_ctx *c, const double *V0, double dt, int nsteps, int every, double *Vfinal,
+                                   double *work, void *stream) {
+  int r = adj_supported(c, "cgd_rw_forward_ckpt");
+  if (r || (r = ckpt_args(c, "cgd_rw_forward_ckpt", nsteps, every))) return r;
+  if (!V0 || !Vfinal || !work) return fail(c, CGD_ERR_ARG, "cgd_rw_forward_ckpt: null array");
+  if (!(dt > 0.0)) return fail(c, CGD_ERR_ARG, "cgd_rw_forward_ckpt: dt must be positive");
+  c->ck_work = nullptr;
+  return ckpt_forward(c, V0, dt, nsteps, every, Vfinal, work, S(stream));
+}
+
+extern "C" int cgd_rw_adjoint_ckpt(cgd_ctx *c, double dt, int nsteps, int every, double *lamV, double *work,
+                                   void *stream) {
+  int r = adj_supported(c, "cgd_rw_adjoint_ckpt");
+  if (r || (r = ckpt_args(c, "cgd_rw_adjoint_ckpt", nsteps, every))) return r;
+  if (!lamV || !work) return fail(c, CGD_ERR_ARG, "cgd_rw_adjoint_ckpt: null array");
+  if (c->ck_work != work || c->ck_dt != dt || c->ck_nsteps != nsteps || c->ck_every != every)
+    return fail(c, CGD_ERR_STATE,
+                "cgd_rw_adjoint_ckpt: no checkpoints for these dt, nsteps, every in this work array "
+                "(run cgd_rw_forward_ckpt with them first)");
+  return ckpt_reverse(c, dt, nsteps, every, lamV, work, S(stream));
 }

@@ -118,6 +118,8 @@ def native():
     L.cgd_jac_adjoint.argtypes = [_vp] * 8
     L.cgd_rw_step_adjoint.argtypes = [_vp, _vp, C.c_double, _vp, _vp, _vp]
     L.cgd_rw_adjoint.argtypes = [_vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp, _vp, _vp]
+    L.cgd_rw_forward_ckpt.argtypes = [_vp, _vp, C.c_double, C.c_int, C.c_int, _vp, _vp, _vp]
+    L.cgd_rw_adjoint_ckpt.argtypes = [_vp, C.c_double, C.c_int, C.c_int, _vp, _vp, _vp]
     L.cgd_setup_error.restype = C.c_char_p
     L.cgd_jacobi_sphere3.argtypes = [C.c_int, C.c_int, C.c_int] + [_vp] * 5 + [C.c_double] * 3 + [_vp] * 4
     L.cgd_mass_cg.argtypes = [C.c_int] * 4 + [_vp] * 6
@@ -133,7 +135,7 @@ def exported_symbols():
             "cgd_autotune", "cgd_rw_graph_create", "cgd_rw_graph_launch", "cgd_setup_error",
             "cgd_jacobi_sphere3", "cgd_mass_cg", "cgd_grid_size_sphere", "cgd_rw_run", "cgd_output_error",
             "cgd_interpolate", "cgd_adjoint_work_size", "cgd_fcn_adjoint", "cgd_schur_adjoint", "cgd_jac_adjoint",
-            "cgd_rw_step_adjoint", "cgd_rw_adjoint", "cgd_set_dtau"]
+            "cgd_rw_step_adjoint", "cgd_rw_adjoint", "cgd_rw_forward_ckpt", "cgd_rw_adjoint_ckpt", "cgd_set_dtau"]
 
 
 def color_elements(glob: np.ndarray, num_g: int):
@@ -465,6 +467,18 @@ class NativeContext:
                                          _ptr(Vfinal) if Vfinal is not None else None, _ptr(work), _stream()),
                    "cgd_rw_adjoint")
 
+    def rw_forward_ckpt(self, V0, dt, nsteps, every, Vfinal, work):
+        """nsteps steps from V0 into Vfinal, keeping a checkpoint every
+        ``every`` steps in ``work`` for rw_adjoint_ckpt."""
+        self.check(self.L.cgd_rw_forward_ckpt(self.h, _ptr(V0), C.c_double(dt), int(nsteps), int(every),
+                                              _ptr(Vfinal), _ptr(work), _stream()), "cgd_rw_forward_ckpt")
+
+    def rw_adjoint_ckpt(self, dt, nsteps, every, lamV, work):
+        """In place: lamV (cotangent of the final state) -> cotangent of V0,
+        from the checkpoints of the last rw_forward_ckpt in ``work``."""
+        self.check(self.L.cgd_rw_adjoint_ckpt(self.h, C.c_double(dt), int(nsteps), int(every), _ptr(lamV),
+                                              _ptr(work), _stream()), "cgd_rw_adjoint_ckpt")
+
     def profile(self, mask=7):
         """Time the passes in ``mask`` (bit 0 pass 1, bit 1 pass 2, bit 2 column; 0 = off)."""
         self.check(self.L.cgd_profile_enable(self.h, int(mask)), "cgd_profile_enable")
@@ -599,27 +613,34 @@ def TimeStepper(U, dt, nIter, FE, Metric, Phys, Cache, JCache, Global, Exchange=
 class RosenbrockSteps(torch.autograd.Function):
     """``nsteps`` Rosenbrock-W steps of a state as a differentiable function
     (the counterpart of a ChainRulesCore ``rrule`` around ``TimeStepper!``):
-    the forward is ``cgd_rw_run`` on a copy of ``V0``; the backward is
-    ``cgd_rw_adjoint`` with a checkpoint every ``every`` steps (it recomputes
-    the trajectory with the composed FP64 calls, which agree with the fused
-    step to rounding).  ``Cache``/``JCache`` are the forward's work arrays."""
+    the forward is ``cgd_rw_forward_ckpt`` (the fused steps, keeping a
+    checkpoint every ``every`` steps in its own work array); the backward is
+    ``cgd_rw_adjoint_ckpt``, which recomputes each segment from its
+    checkpoint and runs the steps' adjoints.  ``Cache``/``JCache`` are kept
+    in the signature for callers of the plain stepper; the forward does not
+    use them.  A second backward through the same forward (retain_graph)
+    runs the forward again first, since the reverse may only use the
+    checkpoints of the last forward on the context."""
 
     @staticmethod
     def forward(fctx, V0, dt, nsteps, every, nctx, Cache, JCache):
-        V = V0.detach().clone()
-        w = _rw_work(Cache, JCache)
-        nctx.check(nctx.L.cgd_rw_run(nctx.h, _ptr(V), C.c_double(dt), C.byref(w), int(nsteps), _stream()),
-                   "cgd_rw_run")
-        fctx.save_for_backward(V0.detach())
-        fctx.args = (dt, nsteps, every, nctx)
+        V0d = V0.detach().contiguous()
+        V = torch.empty_like(V0d)
+        work = nctx.adjoint_work(nsteps, every, device=V0d.device)
+        nctx.rw_forward_ckpt(V0d, dt, nsteps, every, V, work)
+        fctx.save_for_backward(V0d)
+        fctx.args = (dt, nsteps, every, nctx, work)
         return V
 
     @staticmethod
     def backward(fctx, gV):
         (V0,) = fctx.saved_tensors
-        dt, nsteps, every, nctx = fctx.args
+        dt, nsteps, every, nctx, work = fctx.args
         lam = gV.detach().contiguous().clone()
-        nctx.rw_adjoint(V0, dt, nsteps, every, lam)
+        try:
+            nctx.rw_adjoint_ckpt(dt, nsteps, every, lam, work)
+        except CGDError:  # another forward used the context since: start over
+            nctx.rw_adjoint(V0, dt, nsteps, every, lam, work=work)
         return lam, None, None, None, None, None, None
 
 

@@ -329,6 +329,19 @@ int cgd_rw_step_adjoint(cgd_ctx *ctx, const double *Vn, double dt, double *lamV,
 int cgd_rw_adjoint(cgd_ctx *ctx, const double *V0, double dt, int nsteps, int every, double *lamV, double *Vfinal,
                    double *work, void *stream);
 
+/* cgd_rw_adjoint in two halves, for a caller that runs the forward itself
+ * and differentiates later (a ChainRulesCore rrule: the forward pass keeps
+ * its checkpoints, the pullback uses them).  cgd_rw_forward_ckpt: nsteps
+ * steps from V0 into Vfinal, keeping a checkpoint every `every` steps in
+ * work.  cgd_rw_adjoint_ckpt: the reverse sweep from those checkpoints;
+ * lamV as in cgd_rw_adjoint.  The reverse needs the checkpoints of the last
+ * cgd_rw_forward_ckpt (or cgd_rw_adjoint) on this context, with the same
+ * dt, nsteps, every and work array, and fails otherwise; work must not be
+ * used by other calls in between. */
+int cgd_rw_forward_ckpt(cgd_ctx *ctx, const double *V0, double dt, int nsteps, int every, double *Vfinal,
+                        double *work, void *stream);
+int cgd_rw_adjoint_ckpt(cgd_ctx *ctx, double dt, int nsteps, int every, double *lamV, double *work, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -293,6 +293,41 @@ def test_rw_adjoint_checkpoints(pkg, oracle_mod, torch_cuda):
     assert _rel(f, ad) <= FD_TOL
 
 
+def test_rw_adjoint_halves(pkg, oracle_mod, torch_cuda):
+    """cgd_rw_forward_ckpt + cgd_rw_adjoint_ckpt give cgd_rw_adjoint's
+    final state and gradient (the state cotangents are summed with atomics,
+    so to rounding); the reverse refuses checkpoints of another work array,
+    of other dt / nsteps / every, or overwritten by a later forward."""
+    torch = torch_cuda
+    host = pkg.host
+    disc, phys, U0, (FE, Metric, Global, Cache, JCache), orc, ctx = _setup(pkg, oracle_mod, 3, 8)
+    print("rw adjoint halves ne=3 nz=8 nsteps=5", flush=True)
+    dt, nsteps, every = 150.0, 5, 2
+    lamV = torch.as_tensor(np.random.default_rng(11).standard_normal(U0.shape), device="cuda")
+    V0 = torch.as_tensor(U0, device="cuda")
+    lam1, Vf1 = lamV.clone(), torch.empty_like(V0)
+    ctx.rw_adjoint(V0, dt, nsteps, every, lam1, Vfinal=Vf1)
+    work = ctx.adjoint_work(nsteps, every)
+    Vf2, lam2 = torch.empty_like(V0), lamV.clone()
+    ctx.rw_forward_ckpt(V0, dt, nsteps, every, Vf2, work)
+    ctx.rw_adjoint_ckpt(dt, nsteps, every, lam2, work)
+    torch.cuda.synchronize()
+    assert torch.equal(Vf1, Vf2)
+    e = float((lam2 - lam1).norm() / lam1.norm())
+    print(f"  halves vs one call: rel {e:.2e}", flush=True)
+    assert e <= 1e-12
+    for args in ((dt * 2, nsteps, every), (dt, nsteps + 1, every), (dt, nsteps, every + 1)):
+        with pytest.raises(host.CGDError, match="no checkpoints"):
+            ctx.rw_adjoint_ckpt(*args, lamV.clone(), work)
+    with pytest.raises(host.CGDError, match="no checkpoints"):
+        ctx.rw_adjoint_ckpt(dt, nsteps, every, lamV.clone(), ctx.adjoint_work(nsteps, every))
+    other = ctx.adjoint_work(nsteps, every)
+    ctx.rw_forward_ckpt(V0, dt, nsteps, every, torch.empty_like(V0), other)
+    with pytest.raises(host.CGDError, match="no checkpoints"):
+        ctx.rw_adjoint_ckpt(dt, nsteps, every, lamV.clone(), work)
+    torch.cuda.synchronize()
+
+
 def test_autograd_function(pkg, oracle_mod, torch_cuda):
     """RosenbrockSteps as a torch.autograd.Function: the gradient of the sum
     of rho*theta^2 = (rho theta)^2 / rho over the lowest level against
@@ -308,8 +343,16 @@ def test_autograd_function(pkg, oracle_mod, torch_cuda):
 
     V0 = torch.as_tensor(U0, device="cuda").clone().requires_grad_(True)
     V = host.RosenbrockSteps.apply(V0, dt, nsteps, every, ctx, Cache, JCache)
-    functional(V).backward()
+    functional(V).backward(retain_graph=True)
     g = V0.grad.detach().cpu().numpy()
+    # a second forward on the context, then the first graph's backward again:
+    # its checkpoints are no longer the context's last, so it starts over
+    V0b = torch.as_tensor(U0, device="cuda").clone().requires_grad_(True)
+    host.RosenbrockSteps.apply(V0b, dt, nsteps, every, ctx, Cache, JCache)
+    V0.grad = None
+    functional(V).backward()
+    g2 = V0.grad.detach().cpu().numpy()
+    assert np.linalg.norm(g2 - g) <= 1e-12 * np.linalg.norm(g)
     d = direction(U0.shape, 6)
     dev = (FE, Metric, Global, Cache, JCache, phys)
 
