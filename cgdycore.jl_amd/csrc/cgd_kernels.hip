@@ -485,7 +485,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
 // gradient and gravity (rho, p, J and dXdxI[3,3] in FP64) and the
 // horizontal pressure gradient (an FP64 tile and derivative).
 template <int N, int NZT, bool CM, typename R>
-__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, int slot0, int nchunk,
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) __attribute__((amdgpu_waves_per_eu(4))) k_fcn2(KArgs a, int slot0, int nchunk,
                                                const double *__restrict__ U, const R *__restrict__ UR,
                                                const R *__restrict__ C, const double *__restrict__ P,
                                                R *__restrict__ F, R *__restrict__ F4x) {
@@ -500,11 +500,12 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   Tiles<N, NZT, double> TP{smem + NN};
   R *Ds = reinterpret_cast<R *>(smem + NN + NN * NZT), *DWs = Ds + NN;
   Tiles<N, NZT, R> T{Ds + 2 * NN}, TB{Ds + 2 * NN + 7 * NN * NZT};
-  // compact metric layout in FP32: the horizontal components d[1:2][1:2]K
-  // and their sums S_ab are per-node constants first used in phase 1, so
-  // they live in an LDS table (written now, read after phase 1's barrier)
-  // instead of registers for the whole kernel (5 waves per SIMD instead of
-  // 4); in FP64 the extra LDS reads cost more than the registers save
+  // compact metric layout: the horizontal components d[1:2][1:2]K and their
+  // sums S_ab are per-node constants first used in phase 1, so they live in
+  // an LDS table (written now, read after phase 1's barrier) instead of
+  // registers for the whole kernel.  With the DSS accumulators read only at
+  // the end, that brings the FP64 kernel to 128 VGPRs and 4 waves per SIMD
+  // (from 160 and 3), and the FP32 one to 84 (5 waves)
   R *mt = TB.t + 7 * NN * NZT;
   load_dmats<N>(a, Ds, DWs);
   if (threadIdx.x < NN) Dd[threadIdx.x] = a.D[threadIdx.x];
@@ -573,14 +574,14 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     M1 = R(at(M1b, ui));
     M2 = R(at(M2b, ui));
     p = at(P, ui);
-    if constexpr (!(CM && F32)) {
+    if constexpr (!CM) {
       d111 = R(MET0(M111)); d112 = R(MET0(M112)); d121 = R(MET0(M121)); d122 = R(MET0(M122));
       d211 = R(MET0(M211)); d212 = R(MET0(M212)); d221 = R(MET0(M221)); d222 = R(MET0(M222));
     }
     d311 = R(MET0(M311)); d312 = R(MET0(M312)); d321 = R(MET0(M321)); d322 = R(MET0(M322));
     d331 = MET0(M331); d332 = MET0(M332); J1 = MET0(MJ1); J2 = MET0(MJ2); Wc = R(MET0(MCOR));
   }
-  if constexpr (CM && F32) {
+  if constexpr (CM) {
     if (in_tile && l == 0) {
       R hm[8];
 #pragma unroll
@@ -601,16 +602,6 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
   // hyperviscosity of this element's grid (members of an ensemble differ)
   const double *kv = a.kvis + 3 * e;
   const R Kc = R(kv[0]), Kg = R(kv[1]), Kd = R(kv[2]);
-  // the DSS accumulators of this node (untouched by the other elements of
-  // the colour), read with the state rather than after the LDS phases
-  R o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0;
-  if (valid && !first) {
-    o1 = at(F, ui);
-    o2 = at(F + vs, ui);
-    o3 = at(F + 2 * vs, ui);
-    o4 = at(F + 3 * vs, ui);
-    o5 = at(F + 4 * vs, ui);
-  }
 
   // ---- vertical neighbours ----
   // A node's chunk is one 16-lane DPP row (or half of one for 8-level
@@ -763,7 +754,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     W1 = h * ((d322 * u - d312 * v) - (d321 * u - d311 * v));
   }
   const R S11r = d111 + d112, S12r = d121 + d122, S21r = d211 + d212, S22r = d221 + d222;
-#define HM(c, reg) (CM && F32 ? mt[(c) * NN + node] : (reg))
+#define HM(c, reg) (CM ? mt[(c) * NN + node] : (reg))
 #define D111 HM(0, d111)
 #define D112 HM(1, d112)
 #define D121 HM(2, d121)
@@ -931,6 +922,16 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn2(KArgs a, 
     // DSS: the first element (in colour order) holding the node writes, the
     // others accumulate; the w lid (iz = Nz-1) receives no tendency
     R *Fr = F, *Fu = F + vs, *Fv = F + 2 * vs, *Fw = F + 3 * vs, *Ft = F + 4 * vs;
+    // the DSS accumulators of this node (no other element of the colour
+    // touches them), read only now so they hold no registers meanwhile
+    R o1 = 0, o2 = 0, o3 = 0, o4 = 0, o5 = 0;
+    if (!first) {
+      o1 = at(F, ui);
+      o2 = at(F + vs, ui);
+      o3 = at(F + 2 * vs, ui);
+      o4 = at(F + 3 * vs, ui);
+      o5 = at(F + 4 * vs, ui);
+    }
     at(Fr, ui) = first ? f1 : o1 + f1;
     at(Fu, ui) = first ? f2 : o2 + f2;
     at(Fv, ui) = first ? f3 : o3 + f3;
