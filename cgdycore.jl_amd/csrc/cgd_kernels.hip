@@ -504,8 +504,9 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) __attribute__((a
   // sums S_ab are per-node constants first used in phase 1, so they live in
   // an LDS table (written now, read after phase 1's barrier) instead of
   // registers for the whole kernel.  With the DSS accumulators read only at
-  // the end, that brings the FP64 kernel to 128 VGPRs and 4 waves per SIMD
-  // (from 160 and 3), and the FP32 one to 84 (5 waves)
+  // the end and d331, d332 read again for the gravity terms, that brings the
+  // FP64 kernel to 125 VGPRs and 4 waves per SIMD without spills (from 160
+  // and 3), and the FP32 one to 84 (5 waves)
   R *mt = TB.t + 7 * NN * NZT;
   load_dmats<N>(a, Ds, DWs);
   if (threadIdx.x < NN) Dd[threadIdx.x] = a.D[threadIdx.x];
@@ -853,28 +854,32 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) __attribute__((a
     R GuF2 = -(D112 * DXK2 + D212 * DYK2);
     R GvF2 = -(D122 * DXK2 + D222 * DYK2);
     R GwF1 = 0, GwF2 = 0;
+    // d331, d332 again (per node in the compact layout: a cached load
+    // rather than two registers held since phase 0)
+    const double d331L = CM ? double(METN(M331)) : d331, d332L = CM ? double(METN(M332)) : d332;
+    const R d331Lr = R(d331L), d332Lr = R(d332L);
     if (has_lo) {
       GuF1 += -GZlo * d311;
       GvF1 += -GZlo * d321;
-      GwF1 += -rho_r * GZlo * d331_r;
+      GwF1 += -rho_r * GZlo * d331Lr;
     }
     if (has_hi) {
       GuF2 += -GZup * d312;
       GvF2 += -GZup * d322;
-      GwF2 += -rho_r * GZup * d332_r;
+      GwF2 += -rho_r * GZup * d332Lr;
     }
     const R GradZ = h * (K2 - K1);
     GuF2 += -GradZ * d312;
     GuF1 += -GradZ * d311;
     GvF2 += -GradZ * d322;
     GvF1 += -GradZ * d321;
-    GwF2 += -rho_r * GradZ * d332_r;
-    GwF1 += -rho_r * GradZ * d331_r;
+    GwF2 += -rho_r * GradZ * d332Lr;
+    GwF1 += -rho_r * GradZ * d331Lr;
     // horizontal pressure gradient from the FP64 pressure tile
     const double DXp = dx<N, NZT, double>(Dd, TP, 0, I, J, l), DYp = dy<N, NZT, double>(Dd, TP, 0, I, J, l);
     double Gu = S11 * DXp + S21 * DYp;
     double Gv = S12 * DXp + S22 * DYp;
-    const R GZg = R(-Grav * rho * double(Js) * rcp(d331 + d332));
+    const R GZg = R(-Grav * rho * double(Js) * rcp(d331L + d332L));
     Gu += GZg * (d311 + d312);
     Gv += GZg * (d321 + d322);
     f2 += (GuF1 + GuF2 - R(Gu) * irho) * iMs;
