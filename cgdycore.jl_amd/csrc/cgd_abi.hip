@@ -1272,7 +1272,15 @@ extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work 
     // the fastest of a set of shapes: two sweeps over all of them, each
     // shape keeping its better time, so a drift of the device's speed during
     // a sweep does not favour the shapes timed in its fast part
-    auto pick = [&](const std::vector<cgd_launch_shape> &cand) -> int {
+    // The shape the context had keeps its place unless another is faster by
+    // more than 1 %: short timings rank shapes within that margin in either
+    // order, and a switch on noise has cost more over long runs than it won.
+    const cgd_launch_shape entry = best;
+    auto same = [](const cgd_launch_shape &x, const cgd_launch_shape &y) {
+      return x.chunk_levels == y.chunk_levels && x.pass1_chunk_levels == y.pass1_chunk_levels &&
+             x.pass1_items == y.pass1_items && x.col_lanes == y.col_lanes && x.col_block == y.col_block;
+    };
+    auto pick = [&](const std::vector<cgd_launch_shape> &cand, const cgd_launch_shape &keep) -> int {
       std::vector<double> tmin(cand.size(), 1e300);
       for (int sweep = 0; sweep < 2; ++sweep)
         for (size_t i = 0; i < cand.size(); ++i) {
@@ -1281,11 +1289,15 @@ extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work 
           if ((rr = time_shape(cand[i], per))) return rr;
           tmin[i] = std::min(tmin[i], per);
         }
+      size_t ib = 0;
+      for (size_t i = 1; i < cand.size(); ++i)
+        if (tmin[i] < tmin[ib]) ib = i;
       for (size_t i = 0; i < cand.size(); ++i)
-        if (tmin[i] < bms) {
-          bms = tmin[i];
-          best = cand[i];
-        }
+        if (same(cand[i], keep) && tmin[i] <= 1.01 * tmin[ib]) ib = i;
+      if (!cand.empty()) {
+        bms = tmin[ib];
+        best = cand[ib];
+      }
       return CGD_OK;
     };
     // the element kernels' shapes jointly (both passes' chunk heights and
@@ -1301,7 +1313,7 @@ extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work 
           sh.pass1_items = it;
           if (shape_ok(c, sh)) cand.push_back(sh);
         }
-    if ((rr = pick(cand))) return rr;
+    if ((rr = pick(cand, entry))) return rr;
     const cgd_launch_shape b1 = best;
     cand.clear();
     for (int l : {64, 32})
@@ -1311,7 +1323,12 @@ extern "C" int cgd_autotune(cgd_ctx *c, double *V, double dt, const cgd_rw_work 
         sh.col_block = b;
         if (shape_ok(c, sh)) cand.push_back(sh);
       }
-    if ((rr = pick(cand))) return rr;
+    {
+      cgd_launch_shape keep = b1;  // the entry shape's columns with the chosen element shape
+      keep.col_lanes = entry.col_lanes;
+      keep.col_block = entry.col_block;
+      if ((rr = pick(cand, keep))) return rr;
+    }
     return CGD_OK;
   };
   r = run();

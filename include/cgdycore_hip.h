@@ -201,10 +201,12 @@ int cgd_set_launch_shape(cgd_ctx *ctx, const cgd_launch_shape *shape);
 int cgd_get_launch_shape(const cgd_ctx *ctx, cgd_launch_shape *shape);
 
 /* Autotuner: times cgd_rw_step per candidate launch shape on the caller's V
- * and work buffers (the fastest of three timings of reps steps) and keeps
- * the fastest: the element-kernel shapes jointly (both chunk heights and
- * pass1_items), then the column shape with the best of them.  V is restored afterwards (the
- * work buffers are scratch).  A set-up call: it allocates a copy of V and
+ * and work buffers (per shape the fastest of three timings of reps steps,
+ * over two sweeps through the candidates) and keeps the fastest: the
+ * element-kernel shapes jointly (both chunk heights and pass1_items), then
+ * the column shape with the best of them.  The context's shape on entry is
+ * kept unless a candidate beats it by more than 1 %.  V, all components,
+ * is restored afterwards (the work buffers are scratch).  A set-up call: it allocates a copy of V and
  * synchronises the stream.  chosen (may be null) receives the shape, best_ms
  * the time per step.  No reference counterpart. */
 int cgd_autotune(cgd_ctx *ctx, double *V, double dt, const cgd_rw_work *work, int reps, void *stream,
