@@ -485,7 +485,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) k_fcn1(KArgs a, 
 // gradient and gravity (rho, p, J and dXdxI[3,3] in FP64) and the
 // horizontal pressure gradient (an FP64 tile and derivative).
 template <int N, int NZT, bool CM, typename R>
-__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) __attribute__((amdgpu_waves_per_eu(4))) k_fcn2(KArgs a, int slot0, int nchunk,
+__global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) __attribute__((amdgpu_waves_per_eu(std::is_same<R, float>::value ? 6 : 4))) k_fcn2(KArgs a, int slot0, int nchunk,
                                                const double *__restrict__ U, const R *__restrict__ UR,
                                                const R *__restrict__ C, const double *__restrict__ P,
                                                R *__restrict__ F, R *__restrict__ F4x) {
@@ -506,7 +506,7 @@ __global__ void __launch_bounds__((N * N * NZT + 63) / 64 * 64) __attribute__((a
   // registers for the whole kernel.  With the DSS accumulators read only at
   // the end and d331, d332 read again for the gravity terms, that brings the
   // FP64 kernel to 125 VGPRs and 4 waves per SIMD without spills (from 160
-  // and 3), and the FP32 one to 84 (5 waves)
+  // and 3), and the FP32 one to 77 and 6 waves
   R *mt = TB.t + 7 * NN * NZT;
   load_dmats<N>(a, Ds, DWs);
   if (threadIdx.x < NN) Dd[threadIdx.x] = a.D[threadIdx.x];
