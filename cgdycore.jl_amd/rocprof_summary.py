@@ -5,8 +5,12 @@
         --fetch DB --write DB --cal-fetch DB --cal-write DB --copy-bytes B \
         --workload W --algorithmic BYTES --command-fetch CMD --command-write CMD --out FILE.json
 
+    python cgdycore.jl_amd/rocprof_summary.py sq DB --kernel k_fcn2 --command CMD --out FILE
+
 ``kernels`` tabulates calls, total and average duration per kernel from a
-``--kernel-trace --stats`` run.  ``traffic`` turns the per-dispatch
+``--kernel-trace --stats`` run.  ``sq`` sums the SQ counters of one ``--pmc``
+run over a kernel's dispatches and gives each wave-cycle counter's share of
+SQ_WAVE_CYCLES.  ``traffic`` turns the per-dispatch
 FETCH_SIZE / WRITE_SIZE counters (KiB) of two ``--pmc`` runs into HBM bytes
 per tendency evaluation, each scaled by what the same counter reads for a
 plain device copy of known size (the calibration runs).
@@ -85,6 +89,30 @@ def write_traffic(a):
     return res
 
 
+def write_sq(db, kernel, command, out):
+    """Sums of the SQ counters over the dispatches of `kernel`, and the wave
+    time shares (waiting, waiting for issue, issuing)."""
+    rows = _rows(db, "select kernel_name, counter_name, value from counters_collection where counter_name like 'SQ_%'")
+    sums, disp = {}, {}
+    for n, c, v in rows:
+        if kernel in n:
+            sums[c] = sums.get(c, 0.0) + float(v)
+            disp[c] = disp.get(c, 0) + 1
+    name = next((n for n, _, _ in rows if kernel in n), kernel)
+    lines = [f"# rocprofv3 SQ counters of {short_name(name)}, {max(disp.values(), default=0)} dispatches",
+             f"# command: {command}",
+             "# wave-cycle counters are summed over waves (quad-cycle units); shares are of SQ_WAVE_CYCLES"]
+    for c in sorted(sums):
+        lines.append(f"{c:22s} {sums[c]:.4e}")
+    wc = sums.get("SQ_WAVE_CYCLES", 0.0)
+    if wc > 0:
+        for c in ("SQ_WAIT_ANY", "SQ_WAIT_INST_ANY", "SQ_ACTIVE_INST_ANY", "SQ_ACTIVE_INST_VALU", "SQ_ACTIVE_INST_LDS"):
+            if c in sums:
+                lines.append(f"share {c:20s} {sums[c] / wc:.3f}")
+    open(out, "w").write("\n".join(lines) + "\n")
+    return lines
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -99,8 +127,15 @@ def main(argv=None):
     t.add_argument("--copy-bytes", type=int, required=True)
     t.add_argument("--per-eval", type=int, required=True)
     t.add_argument("--algorithmic", type=int, default=0)
+    q = sub.add_parser("sq")
+    q.add_argument("db")
+    q.add_argument("--kernel", required=True)
+    q.add_argument("--command", required=True)
+    q.add_argument("--out", required=True)
     a = ap.parse_args(argv)
-    if a.cmd == "kernels":
+    if a.cmd == "sq":
+        print("\n".join(write_sq(a.db, a.kernel, a.command, a.out)))
+    elif a.cmd == "kernels":
         for row in write_kernels(a.db, a.command, a.out)[:8]:
             print(f"{short_name(row[0]):62s} {row[1]:6d} {row[3]:9.2f}")
     else:
