@@ -981,6 +981,30 @@ __global__ void k_adj_axpy(double *__restrict__ y, const double *__restrict__ x,
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
     y[i] += s * x[i];
 }
+// y_j += s_j x for j < nout (up to three cotangents fed by one x)
+__global__ void k_adj_axpy_multi(double *__restrict__ y0, double s0, double *__restrict__ y1, double s1,
+                                 double *__restrict__ y2, double s2, int nout, const double *__restrict__ x, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double xi = x[i];
+    y0[i] += s0 * xi;
+    if (nout > 1) y1[i] += s1 * xi;
+    if (nout > 2) y2[i] += s2 * xi;
+  }
+}
+
+// k_axpy into two outputs: y = y2 = x0 + a1 x1 + a2 x2 (first nterm terms)
+__global__ void k_axpy_dual(double *__restrict__ y, double *__restrict__ y2, const double *__restrict__ x0,
+                            const double *__restrict__ x1, double a1, const double *__restrict__ x2, double a2,
+                            int nterm, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double r = x0[i];
+    if (nterm >= 1) r = r + a1 * x1[i];
+    if (nterm >= 2) r = r + a2 * x2[i];
+    y[i] = r;
+    y2[i] = r;
+  }
+}
+
 // y = a x
 __global__ void k_adj_scale(double *__restrict__ y, const double *__restrict__ x, double s, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -1018,6 +1042,7 @@ struct AdjLayout {
   size_t F, T1, AuxG, Lam, lT1;                        // tendency adjoint
   size_t bw, Vi, fV, k, lk, lVi, lfV, Jb, tri, lJ;     // step adjoint
   size_t Fs, T1s;                                      // the three stages' tendencies and caches
+  size_t Vs, fVs;                                      // the three stage states and solve right-hand sides
   size_t ck, seg;                                      // runs: checkpoints, segment states
   size_t total;
 };
@@ -1044,6 +1069,8 @@ static AdjLayout adj_layout(const cgd_ctx *c, int nsteps, int every) {
   L.lJ = o; o += 4 * L.nj;
   L.Fs = o; o += 3 * L.n5;
   L.T1s = o; o += 3 * L.n5;
+  L.Vs = o; o += 3 * L.n5;
+  L.fVs = o; o += 3 * L.n5;
   L.ck = o;
   if (nsteps > 0) {
     const int nck = (nsteps + every - 1) / every;
@@ -1178,7 +1205,7 @@ static int rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, double *lamV
   const double fac = dt;  // dt * gamma, gamma = 1
   const AdjLayout L = adj_layout(c, 0, 1);
   const size_t n5 = L.n5, nj = L.nj;
-  double *AuxG = work + L.AuxG, *F = work + L.F, *T1 = work + L.T1, *Vi = work + L.Vi, *fV = work + L.fV;
+  double *AuxG = work + L.AuxG, *F = work + L.F, *T1 = work + L.T1, *fV = work + L.fV;
   double *k[3] = {work + L.k, work + L.k + n5, work + L.k + 2 * n5};
   double *lk[3] = {work + L.lk, work + L.lk + n5, work + L.lk + 2 * n5};
   double *lVi = work + L.lVi, *lfV = work + L.lfV;
@@ -1188,18 +1215,23 @@ static int rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, double *lamV
   const dim3 gb(grid_for(n5, 256)), bb(256);
   double *Fs[3] = {work + L.Fs, work + L.Fs + n5, work + L.Fs + 2 * n5};
   double *T1s[3] = {work + L.T1s, work + L.T1s + n5, work + L.T1s + 2 * n5};
+  double *Vs[3] = {work + L.Vs, work + L.Vs + n5, work + L.Vs + 2 * n5};
+  double *fVs[3] = {work + L.fVs, work + L.fVs + n5, work + L.fVs + 2 * n5};
   (void)F;
   (void)T1;
   // forward: the Jacobian of Vn and the three stage increments; each
   // stage's tendency and pass-1 cache are kept for its reverse
   if ((r = cgd_prepare(c, Vn, AuxG, st))) return r;
   if ((r = cgd_jac(c, Jb[0], Jb[1], Jb[2], Jb[3], Vn, AuxG, st))) return r;
+  // (the stage states and the solves' right-hand sides are kept too, for
+  // the reverse; the solve overwrites its own copy of the latter)
   for (int is = 0; is < 3; ++is) {
-    hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, Vi, Vn, k[0], A[is][0], k[1], A[is][1], is, n5);
+    hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, Vs[is], Vn, k[0], A[is][0], k[1], A[is][1], is, n5);
     HIPCHK(c, hipGetLastError());
-    if ((r = cgd_prepare(c, Vi, AuxG, st))) return r;
-    if ((r = run_fcn<double>(c, Fs[is], Vi, Vi, T1s[is], AuxG, st, true))) return r;
-    hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, fV, Fs[is], k[0], Cc[is][0] / dt, k[1], Cc[is][1] / dt, is, n5);
+    if ((r = cgd_prepare(c, Vs[is], AuxG, st))) return r;
+    if ((r = run_fcn<double>(c, Fs[is], Vs[is], Vs[is], T1s[is], AuxG, st, true))) return r;
+    hipLaunchKernelGGL(k_axpy_dual, gb, bb, 0, st, fV, fVs[is], Fs[is], k[0], Cc[is][0] / dt, k[1], Cc[is][1] / dt,
+                       is, n5);
     HIPCHK(c, hipGetLastError());
     if ((r = cgd_schur(c, k[is], fV, Jb[0], Jb[1], Jb[2], Jb[3], tri, fac, is == 0, st))) return r;
   }
@@ -1210,18 +1242,16 @@ static int rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, double *lamV
   }
   if ((r = adj_zero(c, lJ[0], 4 * nj, st))) return r;
   for (int is = 2; is >= 0; --is) {
-    hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, Vi, Vn, k[0], A[is][0], k[1], A[is][1], is, n5);
+    const double *Vi_ = Vs[is];
+    launch_schur_adj(c, st, lk[is], k[is], fVs[is], Jb, fac, lfV, lJ, work + L.bw);
     HIPCHK(c, hipGetLastError());
-    hipLaunchKernelGGL(k_axpy, gb, bb, 0, st, fV, Fs[is], k[0], Cc[is][0] / dt, k[1], Cc[is][1] / dt, is, n5);
-    HIPCHK(c, hipGetLastError());
-    launch_schur_adj(c, st, lk[is], k[is], fV, Jb, fac, lfV, lJ, work + L.bw);
-    HIPCHK(c, hipGetLastError());
-    for (int j = 0; j < is; ++j) {
-      hipLaunchKernelGGL(k_adj_axpy, gb, bb, 0, st, lk[j], lfV, Cc[is][j] / dt, n5);
+    if (is > 0) {  // lk[j] += (c_ij / dt) lfV, j < is
+      hipLaunchKernelGGL(k_adj_axpy_multi, gb, bb, 0, st, lk[0], Cc[is][0] / dt, lk[1], is > 1 ? Cc[is][1] / dt : 0.0,
+                         lk[0], 0.0, is, lfV, n5);
       HIPCHK(c, hipGetLastError());
     }
-    // the tendency's adjoint at Vi from its kept forward values
-    hipLaunchKernelGGL(k_adj_pre, dim3(grid_for(L.ng, 256)), dim3(256), 0, st, c->ka, Vi, Fs[is], lfV, work + L.Lam,
+    // the tendency's adjoint at the stage state from its kept forward values
+    hipLaunchKernelGGL(k_adj_pre, dim3(grid_for(L.ng, 256)), dim3(256), 0, st, c->ka, Vi_, Fs[is], lfV, work + L.Lam,
                        lVi, work + L.lT1, L.ng);
     HIPCHK(c, hipGetLastError());
     {
@@ -1230,29 +1260,27 @@ static int rw_step_adjoint(cgd_ctx *c, const double *Vn, double dt, double *lamV
       double *Lam = work + L.Lam, *lT1 = work + L.lT1;
       if (c->N == 5) {
         if (c->cm) {
-          hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1s[is], Lam, lVi, lT1);
-          hipLaunchKernelGGL((k_adj1<5, ADJ_NZT, true>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
+          hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi_, T1s[is], Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj1<5, ADJ_NZT, true>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi_, lT1, lVi);
         } else {
-          hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1s[is], Lam, lVi, lT1);
-          hipLaunchKernelGGL((k_adj1<5, ADJ_NZT, false>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
+          hipLaunchKernelGGL((k_adj2<5, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi_, T1s[is], Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj1<5, ADJ_NZT, false>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi_, lT1, lVi);
         }
       } else {
         if (c->cm) {
-          hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1s[is], Lam, lVi, lT1);
-          hipLaunchKernelGGL((k_adj1<4, ADJ_NZT, true>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
+          hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, true>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi_, T1s[is], Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj1<4, ADJ_NZT, true>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi_, lT1, lVi);
         } else {
-          hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi, T1s[is], Lam, lVi, lT1);
-          hipLaunchKernelGGL((k_adj1<4, ADJ_NZT, false>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi, lT1, lVi);
+          hipLaunchKernelGGL((k_adj2<4, ADJ_NZT, false>), grid, block, adj_lds2(c), st, c->ka, nchunk, Vi_, T1s[is], Lam, lVi, lT1);
+          hipLaunchKernelGGL((k_adj1<4, ADJ_NZT, false>), grid, block, adj_lds1(c), st, c->ka, nchunk, Vi_, lT1, lVi);
         }
       }
       HIPCHK(c, hipGetLastError());
     }
-    hipLaunchKernelGGL(k_adj_axpy, gb, bb, 0, st, lamV, lVi, 1.0, n5);
+    // lamV += lVi and lk[j] += a_ij lVi, j < is
+    hipLaunchKernelGGL(k_adj_axpy_multi, gb, bb, 0, st, lamV, 1.0, lk[0], is > 0 ? A[is][0] : 0.0, lk[1],
+                       is > 1 ? A[is][1] : 0.0, 1 + is, lVi, n5);
     HIPCHK(c, hipGetLastError());
-    for (int j = 0; j < is; ++j) {
-      hipLaunchKernelGGL(k_adj_axpy, gb, bb, 0, st, lk[j], lVi, A[is][j], n5);
-      HIPCHK(c, hipGetLastError());
-    }
   }
   launch_jac_adj(c, st, Vn, lJ, lamV);
   HIPCHK(c, hipGetLastError());
